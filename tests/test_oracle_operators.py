@@ -1,0 +1,189 @@
+"""Pins the oracle's operator applies against mathematics that does not
+restate the kernel formula: sphere spectra, transposition identities, the
+Gauss law, Funk-Hecke, exterior closed forms and finite differences."""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import spherical_jn
+
+import oracle
+from baemm_amd.meshes import real_sph_harm
+from conftest import oracle_mesh, relerr
+
+SL = [0, 1, 0, 0]
+DL = [0, 0, 1, 0]
+ADL = [0, 0, 0, 1]
+
+
+def _apply(m, coeff, X, kappa):
+    X = np.asarray(X, dtype=np.complex128).reshape(m.nv, -1)
+    W = X.shape[1]
+    return oracle.apply_boundary(m, 1.0, X, 0.0, np.zeros_like(X), [kappa], coeff, W, W)
+
+
+@pytest.mark.parametrize("kappa", [1e-9, 2.0, 4.0])
+def test_sphere_spectra_converge(kappa):
+    """A_S, A_K, A_K' applied to the P1 interpolant of Y_l approximate
+    lambda_l M u (Bessel/Hankel eigenvalues); the error shrinks under refinement."""
+    errs = {}
+    for L in (3, 4):
+        m = oracle_mesh("sphere", L)
+        Y = real_sph_harm(2, m.V)
+        for (l, mm, y) in (Y[0], Y[2], Y[6]):
+            lam = oracle.sphere_eigenvalues(l, 0.0 if kappa < 1e-6 else kappa)
+            for name, coeff, ev in (("S", SL, lam[0]), ("K", DL, lam[1]), ("K'", ADL, lam[2])):
+                C = _apply(m, coeff, y, kappa)
+                ref = ev * (m.M @ y)
+                errs[(L, l, name)] = relerr(C[:, 0], ref)
+    for key, e in errs.items():
+        if key[0] == 4:
+            assert e < (0.08 if key[2] != "S" else 0.02), (key, e)
+    mean3 = np.mean([e for k, e in errs.items() if k[0] == 3])
+    mean4 = np.mean([e for k, e in errs.items() if k[0] == 4])
+    assert mean4 < 0.75 * mean3
+
+
+def test_transposition_identities():
+    """A_S^T = A_S and A_K^T = A_K' (bilinear, no conjugation)."""
+    m = oracle_mesh("perturbed", 3)
+    rng = np.random.default_rng(0)
+    u = rng.standard_normal(m.nv) + 1j * rng.standard_normal(m.nv)
+    v = rng.standard_normal(m.nv) + 1j * rng.standard_normal(m.nv)
+    for kappa in (0.7, 3.0):
+        a = v @ _apply(m, SL, u, kappa)[:, 0]
+        b = u @ _apply(m, SL, v, kappa)[:, 0]
+        assert abs(a - b) < 1e-12 * abs(a)
+        a = v @ _apply(m, DL, u, kappa)[:, 0]
+        b = u @ _apply(m, ADL, v, kappa)[:, 0]
+        assert abs(a - b) < 1e-12 * abs(a)
+
+
+def test_gauss_law_pins_double_layer():
+    """K 1 = -1/2 on any closed surface at kappa = 0 (outward normal); the adjoint
+    double layer does not satisfy it on a torus, so DL and adjDL cannot be swapped."""
+    res = []
+    for L in (10, 20):
+        m = oracle_mesh("torus", L)
+        one = np.ones(m.nv)
+        Kone = m.mass_solve(_apply(m, DL, one, 1e-9)[:, 0])
+        Kpone = m.mass_solve(_apply(m, ADL, one, 1e-9)[:, 0])
+        res.append((np.sqrt(np.mean(np.abs(Kone + 0.5) ** 2)), np.sqrt(np.mean(np.abs(Kpone + 0.5) ** 2))))
+    assert res[1][0] < 0.012 and res[1][0] < 0.7 * res[0][0]
+    assert res[1][1] > 0.1
+
+
+@pytest.mark.parametrize("kappa", [2.0, 5.0])
+def test_far_field_funk_hecke(kappa):
+    """FF of the density e^{i k d.y} on S^2:  SL: j0(k|xhat - d|);
+    DL: -i k xhat.(w/|w|) * (-j1(k|w|)) / i, w = d - xhat."""
+    d = np.array([0.3, 0.5, math.sqrt(1 - 0.34)])
+    prev = None
+    for L in (3, 4):
+        m = oracle_mesh("sphere", L, 64)
+        B = np.exp(1j * kappa * m.V @ d)[:, None]
+        F = oracle.apply_far_field(m, 1.0, B, 0.0, np.zeros((64, 1)), [kappa], SL, 1, 1)
+        e1 = relerr(F[:, 0], oracle.j0(kappa * np.linalg.norm(m.meas - d, axis=1)))
+        F2 = oracle.apply_far_field(m, 1.0, B, 0.0, np.zeros((64, 1)), [kappa], DL, 1, 1)
+        w = d - m.meas
+        wn = np.linalg.norm(w, axis=1)
+        g = -spherical_jn(1, kappa * wn) / 1j
+        ref2 = -1j * kappa * np.einsum("ij,ij->i", m.meas, w / wn[:, None]) * g
+        e2 = relerr(F2[:, 0], ref2)
+        if prev is not None:
+            assert e1 < 0.35 * prev[0] and e2 < 0.35 * prev[1]
+        prev = (e1, e2)
+    assert prev[0] < 0.01 and prev[1] < 0.01
+
+
+@pytest.mark.parametrize("kappa", [1.5, 4.0])
+def test_near_field_exterior_closed_forms(kappa):
+    """Density 1 on the unit sphere, exterior point p (r = |p|):
+    S[1](p) = j0(k) e^{ikr}/r,   D[1](p) = -k j1(k) e^{ikr}/r;  O(h^2) convergence."""
+    rng = np.random.default_rng(5)
+    P = rng.standard_normal((20, 3))
+    P *= (1.5 + 2 * rng.random(20))[:, None] / np.linalg.norm(P, axis=1, keepdims=True)
+    r = np.linalg.norm(P, axis=1)
+    errs = []
+    for L in (3, 4):
+        m = oracle_mesh("sphere", L)
+        one = np.ones((m.nv, 1))
+        U = oracle.apply_near_field(m, 1.0, one, 0.0, np.zeros((20, 1)), [kappa], SL, 1, 1, P)
+        e1 = relerr(U[:, 0], oracle.j0(kappa) * np.exp(1j * kappa * r) / r)
+        U = oracle.apply_near_field(m, 1.0, one, 0.0, np.zeros((20, 1)), [kappa], DL, 1, 1, P)
+        e2 = relerr(U[:, 0], -kappa * spherical_jn(1, kappa) * np.exp(1j * kappa * r) / r)
+        errs.append((e1, e2))
+    assert errs[1][0] < 0.35 * errs[0][0] and errs[1][1] < 0.35 * errs[0][1]
+    assert max(errs[1]) < 1e-2
+
+
+@pytest.mark.parametrize("kappa", [2.0, 4.0])
+def test_herglotz_funk_hecke(kappa):
+    """(1/m) sum_j e^{-i k d_j.x} -> j0(k|x|); the DL part -> -k j1(k|x|) n.xhat."""
+    m = oracle_mesh("perturbed", 3, 256)
+    M = m.meas.shape[0]
+    g = np.ones((M, 1))
+    kap, c = oracle.load_parameters([kappa], SL, 1, 1, factor=4 * math.pi / M)
+    H = oracle.herglotz_f64(m, m.meas, kap, c, g, 1)[:, 0]
+    rad = np.linalg.norm(m.mid, axis=1)
+    assert relerr(H, oracle.j0(kappa * rad)) < 5e-3
+    kap, c = oracle.load_parameters([kappa], DL, 1, 1, factor=4 * math.pi / M)
+    H = oracle.herglotz_f64(m, m.meas, kap, c, g, 1)[:, 0]
+    ref = -kappa * spherical_jn(1, kappa * rad) * np.einsum("ij,ij->i", m.nrm, m.mid / rad[:, None])
+    assert relerr(H, ref) < 5e-3
+    # conj(g): a phase on g comes out conjugated
+    kap, c = oracle.load_parameters([kappa], SL, 1, 1, factor=4 * math.pi / M)
+    H2 = oracle.herglotz_f64(m, m.meas, kap, c, g * np.exp(0.3j), 1)[:, 0]
+    np.testing.assert_allclose(H2, oracle.herglotz_f64(m, m.meas, kap, c, g, 1)[:, 0] * np.exp(-0.3j), rtol=1e-12)
+
+
+def test_incident_waves_vs_finite_differences():
+    m = oracle_mesh("perturbed", 2)
+    kappa = 3.0
+    d = np.array([[0.6, 0.0, 0.8], [0.0, -1.0, 0.0]])
+    eps = 1e-5
+    # plane: c1 u + c2 du/dn, u = e^{i k d.x}
+    U = oracle.incident_wave_p0(m, d, [kappa], [0, 0.7, -0.2j, 0], 2, 2, "plane")
+    for j in range(2):
+        u = lambda X: np.exp(1j * kappa * X @ d[j])
+        dn = (u(m.mid + eps * m.nrm) - u(m.mid - eps * m.nrm)) / (2 * eps)
+        assert relerr(U[:, j], 0.7 * u(m.mid) - 0.2j * dn) < 1e-8
+    # radial: c1 G(x, s) + c2 dG/dn_x, G = e^{ikR}/(4 pi R)
+    s = np.array([[0.1, 0.2, -0.1], [0.0, 0.0, 0.3]])
+    U = oracle.incident_wave_p0(m, s, [kappa], [0, 1.3, 0.4 + 0.1j, 0], 2, 2, "radial")
+    for j in range(2):
+        G = lambda X: np.exp(1j * kappa * np.linalg.norm(X - s[j], axis=1)) / (4 * math.pi * np.linalg.norm(X - s[j], axis=1))
+        dn = (G(m.mid + eps * m.nrm) - G(m.mid - eps * m.nrm)) / (2 * eps)
+        assert relerr(U[:, j], 1.3 * G(m.mid) + (0.4 + 0.1j) * dn) < 1e-7
+
+
+def test_alpha_beta_and_empty_operator():
+    m = oracle_mesh("sphere", 2)
+    rng = np.random.default_rng(2)
+    B = rng.standard_normal((m.nv, 2)) + 1j * rng.standard_normal((m.nv, 2))
+    C0 = rng.standard_normal((m.nv, 2)) + 1j * rng.standard_normal((m.nv, 2))
+    coeff = [0.5, 1 - 0.5j, 0.3, 0.2j]
+    A = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), [2.0], coeff, 2, 2)
+    C = oracle.apply_boundary(m, 0.3 - 0.1j, B, 2.0 + 1j, C0, [2.0], coeff, 2, 2)
+    np.testing.assert_allclose(C, (0.3 - 0.1j) * A + (2.0 + 1j) * C0, rtol=1e-12, atol=1e-14)
+    # all-zero coefficients leave C untouched (beta not applied)
+    C = oracle.apply_boundary(m, 1.0, B, 0.5, C0, [2.0], [0, 0, 0, 0], 2, 2)
+    np.testing.assert_array_equal(C, C0)
+    # mass only: alpha c0 M B + beta C
+    C = oracle.apply_boundary(m, 2.0, B, 0.5, C0, [2.0], [0.25, 0, 0, 0], 2, 2)
+    np.testing.assert_allclose(C, 0.5 * (m.M @ B) + 0.5 * C0, rtol=1e-12)
+
+
+def test_f32_restatement_matches_f64():
+    """The FP32 restatement of the reference chain agrees with the FP64 oracle
+    to FP32 accuracy, with all eight Re/Im branches active."""
+    m = oracle_mesh("sphere", 3)
+    ch = oracle.F32Chain(m)
+    rng = np.random.default_rng(4)
+    W, wcs = 8, 4
+    B = rng.standard_normal((m.nv, W)) + 1j * rng.standard_normal((m.nv, W))
+    coeff = np.array([0.5 + 0.25j, 1 - 0.5j, -0.75 + 1j, 0.3 - 0.9j, 1, 1, 1, 1], np.complex128)
+    kap = [4.0, 2.5]
+    ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kap, coeff, W, wcs)
+    got = ch.apply_boundary(1.0, B, 0.0, np.zeros_like(B), kap, coeff, W, wcs)
+    assert relerr(got, ref) < 2e-6
