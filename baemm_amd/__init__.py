@@ -1,0 +1,13 @@
+"""baemm_amd -- MI355X-native brute-force boundary-element hot path of BAEMM.
+
+The numerics live in ``libbaemm_hip.so`` (HIP kernels for gfx950 behind the C
+ABI in ``include/baemm.h``).  This package holds its Python binding
+(:class:`Helmholtz_HIP`, a mirror of ``BAEMM::Helmholtz_OpenCL``) and mesh
+utilities.  Importing the package does not load the GPU library; creating a
+:class:`Helmholtz_HIP` does, and fails loudly if it is missing.
+"""
+from ._lib import BaemmError, LIB_PATH
+from .helmholtz import Helmholtz_HIP, WaveType
+from . import meshes
+
+__all__ = ["Helmholtz_HIP", "WaveType", "BaemmError", "LIB_PATH", "meshes"]

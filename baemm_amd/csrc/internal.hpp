@@ -1,0 +1,174 @@
+// internal.hpp -- shared declarations of libbaemm_hip (not installed).
+//
+// Device data layout (all resident in HBM for the lifetime of a handle):
+//   mid[nt_pad], nrm[nt_pad]   float4 centroids / unit normals (w unused);
+//                              padded rows hold the last triangle's data
+//   area[nt], sdiag[nt]        float areas and single-layer self terms
+//   tris[nt]                   int4 vertex ids (w unused)
+//   vt_ptr[nv+1], vt_idx[3nt]  vertex -> incident triangles (AvT pattern)
+//   M_ptr[nv+1], M_idx, M_val  P1 mass matrix CSR (double values)
+//   meas[m_pad]                float4 measurement directions
+// Per-apply workspaces grow on demand and are reused (see Workspace).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/baemm.h"
+
+namespace baemm {
+
+using cdouble = std::complex<double>;
+using cfloat = std::complex<float>;
+
+constexpr int kTile = 256;          // target rows per plain-kernel block
+constexpr int kMixTile = 128;       // target rows per mixed-kernel block (4 waves x 32)
+constexpr int kSrcBlock = 64;       // sources staged per LDS block (plain kernels)
+constexpr int kMaxCols = 64;        // padded columns handled per kernel launch
+
+struct HipError {
+    hipError_t err;
+    const char* what;
+    int line;
+};
+
+#define BAEMM_HIP(x)                                                   \
+    do {                                                               \
+        hipError_t e_ = (x);                                           \
+        if (e_ != hipSuccess) throw ::baemm::HipError{e_, #x, __LINE__}; \
+    } while (0)
+
+struct ArgError {
+    std::string msg;
+};
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        if (ptr) BAEMM_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+        BAEMM_HIP(hipMalloc(&ptr, n));
+        bytes = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Loaded operator parameters (LoadParameters.hpp:21-81).
+struct Params {
+    int64_t W = 0, wcs = 0, K = 0;
+    std::vector<float> kappa;            // K
+    std::vector<cfloat> c;               // K x 4, c_1..3 scaled by factor/(4 pi)
+    bool re[4] = {false, false, false, false};
+    bool im[4] = {false, false, false, false};
+    bool any(int j) const { return re[j] || im[j]; }
+    bool kernel_on() const { return any(1) || any(2) || any(3); }
+    bool mass_on() const { return any(0); }
+    bool loaded = false;
+};
+
+struct ChunkCoef {
+    float kap;       // kappa
+    float kap_rev;   // kappa / (2 pi): the hardware sine takes revolutions
+    float c1r, c1i, c2r, c2i, c3r, c3i;
+};
+
+struct Handle {
+    std::mutex mtx;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t nv = 0, nt = 0, m = 0;
+    int64_t nt_pad = 0, m_pad = 0;
+    // host copies
+    std::vector<double> coords;          // nv x 3
+    std::vector<int32_t> tris;           // nt x 3
+    std::vector<double> meas;            // m x 3
+    std::vector<float> h_mid4, h_nrm4, h_area, h_sdiag;
+    std::vector<int32_t> h_vt_ptr, h_vt_idx;
+    std::vector<int32_t> h_M_ptr, h_M_idx;
+    std::vector<double> h_M_val;
+    std::vector<double> h_lumped_inv;    // areas_lumped_inv (Initialize.hpp:337-353)
+    // device geometry
+    float4* d_mid = nullptr;
+    float4* d_nrm = nullptr;
+    float* d_area = nullptr;
+    float* d_sdiag = nullptr;
+    int4* d_tris = nullptr;
+    int32_t* d_vt_ptr = nullptr;
+    int32_t* d_vt_idx = nullptr;
+    int32_t* d_M_ptr = nullptr;
+    int32_t* d_M_idx = nullptr;
+    double* d_M_val = nullptr;
+    double* d_lumped_inv = nullptr;
+    float4* d_meas = nullptr;
+    float rmin = -1.0f;                  // min centroid distance (mixed scaling), lazily computed
+    // settings (GetSetters.hpp)
+    bool use_diagonal = true;
+    int block_size = 64;
+    int wave_chunk_size = 16;
+    int precision = BAEMM_PRECISION_FP32;
+    int segments = 0;                    // 0 = automatic split of the source range
+    int gmres_max_iter = 30;
+    int gmres_max_restarts = 3;
+    Params prm;
+    // workspaces
+    DevBuf ws_B, ws_out, ws_b32, ws_bop, ws_part, ws_scale, ws_misc, ws_pts, ws_g;
+    DevBuf ws_solver;
+    // profiling of the dominant pair kernel
+    bool timing = false;
+    double timed_ms = 0.0;
+    int64_t timed_launches = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string last_error;
+};
+
+// ---- geometry / setup (geometry.cpp) ----
+void build_host_geometry(Handle& h);
+void upload_geometry(Handle& h);
+void free_device(Handle& h);
+
+// ---- parameters ----
+void load_params(Handle& h, const double* kappa, const double* coeff, int64_t W, int64_t wcs, double factor);
+std::vector<ChunkCoef> chunk_coefs(const Params& p, int q0, int kg, float scale);
+
+// ---- device ops (ops.hip) ----
+// Complex element types of external device arrays.
+enum ElemType { kC128 = BAEMM_C128, kC64 = BAEMM_C64 };
+
+struct ApplyArgs {
+    cdouble alpha, beta;
+    const void* B; int64_t ldB; int Btype;
+    void* C; int64_t ldC; int Ctype;
+};
+
+void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s);
+void apply_far_dev(Handle& h, const ApplyArgs& a, hipStream_t s);
+void apply_near_dev(Handle& h, const ApplyArgs& a, const float4* d_pts, int64_t P, hipStream_t s);
+void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s);
+void incident_dev(Handle& h, const ApplyArgs& a, const double* dirs_host, int64_t inc_count, int wave_type, hipStream_t s);
+float compute_rmin(Handle& h, hipStream_t s);
+
+// Sparse helpers on device data.
+void spmv_mass_real(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s);
+void av_real(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s);
+void avt_real(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double alpha, hipStream_t s);
+
+// ---- solvers (solvers.cpp) ----
+int mass_inverse_dev(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
+                     hipStream_t s);
+
+}  // namespace baemm

@@ -1,0 +1,798 @@
+// ops.hip -- device-resident operator applies: staging, pair-kernel dispatch,
+// epilogues (AvT, SL diagonal, mass, alpha/beta) and the incident waves.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pair_kernels.hpp"
+
+namespace baemm {
+
+// --------------------------------------------------------------- helpers
+
+template <int T>
+struct Elem;
+template <>
+struct Elem<kC128> {
+    using type = double2;
+};
+template <>
+struct Elem<kC64> {
+    using type = float2;
+};
+
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ void from_d2(double2& dst, double2 v) { dst = v; }
+__device__ __forceinline__ void from_d2(float2& dst, double2 v) { dst = make_float2((float)v.x, (float)v.y); }
+
+static inline int next_pow2(int x)
+{
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// Padded chunk width of the plain kernels (instantiated 4..64).
+static inline int wcsp_plain(int wcs) { return std::max(4, next_pow2(wcs)); }
+// Padded chunk width of the MFMA kernel (16, 32 or 64: 2*WCSP real columns
+// must fill 32-column MFMA blocks).
+static inline int wcsp_mixed(int wcs) { return std::max(16, next_pow2(wcs)); }
+
+// --------------------------------------------------------------- staging
+
+// b[t][q*WCSP+kk] = (a_t/3) (B[i0][k] + B[i1][k] + B[i2][k]),  k = q*wcs + kk
+// (AvOp.Dot, ApplyBoundaryOperators.hpp:120-124; Av values Initialize.hpp:133-143).
+// Rows t >= nt and columns kk >= wcs are zero (the reference leaves padded rows
+// of its host buffer unwritten, Q3; here they are defined).
+template <int BT>
+__global__ void av_gather_kernel(const int4* __restrict__ tris, const float* __restrict__ area, int nt,
+                                 int64_t total, const void* __restrict__ Bv, int64_t ldB, int wcs, int WCSP,
+                                 int ldb, float2* __restrict__ b)
+{
+    using T = typename Elem<BT>::type;
+    const T* B = static_cast<const T*>(Bv);
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int t = (int)(idx / ldb);
+    const int pc = (int)(idx % ldb);
+    const int q = pc / WCSP, kk = pc % WCSP;
+    float2 out = make_float2(0.f, 0.f);
+    if (t < nt && kk < wcs) {
+        const int k = q * wcs + kk;
+        const int4 tv = tris[t];
+        const double2 v0 = to_d2(B[(int64_t)tv.x * ldB + k]);
+        const double2 v1 = to_d2(B[(int64_t)tv.y * ldB + k]);
+        const double2 v2 = to_d2(B[(int64_t)tv.z * ldB + k]);
+        const double w = (double)area[t] / 3.0;
+        out = make_float2((float)(w * (v0.x + v1.x + v2.x)), (float)(w * (v0.y + v1.y + v2.y)));
+    }
+    b[idx] = out;
+}
+
+// b[r][q*WCSP+kk] = (conj?) G[r][q*wcs+kk] for r < nrows, zero padding.
+template <int BT>
+__global__ void stage_rows_kernel(int nrows, int64_t total, const void* __restrict__ Gv, int64_t ldG, int wcs,
+                                  int WCSP, int ldb, int conj, float2* __restrict__ b)
+{
+    using T = typename Elem<BT>::type;
+    const T* G = static_cast<const T*>(Gv);
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int r = (int)(idx / ldb);
+    const int pc = (int)(idx % ldb);
+    const int q = pc / WCSP, kk = pc % WCSP;
+    float2 out = make_float2(0.f, 0.f);
+    if (r < nrows && kk < wcs) {
+        const double2 v = to_d2(G[(int64_t)r * ldG + q * wcs + kk]);
+        out = make_float2((float)v.x, (float)(conj ? -v.y : v.y));
+    }
+    b[idx] = out;
+}
+
+// Per-column max of |re|, |im| (positive floats order like their bit patterns).
+__global__ void colmax_kernel(const float2* __restrict__ b, int nrows, int ldb, unsigned* __restrict__ colmax)
+{
+    const int pc = blockIdx.x;
+    float m = 0.0f;
+    for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+        const float2 v = b[(int64_t)r * ldb + pc];
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    __shared__ float sm[16];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mm = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mm = fmaxf(mm, sm[w]);
+        colmax[pc] = __float_as_uint(mm);
+    }
+}
+
+// Power-of-two scale per column so that max |s b| lies in [2^13, 2^14);
+// unscale[pc] = 1 / (sA_q * s_pc).
+struct ChunkScales {
+    float sA[64];
+};
+
+__global__ void colscale_kernel(const unsigned* __restrict__ colmax, int ncols, int WCSP, ChunkScales cs,
+                                float* __restrict__ scale, float* __restrict__ unscale)
+{
+    const int pc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pc >= ncols) return;
+    const float m = __uint_as_float(colmax[pc]);
+    float s = 1.0f;
+    if (m > 0.0f) {
+        int e;
+        frexpf(m, &e);          // m in [2^(e-1), 2^e)
+        s = ldexpf(1.0f, 14 - e);
+    }
+    scale[pc] = s;
+    unscale[pc] = 1.0f / (s * cs.sA[pc / WCSP]);
+}
+
+// B-operand of the MFMA kernel, 16 bytes per (chunk, step, col-block, part, lane);
+// see bdr_mixed_kernel.  Values are scaled per column, then split hi/lo (RN).
+__global__ void split_kernel(const float2* __restrict__ b, int ldb, int nst, int WCSP, int K,
+                             const float* __restrict__ scale, uint4* __restrict__ bop)
+{
+    const int NCB = 2 * WCSP / 32;
+    const int64_t total = (int64_t)K * nst * NCB * 64;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int lane = (int)(idx % 64);
+    int64_t rest = idx / 64;
+    const int cb = (int)(rest % NCB);
+    rest /= NCB;
+    const int st = (int)(rest % nst);
+    const int q = (int)(rest / nst);
+    const int g = lane >> 5;
+    const int c = cb * 32 + (lane & 31);
+    const bool imcol = c >= WCSP;
+    const int kk = imcol ? c - WCSP : c;
+    const int pc = q * WCSP + kk;
+    const float s = scale[pc];
+    _Float16 hi[8], lo[8];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const int j = 8 * st + 4 * g + jj;
+        const float2 v = b[(int64_t)j * ldb + pc];
+        const float br = v.x * s, bi = v.y * s;
+        // rows (j, re) and (j, im) of [[br, bi], [-bi, br]]
+        const float e0 = imcol ? bi : br;
+        const float e1 = imcol ? br : -bi;
+        const _Float16 h0 = (_Float16)e0, h1 = (_Float16)e1;
+        hi[2 * jj] = h0;
+        hi[2 * jj + 1] = h1;
+        lo[2 * jj] = (_Float16)(e0 - (float)h0);
+        lo[2 * jj + 1] = (_Float16)(e1 - (float)h1);
+    }
+    const int64_t base = ((((int64_t)q * nst + st) * NCB + cb) * 2) * 64 + lane;
+    bop[base] = __builtin_bit_cast(uint4, hi);
+    bop[base + 64] = __builtin_bit_cast(uint4, lo);
+}
+
+// Minimum squared distance between distinct centroids (sets the A scale of
+// the MFMA path).  One target per thread, all sources.
+__global__ void rmin_kernel(const float4* __restrict__ mid, int nt, unsigned* __restrict__ out)
+{
+    __shared__ float4 s[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float4 x = mid[min(i, nt - 1)];
+    float m = 3.0e38f;
+    for (int jb = 0; jb < nt; jb += 256) {
+        __syncthreads();
+        if (jb + threadIdx.x < nt) s[threadIdx.x] = mid[jb + threadIdx.x];
+        __syncthreads();
+        const int jn = min(256, nt - jb);
+        for (int jj = 0; jj < jn; ++jj) {
+            const float4 y = s[jj];
+            const float d0 = y.x - x.x, d1 = y.y - x.y, d2 = y.z - x.z;
+            const float r2 = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
+            if (jb + jj != i) m = fminf(m, r2);
+        }
+    }
+    if (i >= nt) m = 3.0e38f;
+    for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0 && i < nt) atomicMin(out, __float_as_uint(m));
+}
+
+// --------------------------------------------------------------- epilogues
+
+struct EpiArgs {
+    int nv, W, wcs, WCSP, K;
+    int kernel_on, diag_on, mass_on;
+    double ar, ai, br, bi;
+    int S;
+    int64_t seg_stride;
+    int ldp, ldb;
+    const int32_t* vt_ptr;
+    const int32_t* vt_idx;
+    const float* area;
+    const float* sdiag;
+    const float* part;
+    const float2* b;
+    const float* unscale;
+    const int32_t* M_ptr;
+    const int32_t* M_idx;
+    const double* M_val;
+    const void* B;
+    int64_t ldB;
+    void* C;
+    int64_t ldC;
+    float c1r[64], c1i[64], kap[64];
+    double c0r[64], c0i[64];
+};
+
+// C[v][k] = alpha * ( sum_{t ni v} a_t/3 (sum_s part_s[t][k] * unscale + D_t b[t][k])
+//                     + c0_q (M B)[v][k] ) + beta * C[v][k]
+// (AvOpTransp.Dot ApplyBoundaryOperators.hpp:136-140, ApplySingleLayerDiagonal.hpp:36-50,
+//  MassOp.Dot :150-175).  One 64-lane wave per vertex, lanes over columns.
+template <int BT, int CT>
+__global__ __launch_bounds__(256) void bdr_epilogue_kernel(EpiArgs a)
+{
+    using TB = typename Elem<BT>::type;
+    using TC = typename Elem<CT>::type;
+    const int lane = threadIdx.x & 63;
+    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (v >= a.nv) return;
+    for (int k = lane; k < a.W; k += 64) {
+        const int q = k / a.wcs, kk = k % a.wcs;
+        const int pc = q * a.WCSP + kk;
+        double sr = 0.0, si = 0.0;
+        if (a.kernel_on) {
+            const double us = a.unscale ? (double)a.unscale[pc] : 1.0;
+            const int e1 = a.vt_ptr[v + 1];
+            for (int e = a.vt_ptr[v]; e < e1; ++e) {
+                const int t = a.vt_idx[e];
+                double pr = 0.0, pi = 0.0;
+                const float* pp = a.part + (int64_t)t * a.ldp + q * 2 * a.WCSP + kk;
+                for (int s = 0; s < a.S; ++s) {
+                    pr += (double)pp[(int64_t)s * a.seg_stride];
+                    pi += (double)pp[(int64_t)s * a.seg_stride + a.WCSP];
+                }
+                pr *= us;
+                pi *= us;
+                if (a.diag_on) {
+                    const float2 bv = a.b[(int64_t)t * a.ldb + pc];
+                    const double fr = (double)a.c1r[q] * a.sdiag[t] - (double)a.c1i[q] * a.kap[q];
+                    const double fi = (double)a.c1i[q] * a.sdiag[t] + (double)a.c1r[q] * a.kap[q];
+                    pr += fr * bv.x - fi * bv.y;
+                    pi += fr * bv.y + fi * bv.x;
+                }
+                const double w = (double)a.area[t] / 3.0;
+                sr += w * pr;
+                si += w * pi;
+            }
+        }
+        if (a.mass_on) {
+            const TB* B = static_cast<const TB*>(a.B);
+            double mr = 0.0, mi = 0.0;
+            const int e1 = a.M_ptr[v + 1];
+            for (int e = a.M_ptr[v]; e < e1; ++e) {
+                const double2 bv = to_d2(B[(int64_t)a.M_idx[e] * a.ldB + k]);
+                mr += a.M_val[e] * bv.x;
+                mi += a.M_val[e] * bv.y;
+            }
+            sr += a.c0r[q] * mr - a.c0i[q] * mi;
+            si += a.c0r[q] * mi + a.c0i[q] * mr;
+        }
+        TC* C = static_cast<TC*>(a.C);
+        TC& dst = C[(int64_t)v * a.ldC + k];
+        double outr = a.ar * sr - a.ai * si;
+        double outi = a.ar * si + a.ai * sr;
+        if (a.br != 0.0 || a.bi != 0.0) {
+            const double2 c = to_d2(dst);
+            outr += a.br * c.x - a.bi * c.y;
+            outi += a.br * c.y + a.bi * c.x;
+        }
+        from_d2(dst, make_double2(outr, outi));
+    }
+}
+
+// F[i][k] = alpha * unscale * sum_s part_s[i][k] + beta * F[i][k]  (far field, potential).
+template <int CT>
+__global__ void rows_epilogue_kernel(int nrows, int W, int wcs, int WCSP, int S, int64_t seg_stride, int ldp,
+                                     const float* __restrict__ part, double ar, double ai, double br, double bi,
+                                     void* Cv, int64_t ldC)
+{
+    using TC = typename Elem<CT>::type;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nrows * W) return;
+    const int r = (int)(idx / W), k = (int)(idx % W);
+    const int q = k / wcs, kk = k % wcs;
+    const float* pp = part + (int64_t)r * ldp + q * 2 * WCSP + kk;
+    double pr = 0.0, pi = 0.0;
+    for (int s = 0; s < S; ++s) {
+        pr += (double)pp[(int64_t)s * seg_stride];
+        pi += (double)pp[(int64_t)s * seg_stride + WCSP];
+    }
+    TC* C = static_cast<TC*>(Cv);
+    TC& dst = C[(int64_t)r * ldC + k];
+    double outr = ar * pr - ai * pi, outi = ar * pi + ai * pr;
+    if (br != 0.0 || bi != 0.0) {
+        const double2 c = to_d2(dst);
+        outr += br * c.x - bi * c.y;
+        outi += br * c.y + bi * c.x;
+    }
+    from_d2(dst, make_double2(outr, outi));
+}
+
+// Incident-wave values per triangle written as a single "segment" partial
+// (IncidentWaveKernel_Plane.hpp:190-233, IncidentWaveKernel_Radial.hpp:5-57):
+//   plane : (c1 + c2 i kappa (d.n)) exp(i kappa d.x)
+//   radial: (c1 + c2 (-(n.delta)/R^2 + i kappa (n.delta)/R)) exp(i kappa R) / (4 pi R)
+struct IncArgs {
+    int nt, W, wcs, WCSP, ldp, radial;
+    const float4* mid;
+    const float4* nrm;
+    float* part;
+    float dirs[64][3];
+    float kap[64], c1r[64], c1i[64], c2r[64], c2i[64];
+};
+
+__global__ void incident_kernel(IncArgs a)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)a.nt * a.W) return;
+    const int t = (int)(idx / a.W), k = (int)(idx % a.W);
+    const int q = k / a.wcs, j = k % a.wcs;
+    const float4 x = a.mid[t], n = a.nrm[t];
+    const double dx = a.dirs[j][0], dy = a.dirs[j][1], dz = a.dirs[j][2];
+    const double kap = a.kap[q];
+    double ur, ui;
+    if (!a.radial) {
+        const double ph = kap * (x.x * dx + x.y * dy + x.z * dz);
+        const double dn = kap * (n.x * dx + n.y * dy + n.z * dz);
+        const double fr = a.c1r[q] - a.c2i[q] * dn, fi = a.c1i[q] + a.c2r[q] * dn;
+        const double cs = cos(ph), sn = sin(ph);
+        ur = fr * cs - fi * sn;
+        ui = fr * sn + fi * cs;
+    } else {
+        const double e0 = x.x - dx, e1 = x.y - dy, e2 = x.z - dz;
+        const double R = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+        const double nd = n.x * e0 + n.y * e1 + n.z * e2;
+        const double A = 1.0 / (4.0 * M_PI * R);
+        const double Br = -nd / (R * R), Bi = kap * nd / R;
+        const double fr = A * (a.c1r[q] + a.c2r[q] * Br - a.c2i[q] * Bi);
+        const double fi = A * (a.c1i[q] + a.c2r[q] * Bi + a.c2i[q] * Br);
+        const double cs = cos(kap * R), sn = sin(kap * R);
+        ur = fr * cs - fi * sn;
+        ui = fr * sn + fi * cs;
+    }
+    float* out = a.part + (int64_t)t * a.ldp + q * 2 * a.WCSP + j;
+    out[0] = (float)ur;
+    out[a.WCSP] = (float)ui;
+}
+
+// --------------------------------------------------------------- dispatch
+
+static CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk)
+{
+    CoefPack cp{};
+    for (int g = 0; g < kg; ++g) {
+        const float s = scale_per_chunk ? scale_per_chunk[q0 + g] : 1.0f;
+        cp.c[g] = chunk_coefs(p, q0 + g, 1, s)[0];
+    }
+    return cp;
+}
+
+#define PLAIN_CASES(OP, ADJ)                                                                             \
+    if (WCSP == 4 && KG == 1) return &pair_plain_kernel<OP, 4, 1, ADJ>;                               \
+    if (WCSP == 4 && KG == 2) return &pair_plain_kernel<OP, 4, 2, ADJ>;                               \
+    if (WCSP == 8 && KG == 1) return &pair_plain_kernel<OP, 8, 1, ADJ>;                               \
+    if (WCSP == 8 && KG == 2) return &pair_plain_kernel<OP, 8, 2, ADJ>;                               \
+    if (WCSP == 16 && KG == 1) return &pair_plain_kernel<OP, 16, 1, ADJ>;                             \
+    if (WCSP == 16 && KG == 2) return &pair_plain_kernel<OP, 16, 2, ADJ>;                             \
+    if (WCSP == 32 && KG == 1) return &pair_plain_kernel<OP, 32, 1, ADJ>;                             \
+    if (WCSP == 32 && KG == 2) return &pair_plain_kernel<OP, 32, 2, ADJ>;                             \
+    if (WCSP == 64 && KG == 1) return &pair_plain_kernel<OP, 64, 1, ADJ>;
+
+using PlainFn = void (*)(const float4*, const float4*, int, const float4*, const float4*, int, const float2*, int,
+                         CoefPack, float*, int64_t, int, int, int, int);
+
+static PlainFn plain_fn(int op, bool adj, int WCSP, int KG)
+{
+    if (op == OP_BDR && adj) { PLAIN_CASES(OP_BDR, true) }
+    if (op == OP_BDR && !adj) { PLAIN_CASES(OP_BDR, false) }
+    if (op == OP_NEAR) { PLAIN_CASES(OP_NEAR, false) }
+    if (op == OP_FAR) { PLAIN_CASES(OP_FAR, false) }
+    if (op == OP_HERG) { PLAIN_CASES(OP_HERG, false) }
+    throw ArgError{"internal: no plain kernel instance"};
+}
+
+using MixedFn = void (*)(const float4*, const float4*, int, const uint4*, int, CoefPack, float*, int64_t, int, int,
+                         int);
+
+static MixedFn mixed_fn(bool adj, int WCSP, int KG)
+{
+#define MIX(W_, K_)                                                                 \
+    if (WCSP == W_ && KG == K_) return adj ? &bdr_mixed_kernel<W_, K_, true> : &bdr_mixed_kernel<W_, K_, false>;
+    MIX(16, 1)
+    MIX(16, 2)
+    MIX(16, 4)
+    MIX(32, 1)
+    MIX(32, 2)
+    MIX(64, 1)
+#undef MIX
+    throw ArgError{"internal: no mixed kernel instance"};
+}
+
+static int pick_kg(int WCSP, int remaining, bool mixed)
+{
+    const int maxkg = std::max(1, kMaxCols / WCSP);
+    const int lim = mixed ? 4 : 2;
+    int kg = 1;
+    while (kg * 2 <= std::min({maxkg, remaining, lim})) kg *= 2;
+    return kg;
+}
+
+struct Timed {
+    Handle& h;
+    hipStream_t s;
+    Timed(Handle& h_, hipStream_t s_) : h(h_), s(s_)
+    {
+        if (h.timing) BAEMM_HIP(hipEventRecord(h.ev0, s));
+    }
+    void done()
+    {
+        if (!h.timing) return;
+        BAEMM_HIP(hipEventRecord(h.ev1, s));
+        BAEMM_HIP(hipEventSynchronize(h.ev1));
+        float ms = 0.0f;
+        BAEMM_HIP(hipEventElapsedTime(&ms, h.ev0, h.ev1));
+        h.timed_ms += ms;
+        h.timed_launches += 1;
+    }
+};
+
+// Number of source segments: enough work-groups to fill 256 CUs several times.
+static int choose_segments(const Handle& h, int tiles, int max_seg)
+{
+    if (h.segments > 0) return std::min(h.segments, max_seg);
+    const int want = (2048 + tiles - 1) / tiles;
+    return std::max(1, std::min(want, max_seg));
+}
+
+// Runs a plain pair kernel over all chunk groups: part has ntgt_pad rows.
+static void run_plain(Handle& h, int op, bool adj, int WCSP, const float4* tx, const float4* tn, int ntgt,
+                      int ntgt_pad, const float4* sy, const float4* sn, int nsrc, int nsrc_pad, const float2* b,
+                      int ldb, float* part, int64_t seg_stride, int ldp, int S, int seg_len, hipStream_t s)
+{
+    const Params& p = h.prm;
+    Timed tm(h, s);
+    for (int q0 = 0; q0 < p.K;) {
+        const int kg = pick_kg(WCSP, (int)p.K - q0, false);
+        const CoefPack cp = make_pack(p, q0, kg, nullptr);
+        PlainFn fn = plain_fn(op, adj, WCSP, kg);
+        dim3 grid(ntgt_pad / kTile, S);
+        hipLaunchKernelGGL(fn, grid, dim3(kTile), 0, s, tx, tn, ntgt, sy, sn, nsrc, b, ldb, cp, part, seg_stride, ldp,
+                           q0, seg_len, nsrc_pad);
+        BAEMM_HIP(hipGetLastError());
+        q0 += kg;
+    }
+    tm.done();
+}
+
+float compute_rmin(Handle& h, hipStream_t s)
+{
+    if (h.rmin >= 0.0f) return h.rmin;
+    h.ws_misc.reserve(256);
+    unsigned init = 0x7f7fffffu;
+    BAEMM_HIP(hipMemcpyAsync(h.ws_misc.ptr, &init, sizeof(unsigned), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(rmin_kernel, dim3((h.nt + 255) / 256), dim3(256), 0, s, h.d_mid, (int)h.nt,
+                       h.ws_misc.as<unsigned>());
+    BAEMM_HIP(hipGetLastError());
+    unsigned out = 0;
+    BAEMM_HIP(hipMemcpyAsync(&out, h.ws_misc.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    BAEMM_HIP(hipStreamSynchronize(s));
+    float r2;
+    memcpy(&r2, &out, sizeof(float));
+    h.rmin = std::sqrt(std::max(r2, 1e-30f));
+    return h.rmin;
+}
+
+template <int BT, int CT>
+static void launch_bdr_epilogue(EpiArgs& ea, hipStream_t s)
+{
+    hipLaunchKernelGGL((bdr_epilogue_kernel<BT, CT>), dim3((ea.nv + 3) / 4), dim3(256), 0, s, ea);
+}
+
+static void bdr_epilogue(Handle& h, EpiArgs& ea, int Btype, int Ctype, hipStream_t s)
+{
+    if (Btype == kC128 && Ctype == kC128) launch_bdr_epilogue<kC128, kC128>(ea, s);
+    else if (Btype == kC128 && Ctype == kC64) launch_bdr_epilogue<kC128, kC64>(ea, s);
+    else if (Btype == kC64 && Ctype == kC128) launch_bdr_epilogue<kC64, kC128>(ea, s);
+    else launch_bdr_epilogue<kC64, kC64>(ea, s);
+    BAEMM_HIP(hipGetLastError());
+}
+
+static EpiArgs base_epi(Handle& h, const ApplyArgs& a, int WCSP)
+{
+    const Params& p = h.prm;
+    if (p.K > 64) throw ArgError{"at most 64 wave-number chunks are supported"};
+    EpiArgs ea{};
+    ea.nv = (int)h.nv;
+    ea.W = (int)p.W;
+    ea.wcs = (int)p.wcs;
+    ea.WCSP = WCSP;
+    ea.K = (int)p.K;
+    ea.ar = a.alpha.real();
+    ea.ai = a.alpha.imag();
+    ea.br = a.beta.real();
+    ea.bi = a.beta.imag();
+    ea.vt_ptr = h.d_vt_ptr;
+    ea.vt_idx = h.d_vt_idx;
+    ea.area = h.d_area;
+    ea.sdiag = h.d_sdiag;
+    ea.M_ptr = h.d_M_ptr;
+    ea.M_idx = h.d_M_idx;
+    ea.M_val = h.d_M_val;
+    ea.B = a.B;
+    ea.ldB = a.ldB;
+    ea.C = a.C;
+    ea.ldC = a.ldC;
+    for (int q = 0; q < p.K; ++q) {
+        ea.c0r[q] = p.c[4 * q].real();
+        ea.c0i[q] = p.c[4 * q].imag();
+        ea.c1r[q] = p.c[4 * q + 1].real();
+        ea.c1i[q] = p.c[4 * q + 1].imag();
+        ea.kap[q] = p.kappa[q];
+    }
+    return ea;
+}
+
+static void launch_av(Handle& h, const void* B, int64_t ldB, int Btype, int WCSP, float2* b, int ldb,
+                      hipStream_t s)
+{
+    const int64_t total = h.nt_pad * (int64_t)ldb;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (Btype == kC128)
+        hipLaunchKernelGGL(av_gather_kernel<kC128>, grid, dim3(256), 0, s, h.d_tris, h.d_area, (int)h.nt, total, B,
+                           ldB, (int)h.prm.wcs, WCSP, ldb, b);
+    else
+        hipLaunchKernelGGL(av_gather_kernel<kC64>, grid, dim3(256), 0, s, h.d_tris, h.d_area, (int)h.nt, total, B,
+                           ldB, (int)h.prm.wcs, WCSP, ldb, b);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
+{
+    const Params& p = h.prm;
+    if (!p.loaded) throw ArgError{"operators not loaded"};
+    const bool kernel_on = p.kernel_on();
+    const bool mass_on = p.mass_on();
+    if (!kernel_on && !mass_on) return;   // the reference leaves C_out untouched
+    const bool mixed = (h.precision == BAEMM_PRECISION_MIXED) && kernel_on;
+    const int WCSP = mixed ? wcsp_mixed((int)p.wcs) : wcsp_plain((int)p.wcs);
+    const int ldb = (int)p.K * WCSP;
+    const int ldp = 2 * ldb;
+    EpiArgs ea = base_epi(h, a, WCSP);
+    ea.mass_on = mass_on;
+    ea.kernel_on = kernel_on;
+    ea.diag_on = kernel_on && h.use_diagonal && p.any(1);
+    if (kernel_on) {
+        const bool adj = p.any(3);
+        h.ws_b32.reserve(sizeof(float2) * h.nt_pad * ldb);
+        float2* b = h.ws_b32.as<float2>();
+        launch_av(h, a.B, a.ldB, a.Btype, WCSP, b, ldb, s);
+        if (!mixed) {
+            const int tiles = (int)(h.nt_pad / kTile);
+            const int max_seg = (int)(h.nt_pad / kSrcBlock);
+            const int S0 = choose_segments(h, tiles, max_seg);
+            const int seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
+            const int S = (int)((h.nt_pad + seg_len - 1) / seg_len);
+            const int64_t seg_stride = h.nt_pad * (int64_t)ldp;
+            h.ws_part.reserve(sizeof(float) * seg_stride * S);
+            run_plain(h, OP_BDR, adj, WCSP, h.d_mid, h.d_nrm, (int)h.nt, (int)h.nt_pad, h.d_mid, h.d_nrm,
+                      (int)h.nt, (int)h.nt_pad, b, ldb, h.ws_part.as<float>(), seg_stride, ldp, S, seg_len, s);
+            ea.S = S;
+            ea.seg_stride = seg_stride;
+            ea.unscale = nullptr;
+        } else {
+            // scales: A per chunk from a bound at the minimum centroid distance,
+            // b per column from its maximum.
+            const float rmin = compute_rmin(h, s);
+            std::vector<float> sA(p.K);
+            ChunkScales csc{};
+            for (int q = 0; q < p.K; ++q) {
+                const double c1 = std::abs(std::complex<double>(p.c[4 * q + 1]));
+                const double c23 = std::abs(std::complex<double>(p.c[4 * q + 2])) +
+                                   std::abs(std::complex<double>(p.c[4 * q + 3]));
+                const double kr = (double)std::fabs(p.kappa[q]) * rmin;
+                const double amax = c1 / rmin + c23 * (1.0 + kr) / ((double)rmin * rmin);
+                int e = 0;
+                std::frexp(std::max(amax, 1e-30), &e);
+                sA[q] = (float)std::ldexp(1.0, 14 - e);
+                csc.sA[q] = sA[q];
+            }
+            const int ncols = ldb;
+            h.ws_scale.reserve(sizeof(float) * 3 * ncols);
+            unsigned* colmax = h.ws_scale.as<unsigned>();
+            float* scale = reinterpret_cast<float*>(colmax + ncols);
+            float* unscale = scale + ncols;
+            hipLaunchKernelGGL(colmax_kernel, dim3(ncols), dim3(256), 0, s, b, (int)h.nt, ldb, colmax);
+            hipLaunchKernelGGL(colscale_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, colmax, ncols, WCSP, csc,
+                               scale, unscale);
+            const int nst = (int)(h.nt_pad / 8);
+            const int NCB = 2 * WCSP / 32;
+            const int64_t nbop = (int64_t)p.K * nst * NCB * 64;
+            h.ws_bop.reserve(sizeof(uint4) * nbop * 2);
+            hipLaunchKernelGGL(split_kernel, dim3((unsigned)((nbop + 255) / 256)), dim3(256), 0, s, b, ldb, nst, WCSP,
+                               (int)p.K, scale, h.ws_bop.as<uint4>());
+            BAEMM_HIP(hipGetLastError());
+            const int tiles = (int)(h.nt_pad / kMixTile);
+            const int S0 = choose_segments(h, tiles, nst / 8);
+            const int steps = (nst + S0 - 1) / S0;
+            const int S = (nst + steps - 1) / steps;
+            const int64_t seg_stride = h.nt_pad * (int64_t)ldp;
+            h.ws_part.reserve(sizeof(float) * seg_stride * S);
+            Timed tm(h, s);
+            for (int q0 = 0; q0 < p.K;) {
+                const int kg = pick_kg(WCSP, (int)p.K - q0, true);
+                const CoefPack cp = make_pack(p, q0, kg, sA.data());
+                MixedFn fn = mixed_fn(adj, WCSP, kg);
+                hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(256), 0, s, h.d_mid, h.d_nrm, (int)h.nt,
+                                   h.ws_bop.as<uint4>(), nst, cp, h.ws_part.as<float>(), seg_stride, ldp, q0, steps);
+                BAEMM_HIP(hipGetLastError());
+                q0 += kg;
+            }
+            tm.done();
+            ea.S = S;
+            ea.seg_stride = seg_stride;
+            ea.unscale = unscale;
+        }
+        ea.part = h.ws_part.as<float>();
+        ea.b = b;
+        ea.ldp = ldp;
+        ea.ldb = ldb;
+    }
+    bdr_epilogue(h, ea, a.Btype, a.Ctype, s);
+}
+
+static void rows_epilogue(int nrows, const Params& p, int WCSP, int S, int64_t seg_stride, int ldp, const float* part,
+                          const ApplyArgs& a, hipStream_t s)
+{
+    const int64_t total = (int64_t)nrows * p.W;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (a.Ctype == kC128)
+        hipLaunchKernelGGL(rows_epilogue_kernel<kC128>, grid, dim3(256), 0, s, nrows, (int)p.W, (int)p.wcs, WCSP, S,
+                           seg_stride, ldp, part, a.alpha.real(), a.alpha.imag(), a.beta.real(), a.beta.imag(), a.C,
+                           a.ldC);
+    else
+        hipLaunchKernelGGL(rows_epilogue_kernel<kC64>, grid, dim3(256), 0, s, nrows, (int)p.W, (int)p.wcs, WCSP, S,
+                           seg_stride, ldp, part, a.alpha.real(), a.alpha.imag(), a.beta.real(), a.beta.imag(), a.C,
+                           a.ldC);
+    BAEMM_HIP(hipGetLastError());
+}
+
+// Far field / potential share the structure "targets x triangles", b = Av B.
+static void apply_rows_op(Handle& h, int op, const ApplyArgs& a, const float4* tx, int ntgt, int ntgt_pad,
+                          hipStream_t s)
+{
+    const Params& p = h.prm;
+    if (!p.loaded) throw ArgError{"parameters not loaded"};
+    if (!(p.any(1) || p.any(2))) return;   // ApplyFarFieldOperators.hpp:83
+    if (p.K > 64) throw ArgError{"at most 64 wave-number chunks are supported"};
+    const int WCSP = wcsp_plain((int)p.wcs);
+    const int ldb = (int)p.K * WCSP, ldp = 2 * ldb;
+    h.ws_b32.reserve(sizeof(float2) * h.nt_pad * ldb);
+    float2* b = h.ws_b32.as<float2>();
+    launch_av(h, a.B, a.ldB, a.Btype, WCSP, b, ldb, s);
+    const int tiles = ntgt_pad / kTile;
+    const int S0 = choose_segments(h, tiles, (int)(h.nt_pad / kSrcBlock));
+    const int seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
+    const int S = (int)((h.nt_pad + seg_len - 1) / seg_len);
+    const int64_t seg_stride = (int64_t)ntgt_pad * ldp;
+    h.ws_part.reserve(sizeof(float) * seg_stride * S);
+    run_plain(h, op, false, WCSP, tx, nullptr, ntgt, ntgt_pad, h.d_mid, h.d_nrm, (int)h.nt, (int)h.nt_pad, b, ldb,
+              h.ws_part.as<float>(), seg_stride, ldp, S, seg_len, s);
+    rows_epilogue(ntgt, p, WCSP, S, seg_stride, ldp, h.ws_part.as<float>(), a, s);
+}
+
+void apply_far_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
+{
+    if (h.m < 1) throw ArgError{"no measurement directions"};
+    apply_rows_op(h, OP_FAR, a, h.d_meas, (int)h.m, (int)h.m_pad, s);
+}
+
+void apply_near_dev(Handle& h, const ApplyArgs& a, const float4* d_pts, int64_t P, hipStream_t s)
+{
+    const int P_pad = (int)(((P + kTile - 1) / kTile) * kTile);
+    apply_rows_op(h, OP_NEAR, a, d_pts, (int)P, P_pad, s);
+}
+
+void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
+{
+    const Params& p = h.prm;
+    if (!p.loaded) throw ArgError{"parameters not loaded"};
+    if (!(p.any(1) || p.any(2))) return;   // CreateHerglotzWave.hpp:200
+    if (h.m < 1) throw ArgError{"no measurement directions"};
+    if (p.K > 64) throw ArgError{"at most 64 wave-number chunks are supported"};
+    const int WCSP = wcsp_plain((int)p.wcs);
+    const int ldb = (int)p.K * WCSP, ldp = 2 * ldb;
+    // g (m x W) -> conj(g) staged with m_pad rows (HerglotzWaveKernel.cl:143-144)
+    h.ws_g.reserve(sizeof(float2) * h.m_pad * ldb);
+    float2* gb = h.ws_g.as<float2>();
+    const int64_t total = h.m_pad * (int64_t)ldb;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (a.Btype == kC128)
+        hipLaunchKernelGGL(stage_rows_kernel<kC128>, grid, dim3(256), 0, s, (int)h.m, total, a.B, a.ldB, (int)p.wcs,
+                           WCSP, ldb, 1, gb);
+    else
+        hipLaunchKernelGGL(stage_rows_kernel<kC64>, grid, dim3(256), 0, s, (int)h.m, total, a.B, a.ldB, (int)p.wcs,
+                           WCSP, ldb, 1, gb);
+    BAEMM_HIP(hipGetLastError());
+    const int tiles = (int)(h.nt_pad / kTile);
+    const int S0 = choose_segments(h, tiles, (int)(h.m_pad / kSrcBlock));
+    const int seg_len = (int)(((h.m_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
+    const int S = (int)((h.m_pad + seg_len - 1) / seg_len);
+    const int64_t seg_stride = h.nt_pad * (int64_t)ldp;
+    h.ws_part.reserve(sizeof(float) * seg_stride * S);
+    run_plain(h, OP_HERG, false, WCSP, h.d_mid, h.d_nrm, (int)h.nt, (int)h.nt_pad, h.d_meas, nullptr, (int)h.m,
+              (int)h.m_pad, gb, ldb, h.ws_part.as<float>(), seg_stride, ldp, S, seg_len, s);
+    EpiArgs ea = base_epi(h, a, WCSP);
+    ea.kernel_on = 1;
+    ea.diag_on = 0;
+    ea.mass_on = 0;
+    ea.S = S;
+    ea.seg_stride = seg_stride;
+    ea.ldp = ldp;
+    ea.ldb = ldb;
+    ea.part = h.ws_part.as<float>();
+    ea.b = gb;
+    ea.unscale = nullptr;
+    bdr_epilogue(h, ea, kC64, a.Ctype, s);
+}
+
+void incident_dev(Handle& h, const ApplyArgs& a, const double* dirs, int64_t inc_count, int wave_type,
+                  hipStream_t s)
+{
+    const Params& p = h.prm;
+    if (!p.loaded) throw ArgError{"parameters not loaded"};
+    if (!(p.any(1) || p.any(2))) return;   // CreateIncidentWave.hpp:95
+    if (!dirs) throw ArgError{"incident directions must not be NULL"};
+    if (inc_count < p.wcs) throw ArgError{"inc_count must be >= wave_chunk_size"};
+    if (p.wcs > 64 || p.K > 64) throw ArgError{"at most 64 directions and 64 chunks are supported"};
+    if (wave_type != BAEMM_WAVE_PLANE && wave_type != BAEMM_WAVE_RADIAL) throw ArgError{"unknown wave type"};
+    const int WCSP = (int)p.wcs;
+    const int ldp = 2 * (int)p.K * WCSP;
+    h.ws_part.reserve(sizeof(float) * h.nt_pad * ldp);
+    IncArgs ia{};
+    ia.nt = (int)h.nt;
+    ia.W = (int)p.W;
+    ia.wcs = (int)p.wcs;
+    ia.WCSP = WCSP;
+    ia.ldp = ldp;
+    ia.radial = (wave_type == BAEMM_WAVE_RADIAL);
+    ia.mid = h.d_mid;
+    ia.nrm = h.d_nrm;
+    ia.part = h.ws_part.as<float>();
+    for (int j = 0; j < p.wcs; ++j)
+        for (int d = 0; d < 3; ++d) ia.dirs[j][d] = (float)dirs[3 * j + d];
+    for (int q = 0; q < p.K; ++q) {
+        ia.kap[q] = p.kappa[q];
+        ia.c1r[q] = p.c[4 * q + 1].real();
+        ia.c1i[q] = p.c[4 * q + 1].imag();
+        ia.c2r[q] = p.c[4 * q + 2].real();
+        ia.c2i[q] = p.c[4 * q + 2].imag();
+    }
+    const int64_t total = h.nt * p.W;
+    hipLaunchKernelGGL(incident_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, ia);
+    BAEMM_HIP(hipGetLastError());
+    EpiArgs ea = base_epi(h, a, WCSP);
+    ea.kernel_on = 1;
+    ea.diag_on = 0;
+    ea.mass_on = 0;
+    ea.S = 1;
+    ea.seg_stride = 0;
+    ea.ldp = ldp;
+    ea.ldb = 0;
+    ea.part = h.ws_part.as<float>();
+    ea.b = nullptr;
+    ea.unscale = nullptr;
+    bdr_epilogue(h, ea, kC64, a.Ctype, s);
+}
+
+}  // namespace baemm

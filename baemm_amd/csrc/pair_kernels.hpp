@@ -1,0 +1,291 @@
+// pair_kernels.hpp -- triangle-pair interaction kernels for gfx950.
+//
+// Each kernel evaluates kernel values A(target i, source j) on the fly in FP32
+// and contracts them with a block of complex right-hand sides:
+//     part[seg][i][col] = sum_{j in seg} A_q(i, j) * b[j][col],  col in chunk q.
+// The source range is split into segments (grid.y) so that a launch has
+// enough work-groups for 256 CUs; the segment partials are summed in the
+// epilogue kernels (deterministic, no atomics).
+//
+// Partial layout per target row (floats): for chunk q of padded width WCSP,
+// the real parts at [q*2*WCSP, q*2*WCSP+WCSP) and the imaginary parts at
+// [q*2*WCSP+WCSP, (q+1)*2*WCSP).  The MFMA kernel's 32-column output blocks map
+// onto this layout directly.
+#pragma once
+
+#include "internal.hpp"
+
+namespace baemm {
+
+enum PairOp { OP_BDR = 0, OP_NEAR = 1, OP_FAR = 2, OP_HERG = 3 };
+
+struct CoefPack {
+    ChunkCoef c[4];
+};
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float hw_sin_rev(float t) { return __builtin_amdgcn_sinf(t); }
+__device__ __forceinline__ float hw_cos_rev(float t) { return __builtin_amdgcn_cosf(t); }
+
+// kappa-independent geometry of one (target, source) pair for the boundary
+// and potential kernels (BoundaryOperatorKernel.cl:104-112, 141, 151-155, 174-178).
+struct BdrGeo {
+    float rinv;   // 1/r, 0 where the pair is masked
+    float r;      // r (0 where masked)
+    float gj;     // (z . n_j) / r^2
+    float gi;     // -(z . n_i) / r^2  (adjoint double layer)
+};
+
+template <bool ADJ>
+__device__ __forceinline__ BdrGeo bdr_geo(float4 x, float4 ni, float4 y, float4 nj, bool valid)
+{
+    const float z0 = y.x - x.x, z1 = y.y - x.y, z2 = y.z - x.z;
+    const float r2 = fmaf(z0, z0, fmaf(z1, z1, z2 * z2));
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    rinv = valid ? rinv : 0.0f;
+    BdrGeo g;
+    g.rinv = rinv;
+    g.r = r2 * rinv;
+    const float rinv2 = rinv * rinv;
+    g.gj = fmaf(z0, nj.x, fmaf(z1, nj.y, z2 * nj.z)) * rinv2;
+    if constexpr (ADJ) g.gi = -fmaf(z0, ni.x, fmaf(z1, ni.y, z2 * ni.z)) * rinv2;
+    else g.gi = 0.0f;
+    return g;
+}
+
+// A = (E/r) [c1 + (c2 gj + c3 gi)(i kappa r - 1)],  E = exp(i kappa r).
+// Equal to c1 E/r + (c2 (z.n_j) - c3 (z.n_i)) (i kappa r - 1) E / r^3 (formula A1).
+template <bool ADJ>
+__device__ __forceinline__ void bdr_value(const BdrGeo& g, const ChunkCoef& c, float& Ar, float& Ai)
+{
+    const float t = g.r * c.kap_rev;
+    const float cs = hw_cos_rev(t), sn = hw_sin_rev(t);
+    const float p = g.r * c.kap;
+    const float er = cs * g.rinv, ei = sn * g.rinv;
+    float Dr = c.c2r * g.gj, Di = c.c2i * g.gj;
+    if constexpr (ADJ) {
+        Dr = fmaf(c.c3r, g.gi, Dr);
+        Di = fmaf(c.c3i, g.gi, Di);
+    }
+    const float Qr = fmaf(-p, Di, c.c1r - Dr);
+    const float Qi = fmaf(p, Dr, c.c1i - Di);
+    Ar = fmaf(er, Qr, -ei * Qi);
+    Ai = fmaf(er, Qi, ei * Qr);
+}
+
+// Plane-wave kernels (formulas A2, A4): A = (c1 - i kappa (u.n) c2) exp(-i kappa u.y)
+// with (u, y, n) = (xhat_i, y_j, n_j) for the far field and (d_j, x_t, n_t) for Herglotz.
+__device__ __forceinline__ void plane_value(float uy, float un, const ChunkCoef& c, float& Ar, float& Ai)
+{
+    const float t = uy * c.kap_rev;
+    const float cs = hw_cos_rev(t), sn = hw_sin_rev(t);
+    const float kn = c.kap * un;
+    const float wr = fmaf(kn, c.c2i, c.c1r);
+    const float wi = fmaf(-kn, c.c2r, c.c1i);
+    Ar = fmaf(wr, cs, wi * sn);
+    Ai = fmaf(wi, cs, -wr * sn);
+}
+
+// ---------------------------------------------------------------------------
+// Plain FP32 pair kernel: one target per thread, WCSP*KG complex accumulators,
+// sources staged per 64 in LDS and broadcast to the whole block.
+// ---------------------------------------------------------------------------
+template <int OP, int WCSP, int KG, bool ADJ>
+__global__ __launch_bounds__(kTile) void pair_plain_kernel(
+    const float4* __restrict__ tx, const float4* __restrict__ tn, int ntgt,
+    const float4* __restrict__ sy, const float4* __restrict__ sn_, int nsrc,
+    const float2* __restrict__ b, int ldb, CoefPack cp,
+    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int seg_len, int nsrc_pad)
+{
+    constexpr int NC = WCSP * KG;
+    __shared__ float4 s_y[kSrcBlock];
+    __shared__ float4 s_n[kSrcBlock];
+    __shared__ float2 s_b[kSrcBlock * NC];
+
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x * kTile + tid;
+    const float4 x = tx[i];
+    float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (OP == OP_BDR || OP == OP_HERG) ni = tn[i];
+
+    float accr[NC], acci[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) accr[k] = acci[k] = 0.0f;
+
+    const int s0 = blockIdx.y * seg_len;
+    const int s1 = min(s0 + seg_len, nsrc_pad);
+    for (int jb = s0; jb < s1; jb += kSrcBlock) {
+        __syncthreads();
+        if (tid < kSrcBlock) s_y[tid] = sy[jb + tid];
+        else if (tid < 2 * kSrcBlock) {
+            if constexpr (OP != OP_HERG) s_n[tid - kSrcBlock] = sn_[jb + tid - kSrcBlock];
+        }
+        // b rows jb..jb+63, columns q0*WCSP .. q0*WCSP + NC (float4 = 2 complex)
+        constexpr int V4 = kSrcBlock * NC / 2;
+        for (int e = tid; e < V4; e += kTile) {
+            const int row = e / (NC / 2 > 0 ? NC / 2 : 1);
+            if constexpr (NC >= 2) {
+                const int c2 = e % (NC / 2);
+                const float4 v = *reinterpret_cast<const float4*>(b + (int64_t)(jb + row) * ldb + q0 * WCSP + 2 * c2);
+                s_b[row * NC + 2 * c2] = make_float2(v.x, v.y);
+                s_b[row * NC + 2 * c2 + 1] = make_float2(v.z, v.w);
+            }
+        }
+        if constexpr (NC == 1) {
+            if (tid < kSrcBlock) s_b[tid] = b[(int64_t)(jb + tid) * ldb + q0 * WCSP];
+        }
+        __syncthreads();
+        const int jn = min(kSrcBlock, nsrc - jb);
+        for (int jj = 0; jj < jn; ++jj) {
+            const int j = jb + jj;
+            float Ar[KG], Ai[KG];
+            if constexpr (OP == OP_BDR || OP == OP_NEAR) {
+                const bool valid = (OP == OP_NEAR) ? true : (j != i);
+                const BdrGeo g = bdr_geo<ADJ>(x, ni, s_y[jj], s_n[jj], valid);
+#pragma unroll
+                for (int q = 0; q < KG; ++q) bdr_value<ADJ>(g, cp.c[q], Ar[q], Ai[q]);
+            } else if constexpr (OP == OP_FAR) {
+                const float4 y = s_y[jj], n = s_n[jj];
+                const float uy = fmaf(x.x, y.x, fmaf(x.y, y.y, x.z * y.z));
+                const float un = fmaf(x.x, n.x, fmaf(x.y, n.y, x.z * n.z));
+#pragma unroll
+                for (int q = 0; q < KG; ++q) plane_value(uy, un, cp.c[q], Ar[q], Ai[q]);
+            } else {  // OP_HERG: sources are directions d_j
+                const float4 d = s_y[jj];
+                const float uy = fmaf(d.x, x.x, fmaf(d.y, x.y, d.z * x.z));
+                const float un = fmaf(d.x, ni.x, fmaf(d.y, ni.y, d.z * ni.z));
+#pragma unroll
+                for (int q = 0; q < KG; ++q) plane_value(uy, un, cp.c[q], Ar[q], Ai[q]);
+            }
+            const float2* bj = s_b + jj * NC;
+#pragma unroll
+            for (int q = 0; q < KG; ++q) {
+#pragma unroll
+                for (int k = 0; k < WCSP; ++k) {
+                    const float2 v = bj[q * WCSP + k];
+                    accr[q * WCSP + k] = fmaf(Ar[q], v.x, fmaf(-Ai[q], v.y, accr[q * WCSP + k]));
+                    acci[q * WCSP + k] = fmaf(Ar[q], v.y, fmaf(Ai[q], v.x, acci[q * WCSP + k]));
+                }
+            }
+        }
+    }
+    if (i >= ntgt) return;
+    float* out = part + (int64_t)blockIdx.y * seg_stride + (int64_t)i * ldp + q0 * 2 * WCSP;
+#pragma unroll
+    for (int q = 0; q < KG; ++q) {
+#pragma unroll
+        for (int k = 0; k < WCSP; ++k) {
+            out[q * 2 * WCSP + k] = accr[q * WCSP + k];
+            out[q * 2 * WCSP + WCSP + k] = acci[q * WCSP + k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision boundary kernel on MFMA (v_mfma_f32_32x32x16_f16).
+//
+// Complex contraction as a real GEMM per chunk:
+//   [Cr | Ci] = [Ar Ai] (interleaved along K) x [[br, bi], [-bi, br]]
+// A-operand (generated in registers): lane l holds target i = l%32 and the
+// 4 sources 4*(l/32)+0..3 of an 8-source step as (Ar, Ai) pairs.
+// B-operand (precomputed by the split kernel, 16 bytes per lane): rows
+// (j, re/im) x 32 real output columns.  Each value v is split
+// v = hi + lo with hi = fp16(v) (round toward zero), lo = fp16(v - hi);
+// the products hi*hi + hi*lo + lo*hi accumulate in FP32.  A carries a
+// power-of-two scale folded into the coefficients, b a per-column
+// power-of-two scale; the epilogue divides both out.
+// Output lane l, register r: row (r/4)*8 + (l/32)*4 + r%4, column l%32.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pack_rtz(float a, float b)
+{
+    auto h = __builtin_amdgcn_cvt_pkrtz(a, b);
+    return __builtin_bit_cast(uint32_t, h);
+}
+
+__device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t& lo)
+{
+    hi = pack_rtz(a, b);
+    const float ah = __uint_as_float(__float_as_uint(a) & 0xFFFFE000u);
+    const float bh = __uint_as_float(__float_as_uint(b) & 0xFFFFE000u);
+    lo = pack_rtz(a - ah, b - bh);
+}
+
+template <int WCSP, int KG, bool ADJ>
+__global__ __launch_bounds__(256) void bdr_mixed_kernel(
+    const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
+    const uint4* __restrict__ bop, int nst, CoefPack cp,
+    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
+{
+    constexpr int NCB = 2 * WCSP / 32;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 5;
+    const int ibase = blockIdx.x * kMixTile + wave * 32;
+    const int i = ibase + (lane & 31);
+    const float4 x = mid[i];
+    float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (ADJ) ni = nrm[i];
+
+    float16_t acc[KG][NCB];
+#pragma unroll
+    for (int q = 0; q < KG; ++q)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[q][cb][r] = 0.0f;
+
+    const int st0 = blockIdx.y * steps_per_seg;
+    const int st1 = min(st0 + steps_per_seg, nst);
+    for (int st = st0; st < st1; ++st) {
+        uint4 bh[KG][NCB], bl[KG][NCB];
+#pragma unroll
+        for (int q = 0; q < KG; ++q)
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const int64_t base = ((((int64_t)(q0 + q) * nst + st) * NCB + cb) * 2) * 64 + lane;
+                bh[q][cb] = bop[base];
+                bl[q][cb] = bop[base + 64];
+            }
+        uint32_t ah[KG][4], al[KG][4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int j = 8 * st + 4 * g + jj;
+            const float4 y = mid[j];
+            const float4 nj = nrm[j];
+            const BdrGeo geo = bdr_geo<ADJ>(x, ni, y, nj, (j != i) && (j < nt));
+#pragma unroll
+            for (int q = 0; q < KG; ++q) {
+                float Ar, Ai;
+                bdr_value<ADJ>(geo, cp.c[q], Ar, Ai);
+                split2(Ar, Ai, ah[q][jj], al[q][jj]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KG; ++q) {
+            const half8_t a_hi = __builtin_bit_cast(half8_t, make_uint4(ah[q][0], ah[q][1], ah[q][2], ah[q][3]));
+            const half8_t a_lo = __builtin_bit_cast(half8_t, make_uint4(al[q][0], al[q][1], al[q][2], al[q][3]));
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const half8_t b_hi = __builtin_bit_cast(half8_t, bh[q][cb]);
+                const half8_t b_lo = __builtin_bit_cast(half8_t, bl[q][cb]);
+                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_hi, acc[q][cb], 0, 0, 0);
+                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_lo, acc[q][cb], 0, 0, 0);
+                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo, b_hi, acc[q][cb], 0, 0, 0);
+            }
+        }
+    }
+    float* out = part + (int64_t)blockIdx.y * seg_stride;
+#pragma unroll
+    for (int q = 0; q < KG; ++q)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = ibase + (r >> 2) * 8 + g * 4 + (r & 3);
+                out[(int64_t)row * ldp + (q0 + q) * 2 * WCSP + cb * 32 + (lane & 31)] = acc[q][cb][r];
+            }
+}
+
+}  // namespace baemm

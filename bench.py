@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark of the BAEMM hot path on one MI355X.
+
+Workload (the operator-apply configuration of the metric): icosphere with
+81,920 triangles / 40,962 vertices (6-fold midpoint subdivision of the unit
+icosahedron), wave numbers kappa in {8, 16} with 32 right-hand sides each
+(W = 64 columns, chunk size 32), vertex values i.i.d. complex N(0,1) from seed 1,
+single + double layer with coefficient rows (0, 1, 1, 0) and (1/2, -i kappa, 1, 0).
+
+One step = one device-resident SL+DL boundary-operator apply
+C = A B  (Av gather, pair kernel, SL diagonal + AvT + mass epilogue), inputs and
+outputs resident in HBM.  Reported:
+
+  value              mixed-precision (FP16 hi+lo MFMA) pair*RHS/s = N_t^2 W / t_step
+  fp32_value         plain FP32 apply, same workload (the >= 3x target's denominator)
+  roofline           dominant kernel (MFMA pair kernel): algorithmic FLOP/s vs dense
+                     FP16 MFMA peak; roofline_valu: kernel-generation FLOP/s vs FP32 peak
+  cpu_baseline       FP32 restatement of the reference OpenCL kernel (oracle) on the
+                     host cores, bounded sample of target rows, same workload
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "pair·RHS/s for SL+DL apply (mixed vs FP32); solves/s + p50/p99 at 1e-6"
+PEAK_MFMA_F16 = 2500.0   # TFLOP/s, dense FP16 MFMA, MI355X
+PEAK_FP32 = 157.3        # TFLOP/s, FP32 vector, MI355X
+G_SLDL = 43              # algorithmic flops per pair per wave number, SL+DL (BoundaryOperatorKernel.cl:104-170)
+
+
+def workload():
+    from baemm_amd.meshes import icosphere
+
+    V, T = icosphere(6)
+    W, wcs = 64, 32
+    kappa = [8.0, 16.0]
+    coeff = np.array([0, 1, 1, 0, 0.5, -1j * 16.0, 1, 0], np.complex128)
+    rng = np.random.default_rng(1)
+    B = rng.standard_normal((V.shape[0], W)) + 1j * rng.standard_normal((V.shape[0], W))
+    return V, T, W, wcs, kappa, coeff, B
+
+
+def run_gpu(args):
+    import ctypes
+
+    from baemm_amd import Helmholtz_HIP, _lib
+
+    V, T, W, wcs, kappa, coeff, B = workload()
+    nt, nv = T.shape[0], V.shape[0]
+    L = _lib.load()
+    H = Helmholtz_HIP(V, T)
+    h = H._h
+    nbytes = nv * W * 16
+    dB = ctypes.c_void_p()
+    dC = ctypes.c_void_p()
+    _lib.check(L.baemm_dev_alloc(ctypes.byref(dB), nbytes))
+    _lib.check(L.baemm_dev_alloc(ctypes.byref(dC), nbytes))
+    Bc = np.ascontiguousarray(B)
+    _lib.check(L.baemm_dev_upload(dB, Bc.ctypes.data_as(ctypes.c_void_p), nbytes))
+    one = (ctypes.c_double * 2)(1.0, 0.0)
+    zero = (ctypes.c_double * 2)(0.0, 0.0)
+    H.LoadBoundaryOperators_PL(kappa, coeff, W, wcs)
+
+    def step():
+        _lib.check(L.baemm_apply_boundary_dev(h, one, dB, W, _lib.C128, zero, dC, W, _lib.C128, None), h)
+
+    results = {}
+    for mode in ("mixed", "fp32"):
+        H.SetPrecision(mode)
+        for _ in range(args.warmup):
+            step()
+        _lib.check(L.baemm_dev_synchronize())
+        H.SetTiming(True)
+        _lib.check(L.baemm_dev_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        _lib.check(L.baemm_dev_synchronize())
+        t1 = time.perf_counter()
+        kms, launches = H.GetTiming()
+        H.SetTiming(False)
+        out = np.zeros((nv, W), np.complex128)
+        _lib.check(L.baemm_dev_download(out.ctypes.data_as(ctypes.c_void_p), dC, nbytes))
+        results[mode] = {
+            "ms_per_step": (t1 - t0) * 1e3 / args.steps,
+            "kernel_ms": kms / max(1, args.steps),
+            "kernel_launch_groups": launches,
+            "out": out,
+        }
+    # accuracy of the mixed apply relative to the FP32 apply (same inputs)
+    dm = results["mixed"]["out"] - results["fp32"]["out"]
+    rel_mixed_vs_fp32 = float(np.linalg.norm(dm) / np.linalg.norm(results["fp32"]["out"]))
+    L.baemm_dev_free(dB)
+    L.baemm_dev_free(dC)
+    H.close()
+    return nt, nv, W, results, rel_mixed_vs_fp32
+
+
+def cpu_baseline(rows_target_seconds=12.0):
+    """Times the FP32 restatement of BoundaryOperatorKernel.cl (oracle) on the
+    host cores for a bounded sample of target rows of the same workload."""
+    import oracle
+
+    V, T, W, wcs, kappa, coeff, B = workload()
+    m = oracle.Mesh(V, T)
+    ch = oracle.F32Chain(m)
+    kap32, c32, bits, _ = ch.params32(kappa, coeff, W, wcs)
+    b32 = np.ascontiguousarray((ch.Av @ B).astype(np.complex64))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    oracle.set_threads(threads)
+    rows = 64
+    t = 0.0
+    while True:
+        t0 = time.perf_counter()
+        ch.kernel(kap32, c32, bits, b32, wcs, 0, rows)
+        t = time.perf_counter() - t0
+        if t > 1.0 or rows >= m.nt:
+            break
+        rows = min(m.nt, rows * 4)
+    rows = int(min(m.nt, max(rows, rows * rows_target_seconds / max(t, 1e-3))))
+    rows = max(64, (rows // 64) * 64)
+    t0 = time.perf_counter()
+    ch.kernel(kap32, c32, bits, b32, wcs, 0, rows)
+    t = time.perf_counter() - t0
+    value = rows * m.nt * W / t
+    return {
+        "value": value,
+        "unit": "pair·RHS/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP32 restatement of BoundaryOperatorKernel.cl (64-source blocks, per-chunk loop), "
+                  f"{rows} of {m.nt} target rows x all {m.nt} sources x {W} RHS, {t:.1f} s",
+        "seconds": t,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "the apply shards by target rows or by request, see DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+    nt, nv, W, res, rel = run_gpu(args)
+    pairs_rhs = float(nt) * nt * W
+    mixed = res["mixed"]
+    fp32 = res["fp32"]
+    flops = float(nt) * nt * (8 * W + G_SLDL * 2)
+    kt = mixed["kernel_ms"] * 1e-3
+    achieved = flops / kt / 1e12
+    gen_tf = float(nt) * nt * G_SLDL * 2 / kt / 1e12
+    line = {
+        "metric": METRIC,
+        "value": pairs_rhs / (mixed["ms_per_step"] * 1e-3),
+        "unit": "pair·RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 kernel values, f16 hi+lo MFMA contraction, f32 accumulate",
+        "data": "synthetic",
+        "config": {
+            "workload": "SL+DL boundary-operator apply, icosphere 81920 triangles / 40962 vertices, "
+                        "kappa {8,16} x 32 RHS = 64 columns, complex N(0,1) vertex values (seed 1)",
+            "n_triangles": nt,
+            "n_vertices": nv,
+            "rhs_columns": W,
+            "wave_chunk_size": 32,
+        },
+        "fp32_value": pairs_rhs / (fp32["ms_per_step"] * 1e-3),
+        "fp32_ms_per_step": fp32["ms_per_step"],
+        "speedup_mixed_vs_fp32": fp32["ms_per_step"] / mixed["ms_per_step"],
+        "rel_diff_mixed_vs_fp32": rel,
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_MFMA_F16,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_MFMA_F16,
+            "traffic": None,
+            "kernel": "bdr_mixed_kernel",
+            "kernel_ms": mixed["kernel_ms"],
+            "flops_per_launch": flops,
+        },
+        "roofline_valu": {
+            "bound": "fp32-valu",
+            "achieved": gen_tf,
+            "peak": PEAK_FP32,
+            "unit": "TFLOP/s",
+            "frac": gen_tf / PEAK_FP32,
+            "note": "kernel-value generation flops (43 per pair per wave number) of the MFMA kernel",
+        },
+        "roofline_fp32_kernel": {
+            "bound": "fp32-valu",
+            "achieved": flops / (fp32["kernel_ms"] * 1e-3) / 1e12,
+            "peak": PEAK_FP32,
+            "unit": "TFLOP/s",
+            "frac": flops / (fp32["kernel_ms"] * 1e-3) / 1e12 / PEAK_FP32,
+            "kernel": "pair_plain_kernel",
+            "kernel_ms": fp32["kernel_ms"],
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the baseline must not hide the GPU result
+            line["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

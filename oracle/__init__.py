@@ -279,6 +279,41 @@ def apply_boundary(mesh: Mesh, alpha, B, beta, C, kappa, coeff, W, wcs, use_diag
     return C
 
 
+def apply_boundary_rows(mesh: Mesh, alpha, B, beta, C, kappa, coeff, W, wcs, vrows, use_diagonal=True):
+    """apply_boundary restricted to the vertex rows ``vrows`` (C has len(vrows) rows).
+
+    Only the P0 rows of triangles incident to those vertices are evaluated, so
+    the O(N^2) oracle stays cheap on large meshes (sampled-row parity)."""
+    kap, c = load_parameters(kappa, coeff, W, wcs)
+    K = kap.shape[0]
+    f = flags_of(c)
+    B = _c128(B)
+    vrows = np.asarray(vrows, dtype=np.int64)
+    C = np.array(C, dtype=np.complex128, copy=True)
+    kernel_on = any(f[k] for k in f if not k.endswith("mass_matrix"))
+    mass_on = f["Re_mass_matrix"] or f["Im_mass_matrix"]
+    cols = _chunk_cols(W, wcs, K)
+    if kernel_on:
+        AvT_rows = mesh.AvT[vrows]
+        tris = np.unique(AvT_rows.indices)
+        b = mesh.Av @ B
+        Ck = bdr_offdiag_f64(mesh, kap, c, b, wcs, rows=tris)
+        if use_diagonal and (f["Re_single_layer"] or f["Im_single_layer"]):
+            for q, sl in enumerate(cols):
+                Ck[:, sl] += (c[q, 1] * (mesh.sdiag[tris] + 1j * kap[q]))[:, None] * b[tris][:, sl]
+        full = np.zeros((mesh.nt, W), np.complex128)
+        full[tris] = Ck
+        C = alpha * (AvT_rows @ full) + beta * C
+        add_to = 1.0
+    else:
+        add_to = beta
+    if mass_on:
+        MB = mesh.M[vrows] @ B
+        for q, sl in enumerate(cols):
+            C[:, sl] = alpha * c[q, 0] * MB[:, sl] + add_to * C[:, sl]
+    return C
+
+
 def apply_far_field(mesh: Mesh, alpha, B, beta, F, kappa, coeff, W, wcs):
     """ApplyFarFieldOperators_PL (ApplyFarFieldOperators.hpp:49-104), FP64."""
     kap, c = load_parameters(kappa, coeff, W, wcs)

@@ -1,0 +1,164 @@
+"""GPU parity of the boundary-operator apply (ApplyBoundaryOperators_PL)
+against the FP64 oracle and the FP32 restatement of the reference chain.
+
+Tolerance: 1e-5 relative (Frobenius) in both precisions, the bar the reference's
+FP32 apply sets; sampled vertex rows at the 81,920-triangle size."""
+import numpy as np
+import pytest
+
+import oracle
+from baemm_amd import Helmholtz_HIP
+from baemm_amd.meshes import icosphere, perturbed_sphere, torus
+from conftest import oracle_mesh, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+FULL_COEFF = [0.5 + 0.25j, 1 - 0.5j, -0.75 + 1j, 0.3 - 0.9j]
+
+
+def _rand(rng, n, W):
+    return rng.standard_normal((n, W)) + 1j * rng.standard_normal((n, W))
+
+
+@pytest.fixture(scope="module")
+def sphere3():
+    m = oracle_mesh("sphere", 3)
+    H = Helmholtz_HIP(m.V, m.T)
+    yield m, H
+    H.close()
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+@pytest.mark.parametrize(
+    "W,wcs,kappa",
+    [(16, 16, [4.0]), (8, 4, [4.0, 2.5]), (12, 3, [1.0, 2.0, 3.0, 4.0]), (64, 32, [8.0, 16.0]), (1, 1, [3.0]),
+     (64, 16, [1.0, 2.0, 3.0, 5.0])],
+)
+def test_boundary_apply_matches_oracle(sphere3, mode, W, wcs, kappa):
+    m, H = sphere3
+    H.SetPrecision(mode)
+    rng = np.random.default_rng(W * 7 + wcs)
+    B = _rand(rng, m.nv, W)
+    K = W // wcs
+    coeff = np.array(FULL_COEFF * K, np.complex128)
+    coeff[4:] *= 0.7  # distinct rows per chunk
+    ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
+    C = np.zeros_like(B)
+    H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)
+    print(f"{mode} W={W} wcs={wcs} relerr={relerr(C, ref):.3e}")
+    assert relerr(C, ref) < TOL
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+def test_c1_all_operators_vs_fp32_restatement(mode):
+    """Config 1: 5120 triangles, kappa 4, 16 plane-wave RHS, all four operators;
+    compared with the FP32 restatement of the reference kernel and the FP64 oracle."""
+    m = oracle_mesh("sphere", 4)
+    rng = np.random.default_rng(0)
+    d = rng.standard_normal((16, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    B = np.exp(1j * 4.0 * m.V @ d.T)
+    ref64 = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), [4.0], FULL_COEFF, 16, 16)
+    ref32 = oracle.F32Chain(m).apply_boundary(1.0, B, 0.0, np.zeros_like(B), [4.0], FULL_COEFF, 16, 16)
+    with Helmholtz_HIP(m.V, m.T) as H:
+        H.SetPrecision(mode)
+        C = np.zeros_like(B)
+        H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [4.0], FULL_COEFF, 16, 16)
+    e64, e32 = relerr(C, ref64), relerr(C, ref32)
+    print(f"C1 {mode}: vs FP64 {e64:.3e}, vs FP32 restatement {e32:.3e}")
+    assert e64 < TOL and e32 < TOL
+
+
+def test_alpha_beta_and_branch_semantics(sphere3):
+    m, H = sphere3
+    H.SetPrecision("fp32")
+    rng = np.random.default_rng(3)
+    B = _rand(rng, m.nv, 4)
+    C0 = _rand(rng, m.nv, 4)
+    coeff = FULL_COEFF
+    alpha, beta = 0.3 - 0.1j, 2.0 + 1.0j
+    ref = oracle.apply_boundary(m, alpha, B, beta, C0, [2.0], coeff, 4, 4)
+    C = C0.copy()
+    H.ApplyBoundaryOperators_PL(alpha, B, beta, C, [2.0], coeff, 4, 4)
+    assert relerr(C, ref) < TOL
+    # all-zero coefficients: C untouched (beta not applied), as in the reference
+    C = C0.copy()
+    H.ApplyBoundaryOperators_PL(1.0, B, 0.5, C, [2.0], [0, 0, 0, 0], 4, 4)
+    np.testing.assert_array_equal(C, C0)
+    # mass only
+    ref = oracle.apply_boundary(m, 2.0, B, 0.5, C0, [2.0], [0.25, 0, 0, 0], 4, 4)
+    C = C0.copy()
+    H.ApplyBoundaryOperators_PL(2.0, B, 0.5, C, [2.0], [0.25, 0, 0, 0], 4, 4)
+    assert relerr(C, ref) < 1e-7   # mass values from FP32 areas
+    # UseDiagonal(false) drops the single-layer self term
+    H.UseDiagonal(False)
+    ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), [2.0], coeff, 4, 4, use_diagonal=False)
+    C = np.zeros_like(B)
+    H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [2.0], coeff, 4, 4)
+    H.UseDiagonal(True)
+    assert relerr(C, ref) < TOL
+    # strided (ld > W) input and output
+    Bw = np.zeros((m.nv, 7), np.complex128)
+    Bw[:, :4] = B
+    Cw = np.zeros((m.nv, 9), np.complex128)
+    ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), [2.0], coeff, 4, 4)
+    H.LoadBoundaryOperators_PL([2.0], coeff, 4, 4)
+    H.ApplyBoundaryOperators_PL(1.0, Bw[:, :4], 0.0, Cw[:, :4])
+    assert relerr(Cw[:, :4], ref) < TOL
+    assert np.all(Cw[:, 4:] == 0)
+
+
+def test_argument_errors(sphere3):
+    from baemm_amd import BaemmError
+
+    m, H = sphere3
+    with pytest.raises(BaemmError):
+        H.LoadBoundaryOperators_PL([1.0, 2.0], FULL_COEFF * 2, 6, 4)   # W % wcs != 0
+    with pytest.raises(BaemmError):
+        H.SetPrecision(7)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+def test_nonspherical_meshes(mode):
+    for V, T in (perturbed_sphere(3, seed=3), torus(1.0, 0.4, 48, 16)):
+        m = oracle.Mesh(V, T)
+        rng = np.random.default_rng(1)
+        B = _rand(rng, m.nv, 32)
+        ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), [3.0, 7.0], FULL_COEFF * 2, 32, 16)
+        with Helmholtz_HIP(V, T) as H:
+            H.SetPrecision(mode)
+            C = np.zeros_like(B)
+            H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [3.0, 7.0], FULL_COEFF * 2, 32, 16)
+        assert relerr(C, ref) < TOL
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+def test_c2_size_sampled_rows(mode):
+    """Config 2 size: 81,920 triangles, kappa in {8, 16}, 2 x 32 RHS, SL+DL with
+    coefficients (0,1,1,0) and (1/2, -i kappa, 1, 0); 48 sampled vertex rows
+    against the FP64 oracle, plus a full-size bilinear symmetry check."""
+    V, T = icosphere(6)
+    m = oracle.Mesh(V, T)
+    rng = np.random.default_rng(1)
+    W, wcs = 64, 32
+    B = _rand(rng, m.nv, W)
+    coeff = np.array([0, 1, 1, 0, 0.5, -16j, 1, 0], np.complex128)
+    rows = np.sort(rng.choice(m.nv, 48, replace=False))
+    ref = oracle.apply_boundary_rows(m, 1.0, B, 0.0, np.zeros((48, W)), [8.0, 16.0], coeff, W, wcs, rows)
+    with Helmholtz_HIP(V, T) as H:
+        H.SetPrecision(mode)
+        C = np.zeros_like(B)
+        H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [8.0, 16.0], coeff, W, wcs)
+        e = relerr(C[rows], ref)
+        print(f"C2 {mode}: sampled-row rel. error vs FP64 = {e:.3e}")
+        assert e < TOL
+        # symmetry of the single layer at full size: v^T S u = u^T S v
+        u = _rand(rng, m.nv, 1)
+        v = _rand(rng, m.nv, 1)
+        X = np.concatenate([u, v], axis=1)
+        Y = np.zeros_like(X)
+        H.ApplyBoundaryOperators_PL(1.0, X, 0.0, Y, [16.0], [0, 1, 0, 0], 2, 2)
+        a = (v[:, 0] @ Y[:, 0])
+        b = (u[:, 0] @ Y[:, 1])
+        assert abs(a - b) < 1e-5 * abs(a)
