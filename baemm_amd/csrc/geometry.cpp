@@ -211,6 +211,11 @@ std::vector<ChunkCoef> chunk_coefs(const Params& p, int q0, int kg, float scale)
         cc.c2i = p.c[4 * q + 2].imag() * scale;
         cc.c3r = p.c[4 * q + 3].real() * scale;
         cc.c3i = p.c[4 * q + 3].imag() * scale;
+        const double k = (double)p.kappa[q];
+        cc.kc2r = (float)(k * p.c[4 * q + 2].real() * scale);
+        cc.kc2i = (float)(k * p.c[4 * q + 2].imag() * scale);
+        cc.kc3r = (float)(k * p.c[4 * q + 3].real() * scale);
+        cc.kc3i = (float)(k * p.c[4 * q + 3].imag() * scale);
         out[g] = cc;
     }
     return out;

@@ -85,6 +85,7 @@ struct ChunkCoef {
     float kap;       // kappa
     float kap_rev;   // kappa / (2 pi): the hardware sine takes revolutions
     float c1r, c1i, c2r, c2i, c3r, c3i;
+    float kc2r, kc2i, kc3r, kc3i;   // kappa * c2, kappa * c3
 };
 
 struct Handle {

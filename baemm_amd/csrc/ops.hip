@@ -90,23 +90,34 @@ __global__ void stage_rows_kernel(int nrows, int64_t total, const void* __restri
     b[idx] = out;
 }
 
-// Per-column max of |re|, |im| (positive floats order like their bit patterns).
-__global__ void colmax_kernel(const float2* __restrict__ b, int nrows, int ldb, unsigned* __restrict__ colmax)
+__global__ void zero_u32_kernel(unsigned* __restrict__ p, int n)
 {
-    const int pc = blockIdx.x;
-    float m = 0.0f;
-    for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
-        const float2 v = b[(int64_t)r * ldb + pc];
-        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
-    }
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    __shared__ float sm[16];
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float mm = 0.0f;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mm = fmaxf(mm, sm[w]);
-        colmax[pc] = __float_as_uint(mm);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Per-column max of |re|, |im| (positive floats order like their bit patterns).
+// Block: 4 row lanes x 64 columns; blocks stride over rows; one atomicMax per
+// column and block.  colmax must be zero on entry.
+__global__ __launch_bounds__(256) void colmax_kernel(const float2* __restrict__ b, int nrows, int ldb,
+                                                     unsigned* __restrict__ colmax)
+{
+    __shared__ float sm[4][64];
+    const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < ldb; c0 += 64) {
+        float m = 0.0f;
+        if (c0 + c < ldb)
+            for (int r = blockIdx.x * 4 + rl; r < nrows; r += gridDim.x * 4) {
+                const float2 v = b[(int64_t)r * ldb + c0 + c];
+                m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+            }
+        sm[rl][c] = m;
+        __syncthreads();
+        if (rl == 0 && c0 + c < ldb) {
+            m = fmaxf(fmaxf(sm[0][c], sm[1][c]), fmaxf(sm[2][c], sm[3][c]));
+            atomicMax(&colmax[c0 + c], __float_as_uint(m));
+        }
+        __syncthreads();
     }
 }
 
@@ -612,7 +623,8 @@ void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
             unsigned* colmax = h.ws_scale.as<unsigned>();
             float* scale = reinterpret_cast<float*>(colmax + ncols);
             float* unscale = scale + ncols;
-            hipLaunchKernelGGL(colmax_kernel, dim3(ncols), dim3(256), 0, s, b, (int)h.nt, ldb, colmax);
+            hipLaunchKernelGGL(zero_u32_kernel, dim3((ncols + 255) / 256), dim3(256), 0, s, colmax, ncols);
+            hipLaunchKernelGGL(colmax_kernel, dim3(256), dim3(256), 0, s, b, (int)h.nt, ldb, colmax);
             hipLaunchKernelGGL(colscale_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, colmax, ncols, WCSP, csc,
                                scale, unscale);
             const int nst = (int)(h.nt_pad / 8);

@@ -35,44 +35,50 @@ struct BdrGeo {
     float rinv;   // 1/r, 0 where the pair is masked
     float r;      // r (0 where masked)
     float gj;     // (z . n_j) / r^2
+    float hj;     // (z . n_j) / r^3
     float gi;     // -(z . n_i) / r^2  (adjoint double layer)
+    float hi;     // -(z . n_i) / r^3
 };
 
-template <bool ADJ>
+template <bool ADJ, bool MASK>
 __device__ __forceinline__ BdrGeo bdr_geo(float4 x, float4 ni, float4 y, float4 nj, bool valid)
 {
     const float z0 = y.x - x.x, z1 = y.y - x.y, z2 = y.z - x.z;
     const float r2 = fmaf(z0, z0, fmaf(z1, z1, z2 * z2));
     float rinv = __builtin_amdgcn_rsqf(r2);
-    rinv = valid ? rinv : 0.0f;
+    if constexpr (MASK) rinv = valid ? rinv : 0.0f;
     BdrGeo g;
     g.rinv = rinv;
     g.r = r2 * rinv;
     const float rinv2 = rinv * rinv;
     g.gj = fmaf(z0, nj.x, fmaf(z1, nj.y, z2 * nj.z)) * rinv2;
-    if constexpr (ADJ) g.gi = -fmaf(z0, ni.x, fmaf(z1, ni.y, z2 * ni.z)) * rinv2;
-    else g.gi = 0.0f;
+    g.hj = g.gj * rinv;
+    if constexpr (ADJ) {
+        g.gi = -fmaf(z0, ni.x, fmaf(z1, ni.y, z2 * ni.z)) * rinv2;
+        g.hi = g.gi * rinv;
+    } else {
+        g.gi = g.hi = 0.0f;
+    }
     return g;
 }
 
-// A = (E/r) [c1 + (c2 gj + c3 gi)(i kappa r - 1)],  E = exp(i kappa r).
-// Equal to c1 E/r + (c2 (z.n_j) - c3 (z.n_i)) (i kappa r - 1) E / r^3 (formula A1).
+// Formula A1, c1 E/r + (c2 (z.n_j) - c3 (z.n_i)) (i kappa r - 1) E / r^3,
+// rearranged as A = E Q' with E = exp(i kappa r) and
+//   Q' = c1/r - c2 (z.n_j)/r^3 - c3 gi/r + i kappa (c2 (z.n_j)/r^2 + c3 gi)
+// (the kappa r factor of the double layer cancels one power of 1/r).
 template <bool ADJ>
 __device__ __forceinline__ void bdr_value(const BdrGeo& g, const ChunkCoef& c, float& Ar, float& Ai)
 {
     const float t = g.r * c.kap_rev;
     const float cs = hw_cos_rev(t), sn = hw_sin_rev(t);
-    const float p = g.r * c.kap;
-    const float er = cs * g.rinv, ei = sn * g.rinv;
-    float Dr = c.c2r * g.gj, Di = c.c2i * g.gj;
+    float Qr = fmaf(-c.c2r, g.hj, fmaf(-c.kc2i, g.gj, c.c1r * g.rinv));
+    float Qi = fmaf(-c.c2i, g.hj, fmaf(c.kc2r, g.gj, c.c1i * g.rinv));
     if constexpr (ADJ) {
-        Dr = fmaf(c.c3r, g.gi, Dr);
-        Di = fmaf(c.c3i, g.gi, Di);
+        Qr = fmaf(-c.c3r, g.hi, fmaf(-c.kc3i, g.gi, Qr));
+        Qi = fmaf(-c.c3i, g.hi, fmaf(c.kc3r, g.gi, Qi));
     }
-    const float Qr = fmaf(-p, Di, c.c1r - Dr);
-    const float Qi = fmaf(p, Dr, c.c1i - Di);
-    Ar = fmaf(er, Qr, -ei * Qi);
-    Ai = fmaf(er, Qi, ei * Qr);
+    Ar = fmaf(cs, Qr, -sn * Qi);
+    Ai = fmaf(cs, Qi, sn * Qr);
 }
 
 // Plane-wave kernels (formulas A2, A4): A = (c1 - i kappa (u.n) c2) exp(-i kappa u.y)
@@ -143,7 +149,7 @@ __global__ __launch_bounds__(kTile) void pair_plain_kernel(
             float Ar[KG], Ai[KG];
             if constexpr (OP == OP_BDR || OP == OP_NEAR) {
                 const bool valid = (OP == OP_NEAR) ? true : (j != i);
-                const BdrGeo g = bdr_geo<ADJ>(x, ni, s_y[jj], s_n[jj], valid);
+                const BdrGeo g = bdr_geo<ADJ, OP == OP_BDR>(x, ni, s_y[jj], s_n[jj], valid);
 #pragma unroll
                 for (int q = 0; q < KG; ++q) bdr_value<ADJ>(g, cp.c[q], Ar[q], Ai[q]);
             } else if constexpr (OP == OP_FAR) {
@@ -212,13 +218,68 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t&
     lo = pack_rtz(a - ah, b - bh);
 }
 
+template <int KG, bool ADJ, bool MASK>
+__device__ __forceinline__ void mixed_gen(const float4 x, const float4 ni, const float4 (&y)[4], const float4 (&nj)[4],
+                                          int jbase, int i, int nt, const CoefPack& cp, uint32_t (&ah)[KG][4],
+                                          uint32_t (&al)[KG][4])
+{
+    // Written stage by stage across the 4 sources so that the independent
+    // dependency chains interleave (the kernel is bound by VALU latency
+    // otherwise).
+    BdrGeo geo[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const int j = jbase + jj;
+        geo[jj] = bdr_geo<ADJ, MASK>(x, ni, y[jj], nj[jj], (j != i) && (j < nt));
+    }
+#pragma unroll
+    for (int q = 0; q < KG; ++q) {
+        float cs[4], sn[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const float t = geo[jj].r * cp.c[q].kap_rev;
+            cs[jj] = hw_cos_rev(t);
+            sn[jj] = hw_sin_rev(t);
+        }
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const ChunkCoef& c = cp.c[q];
+            const BdrGeo& g = geo[jj];
+            float Qr = fmaf(-c.c2r, g.hj, fmaf(-c.kc2i, g.gj, c.c1r * g.rinv));
+            float Qi = fmaf(-c.c2i, g.hj, fmaf(c.kc2r, g.gj, c.c1i * g.rinv));
+            if constexpr (ADJ) {
+                Qr = fmaf(-c.c3r, g.hi, fmaf(-c.kc3i, g.gi, Qr));
+                Qi = fmaf(-c.c3i, g.hi, fmaf(c.kc3r, g.gi, Qi));
+            }
+            const float Ar = fmaf(cs[jj], Qr, -sn[jj] * Qi);
+            const float Ai = fmaf(cs[jj], Qi, sn[jj] * Qr);
+            split2(Ar, Ai, ah[q][jj], al[q][jj]);
+        }
+    }
+}
+
+#ifndef BAEMM_MIX_NS
+#define BAEMM_MIX_NS 2
+#endif
+#ifndef BAEMM_MIX_MINB
+#define BAEMM_MIX_MINB 3
+#endif
 template <int WCSP, int KG, bool ADJ>
-__global__ __launch_bounds__(256) void bdr_mixed_kernel(
+__global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
     const uint4* __restrict__ bop, int nst, CoefPack cp,
     float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
 {
     constexpr int NCB = 2 * WCSP / 32;
+    constexpr int OPS = KG * NCB * 2;       // 1-KB operand tiles per step (chunk x col-block x hi/lo)
+    constexpr int NS = BAEMM_MIX_NS;        // steps (8 sources each) per LDS stage
+    constexpr int TILES = NS * OPS;
+    // Operand tiles and source geometry of one stage, double buffered; filled by
+    // global_load_lds (16 bytes per lane straight into LDS, no VGPR staging).
+    __shared__ uint4 s_op[2][TILES][64];
+    constexpr int GEO = 2 * NS * 8;         // float4 per stage: NS*8 centroids, then NS*8 normals
+    __shared__ float4 s_geo[2][GEO];
+
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 5;
@@ -238,42 +299,81 @@ __global__ __launch_bounds__(256) void bdr_mixed_kernel(
 
     const int st0 = blockIdx.y * steps_per_seg;
     const int st1 = min(st0 + steps_per_seg, nst);
-    for (int st = st0; st < st1; ++st) {
-        uint4 bh[KG][NCB], bl[KG][NCB];
+    const int nt_pad = nst * 8;
+
+    auto issue_stage = [&](int sbase, int buf) {
+        for (int t = wave; t < TILES; t += 4) {
+            const int st = sbase + t / OPS;
+            if (st >= st1) break;
+            const int o = t % OPS;
+            const int q = o / (NCB * 2), cb = (o / 2) % NCB, pt = o % 2;
+            const uint4* src = bop + ((((int64_t)(q0 + q) * nst + st) * NCB + cb) * 2 + pt) * 64 + lane;
+            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_op[buf][t][0], 16, 0, 0);
+        }
+        if (wave == 0) {
+            // lane e of each 64-entry piece writes s_geo[buf][e0 + e]
 #pragma unroll
-        for (int q = 0; q < KG; ++q)
-#pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) {
-                const int64_t base = ((((int64_t)(q0 + q) * nst + st) * NCB + cb) * 2) * 64 + lane;
-                bh[q][cb] = bop[base];
-                bl[q][cb] = bop[base + 64];
-            }
-        uint32_t ah[KG][4], al[KG][4];
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const int j = 8 * st + 4 * g + jj;
-            const float4 y = mid[j];
-            const float4 nj = nrm[j];
-            const BdrGeo geo = bdr_geo<ADJ>(x, ni, y, nj, (j != i) && (j < nt));
-#pragma unroll
-            for (int q = 0; q < KG; ++q) {
-                float Ar, Ai;
-                bdr_value<ADJ>(geo, cp.c[q], Ar, Ai);
-                split2(Ar, Ai, ah[q][jj], al[q][jj]);
+            for (int e0 = 0; e0 < GEO; e0 += 64) {
+                const int e = e0 + lane;
+                if (e < GEO) {
+                    const int which = e / (NS * 8), k = e % (NS * 8);
+                    const int j = min(sbase * 8 + k, nt_pad - 1);
+                    const float4* src = which ? (nrm + j) : (mid + j);
+                    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_geo[buf][e0], 16,
+                                                     0, 0);
+                }
             }
         }
+    };
+
+    if (st0 < st1) issue_stage(st0, 0);
+    int buf = 0;
+    for (int sb = st0; sb < st1; sb += NS, buf ^= 1) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (sb + NS < st1) issue_stage(sb + NS, buf ^ 1);
+        const int ns = min(NS, st1 - sb);
+        for (int s = 0; s < ns; ++s) {
+            const int st = sb + s;
+            float4 y[4], nj[4];
 #pragma unroll
-        for (int q = 0; q < KG; ++q) {
-            const half8_t a_hi = __builtin_bit_cast(half8_t, make_uint4(ah[q][0], ah[q][1], ah[q][2], ah[q][3]));
-            const half8_t a_lo = __builtin_bit_cast(half8_t, make_uint4(al[q][0], al[q][1], al[q][2], al[q][3]));
-#pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) {
-                const half8_t b_hi = __builtin_bit_cast(half8_t, bh[q][cb]);
-                const half8_t b_lo = __builtin_bit_cast(half8_t, bl[q][cb]);
-                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_hi, acc[q][cb], 0, 0, 0);
-                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_lo, acc[q][cb], 0, 0, 0);
-                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo, b_hi, acc[q][cb], 0, 0, 0);
+            for (int jj = 0; jj < 4; ++jj) {
+                y[jj] = s_geo[buf][s * 8 + 4 * g + jj];
+                nj[jj] = s_geo[buf][NS * 8 + s * 8 + 4 * g + jj];
             }
+            uint32_t ah[KG][4], al[KG][4];
+            // Sources of this step overlap the wave's targets or the padding: mask.
+            const bool masked = (8 * st < ibase + 32 && 8 * st + 8 > ibase) || (8 * st + 8 > nt);
+            if (masked) mixed_gen<KG, ADJ, true>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
+            else mixed_gen<KG, ADJ, false>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
+            half8_t a_hi[KG], a_lo[KG], b_hi[KG][NCB], b_lo[KG][NCB];
+#pragma unroll
+            for (int q = 0; q < KG; ++q) {
+                a_hi[q] = __builtin_bit_cast(half8_t, make_uint4(ah[q][0], ah[q][1], ah[q][2], ah[q][3]));
+                a_lo[q] = __builtin_bit_cast(half8_t, make_uint4(al[q][0], al[q][1], al[q][2], al[q][3]));
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb) {
+                    const int o = (q * NCB + cb) * 2;
+                    b_hi[q][cb] = __builtin_bit_cast(half8_t, s_op[buf][s * OPS + o][lane]);
+                    b_lo[q][cb] = __builtin_bit_cast(half8_t, s_op[buf][s * OPS + o + 1][lane]);
+                }
+            }
+            // Product-major order: consecutive MFMAs write different accumulators.
+#pragma unroll
+            for (int q = 0; q < KG; ++q)
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb)
+                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi[q], b_hi[q][cb], acc[q][cb], 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < KG; ++q)
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb)
+                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi[q], b_lo[q][cb], acc[q][cb], 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < KG; ++q)
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb)
+                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo[q], b_hi[q][cb], acc[q][cb], 0, 0, 0);
         }
     }
     float* out = part + (int64_t)blockIdx.y * seg_stride;
