@@ -57,6 +57,21 @@ SIGNATURES = {
     "baemm_create_incident_wave": (_INT, [_P, _P, _P, _I64, _P, _P, _I64, _INT]),
     "baemm_apply_boundary_dev": (_INT, [_P, _P, _P, _I64, _INT, _P, _P, _I64, _INT, _P]),
     "baemm_apply_far_field_dev": (_INT, [_P, _P, _P, _I64, _INT, _P, _P, _I64, _INT, _P]),
+    "baemm_apply_mass_inverse": (_INT, [_P, _P, _I64, _P, _I64, _I64, _D]),
+    "baemm_apply_mass_inverse_complex": (_INT, [_P, _P, _I64, _P, _I64, _I64, _D]),
+    "baemm_apply_lumped_mass_inverse": (_INT, [_P, _P, _I64, _P, _I64, _I64]),
+    "baemm_boundary_potential": (_INT, [_P, _P, _I64, _P, _P, _P, _P, _I64, _D, _D]),
+    "baemm_dirichlet_to_neumann": (_INT, [_P, _P, _I64, _P, _P, _I64, _D, _D]),
+    "baemm_far_field": (_INT, [_P, _P, _I64, _P, _I64, _P, _INT, _D, _D]),
+    "baemm_far_field_parameters": (_INT, [_P, _P, _I64, _P, _I64, _P, _INT, _P, _D, _D]),
+    "baemm_derivative_ff_helper": (_INT, [_P, _P, _I64, _P, _I64, _P, _INT, _D, _D]),
+    "baemm_derivative_ff": (_INT, [_P, _P, _I64, _P, _I64, _P, _P, ctypes.POINTER(_P), _INT, _D, _D]),
+    "baemm_adjoint_derivative_ff": (_INT, [_P, _P, _I64, _P, _I64, _P, _P, ctypes.POINTER(_P), _INT, _D, _D]),
+    "baemm_gauss_newton_solve": (_INT, [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _D, _P, _I64, _D, _P, _I64,
+                                        ctypes.POINTER(_P), _INT, _D, _D, _D, ctypes.POINTER(_INT)]),
+    "baemm_get_solver_stats": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_D),
+                                      ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "baemm_reset_solver_stats": (_INT, [_P]),
     "baemm_set_timing": (_INT, [_P, _INT]),
     "baemm_get_timing": (_INT, [_P, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
     "baemm_dev_alloc": (_INT, [ctypes.POINTER(_P), ctypes.c_size_t]),
@@ -72,6 +87,8 @@ class BaemmError(RuntimeError):
         super().__init__(f"baemm status {status}: {msg}")
         self.status = status
 
+
+REAL_OP = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
 
 _lib = None
 

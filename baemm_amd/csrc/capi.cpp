@@ -5,65 +5,44 @@
 #include <new>
 #include <string>
 
-#include "internal.hpp"
+#include "capi_util.hpp"
 
 using namespace baemm;
 
-struct StateError {
-    std::string msg;
-};
-
 namespace {
 
-Handle* H(baemm_handle h) { return reinterpret_cast<Handle*>(h); }
-
-template <class F>
-int guard(baemm_handle h, F&& f)
-{
-    if (!h) return BAEMM_ERR_ARG;
-    Handle& hh = *H(h);
-    std::lock_guard<std::mutex> lk(hh.mtx);
-    hh.last_error.clear();
-    try {
-        BAEMM_HIP(hipSetDevice(hh.device));
-        return f(hh);
-    } catch (const ArgError& e) {
-        hh.last_error = e.msg;
-        return BAEMM_ERR_ARG;
-    } catch (const StateError& e) {
-        hh.last_error = e.msg;
-        return BAEMM_ERR_STATE;
-    } catch (const HipError& e) {
-        hh.last_error = std::string("HIP error '") + hipGetErrorString(e.err) + "' in " + e.what;
-        return BAEMM_ERR_HIP;
-    } catch (const std::bad_alloc&) {
-        hh.last_error = "host allocation failed";
-        return BAEMM_ERR_INTERNAL;
-    } catch (const std::exception& e) {
-        hh.last_error = e.what();
-        return BAEMM_ERR_INTERNAL;
-    }
-}
-
-cdouble cplx(const double* z, double dflt_re)
-{
-    return z ? cdouble(z[0], z[1]) : cdouble(dflt_re, 0.0);
-}
-
-hipStream_t stream_of(Handle& h, void* s) { return s ? static_cast<hipStream_t>(s) : h.stream; }
+// Message of the last failed baemm_create* on this thread (there is no handle
+// to carry it); returned by baemm_last_error(NULL).
+thread_local std::string g_create_error;
 
 template <class I>
 int create_impl(const double* coords, int64_t nv, const I* tris, int64_t nt, const double* meas, int64_t m,
                 int device, baemm_handle* out)
 {
-    if (!out) return BAEMM_ERR_ARG;
+    g_create_error.clear();
+    auto fail = [](const char* msg) {
+        g_create_error = msg;
+        return BAEMM_ERR_ARG;
+    };
+    if (!out) return fail("out is NULL");
     *out = nullptr;
-    if (!coords || !tris || nv < 3 || nt < 1) return BAEMM_ERR_ARG;
-    if (m < 0 || (m > 0 && !meas)) return BAEMM_ERR_ARG;
-    if (nt > (int64_t)1 << 28 || nv > (int64_t)1 << 28) return BAEMM_ERR_ARG;
+    if (!coords || !tris || nv < 3 || nt < 1) return fail("empty or NULL mesh");
+    if (m < 0 || (m > 0 && !meas)) return fail("invalid measurement directions");
+    if (nt > (int64_t)1 << 28 || nv > (int64_t)1 << 28) return fail("mesh larger than 2^28 vertices or triangles");
     Handle* h = new (std::nothrow) Handle();
-    if (!h) return BAEMM_ERR_INTERNAL;
+    if (!h) {
+        g_create_error = "host allocation failed";
+        return BAEMM_ERR_INTERNAL;
+    }
+    int st = BAEMM_OK;
     try {
+        // Arguments are checked before the first HIP call.
+        h->tris.resize(3 * nt);
+        for (int64_t k = 0; k < 3 * nt; ++k) {
+            const int64_t v = (int64_t)tris[k];
+            if (v < 0 || v >= nv) throw ArgError{"triangle index out of range"};
+            h->tris[k] = (int32_t)v;
+        }
         int ndev = 0;
         BAEMM_HIP(hipGetDeviceCount(&ndev));
         // Helmholtz_OpenCL.hpp:216-223: an out-of-range device index falls back to device 0.
@@ -73,22 +52,29 @@ int create_impl(const double* coords, int64_t nv, const I* tris, int64_t nt, con
         h->nt = nt;
         h->m = m;
         h->coords.assign(coords, coords + 3 * nv);
-        h->tris.resize(3 * nt);
-        for (int64_t k = 0; k < 3 * nt; ++k) {
-            const int64_t v = (int64_t)tris[k];
-            if (v < 0 || v >= nv) throw ArgError{"triangle index out of range"};
-            h->tris[k] = (int32_t)v;
-        }
         if (m > 0) h->meas.assign(meas, meas + 3 * m);
         build_host_geometry(*h);
         upload_geometry(*h);
         BAEMM_HIP(hipStreamCreate(&h->stream));
         BAEMM_HIP(hipEventCreate(&h->ev0));
         BAEMM_HIP(hipEventCreate(&h->ev1));
-    } catch (...) {
+    } catch (const ArgError& e) {
+        g_create_error = e.msg;
+        st = BAEMM_ERR_ARG;
+    } catch (const HipError& e) {
+        g_create_error = std::string("HIP error '") + hipGetErrorString(e.err) + "' in " + e.what;
+        st = BAEMM_ERR_HIP;
+    } catch (const std::bad_alloc&) {
+        g_create_error = "host allocation failed";
+        st = BAEMM_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        st = BAEMM_ERR_INTERNAL;
+    }
+    if (st != BAEMM_OK) {
         free_device(*h);
         delete h;
-        return BAEMM_ERR_HIP;
+        return st;
     }
     *out = reinterpret_cast<baemm_handle>(h);
     return BAEMM_OK;
@@ -161,7 +147,11 @@ int baemm_destroy(baemm_handle h)
     return BAEMM_OK;
 }
 
-const char* baemm_last_error(baemm_handle h) { return h ? H(h)->last_error.c_str() : "null handle"; }
+const char* baemm_last_error(baemm_handle h)
+{
+    if (h) return H(h)->last_error.c_str();
+    return g_create_error.empty() ? "null handle" : g_create_error.c_str();
+}
 
 int64_t baemm_vertex_count(baemm_handle h) { return h ? H(h)->nv : -1; }
 int64_t baemm_simplex_count(baemm_handle h) { return h ? H(h)->nt : -1; }

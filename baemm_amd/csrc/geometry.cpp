@@ -160,8 +160,10 @@ void free_device(Handle& h)
                     h.d_M_ptr, h.d_M_idx, h.d_M_val, h.d_lumped_inv, h.d_meas};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_scale,
-                      &h.ws_misc, &h.ws_pts, &h.ws_g, &h.ws_solver};
+    DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_scale, &h.ws_misc,
+                      &h.ws_pts, &h.ws_g, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
+                      &h.sws.t, &h.sws.x, &h.sws.r, &h.sws.hc, &h.sws.a, &h.sws.b, &h.sws.c,
+                      &h.sws.d, &h.sws.e};
     for (DevBuf* b : bufs) b->release();
 }
 

@@ -88,6 +88,22 @@ struct ChunkCoef {
     float kc2r, kc2i, kc3r, kc3i;   // kappa * c2, kappa * c3
 };
 
+// Device workspaces of the solvers (grow-only, reused across calls).
+struct Workspace {
+    DevBuf red, scal, cg;                 // reduction partials, per-column scalars, CG vectors
+    DevBuf V, w, t, x, r, hc;             // GMRES basis and vectors, Gram-Schmidt coefficients
+    DevBuf a, b, c, d, e;                 // composition temporaries
+};
+
+struct SolverStats {
+    int64_t gmres_iterations = 0;
+    int64_t gmres_cycles = 0;
+    double gmres_rel_residual = 0.0;
+    int64_t cg_iterations = 0;
+    int64_t operator_applies = 0;
+    bool converged = true;
+};
+
 struct Handle {
     std::mutex mtx;
     int device = 0;
@@ -128,7 +144,8 @@ struct Handle {
     Params prm;
     // workspaces
     DevBuf ws_B, ws_out, ws_b32, ws_bop, ws_part, ws_scale, ws_misc, ws_pts, ws_g;
-    DevBuf ws_solver;
+    Workspace sws;
+    SolverStats stats;
     // profiling of the dominant pair kernel
     bool timing = false;
     double timed_ms = 0.0;
@@ -163,13 +180,6 @@ void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s);
 void incident_dev(Handle& h, const ApplyArgs& a, const double* dirs_host, int64_t inc_count, int wave_type, hipStream_t s);
 float compute_rmin(Handle& h, hipStream_t s);
 
-// Sparse helpers on device data.
-void spmv_mass_real(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s);
-void av_real(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s);
-void avt_real(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double alpha, hipStream_t s);
 
-// ---- solvers (solvers.cpp) ----
-int mass_inverse_dev(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
-                     hipStream_t s);
 
 }  // namespace baemm

@@ -1,2 +1,462 @@
-// solvers.hip -- device Krylov kernels (filled in below).
-#include "internal.hpp"
+// solvers.hip -- device kernels of the Krylov solvers and the small
+// per-vertex / per-triangle operators used by the solver compositions.
+//
+// All solver vectors are FP64, row-major (vertex-major, column-minor) with a
+// leading dimension; complex vectors are viewed as 2x as many real columns
+// where the arithmetic is real (CG), as ApplyMassInverse.hpp:71-89 does.
+// Column reductions are two-pass (block partials, then a per-column sum) and
+// deterministic.
+#include <algorithm>
+#include <vector>
+
+#include "solvers.hpp"
+
+namespace baemm {
+
+
+// ---------------------------------------------------------------- sparse
+
+// Y = alpha M X + beta Y, real columns (MassOp.Dot).  Thread per (row, col).
+__global__ void mass_spmm_kernel(int nv, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                                 const double* __restrict__ val, const double* __restrict__ X, int64_t ldX,
+                                 double* __restrict__ Y, int64_t ldY, int ncols, double alpha, double beta)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nv * ncols) return;
+    const int v = (int)(t / ncols), c = (int)(t % ncols);
+    double s = 0.0;
+    for (int e = ptr[v]; e < ptr[v + 1]; ++e) s += val[e] * X[(int64_t)idx[e] * ldX + c];
+    double* y = Y + (int64_t)v * ldY + c;
+    *y = alpha * s + (beta != 0.0 ? beta * *y : 0.0);
+}
+
+// Y = diag(d) X (ApplyLumpedMassInverse, ApplyMassInverse.hpp:178-194).
+__global__ void diag_scale_kernel(int nv, const double* __restrict__ d, const double* __restrict__ X, int64_t ldX,
+                                  double* __restrict__ Y, int64_t ldY, int ncols)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nv * ncols) return;
+    const int v = (int)(t / ncols), c = (int)(t % ncols);
+    Y[(int64_t)v * ldY + c] = d[v] * X[(int64_t)v * ldX + c];
+}
+
+// P0 <- P1: Y[t][c] = (a_t/3) sum_k X[tri_k][c]  (AvOp.Dot on real columns).
+__global__ void av_real_kernel(int nt, const int4* __restrict__ tris, const float* __restrict__ area,
+                               const double* __restrict__ X, int64_t ldX, double* __restrict__ Y, int64_t ldY,
+                               int ncols)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nt * ncols) return;
+    const int tr = (int)(t / ncols), c = (int)(t % ncols);
+    const int4 v = tris[tr];
+    Y[(int64_t)tr * ldY + c] =
+        (double)area[tr] / 3.0 * (X[(int64_t)v.x * ldX + c] + X[(int64_t)v.y * ldX + c] + X[(int64_t)v.z * ldX + c]);
+}
+
+// P1 <- P0: Y[v][c] = alpha sum_{t ni v} (a_t/3) X[t][c] + beta Y  (AvOpTransp.Dot).
+__global__ void avt_real_kernel(int nv, const int32_t* __restrict__ vt_ptr, const int32_t* __restrict__ vt_idx,
+                                const float* __restrict__ area, const double* __restrict__ X, int64_t ldX,
+                                double* __restrict__ Y, int64_t ldY, int ncols, double alpha, double beta)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nv * ncols) return;
+    const int v = (int)(t / ncols), c = (int)(t % ncols);
+    double s = 0.0;
+    for (int e = vt_ptr[v]; e < vt_ptr[v + 1]; ++e) {
+        const int tr = vt_idx[e];
+        s += (double)area[tr] / 3.0 * X[(int64_t)tr * ldX + c];
+    }
+    double* y = Y + (int64_t)v * ldY + c;
+    *y = alpha * s + (beta != 0.0 ? beta * *y : 0.0);
+}
+
+// DotWithNormals_PL pointwise part (FarField.hpp:990-1005): y_t = (n_t . X_t) / a_t, X: nt x 3.
+__global__ void dot_normals_kernel(int nt, const float4* __restrict__ nrm, const float* __restrict__ area,
+                                   const double* __restrict__ X, double* __restrict__ y)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const float4 n = nrm[t];
+    y[t] = ((double)n.x * X[3 * t] + (double)n.y * X[3 * t + 1] + (double)n.z * X[3 * t + 2]) / (double)area[t];
+}
+
+// MultiplyWithNormals_PL pointwise part (FarField.hpp:924-935): Y_t = Re(x_t) / a_t n_t.
+__global__ void mul_normals_kernel(int nt, const float4* __restrict__ nrm, const float* __restrict__ area,
+                                   const double2* __restrict__ x, double* __restrict__ Y)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const float4 n = nrm[t];
+    const double mul = x[t].x / (double)area[t];
+    Y[3 * t] = mul * n.x;
+    Y[3 * t + 1] = mul * n.y;
+    Y[3 * t + 2] = mul * n.z;
+}
+
+// bdr[v][k] = -s[v] du[v][k]  (Derivative_FF, FarField.hpp:288-300).
+__global__ void scale_rows_neg_kernel(int nv, int W, const double* __restrict__ s, const double2* __restrict__ du,
+                                      double2* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nv * W) return;
+    const int v = (int)(t / W);
+    const double2 d = du[t];
+    out[t] = make_double2(-s[v] * d.x, -s[v] * d.y);
+}
+
+// p[v] = sum_k a[v][k] b[v][k] (unconjugated; HadamardProduct with ld_sum, HadamardProduct.hpp:18-25).
+__global__ void row_product_sum_kernel(int nv, int W, const double2* __restrict__ a, const double2* __restrict__ b,
+                                       double2* __restrict__ p)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double sr = 0.0, si = 0.0;
+    for (int k = 0; k < W; ++k) {
+        const double2 x = a[(int64_t)v * W + k], y = b[(int64_t)v * W + k];
+        sr += x.x * y.x - x.y * y.y;
+        si += x.x * y.y + x.y * y.x;
+    }
+    p[v] = make_double2(sr, si);
+}
+
+// ---------------------------------------------------------------- reductions
+
+// part[b][c] = sum over this block's rows of X[r][c] * Y[r][c]  (real columns).
+__global__ __launch_bounds__(256) void coldot_part_kernel(int n, int ncols, const double* __restrict__ X,
+                                                          int64_t ldX, const double* __restrict__ Y, int64_t ldY,
+                                                          double* __restrict__ part)
+{
+    __shared__ double sm[4][64];
+    const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < ncols; c0 += 64) {
+        double s = 0.0;
+        if (c0 + c < ncols)
+            for (int r = blockIdx.x * 4 + rl; r < n; r += gridDim.x * 4)
+                s += X[(int64_t)r * ldX + c0 + c] * Y[(int64_t)r * ldY + c0 + c];
+        sm[rl][c] = s;
+        __syncthreads();
+        if (rl == 0 && c0 + c < ncols) part[(int64_t)blockIdx.x * ncols + c0 + c] = sm[0][c] + sm[1][c] + sm[2][c] + sm[3][c];
+        __syncthreads();
+    }
+}
+
+// out[c] = sum_b part[b][c]
+__global__ void colsum_kernel(int nblocks, int ncols, const double* __restrict__ part, double* __restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * ncols + c];
+    out[c] = s;
+}
+
+// Complex Gram-Schmidt dots: part[b][i][k] = sum_rows conj(V_i[r][k]) w[r][k], i < nb.
+// V: basis vectors with stride vstride (elements) between vectors, ld W.
+__global__ __launch_bounds__(256) void gs_dots_part_kernel(int n, int W, int nb, const double2* __restrict__ V,
+                                                           int64_t vstride, const double2* __restrict__ w,
+                                                           double2* __restrict__ part)
+{
+    __shared__ double2 sm[4][64];
+    const int k = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    for (int i = 0; i < nb; ++i) {
+        for (int k0 = 0; k0 < W; k0 += 64) {
+            double sr = 0.0, si = 0.0;
+            if (k0 + k < W)
+                for (int r = blockIdx.x * 4 + rl; r < n; r += gridDim.x * 4) {
+                    const double2 a = V[(int64_t)i * vstride + (int64_t)r * W + k0 + k];
+                    const double2 b = w[(int64_t)r * W + k0 + k];
+                    sr += a.x * b.x + a.y * b.y;
+                    si += a.x * b.y - a.y * b.x;
+                }
+            sm[rl][k] = make_double2(sr, si);
+            __syncthreads();
+            if (rl == 0 && k0 + k < W) {
+                double2 s = sm[0][k];
+                for (int q = 1; q < 4; ++q) {
+                    s.x += sm[q][k].x;
+                    s.y += sm[q][k].y;
+                }
+                part[((int64_t)blockIdx.x * nb + i) * W + k0 + k] = s;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void gs_dots_sum_kernel(int nblocks, int nbW, const double2* __restrict__ part, double2* __restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nbW) return;
+    double sr = 0.0, si = 0.0;
+    for (int b = 0; b < nblocks; ++b) {
+        const double2 p = part[(int64_t)b * nbW + e];
+        sr += p.x;
+        si += p.y;
+    }
+    out[e] = make_double2(sr, si);
+}
+
+// w[r][k] -= sum_{i<nb} h[i][k] V_i[r][k]   (or x += ... with sign = +1)
+__global__ void gs_update_kernel(int n, int W, int nb, const double2* __restrict__ V, int64_t vstride,
+                                 const double2* __restrict__ h, double2* __restrict__ w, double sign)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * W) return;
+    const int k = (int)(t % W);
+    double sr = 0.0, si = 0.0;
+    for (int i = 0; i < nb; ++i) {
+        const double2 a = V[(int64_t)i * vstride + t];
+        const double2 c = h[i * W + k];
+        sr += c.x * a.x - c.y * a.y;
+        si += c.x * a.y + c.y * a.x;
+    }
+    double2 o = w[t];
+    o.x += sign * sr;
+    o.y += sign * si;
+    w[t] = o;
+}
+
+// dst[r][k] = src[r][k] * s[k]  (complex vector, real per-column scale)
+__global__ void scale_cols_kernel(int n, int W, const double2* __restrict__ src, const double* __restrict__ s,
+                                  double2* __restrict__ dst)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * W) return;
+    const double f = s[t % W];
+    const double2 v = src[t];
+    dst[t] = make_double2(f * v.x, f * v.y);
+}
+
+// ---------------------------------------------------------------- CG steps (real columns)
+
+// alpha = rz / pq per column (0 when a column is frozen)
+__global__ void cg_alpha_kernel(int ncols, const double* __restrict__ rz, const double* __restrict__ pq,
+                                const int* __restrict__ active, double* __restrict__ alpha)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    alpha[c] = (active[c] && pq[c] != 0.0) ? rz[c] / pq[c] : 0.0;
+}
+
+// x += alpha p; r -= alpha q
+__global__ void cg_update_xr_kernel(int64_t n, int ncols, const double* __restrict__ alpha,
+                                    const double* __restrict__ p, const double* __restrict__ q,
+                                    double* __restrict__ x, int64_t ldx, double* __restrict__ r)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * ncols) return;
+    const int64_t v = t / ncols;
+    const int c = (int)(t % ncols);
+    const double a = alpha[c];
+    x[v * ldx + c] += a * p[t];
+    r[t] -= a * q[t];
+}
+
+// beta = rz_new / rz; rz = rz_new; p = z + beta p
+__global__ void cg_beta_kernel(int ncols, double* __restrict__ rz, const double* __restrict__ rz_new,
+                               double* __restrict__ beta)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    beta[c] = (rz[c] != 0.0) ? rz_new[c] / rz[c] : 0.0;
+    rz[c] = rz_new[c];
+}
+
+__global__ void cg_update_p_kernel(int64_t n, int ncols, const double* __restrict__ beta, const double* __restrict__ z,
+                                   double* __restrict__ p)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * ncols) return;
+    p[t] = z[t] + beta[t % ncols] * p[t];
+}
+
+// ---------------------------------------------------------------- host wrappers
+
+static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+void dev_mass(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
+              hipStream_t s)
+{
+    hipLaunchKernelGGL(mass_spmm_kernel, grid_for(h.nv * ncols), dim3(256), 0, s, (int)h.nv, h.d_M_ptr, h.d_M_idx,
+                       h.d_M_val, X, ldX, Y, ldY, ncols, alpha, beta);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_lumped(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s)
+{
+    hipLaunchKernelGGL(diag_scale_kernel, grid_for(h.nv * ncols), dim3(256), 0, s, (int)h.nv, h.d_lumped_inv, X, ldX,
+                       Y, ldY, ncols);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_av(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s)
+{
+    hipLaunchKernelGGL(av_real_kernel, grid_for(h.nt * ncols), dim3(256), 0, s, (int)h.nt, h.d_tris, h.d_area, X, ldX,
+                       Y, ldY, ncols);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_avt(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
+             hipStream_t s)
+{
+    hipLaunchKernelGGL(avt_real_kernel, grid_for(h.nv * ncols), dim3(256), 0, s, (int)h.nv, h.d_vt_ptr, h.d_vt_idx,
+                       h.d_area, X, ldX, Y, ldY, ncols, alpha, beta);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_dot_normals(Handle& h, const double* X, double* y, hipStream_t s)
+{
+    hipLaunchKernelGGL(dot_normals_kernel, grid_for(h.nt), dim3(256), 0, s, (int)h.nt, h.d_nrm, h.d_area, X, y);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_mul_normals(Handle& h, const double2* x, double* Y, hipStream_t s)
+{
+    hipLaunchKernelGGL(mul_normals_kernel, grid_for(h.nt), dim3(256), 0, s, (int)h.nt, h.d_nrm, h.d_area, x, Y);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_scale_rows_neg(Handle& h, int W, const double* sv, const double2* du, double2* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(scale_rows_neg_kernel, grid_for(h.nv * W), dim3(256), 0, s, (int)h.nv, W, sv, du, out);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_row_product_sum(Handle& h, int W, const double2* a, const double2* b, double2* p, hipStream_t s)
+{
+    hipLaunchKernelGGL(row_product_sum_kernel, grid_for(h.nv), dim3(256), 0, s, (int)h.nv, W, a, b, p);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_coldot(Workspace& ws, int64_t n, int ncols, const double* X, int64_t ldX, const double* Y, int64_t ldY,
+                double* out, hipStream_t s)
+{
+    double* part = ws.red.as<double>();
+    hipLaunchKernelGGL(coldot_part_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, ncols, X, ldX, Y, ldY, part);
+    hipLaunchKernelGGL(colsum_kernel, grid_for(ncols), dim3(256), 0, s, kRedBlocks, ncols, part, out);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_gs_dots(Workspace& ws, int64_t n, int W, int nb, const double2* V, int64_t vstride, const double2* w,
+                 double2* out, hipStream_t s)
+{
+    double2* part = ws.red.as<double2>();
+    hipLaunchKernelGGL(gs_dots_part_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, W, nb, V, vstride, w, part);
+    hipLaunchKernelGGL(gs_dots_sum_kernel, grid_for((int64_t)nb * W), dim3(256), 0, s, kRedBlocks, nb * W, part, out);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_gs_update(int64_t n, int W, int nb, const double2* V, int64_t vstride, const double2* hcoef, double2* w,
+                   double sign, hipStream_t s)
+{
+    hipLaunchKernelGGL(gs_update_kernel, grid_for(n * W), dim3(256), 0, s, (int)n, W, nb, V, vstride, hcoef, w, sign);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, double2* dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(scale_cols_kernel, grid_for(n * W), dim3(256), 0, s, (int)n, W, src, sc, dst);
+    BAEMM_HIP(hipGetLastError());
+}
+
+// PCG for M Y = X on real columns (ApplyMassInverse.hpp:6-156): zero initial
+// guess, lumped-diagonal preconditioner, at most max_iter iterations, per
+// column relative residual ||r|| <= tol ||X||.  X: n x ncols (ldX), Y: (ldY).
+// Returns the number of iterations; *converged tells whether all columns met tol.
+int dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
+                int max_iter, bool* converged, hipStream_t s)
+{
+    const int64_t n = h.nv;
+    const int64_t N = n * ncols;
+    ws.cg.reserve(sizeof(double) * (4 * N));
+    ws.red.reserve(sizeof(double) * kRedBlocks * ncols);
+    ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
+    double* r = ws.cg.as<double>();
+    double* z = r + N;
+    double* p = z + N;
+    double* q = p + N;
+    double* rz = ws.scal.as<double>();
+    double* rz_new = rz + ncols;
+    double* pq = rz_new + ncols;
+    double* alpha = pq + ncols;
+    double* beta = alpha + ncols;
+    double* rr = beta + ncols;
+    double* bb = rr + ncols;
+    int* active = reinterpret_cast<int*>(bb + 2 * ncols);
+    // r = X (contiguous copy), Y = 0
+    BAEMM_HIP(hipMemcpy2DAsync(r, sizeof(double) * ncols, X, sizeof(double) * ldX, sizeof(double) * ncols, n,
+                               hipMemcpyDeviceToDevice, s));
+    BAEMM_HIP(hipMemset2DAsync(Y, sizeof(double) * ldY, 0, sizeof(double) * ncols, n, s));
+    dev_coldot(ws, n, ncols, r, ncols, r, ncols, bb, s);
+    std::vector<double> h_bb(ncols), h_rr(ncols);
+    BAEMM_HIP(hipMemcpyAsync(h_bb.data(), bb, sizeof(double) * ncols, hipMemcpyDeviceToHost, s));
+    BAEMM_HIP(hipStreamSynchronize(s));
+    std::vector<int> h_act(ncols);
+    int nact = 0;
+    for (int c = 0; c < ncols; ++c) {
+        h_act[c] = h_bb[c] > 0.0;
+        nact += h_act[c];
+    }
+    int it = 0;
+    if (nact == 0) {
+        *converged = true;
+        return 0;
+    }
+    BAEMM_HIP(hipMemcpyAsync(active, h_act.data(), sizeof(int) * ncols, hipMemcpyHostToDevice, s));
+    dev_lumped(h, r, ncols, z, ncols, ncols, s);
+    BAEMM_HIP(hipMemcpyAsync(p, z, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    dev_coldot(ws, n, ncols, r, ncols, z, ncols, rz, s);
+    bool done = false;
+    while (it < max_iter) {
+        ++it;
+        dev_mass(h, 1.0, p, ncols, 0.0, q, ncols, ncols, s);
+        dev_coldot(ws, n, ncols, p, ncols, q, ncols, pq, s);
+        hipLaunchKernelGGL(cg_alpha_kernel, grid_for(ncols), dim3(256), 0, s, ncols, rz, pq, active, alpha);
+        hipLaunchKernelGGL(cg_update_xr_kernel, grid_for(N), dim3(256), 0, s, n, ncols, alpha, p, q, Y, ldY, r);
+        BAEMM_HIP(hipGetLastError());
+        dev_coldot(ws, n, ncols, r, ncols, r, ncols, rr, s);
+        BAEMM_HIP(hipMemcpyAsync(h_rr.data(), rr, sizeof(double) * ncols, hipMemcpyDeviceToHost, s));
+        BAEMM_HIP(hipStreamSynchronize(s));
+        done = true;
+        bool changed = false;
+        for (int c = 0; c < ncols; ++c) {
+            if (!h_act[c]) continue;
+            if (h_rr[c] <= tol * tol * h_bb[c]) {
+                h_act[c] = 0;
+                changed = true;
+            } else {
+                done = false;
+            }
+        }
+        if (done) break;
+        if (changed) BAEMM_HIP(hipMemcpyAsync(active, h_act.data(), sizeof(int) * ncols, hipMemcpyHostToDevice, s));
+        dev_lumped(h, r, ncols, z, ncols, ncols, s);
+        dev_coldot(ws, n, ncols, r, ncols, z, ncols, rz_new, s);
+        hipLaunchKernelGGL(cg_beta_kernel, grid_for(ncols), dim3(256), 0, s, ncols, rz, rz_new, beta);
+        hipLaunchKernelGGL(cg_update_p_kernel, grid_for(N), dim3(256), 0, s, n, ncols, beta, z, p);
+        BAEMM_HIP(hipGetLastError());
+    }
+    *converged = done;
+    return it;
+}
+
+}  // namespace baemm
+
+namespace baemm {
+
+// r = b - t  (complex vectors of N elements)
+__global__ void sub_c_kernel(int64_t N, const double2* __restrict__ b, const double2* __restrict__ t,
+                             double2* __restrict__ r)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    r[i] = make_double2(b[i].x - t[i].x, b[i].y - t[i].y);
+}
+
+void dev_sub_c(int64_t N, const double2* b, const double2* t, double2* r, hipStream_t s)
+{
+    hipLaunchKernelGGL(sub_c_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, b, t, r);
+    BAEMM_HIP(hipGetLastError());
+}
+
+}  // namespace baemm

@@ -1,0 +1,35 @@
+// solvers.hpp -- device primitives of the Krylov solvers (solvers.hip) and the
+// host drivers / compositions built on them (solvers_host.cpp).
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace baemm {
+
+constexpr int kRedBlocks = 240;   // blocks of the first pass of column reductions
+
+void dev_mass(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
+              hipStream_t s);
+void dev_lumped(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s);
+void dev_av(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s);
+void dev_avt(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
+             hipStream_t s);
+void dev_dot_normals(Handle& h, const double* X, double* y, hipStream_t s);
+void dev_mul_normals(Handle& h, const double2* x, double* Y, hipStream_t s);
+void dev_scale_rows_neg(Handle& h, int W, const double* sv, const double2* du, double2* out, hipStream_t s);
+void dev_row_product_sum(Handle& h, int W, const double2* a, const double2* b, double2* p, hipStream_t s);
+void dev_coldot(Workspace& ws, int64_t n, int ncols, const double* X, int64_t ldX, const double* Y, int64_t ldY,
+                double* out, hipStream_t s);
+void dev_gs_dots(Workspace& ws, int64_t n, int W, int nb, const double2* V, int64_t vstride, const double2* w,
+                 double2* out, hipStream_t s);
+void dev_gs_update(int64_t n, int W, int nb, const double2* V, int64_t vstride, const double2* hcoef, double2* w,
+                   double sign, hipStream_t s);
+void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, double2* dst, hipStream_t s);
+void dev_sub_c(int64_t N, const double2* b, const double2* t, double2* r, hipStream_t s);
+int dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
+                int max_iter, bool* converged, hipStream_t s);
+
+}  // namespace baemm

@@ -1,2 +1,688 @@
-// solvers_host.cpp -- host drivers of the Krylov solvers.
-#include "internal.hpp"
+// solvers_host.cpp -- host drivers of the device Krylov solvers and the solver
+// compositions of FarField.hpp (FarField, BoundaryPotential, DirichletToNeumann,
+// Derivative_FF, AdjointDerivative_FF, GaussNewtonSolve), with their C ABI.
+//
+// Vectors live on the device for the whole composition; only the caller's
+// inputs and outputs cross PCIe.  GMRES: left-preconditioned, restarted
+// (restart length gmres_max_iter, at most gmres_max_restarts cycles), one
+// Krylov space per column advanced together, classical Gram-Schmidt with
+// re-orthogonalisation on the device, Hessenberg / Givens / least squares in
+// FP64 on the host (W x 31 x 30 numbers).  The preconditioner is the mass
+// inverse by PCG, as in the reference (lumped_mass_as_prec_for_intopsQ false,
+// MemberVariables.hpp:95).
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "capi_util.hpp"
+#include "solvers.hpp"
+
+namespace baemm {
+
+namespace {
+
+struct GmresResult {
+    int iterations = 0;
+    int cycles = 0;
+    double rel_residual = 0.0;
+    bool converged = false;
+};
+
+void apply_op(Handle& h, const double2* in, double2* out, hipStream_t s)
+{
+    ApplyArgs a{cdouble(1.0, 0.0), cdouble(0.0, 0.0), in, h.prm.W, kC128, out, h.prm.W, kC128};
+    apply_boundary_dev(h, a, s);
+    h.stats.operator_applies++;
+}
+
+void apply_prec(Handle& h, const double2* in, double2* out, double cg_tol, hipStream_t s)
+{
+    const int W = (int)h.prm.W;
+    bool conv = false;
+    const int it = dev_mass_cg(h, h.sws, reinterpret_cast<const double*>(in), 2 * W, reinterpret_cast<double*>(out),
+                               2 * W, 2 * W, cg_tol, 20, &conv, s);
+    h.stats.cg_iterations += it;
+}
+
+// Solves (P A) X = P B for the loaded boundary operator A on W columns.
+GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, double cg_tol, hipStream_t s)
+{
+    Workspace& ws = h.sws;
+    const int W = (int)h.prm.W;
+    const int64_t n = h.nv, NW = n * W;
+    const int m = h.gmres_max_iter;
+    const int cycles = h.gmres_max_restarts;
+    ws.V.reserve(sizeof(double2) * NW * (m + 1));
+    ws.w.reserve(sizeof(double2) * NW);
+    ws.t.reserve(sizeof(double2) * NW);
+    ws.hc.reserve(sizeof(double2) * (2 * (m + 2) * W) + sizeof(double) * W);
+    ws.red.reserve(sizeof(double2) * kRedBlocks * (m + 2) * W);
+    double2* V = ws.V.as<double2>();
+    double2* w = ws.w.as<double2>();
+    double2* t = ws.t.as<double2>();
+    double2* hc1 = ws.hc.as<double2>();
+    double2* hc2 = hc1 + (m + 2) * W;
+    double* sc = reinterpret_cast<double*>(hc2 + (m + 2) * W);
+
+    BAEMM_HIP(hipMemsetAsync(X, 0, sizeof(double2) * NW, s));
+    std::vector<double> bnorm(W, 0.0), beta(W, 0.0), res(W, 0.0), scale(W);
+    std::vector<cdouble> H((size_t)W * (m + 1) * m), g((size_t)W * (m + 1)), cs_((size_t)W * m), sn_((size_t)W * m);
+    std::vector<cdouble> hh1((size_t)(m + 2) * W), hh2((size_t)(m + 2) * W), y((size_t)(m + 1) * W);
+    auto Hk = [&](int k, int i, int j) -> cdouble& { return H[((size_t)k * (m + 1) + i) * m + j]; };
+
+    GmresResult out;
+    for (int cyc = 0; cyc < cycles; ++cyc) {
+        out.cycles = cyc + 1;
+        if (cyc == 0) {
+            apply_prec(h, B, V, cg_tol, s);
+        } else {
+            apply_op(h, X, t, s);
+            dev_sub_c(NW, B, t, w, s);
+            apply_prec(h, w, V, cg_tol, s);
+        }
+        dev_gs_dots(ws, n, W, 1, V, NW, V, hc1, s);
+        BAEMM_HIP(hipMemcpyAsync(hh1.data(), hc1, sizeof(double2) * W, hipMemcpyDeviceToHost, s));
+        BAEMM_HIP(hipStreamSynchronize(s));
+        double worst = 0.0;
+        for (int k = 0; k < W; ++k) {
+            beta[k] = std::sqrt(std::max(0.0, hh1[k].real()));
+            if (cyc == 0) bnorm[k] = beta[k];
+            res[k] = bnorm[k] > 0.0 ? beta[k] / bnorm[k] : 0.0;
+            worst = std::max(worst, res[k]);
+        }
+        out.rel_residual = worst;
+        if (worst <= tol) {
+            out.converged = true;
+            break;
+        }
+        for (int k = 0; k < W; ++k) scale[k] = beta[k] > 0.0 ? 1.0 / beta[k] : 0.0;
+        BAEMM_HIP(hipMemcpyAsync(sc, scale.data(), sizeof(double) * W, hipMemcpyHostToDevice, s));
+        dev_scale_cols(n, W, V, sc, V, s);
+        std::fill(g.begin(), g.end(), cdouble(0.0, 0.0));
+        for (int k = 0; k < W; ++k) g[(size_t)k * (m + 1)] = beta[k];
+        int jdone = 0;
+        for (int j = 0; j < m; ++j) {
+            double2* Vj = V + (int64_t)j * NW;
+            double2* Vn = V + (int64_t)(j + 1) * NW;
+            apply_op(h, Vj, t, s);
+            apply_prec(h, t, w, cg_tol, s);
+            // classical Gram-Schmidt, twice
+            dev_gs_dots(ws, n, W, j + 1, V, NW, w, hc1, s);
+            dev_gs_update(n, W, j + 1, V, NW, hc1, w, -1.0, s);
+            dev_gs_dots(ws, n, W, j + 1, V, NW, w, hc2, s);
+            dev_gs_update(n, W, j + 1, V, NW, hc2, w, -1.0, s);
+            dev_gs_dots(ws, n, W, 1, w, NW, w, hc2 + (j + 1) * W, s);
+            BAEMM_HIP(hipMemcpyAsync(hh1.data(), hc1, sizeof(double2) * (j + 1) * W, hipMemcpyDeviceToHost, s));
+            BAEMM_HIP(hipMemcpyAsync(hh2.data(), hc2, sizeof(double2) * (j + 2) * W, hipMemcpyDeviceToHost, s));
+            BAEMM_HIP(hipStreamSynchronize(s));
+            worst = 0.0;
+            for (int k = 0; k < W; ++k) {
+                for (int i = 0; i <= j; ++i) Hk(k, i, j) = hh1[(size_t)i * W + k] + hh2[(size_t)i * W + k];
+                const double hn = std::sqrt(std::max(0.0, hh2[(size_t)(j + 1) * W + k].real()));
+                scale[k] = hn > 0.0 ? 1.0 / hn : 0.0;
+                // previous rotations
+                for (int i = 0; i < j; ++i) {
+                    const cdouble c = cs_[(size_t)k * m + i], sn = sn_[(size_t)k * m + i];
+                    const cdouble a = Hk(k, i, j), b = Hk(k, i + 1, j);
+                    Hk(k, i, j) = c * a + sn * b;
+                    Hk(k, i + 1, j) = -std::conj(sn) * a + c * b;
+                }
+                const cdouble a = Hk(k, j, j);
+                const double b = hn;
+                const double na = std::abs(a), nu = std::sqrt(na * na + b * b);
+                cdouble c, sn;
+                if (nu == 0.0) {
+                    c = 1.0;
+                    sn = 0.0;
+                } else if (na == 0.0) {
+                    c = 0.0;
+                    sn = 1.0;
+                } else {
+                    c = na / nu;
+                    sn = (a / na) * b / nu;
+                }
+                cs_[(size_t)k * m + j] = c;
+                sn_[(size_t)k * m + j] = sn;
+                Hk(k, j, j) = c * a + sn * b;
+                Hk(k, j + 1, j) = 0.0;
+                const cdouble gj = g[(size_t)k * (m + 1) + j];
+                g[(size_t)k * (m + 1) + j] = c * gj;
+                g[(size_t)k * (m + 1) + j + 1] = -std::conj(sn) * gj;
+                res[k] = bnorm[k] > 0.0 ? std::abs(g[(size_t)k * (m + 1) + j + 1]) / bnorm[k] : 0.0;
+                worst = std::max(worst, res[k]);
+            }
+            BAEMM_HIP(hipMemcpyAsync(sc, scale.data(), sizeof(double) * W, hipMemcpyHostToDevice, s));
+            dev_scale_cols(n, W, w, sc, Vn, s);
+            out.iterations++;
+            h.stats.gmres_iterations++;
+            jdone = j + 1;
+            out.rel_residual = worst;
+            if (worst <= tol) break;
+        }
+        // y = R^{-1} g per column (R upper triangular jdone x jdone); X += V y
+        for (int k = 0; k < W; ++k) {
+            for (int i = jdone - 1; i >= 0; --i) {
+                cdouble acc = g[(size_t)k * (m + 1) + i];
+                for (int l = i + 1; l < jdone; ++l) acc -= Hk(k, i, l) * y[(size_t)l * W + k];
+                const cdouble d = Hk(k, i, i);
+                y[(size_t)i * W + k] = (std::abs(d) > 0.0) ? acc / d : cdouble(0.0, 0.0);
+            }
+        }
+        BAEMM_HIP(hipMemcpyAsync(hc1, y.data(), sizeof(double2) * jdone * W, hipMemcpyHostToDevice, s));
+        dev_gs_update(n, W, jdone, V, NW, hc1, X, 1.0, s);
+        BAEMM_HIP(hipStreamSynchronize(s));
+        if (out.rel_residual <= tol) {
+            out.converged = true;
+            break;
+        }
+    }
+    h.stats.gmres_cycles += out.cycles;
+    h.stats.gmres_rel_residual = out.rel_residual;
+    h.stats.converged = h.stats.converged && out.converged;
+    return out;
+}
+
+std::vector<double> rows_coeff(int64_t K, const std::vector<cdouble>& row_of)
+{
+    std::vector<double> c(8 * K);
+    for (int64_t q = 0; q < K; ++q)
+        for (int j = 0; j < 4; ++j) {
+            c[8 * q + 2 * j] = row_of[4 * q + j].real();
+            c[8 * q + 2 * j + 1] = row_of[4 * q + j].imag();
+        }
+    return c;
+}
+
+// BoundaryPotential_parameters (FarField.hpp:678-771): (1/2 M - i eta S + K) phi = wave.
+GmresResult boundary_potential_dev(Handle& h, const double* kappa, const double* eta, int64_t K, int64_t wcs,
+                                   const double2* wave, double2* phi, double cg_tol, double gmres_tol, hipStream_t s,
+                                   double* coeff_out = nullptr)
+{
+    std::vector<cdouble> rows(4 * K);
+    for (int64_t q = 0; q < K; ++q) {
+        rows[4 * q] = {0.5, 0.0};
+        rows[4 * q + 1] = {0.0, -eta[q]};
+        rows[4 * q + 2] = {1.0, 0.0};
+        rows[4 * q + 3] = {0.0, 0.0};
+    }
+    const std::vector<double> c = rows_coeff(K, rows);
+    if (coeff_out) std::memcpy(coeff_out, c.data(), sizeof(double) * c.size());
+    load_params(h, kappa, c.data(), K * wcs, wcs, 1.0);
+    return gmres_boundary(h, wave, phi, gmres_tol, cg_tol, s);
+}
+
+// DirichletToNeumann (FarField.hpp:790-879): (1/2 M - i kappa S + K') psi = wave.
+GmresResult dtn_dev(Handle& h, const double* kappa, int64_t K, int64_t wcs, const double2* wave, double2* psi,
+                    double cg_tol, double gmres_tol, hipStream_t s)
+{
+    std::vector<cdouble> rows(4 * K);
+    for (int64_t q = 0; q < K; ++q) {
+        rows[4 * q] = {0.5, 0.0};
+        rows[4 * q + 1] = {0.0, -kappa[q]};
+        rows[4 * q + 2] = {0.0, 0.0};
+        rows[4 * q + 3] = {1.0, 0.0};
+    }
+    const std::vector<double> c = rows_coeff(K, rows);
+    load_params(h, kappa, c.data(), K * wcs, wcs, 1.0);
+    return gmres_boundary(h, wave, psi, gmres_tol, cg_tol, s);
+}
+
+// CreateIncidentWave_PL with coefficient rows (0, c1_q, c2_q, 0), factor 4 pi (alpha 1, beta 0).
+void incident_rows_dev(Handle& h, const double* kappa, int64_t K, int64_t wcs, const double* dirs, int wave_type,
+                       const std::vector<cdouble>& c12, double2* out, hipStream_t s)
+{
+    std::vector<cdouble> rows(4 * K);
+    for (int64_t q = 0; q < K; ++q) {
+        rows[4 * q] = 0.0;
+        rows[4 * q + 1] = c12[2 * q];
+        rows[4 * q + 2] = c12[2 * q + 1];
+        rows[4 * q + 3] = 0.0;
+    }
+    const std::vector<double> c = rows_coeff(K, rows);
+    load_params(h, kappa, c.data(), K * wcs, wcs, 4.0 * M_PI);
+    ApplyArgs a{cdouble(1.0, 0.0), cdouble(0.0, 0.0), nullptr, 0, kC128, out, K * wcs, kC128};
+    incident_dev(h, a, dirs, wcs, wave_type, s);
+}
+
+// ApplyFarFieldOperators_PL with coefficient rows `c` (FarField.hpp:139-143): F (device, m x W).
+void far_field_dev(Handle& h, const double* kappa, int64_t K, int64_t wcs, const std::vector<double>& c,
+                   const double2* phi, double2* F, hipStream_t s)
+{
+    load_params(h, kappa, c.data(), K * wcs, wcs, 1.0);
+    ApplyArgs a{cdouble(1.0, 0.0), cdouble(0.0, 0.0), phi, K * wcs, kC128, F, K * wcs, kC128};
+    apply_far_dev(h, a, s);
+}
+
+// Derivative_FF_Helper (FarField.hpp:148-207): du_dn = DtN(du_i/dn - i kappa u_i).
+void du_dn_dev(Handle& h, const double* kappa, int64_t K, int64_t wcs, const double* dirs, int wave_type,
+               double2* du, double cg_tol, double gmres_tol, hipStream_t s)
+{
+    const int64_t W = K * wcs;
+    h.sws.a.reserve(sizeof(double2) * h.nv * W);
+    double2* inc = h.sws.a.as<double2>();
+    std::vector<cdouble> c12(2 * K);
+    for (int64_t q = 0; q < K; ++q) {
+        c12[2 * q] = {0.0, -kappa[q]};
+        c12[2 * q + 1] = {1.0, 0.0};
+    }
+    incident_rows_dev(h, kappa, K, wcs, dirs, wave_type, c12, inc, s);
+    dtn_dev(h, kappa, K, wcs, inc, du, cg_tol, gmres_tol, s);
+}
+
+// PCG mass solve for n_cols real columns of a device array (ApplyMassInverse).
+void mass_inverse_dev(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
+                      hipStream_t s)
+{
+    bool conv = false;
+    h.stats.cg_iterations += dev_mass_cg(h, h.sws, X, ldX, Y, ldY, ncols, tol, 20, &conv, s);
+}
+
+// DotWithNormals_PL (FarField.hpp:956-1022): Xn (device, V) = M^{-1} AvT ((n . Av X)/a).
+void dot_with_normals_dev(Handle& h, const double* X, double* Xn, double cg_tol, hipStream_t s)
+{
+    h.sws.b.reserve(sizeof(double) * (4 * h.nt + h.nv));
+    double* Xt = h.sws.b.as<double>();
+    double* yt = Xt + 3 * h.nt;
+    double* yw = yt + h.nt;
+    dev_av(h, X, 3, Xt, 3, 3, s);
+    dev_dot_normals(h, Xt, yt, s);
+    dev_avt(h, 1.0, yt, 1, 0.0, yw, 1, 1, s);
+    mass_inverse_dev(h, yw, 1, Xn, 1, 1, cg_tol, s);
+}
+
+// MultiplyWithNormals_PL (FarField.hpp:886-952): Y (device, V x 3) = M^{-1} (factor AvT (Re(Av x)/a n)).
+void multiply_with_normals_dev(Handle& h, const double2* x, double* Y, double factor, double cg_tol, hipStream_t s)
+{
+    h.sws.b.reserve(sizeof(double) * (2 * h.nt + 3 * h.nt + 3 * h.nv));
+    double* xt = h.sws.b.as<double>();               // nt complex (as 2 real columns)
+    double* Yt = xt + 2 * h.nt;
+    double* Yw = Yt + 3 * h.nt;
+    dev_av(h, reinterpret_cast<const double*>(x), 2, xt, 2, 2, s);
+    dev_mul_normals(h, reinterpret_cast<const double2*>(xt), Yt, s);
+    dev_avt(h, factor, Yt, 3, 0.0, Yw, 3, 3, s);
+    mass_inverse_dev(h, Yw, 3, Y, 3, 3, cg_tol, s);
+}
+
+// Derivative_FF core (FarField.hpp:285-318) with du_dn on the device: F = DF X.
+void derivative_dev(Handle& h, const double* kappa, int64_t K, int64_t wcs, const double* Xdev, const double2* du,
+                    double2* F, double cg_tol, double gmres_tol, hipStream_t s)
+{
+    const int64_t W = K * wcs, NW = h.nv * W;
+    h.sws.c.reserve(sizeof(double2) * 2 * NW + sizeof(double) * h.nv);
+    double2* bdr = h.sws.c.as<double2>();
+    double2* phi = bdr + NW;
+    double* xn = reinterpret_cast<double*>(phi + NW);
+    dot_with_normals_dev(h, Xdev, xn, cg_tol, s);
+    dev_scale_rows_neg(h, (int)W, xn, du, bdr, s);
+    h.sws.a.reserve(sizeof(double2) * NW);
+    double2* weak = h.sws.a.as<double2>();
+    dev_mass(h, 1.0, reinterpret_cast<const double*>(bdr), 2 * W, 0.0, reinterpret_cast<double*>(weak), 2 * W,
+             (int)(2 * W), s);
+    std::vector<double> coeff(8 * K);
+    boundary_potential_dev(h, kappa, kappa, K, wcs, weak, phi, cg_tol, gmres_tol, s, coeff.data());
+    far_field_dev(h, kappa, K, wcs, coeff, phi, F, s);
+}
+
+// AdjointDerivative_FF core (FarField.hpp:394-425) with du_dn on the device: Y (V x 3) = DF* g.
+void adjoint_dev(Handle& h, const double* kappa, int64_t K, int64_t wcs, const double2* g, const double2* du,
+                 double* Y, double cg_tol, double gmres_tol, hipStream_t s)
+{
+    const int64_t W = K * wcs, NW = h.nv * W;
+    h.sws.d.reserve(sizeof(double2) * (2 * NW + h.nv));
+    double2* herg = h.sws.d.as<double2>();
+    double2* dv = herg + NW;
+    double2* prod = dv + NW;
+    std::vector<cdouble> rows(4 * K);
+    for (int64_t q = 0; q < K; ++q) {
+        rows[4 * q] = 0.0;
+        rows[4 * q + 1] = {0.0, -kappa[q]};
+        rows[4 * q + 2] = 1.0;
+        rows[4 * q + 3] = 0.0;
+    }
+    const std::vector<double> c = rows_coeff(K, rows);
+    load_params(h, kappa, c.data(), W, wcs, 4.0 * M_PI / (double)h.m);
+    ApplyArgs a{cdouble(1.0, 0.0), cdouble(0.0, 0.0), g, W, kC128, herg, W, kC128};
+    herglotz_dev(h, a, s);
+    dtn_dev(h, kappa, K, wcs, herg, dv, cg_tol, gmres_tol, s);
+    dev_row_product_sum(h, (int)W, du, dv, prod, s);
+    multiply_with_normals_dev(h, prod, Y, -1.0 / (double)W, cg_tol, s);
+}
+
+void check_common(Handle& h, const double* kappa, int64_t K, int64_t wcs)
+{
+    if (!kappa) throw ArgError{"kappa must not be NULL"};
+    if (K < 1 || wcs < 1) throw ArgError{"wave_chunk_count and wave_chunk_size must be >= 1"};
+    if (K > 64) throw ArgError{"at most 64 wave-number chunks are supported"};
+}
+
+struct DevArray {
+    void* p = nullptr;
+    explicit DevArray(size_t bytes) { BAEMM_HIP(hipMalloc(&p, bytes > 0 ? bytes : 16)); }
+    ~DevArray() { if (p) (void)hipFree(p); }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+void upload(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    BAEMM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+}
+
+void download(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    BAEMM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    BAEMM_HIP(hipStreamSynchronize(s));
+}
+
+// du_dn of the reference's C_ext*& argument: allocated with calloc when NULL
+// (FarField.hpp:273-283), filled by Derivative_FF_Helper.
+double2* ensure_du_dn(Handle& h, double** du_dn, const double* kappa, int64_t K, int64_t wcs, const double* dirs,
+                      int wave_type, double cg_tol, double gmres_tol, DevArray& ddu, hipStream_t s)
+{
+    const int64_t W = K * wcs, NW = h.nv * W;
+    if (!du_dn) throw ArgError{"du_dn must point to a pointer (which may be NULL)"};
+    if (*du_dn == nullptr) {
+        *du_dn = static_cast<double*>(std::calloc((size_t)NW * 2, sizeof(double)));
+        if (!*du_dn) throw std::bad_alloc();
+        du_dn_dev(h, kappa, K, wcs, dirs, wave_type, ddu.as<double2>(), cg_tol, gmres_tol, s);
+        download(*du_dn, ddu.p, sizeof(double2) * NW, s);
+    } else {
+        upload(ddu.p, *du_dn, sizeof(double2) * NW, s);
+    }
+    return ddu.as<double2>();
+}
+
+}  // namespace
+
+}  // namespace baemm
+
+using namespace baemm;
+
+extern "C" {
+
+int baemm_apply_mass_inverse(baemm_handle h, const double* X, int64_t ldX, double* Y, int64_t ldY, int64_t nrhs,
+                             double cg_tol)
+{
+    return guard(h, [&](Handle& hh) {
+        if (!X || !Y || nrhs < 1 || ldX < nrhs || ldY < nrhs) throw ArgError{"invalid mass-inverse arguments"};
+        hipStream_t s = hh.stream;
+        DevArray dx(sizeof(double) * hh.nv * nrhs), dy(sizeof(double) * hh.nv * nrhs);
+        BAEMM_HIP(hipMemcpy2DAsync(dx.p, sizeof(double) * nrhs, X, sizeof(double) * ldX, sizeof(double) * nrhs, hh.nv,
+                                   hipMemcpyHostToDevice, s));
+        bool conv = false;
+        hh.stats.cg_iterations += dev_mass_cg(hh, hh.sws, dx.as<double>(), nrhs, dy.as<double>(), nrhs, (int)nrhs,
+                                              cg_tol, 20, &conv, s);
+        BAEMM_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldY, dy.p, sizeof(double) * nrhs, sizeof(double) * nrhs, hh.nv,
+                                   hipMemcpyDeviceToHost, s));
+        BAEMM_HIP(hipStreamSynchronize(s));
+        return BAEMM_OK;
+    });
+}
+
+int baemm_apply_mass_inverse_complex(baemm_handle h, const double* X, int64_t ldX, double* Y, int64_t ldY,
+                                     int64_t nrhs, double cg_tol)
+{
+    if (nrhs < 1 || ldX < nrhs || ldY < nrhs) return BAEMM_ERR_ARG;
+    return baemm_apply_mass_inverse(h, X, 2 * ldX, Y, 2 * ldY, 2 * nrhs, cg_tol);
+}
+
+int baemm_apply_lumped_mass_inverse(baemm_handle h, const double* X, int64_t ldX, double* Y, int64_t ldY,
+                                    int64_t nrhs)
+{
+    return guard(h, [&](Handle& hh) {
+        if (!X || !Y || nrhs < 1 || ldX < nrhs || ldY < nrhs) throw ArgError{"invalid arguments"};
+        for (int64_t v = 0; v < hh.nv; ++v)
+            for (int64_t k = 0; k < nrhs; ++k) Y[v * ldY + k] = hh.h_lumped_inv[v] * X[v * ldX + k];
+        return BAEMM_OK;
+    });
+}
+
+int baemm_boundary_potential(baemm_handle h, const double* kappa, int64_t K, const double* eta, double* coeff,
+                             const double* wave, double* phi, int64_t wcs, double cg_tol, double gmres_tol)
+{
+    return guard(h, [&](Handle& hh) {
+        check_common(hh, kappa, K, wcs);
+        if (!wave || !phi) throw ArgError{"wave and phi must not be NULL"};
+        const int64_t NW = hh.nv * K * wcs;
+        hipStream_t s = hh.stream;
+        DevArray dw(sizeof(double2) * NW), dp(sizeof(double2) * NW);
+        upload(dw.p, wave, sizeof(double2) * NW, s);
+        const GmresResult r = boundary_potential_dev(hh, kappa, eta ? eta : kappa, K, wcs, dw.as<double2>(),
+                                                     dp.as<double2>(), cg_tol, gmres_tol, s, coeff);
+        download(phi, dp.p, sizeof(double2) * NW, s);
+        return r.converged ? BAEMM_OK : BAEMM_ERR_NOCONV;
+    });
+}
+
+int baemm_dirichlet_to_neumann(baemm_handle h, const double* kappa, int64_t K, const double* wave, double* neumann,
+                               int64_t wcs, double cg_tol, double gmres_tol)
+{
+    return guard(h, [&](Handle& hh) {
+        check_common(hh, kappa, K, wcs);
+        if (!wave || !neumann) throw ArgError{"wave and neumann_trace must not be NULL"};
+        const int64_t NW = hh.nv * K * wcs;
+        hipStream_t s = hh.stream;
+        DevArray dw(sizeof(double2) * NW), dp(sizeof(double2) * NW);
+        upload(dw.p, wave, sizeof(double2) * NW, s);
+        const GmresResult r = dtn_dev(hh, kappa, K, wcs, dw.as<double2>(), dp.as<double2>(), cg_tol, gmres_tol, s);
+        download(neumann, dp.p, sizeof(double2) * NW, s);
+        return r.converged ? BAEMM_OK : BAEMM_ERR_NOCONV;
+    });
+}
+
+int baemm_far_field_parameters(baemm_handle h, const double* kappa, int64_t K, const double* dirs, int64_t wcs,
+                               double* Y, int wave_type, const double* eta, double cg_tol, double gmres_tol)
+{
+    return guard(h, [&](Handle& hh) {
+        check_common(hh, kappa, K, wcs);
+        if (!dirs || !Y) throw ArgError{"directions and output must not be NULL"};
+        if (hh.m < 1) throw ArgError{"no measurement directions"};
+        const int64_t W = K * wcs, NW = hh.nv * W;
+        hipStream_t s = hh.stream;
+        DevArray dwave(sizeof(double2) * NW), dphi(sizeof(double2) * NW), dF(sizeof(double2) * hh.m * W);
+        std::vector<cdouble> c12(2 * K, cdouble(0.0, 0.0));
+        for (int64_t q = 0; q < K; ++q) c12[2 * q] = -1.0;     // (0, -1, 0, 0): FarField.hpp:120-126
+        incident_rows_dev(hh, kappa, K, wcs, dirs, wave_type, c12, dwave.as<double2>(), s);
+        std::vector<double> coeff(8 * K);
+        const GmresResult r = boundary_potential_dev(hh, kappa, eta ? eta : kappa, K, wcs, dwave.as<double2>(),
+                                                     dphi.as<double2>(), cg_tol, gmres_tol, s, coeff.data());
+        far_field_dev(hh, kappa, K, wcs, coeff, dphi.as<double2>(), dF.as<double2>(), s);
+        download(Y, dF.p, sizeof(double2) * hh.m * W, s);
+        return r.converged ? BAEMM_OK : BAEMM_ERR_NOCONV;
+    });
+}
+
+int baemm_far_field(baemm_handle h, const double* kappa, int64_t K, const double* dirs, int64_t wcs, double* Y,
+                    int wave_type, double cg_tol, double gmres_tol)
+{
+    return baemm_far_field_parameters(h, kappa, K, dirs, wcs, Y, wave_type, kappa, cg_tol, gmres_tol);
+}
+
+int baemm_derivative_ff_helper(baemm_handle h, const double* kappa, int64_t K, const double* dirs, int64_t wcs,
+                               double* du_dn, int wave_type, double cg_tol, double gmres_tol)
+{
+    return guard(h, [&](Handle& hh) {
+        check_common(hh, kappa, K, wcs);
+        if (!dirs || !du_dn) throw ArgError{"directions and du_dn must not be NULL"};
+        const int64_t NW = hh.nv * K * wcs;
+        hipStream_t s = hh.stream;
+        DevArray ddu(sizeof(double2) * NW);
+        hh.stats.converged = true;
+        du_dn_dev(hh, kappa, K, wcs, dirs, wave_type, ddu.as<double2>(), cg_tol, gmres_tol, s);
+        download(du_dn, ddu.p, sizeof(double2) * NW, s);
+        return hh.stats.converged ? BAEMM_OK : BAEMM_ERR_NOCONV;
+    });
+}
+
+int baemm_derivative_ff(baemm_handle h, const double* kappa, int64_t K, const double* dirs, int64_t wcs,
+                        const double* X_in, double* Y_out, double** du_dn, int wave_type, double cg_tol,
+                        double gmres_tol)
+{
+    return guard(h, [&](Handle& hh) {
+        check_common(hh, kappa, K, wcs);
+        if (!dirs || !X_in || !Y_out) throw ArgError{"directions, X_in and Y_out must not be NULL"};
+        if (hh.m < 1) throw ArgError{"no measurement directions"};
+        const int64_t W = K * wcs, NW = hh.nv * W;
+        hipStream_t s = hh.stream;
+        hh.stats.converged = true;
+        DevArray ddu(sizeof(double2) * NW), dX(sizeof(double) * 3 * hh.nv), dF(sizeof(double2) * hh.m * W);
+        const double2* du = ensure_du_dn(hh, du_dn, kappa, K, wcs, dirs, wave_type, cg_tol, gmres_tol, ddu, s);
+        upload(dX.p, X_in, sizeof(double) * 3 * hh.nv, s);
+        derivative_dev(hh, kappa, K, wcs, dX.as<double>(), du, dF.as<double2>(), cg_tol, gmres_tol, s);
+        download(Y_out, dF.p, sizeof(double2) * hh.m * W, s);
+        return hh.stats.converged ? BAEMM_OK : BAEMM_ERR_NOCONV;
+    });
+}
+
+int baemm_adjoint_derivative_ff(baemm_handle h, const double* kappa, int64_t K, const double* dirs, int64_t wcs,
+                                const double* g, double* Y_out, double** du_dn, int wave_type, double cg_tol,
+                                double gmres_tol)
+{
+    return guard(h, [&](Handle& hh) {
+        check_common(hh, kappa, K, wcs);
+        if (!dirs || !g || !Y_out) throw ArgError{"directions, g and Y_out must not be NULL"};
+        if (hh.m < 1) throw ArgError{"no measurement directions"};
+        const int64_t W = K * wcs, NW = hh.nv * W;
+        hipStream_t s = hh.stream;
+        hh.stats.converged = true;
+        DevArray ddu(sizeof(double2) * NW), dg(sizeof(double2) * hh.m * W), dY(sizeof(double) * 3 * hh.nv);
+        const double2* du = ensure_du_dn(hh, du_dn, kappa, K, wcs, dirs, wave_type, cg_tol, gmres_tol, ddu, s);
+        upload(dg.p, g, sizeof(double2) * hh.m * W, s);
+        adjoint_dev(hh, kappa, K, wcs, dg.as<double2>(), du, dY.as<double>(), cg_tol, gmres_tol, s);
+        download(Y_out, dY.p, sizeof(double) * 3 * hh.nv, s);
+        return hh.stats.converged ? BAEMM_OK : BAEMM_ERR_NOCONV;
+    });
+}
+
+int baemm_gauss_newton_solve(baemm_handle h, const double* kappa, int64_t K, const double* dirs, int64_t wcs,
+                             baemm_real_op M, void* M_user, baemm_real_op P, void* P_user, double alpha,
+                             const double* B, int64_t ldB, double beta, double* X, int64_t ldX, double** du_dn,
+                             int wave_type, double cg_tol, double gmres_tol_inner, double gmres_tol_outer,
+                             int* succeeded)
+{
+    return guard(h, [&](Handle& hh) {
+        check_common(hh, kappa, K, wcs);
+        if (!dirs || !M || !P || !B || !X || ldB < 3 || ldX < 3) throw ArgError{"invalid Gauss-Newton arguments"};
+        if (hh.m < 1) throw ArgError{"no measurement directions"};
+        const int64_t W = K * wcs, NW = hh.nv * W, n = hh.nv, N = 3 * n;
+        hipStream_t s = hh.stream;
+        hh.stats.converged = true;
+        DevArray ddu(sizeof(double2) * NW), dX(sizeof(double) * N), dF(sizeof(double2) * hh.m * W),
+            dY(sizeof(double) * N);
+        const double2* du = ensure_du_dn(hh, du_dn, kappa, K, wcs, dirs, wave_type, cg_tol, gmres_tol_inner, ddu, s);
+        std::vector<double> tmp(N), ys(N);
+        // A x = M x + Mass DF* DF x  (FarField.hpp:551-572)
+        auto A = [&](const double* x, double* y) {
+            M(M_user, x, y);
+            upload(dX.p, x, sizeof(double) * N, s);
+            derivative_dev(hh, kappa, K, wcs, dX.as<double>(), du, dF.as<double2>(), cg_tol, gmres_tol_inner, s);
+            adjoint_dev(hh, kappa, K, wcs, dF.as<double2>(), du, dY.as<double>(), cg_tol, gmres_tol_inner, s);
+            download(ys.data(), dY.p, sizeof(double) * N, s);
+            for (int64_t v = 0; v < n; ++v)
+                for (int d = 0; d < 3; ++d) {
+                    double acc = 0.0;
+                    for (int32_t e = hh.h_M_ptr[v]; e < hh.h_M_ptr[v + 1]; ++e)
+                        acc += hh.h_M_val[e] * ys[3 * (int64_t)hh.h_M_idx[e] + d];
+                    y[3 * v + d] += acc;
+                }
+        };
+        // Left-preconditioned restarted GMRES on the host (one real column of length 3V).
+        std::vector<double> b(N), xs(N, 0.0), r(N), z(N);
+        for (int64_t v = 0; v < n; ++v)
+            for (int d = 0; d < 3; ++d) b[3 * v + d] = B[v * ldB + d];
+        const int m = hh.gmres_max_iter;
+        std::vector<double> Vb((size_t)(m + 1) * N), Hm((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), yv(m);
+        auto dot = [&](const double* a, const double* c) {
+            double acc = 0.0;
+            for (int64_t i = 0; i < N; ++i) acc += a[i] * c[i];
+            return acc;
+        };
+        bool ok = false;
+        double bnorm = 0.0;
+        for (int cyc = 0; cyc < hh.gmres_max_restarts && !ok; ++cyc) {
+            if (cyc == 0) {
+                P(P_user, b.data(), z.data());
+            } else {
+                A(xs.data(), tmp.data());
+                for (int64_t i = 0; i < N; ++i) r[i] = b[i] - tmp[i];
+                P(P_user, r.data(), z.data());
+            }
+            const double bt = std::sqrt(dot(z.data(), z.data()));
+            if (cyc == 0) bnorm = bt;
+            if (bnorm == 0.0 || bt <= gmres_tol_outer * bnorm) {
+                ok = true;
+                break;
+            }
+            for (int64_t i = 0; i < N; ++i) Vb[i] = z[i] / bt;
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = bt;
+            int jd = 0;
+            for (int j = 0; j < m; ++j) {
+                A(&Vb[(size_t)j * N], tmp.data());
+                P(P_user, tmp.data(), z.data());
+                for (int i = 0; i <= j; ++i) {   // modified Gram-Schmidt
+                    const double hij = dot(&Vb[(size_t)i * N], z.data());
+                    Hm[(size_t)i * m + j] = hij;
+                    for (int64_t l = 0; l < N; ++l) z[l] -= hij * Vb[(size_t)i * N + l];
+                }
+                const double hn = std::sqrt(dot(z.data(), z.data()));
+                for (int64_t l = 0; l < N; ++l) Vb[(size_t)(j + 1) * N + l] = hn > 0.0 ? z[l] / hn : 0.0;
+                for (int i = 0; i < j; ++i) {
+                    const double a0 = Hm[(size_t)i * m + j], a1 = Hm[(size_t)(i + 1) * m + j];
+                    Hm[(size_t)i * m + j] = cs[i] * a0 + sn[i] * a1;
+                    Hm[(size_t)(i + 1) * m + j] = -sn[i] * a0 + cs[i] * a1;
+                }
+                const double a0 = Hm[(size_t)j * m + j];
+                const double nu = std::sqrt(a0 * a0 + hn * hn);
+                cs[j] = nu > 0.0 ? a0 / nu : 1.0;
+                sn[j] = nu > 0.0 ? hn / nu : 0.0;
+                Hm[(size_t)j * m + j] = nu;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                jd = j + 1;
+                hh.stats.gmres_iterations++;
+                if (std::fabs(g[j + 1]) <= gmres_tol_outer * bnorm || hn == 0.0) break;
+            }
+            for (int i = jd - 1; i >= 0; --i) {
+                double acc = g[i];
+                for (int l = i + 1; l < jd; ++l) acc -= Hm[(size_t)i * m + l] * yv[l];
+                yv[i] = Hm[(size_t)i * m + i] != 0.0 ? acc / Hm[(size_t)i * m + i] : 0.0;
+            }
+            for (int i = 0; i < jd; ++i)
+                for (int64_t l = 0; l < N; ++l) xs[l] += yv[i] * Vb[(size_t)i * N + l];
+            hh.stats.gmres_rel_residual = std::fabs(g[jd]) / bnorm;
+            if (std::fabs(g[jd]) <= gmres_tol_outer * bnorm) ok = true;
+        }
+        // X = alpha (M + Mass DF* DF)^{-1} B + beta X   (FarField.hpp:433-435, 581-599)
+        for (int64_t v = 0; v < n; ++v)
+            for (int d = 0; d < 3; ++d)
+                X[v * ldX + d] = alpha * xs[3 * v + d] + (beta != 0.0 ? beta * X[v * ldX + d] : 0.0);
+        if (succeeded) *succeeded = ok ? 1 : 0;
+        return BAEMM_OK;
+    });
+}
+
+int baemm_get_solver_stats(baemm_handle h, int64_t* gmres_iterations, int64_t* gmres_cycles, double* rel_residual,
+                           int64_t* cg_iterations, int64_t* operator_applies)
+{
+    return guard(h, [&](Handle& hh) {
+        if (gmres_iterations) *gmres_iterations = hh.stats.gmres_iterations;
+        if (gmres_cycles) *gmres_cycles = hh.stats.gmres_cycles;
+        if (rel_residual) *rel_residual = hh.stats.gmres_rel_residual;
+        if (cg_iterations) *cg_iterations = hh.stats.cg_iterations;
+        if (operator_applies) *operator_applies = hh.stats.operator_applies;
+        return BAEMM_OK;
+    });
+}
+
+int baemm_reset_solver_stats(baemm_handle h)
+{
+    return guard(h, [&](Handle& hh) {
+        hh.stats = SolverStats{};
+        return BAEMM_OK;
+    });
+}
+
+}  // extern "C"

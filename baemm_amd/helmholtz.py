@@ -55,8 +55,8 @@ def _c128_out(a, what, rows):
     return a
 
 
-class Helmholtz_HIP:
-    """Drop-in for ``BAEMM::Helmholtz_OpenCL`` (Helmholtz_OpenCL.hpp:34-399)."""
+class _HelmholtzBase:
+    """Construction, getters/setters and operator applies."""
 
     def __init__(self, vertex_coords, triangles, meas_directions=None, device=0, CPU_thread_count=1):
         self._L = _lib.load()
@@ -67,7 +67,8 @@ class Helmholtz_HIP:
         st = self._L.baemm_create(_ptr(V), V.shape[0], _ptr(T), T.shape[0], _ptr(M), M.shape[0], int(device),
                                   ctypes.byref(h))
         if st != _lib.OK:
-            raise BaemmError(st, "baemm_create failed")
+            raw = self._L.baemm_last_error(None)
+            raise BaemmError(st, "baemm_create failed: " + (raw.decode() if raw else ""))
         self._h = h
         self.CPU_thread_count = int(CPU_thread_count)
         self._V = V
@@ -276,3 +277,181 @@ class Helmholtz_HIP:
         n = ctypes.c_int64()
         self._check(self._L.baemm_get_timing(self._h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+def _libc_free(ptr):
+    ctypes.CDLL(None).free(ctypes.c_void_p(ptr))
+
+
+class _SolverMixin:
+    """Solver compositions of FarField.hpp on the device (see include/baemm.h)."""
+
+    def SetGmresLimits(self, max_iter=30, max_restarts=3):
+        """gmres_max_iter / gmres_max_restarts (MemberVariables.hpp:64-65)."""
+        self._check(self._L.baemm_set_gmres_limits(self._h, int(max_iter), int(max_restarts)))
+
+    def SolverStats(self):
+        it, cyc, cg, ap = (ctypes.c_int64() for _ in range(4))
+        res = ctypes.c_double()
+        self._check(self._L.baemm_get_solver_stats(self._h, ctypes.byref(it), ctypes.byref(cyc), ctypes.byref(res),
+                                                   ctypes.byref(cg), ctypes.byref(ap)))
+        return {"gmres_iterations": it.value, "gmres_cycles": cyc.value, "rel_residual": res.value,
+                "cg_iterations": cg.value, "operator_applies": ap.value}
+
+    def ResetSolverStats(self):
+        self._check(self._L.baemm_reset_solver_stats(self._h))
+
+    def _solve_status(self, st):
+        if st not in (_lib.OK, _lib.ERR_NOCONV):
+            self._check(st)
+        return st == _lib.OK
+
+    def ApplyMassInverse(self, X, Y, cg_tol):
+        """ApplyMassInverse.hpp:6-156: Y = M^{-1} X (real or complex columns)."""
+        X = np.asarray(X)
+        if np.iscomplexobj(X):
+            Xc = np.require(X.reshape(X.shape[0], -1), dtype=np.complex128, requirements=["C"])
+            Yc = Y.reshape(Y.shape[0], -1)
+            self._check(self._L.baemm_apply_mass_inverse_complex(self._h, _ptr(Xc), Xc.shape[1], _ptr(Yc),
+                                                                 _ld(Yc, "Y"), Xc.shape[1], float(cg_tol)))
+        else:
+            Xr = np.require(X.reshape(X.shape[0], -1), dtype=np.float64, requirements=["C"])
+            Yr = Y.reshape(Y.shape[0], -1)
+            self._check(self._L.baemm_apply_mass_inverse(self._h, _ptr(Xr), Xr.shape[1], _ptr(Yr), _ld(Yr, "Y"),
+                                                         Xr.shape[1], float(cg_tol)))
+        return Y
+
+    def ApplyLumpedMassInverse(self, X, Y):
+        Xr = np.require(np.asarray(X, dtype=np.float64).reshape(len(X), -1), requirements=["C"])
+        Yr = Y.reshape(Y.shape[0], -1)
+        self._check(self._L.baemm_apply_lumped_mass_inverse(self._h, _ptr(Xr), Xr.shape[1], _ptr(Yr), _ld(Yr, "Y"),
+                                                            Xr.shape[1]))
+        return Y
+
+    @staticmethod
+    def _kd(kappa, dirs, wcs):
+        k = np.ascontiguousarray(kappa, dtype=np.float64).ravel()
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        if d.shape[0] < wcs:
+            raise ValueError("need wave_chunk_size directions")
+        return k, d
+
+    def BoundaryPotential(self, kappa, wave, wave_chunk_size, cg_tol, gmres_tol, eta=None):
+        """FarField.hpp:626-771; returns (phi, coeff rows, converged)."""
+        k = np.ascontiguousarray(kappa, dtype=np.float64).ravel()
+        wave = np.require(wave, dtype=np.complex128, requirements=["C"])
+        phi = np.zeros_like(wave)
+        coeff = np.zeros((k.size, 4), np.complex128)
+        e = None if eta is None else np.ascontiguousarray(eta, dtype=np.float64)
+        st = self._L.baemm_boundary_potential(self._h, _ptr(k), k.size, None if e is None else _ptr(e), _ptr(coeff),
+                                              _ptr(wave), _ptr(phi), int(wave_chunk_size), float(cg_tol),
+                                              float(gmres_tol))
+        return phi, coeff, self._solve_status(st)
+
+    def DirichletToNeumann(self, kappa, wave, wave_chunk_size, cg_tol, gmres_tol):
+        """FarField.hpp:790-879; returns (neumann_trace, converged)."""
+        k = np.ascontiguousarray(kappa, dtype=np.float64).ravel()
+        wave = np.require(wave, dtype=np.complex128, requirements=["C"])
+        out = np.zeros_like(wave)
+        st = self._L.baemm_dirichlet_to_neumann(self._h, _ptr(k), k.size, _ptr(wave), _ptr(out),
+                                                int(wave_chunk_size), float(cg_tol), float(gmres_tol))
+        return out, self._solve_status(st)
+
+    def FarField(self, kappa, wave_chunk_count, inc_directions, wave_chunk_size, Y_out=None,
+                 type=WaveType.Plane, cg_tol=1e-5, gmres_tol=1e-4, eta=None):
+        """FarField / FarField_parameters (FarField.hpp:28-144)."""
+        k, d = self._kd(kappa, inc_directions, wave_chunk_size)
+        W = int(wave_chunk_count) * int(wave_chunk_size)
+        Y = np.zeros((self.GetMeasCount(), W), np.complex128) if Y_out is None else Y_out
+        e = None if eta is None else np.ascontiguousarray(eta, dtype=np.float64)
+        st = self._L.baemm_far_field_parameters(self._h, _ptr(k), int(wave_chunk_count), _ptr(d),
+                                                int(wave_chunk_size), _ptr(Y), int(type),
+                                                None if e is None else _ptr(e), float(cg_tol), float(gmres_tol))
+        self._solve_status(st)
+        return Y
+
+    def Derivative_FF_Helper(self, kappa, wave_chunk_count, inc_directions, wave_chunk_size,
+                             type=WaveType.Plane, cg_tol=1e-5, gmres_tol=1e-4):
+        k, d = self._kd(kappa, inc_directions, wave_chunk_size)
+        W = int(wave_chunk_count) * int(wave_chunk_size)
+        du = np.zeros((self.VertexCount(), W), np.complex128)
+        st = self._L.baemm_derivative_ff_helper(self._h, _ptr(k), int(wave_chunk_count), _ptr(d),
+                                                int(wave_chunk_size), _ptr(du), int(type), float(cg_tol),
+                                                float(gmres_tol))
+        self._solve_status(st)
+        return du
+
+    def _du_arg(self, du_dn):
+        if du_dn is None:
+            return ctypes.c_void_p(None), None
+        du = np.require(du_dn, dtype=np.complex128, requirements=["C"])
+        return ctypes.c_void_p(du.ctypes.data), du
+
+    def _du_result(self, pp, keep, W):
+        if keep is not None:
+            return keep
+        nv = self.VertexCount()
+        arr = np.ctypeslib.as_array(ctypes.cast(pp, ctypes.POINTER(ctypes.c_double)), shape=(nv * W * 2,))
+        du = arr.view(np.complex128).reshape(nv, W).copy()
+        _libc_free(pp.value)
+        return du
+
+    def Derivative_FF(self, kappa, wave_chunk_count, inc_directions, wave_chunk_size, X_in, Y_out=None, du_dn=None,
+                      type=WaveType.Plane, cg_tol=1e-5, gmres_tol=1e-4):
+        """FarField.hpp:227-321; returns (Y_out, du_dn)."""
+        k, d = self._kd(kappa, inc_directions, wave_chunk_size)
+        W = int(wave_chunk_count) * int(wave_chunk_size)
+        X = np.require(X_in, dtype=np.float64, requirements=["C"]).reshape(-1, 3)
+        Y = np.zeros((self.GetMeasCount(), W), np.complex128) if Y_out is None else Y_out
+        pp, keep = self._du_arg(du_dn)
+        st = self._L.baemm_derivative_ff(self._h, _ptr(k), int(wave_chunk_count), _ptr(d), int(wave_chunk_size),
+                                         _ptr(X), _ptr(Y), ctypes.byref(pp), int(type), float(cg_tol),
+                                         float(gmres_tol))
+        self._solve_status(st)
+        return Y, self._du_result(pp, keep, W)
+
+    def AdjointDerivative_FF(self, kappa, wave_chunk_count, inc_directions, wave_chunk_size, g, Y_out=None,
+                             du_dn=None, type=WaveType.Plane, cg_tol=1e-5, gmres_tol=1e-4):
+        """FarField.hpp:342-428; returns (Y_out (V x 3), du_dn)."""
+        k, d = self._kd(kappa, inc_directions, wave_chunk_size)
+        W = int(wave_chunk_count) * int(wave_chunk_size)
+        gg = np.require(g, dtype=np.complex128, requirements=["C"])
+        Y = np.zeros((self.VertexCount(), 3)) if Y_out is None else Y_out
+        pp, keep = self._du_arg(du_dn)
+        st = self._L.baemm_adjoint_derivative_ff(self._h, _ptr(k), int(wave_chunk_count), _ptr(d),
+                                                 int(wave_chunk_size), _ptr(gg), _ptr(Y), ctypes.byref(pp),
+                                                 int(type), float(cg_tol), float(gmres_tol))
+        self._solve_status(st)
+        return Y, self._du_result(pp, keep, W)
+
+    def GaussNewtonSolve(self, kappa, wave_chunk_count, inc_directions, wave_chunk_size, M, P, alpha, B, beta, X,
+                         du_dn=None, type=WaveType.Plane, cg_tol=1e-5, gmres_tol_inner=5e-3, gmres_tol_outer=1e-2):
+        """FarField.hpp:457-604.  M, P: callables (x, y) on (V x 3) float64 arrays,
+        writing y in place.  Returns (succeeded, du_dn)."""
+        k, d = self._kd(kappa, inc_directions, wave_chunk_size)
+        W = int(wave_chunk_count) * int(wave_chunk_size)
+        nv = self.VertexCount()
+
+        def wrap(fn):
+            def cb(user, x, y):
+                xa = np.ctypeslib.as_array(x, shape=(nv, 3))
+                ya = np.ctypeslib.as_array(y, shape=(nv, 3))
+                fn(xa, ya)
+            return _lib.REAL_OP(cb)
+
+        Mc, Pc = wrap(M), wrap(P)
+        Bv = np.require(B, dtype=np.float64, requirements=["C"])
+        pp, keep = self._du_arg(du_dn)
+        ok = ctypes.c_int(0)
+        st = self._L.baemm_gauss_newton_solve(self._h, _ptr(k), int(wave_chunk_count), _ptr(d), int(wave_chunk_size),
+                                              ctypes.cast(Mc, ctypes.c_void_p), None,
+                                              ctypes.cast(Pc, ctypes.c_void_p), None, float(alpha), _ptr(Bv),
+                                              _ld(Bv, "B"), float(beta), _ptr(X), _ld(X, "X"), ctypes.byref(pp),
+                                              int(type), float(cg_tol), float(gmres_tol_inner),
+                                              float(gmres_tol_outer), ctypes.byref(ok))
+        self._check(st)
+        return bool(ok.value), self._du_result(pp, keep, W)
+
+
+class Helmholtz_HIP(_HelmholtzBase, _SolverMixin):
+    """Drop-in for ``BAEMM::Helmholtz_OpenCL`` (Helmholtz_OpenCL.hpp:34-399)."""

@@ -77,7 +77,8 @@ int baemm_create_i32(const double* coords, int64_t vertex_count,
 /* ~Helmholtz_OpenCL, Helmholtz_OpenCL.hpp:251-282. */
 int baemm_destroy(baemm_handle h);
 
-/* Message of the last failed call on h (empty if none). */
+/* Message of the last failed call on h (empty if none).  With h == NULL: the
+ * message of the last failed baemm_create* on the calling thread. */
 const char* baemm_last_error(baemm_handle h);
 
 /* ------------------------------------------------------ getters / setters */
@@ -185,6 +186,92 @@ int baemm_apply_boundary_dev(baemm_handle h, const double* alpha, const void* B,
                              const double* beta, void* C, int64_t ldC, int C_type, void* stream);
 int baemm_apply_far_field_dev(baemm_handle h, const double* alpha, const void* B, int64_t ldB, int B_type,
                               const double* beta, void* F, int64_t ldF, int F_type, void* stream);
+
+/* ----------------------------------------------------------- mass inverse */
+
+/* ApplyMassInverse<NRHS>(X, ldX, Y, ldY, cg_tol, nrhs), ApplyMassInverse.hpp:6-156:
+ * Y = M^{-1} X by preconditioned CG (lumped-diagonal preconditioner,
+ * at most 20 iterations, relative residual cg_tol per column); real columns. */
+int baemm_apply_mass_inverse(baemm_handle h, const double* X, int64_t ldX, double* Y, int64_t ldY,
+                             int64_t nrhs, double cg_tol);
+
+/* Same for complex columns (re and im solved as 2*nrhs real columns,
+ * ApplyMassInverse.hpp:71-89); ldX, ldY in complex elements. */
+int baemm_apply_mass_inverse_complex(baemm_handle h, const double* X, int64_t ldX, double* Y,
+                                     int64_t ldY, int64_t nrhs, double cg_tol);
+
+/* ApplyLumpedMassInverse, ApplyMassInverse.hpp:161-197 (real columns). */
+int baemm_apply_lumped_mass_inverse(baemm_handle h, const double* X, int64_t ldX, double* Y,
+                                    int64_t ldY, int64_t nrhs);
+
+/* ----------------------------------------------------------------- solvers
+ * kappa[K] wave numbers, wcs = wave_chunk_size, W = K * wcs columns; vertex
+ * arrays are V x W complex (ld = W), far-field arrays m x W complex.  The
+ * boundary integral equations are solved by left-preconditioned restarted
+ * GMRES on the device (restart and cycle limits: baemm_set_gmres_limits,
+ * default 30 and 3, MemberVariables.hpp:64-65) with the mass inverse (PCG,
+ * cg_tol) as preconditioner.  A solve that stops above its tolerance still
+ * writes its result and returns BAEMM_ERR_NOCONV (the reference discards the
+ * GMRES flag, FarField.hpp:764). */
+
+/* BoundaryPotential / BoundaryPotential_parameters, FarField.hpp:626-771:
+ * (1/2 M - i eta S + K) phi = wave.  eta may be NULL (eta = kappa); coeff
+ * (K x 4 complex, may be NULL) receives the coefficient rows. */
+int baemm_boundary_potential(baemm_handle h, const double* kappa, int64_t K, const double* eta,
+                             double* coeff, const double* wave, double* phi, int64_t wcs,
+                             double cg_tol, double gmres_tol);
+
+/* DirichletToNeumann, FarField.hpp:790-879: (1/2 M - i kappa S + K') psi = wave
+ * (wave in weak form, psi strong). */
+int baemm_dirichlet_to_neumann(baemm_handle h, const double* kappa, int64_t K, const double* wave,
+                               double* neumann_trace, int64_t wcs, double cg_tol, double gmres_tol);
+
+/* FarField<WC>, FarField.hpp:28-70, and FarField_parameters (:90-144, eta may
+ * be NULL).  dirs: wcs x 3 incident directions (plane) or point sources
+ * (radial); Y: m x W far field. */
+int baemm_far_field(baemm_handle h, const double* kappa, int64_t K, const double* dirs, int64_t wcs,
+                    double* Y, int wave_type, double cg_tol, double gmres_tol);
+int baemm_far_field_parameters(baemm_handle h, const double* kappa, int64_t K, const double* dirs,
+                               int64_t wcs, double* Y, int wave_type, const double* eta,
+                               double cg_tol, double gmres_tol);
+
+/* Derivative_FF_Helper, FarField.hpp:148-207: du_dn (V x W) = Neumann data
+ * of the total field. */
+int baemm_derivative_ff_helper(baemm_handle h, const double* kappa, int64_t K, const double* dirs,
+                               int64_t wcs, double* du_dn, int wave_type, double cg_tol,
+                               double gmres_tol);
+
+/* Derivative_FF, FarField.hpp:227-321: Y_out (m x W) = DF X_in, X_in V x 3.
+ * du_dn is the reference's C_ext*& argument: if *du_dn is NULL it is
+ * allocated with calloc (free it with free()) and filled by the helper. */
+int baemm_derivative_ff(baemm_handle h, const double* kappa, int64_t K, const double* dirs,
+                        int64_t wcs, const double* X_in, double* Y_out, double** du_dn,
+                        int wave_type, double cg_tol, double gmres_tol);
+
+/* AdjointDerivative_FF, FarField.hpp:342-428: Y_out (V x 3) = DF* g, g m x W. */
+int baemm_adjoint_derivative_ff(baemm_handle h, const double* kappa, int64_t K, const double* dirs,
+                                int64_t wcs, const double* g, double* Y_out, double** du_dn,
+                                int wave_type, double cg_tol, double gmres_tol);
+
+/* Caller-supplied real operator y = Op(x) on V x 3 arrays (ld 3), the M and P
+ * function handles of GaussNewtonSolve. */
+typedef void (*baemm_real_op)(void* user, const double* x, double* y);
+
+/* GaussNewtonSolve, FarField.hpp:457-604: X = alpha (M + Mass DF* DF)^{-1} B
+ * + beta X by left-preconditioned real GMRES (preconditioner P) with tolerance
+ * gmres_tol_outer; the inner solves use gmres_tol_inner.  *succeeded receives
+ * the outer GMRES flag (the reference's return value). */
+int baemm_gauss_newton_solve(baemm_handle h, const double* kappa, int64_t K, const double* dirs,
+                             int64_t wcs, baemm_real_op M, void* M_user, baemm_real_op P, void* P_user,
+                             double alpha, const double* B, int64_t ldB, double beta, double* X,
+                             int64_t ldX, double** du_dn, int wave_type, double cg_tol,
+                             double gmres_tol_inner, double gmres_tol_outer, int* succeeded);
+
+/* Counters of the solvers since the last reset (no reference counterpart). */
+int baemm_get_solver_stats(baemm_handle h, int64_t* gmres_iterations, int64_t* gmres_cycles,
+                           double* last_rel_residual, int64_t* cg_iterations,
+                           int64_t* operator_applies);
+int baemm_reset_solver_stats(baemm_handle h);
 
 /* ------------------------------------------------------------- timing */
 

@@ -41,3 +41,22 @@ def test_null_handle_is_rejected_without_gpu():
     assert L.baemm_use_diagonal(None, 1) == _lib.ERR_ARG
     h = ctypes.c_void_p()
     assert L.baemm_create(None, 0, None, 0, None, 0, 0, ctypes.byref(h)) == _lib.ERR_ARG
+
+
+def test_create_argument_errors_carry_messages_without_gpu():
+    """Argument checks of baemm_create run before any HIP call; the message of
+    a failed creation is readable through baemm_last_error(NULL)."""
+    import numpy as np
+
+    from baemm_amd.meshes import icosphere
+
+    L = _lib.load()
+    V, T = icosphere(1)
+    bad = np.ascontiguousarray(T + V.shape[0])
+    h = ctypes.c_void_p()
+    st = L.baemm_create(V.ctypes.data, V.shape[0], bad.ctypes.data, bad.shape[0], None, 0, 0, ctypes.byref(h))
+    assert st == _lib.ERR_ARG and not h.value
+    assert b"triangle index" in L.baemm_last_error(None)
+    st = L.baemm_create(V.ctypes.data, V.shape[0], T.ctypes.data, T.shape[0], None, -1, 0, ctypes.byref(h))
+    assert st == _lib.ERR_ARG
+    assert b"measurement" in L.baemm_last_error(None)

@@ -1,0 +1,156 @@
+"""GPU solver compositions (device GMRES + PCG) against dense FP64 oracle
+solves, the Mie series and the defining equations of GaussNewtonSolve."""
+import numpy as np
+import pytest
+
+import oracle
+from baemm_amd import Helmholtz_HIP
+from baemm_amd.meshes import fibonacci_sphere, icosphere, write_mesh_file
+from conftest import oracle_mesh, relerr
+
+pytestmark = pytest.mark.gpu
+
+DIRS = np.array([[0.0, 0.0, 1.0], [0.6, 0.0, 0.8], [0.0, 0.6, -0.8]])
+
+
+@pytest.fixture(scope="module")
+def small():
+    m = oracle_mesh("sphere", 2, 64)
+    H = Helmholtz_HIP(m.V, m.T, m.meas)
+    H.SetGmresLimits(60, 4)
+    yield m, H
+    H.close()
+
+
+def test_dirichlet_to_neumann_matches_dense_solve(small):
+    m, H = small
+    rng = np.random.default_rng(0)
+    W, wcs = 6, 3
+    kappa = [2.0, 3.0]
+    wave = rng.standard_normal((m.nv, W)) + 1j * rng.standard_normal((m.nv, W))
+    ref = oracle.dtn(m, kappa, wave, wcs)
+    for mode in ("fp32", "mixed"):
+        H.SetPrecision(mode)
+        psi, ok = H.DirichletToNeumann(kappa, wave, wcs, 1e-12, 1e-10)
+        assert ok
+        assert relerr(psi, ref) < 2e-5, mode
+    H.SetPrecision("fp32")
+
+
+def test_boundary_potential_and_coefficients(small):
+    m, H = small
+    rng = np.random.default_rng(1)
+    wave = rng.standard_normal((m.nv, 4)) + 1j * rng.standard_normal((m.nv, 4))
+    phi, coeff, ok = H.BoundaryPotential([2.5], wave, 4, 1e-12, 1e-10, eta=[1.5])
+    assert ok
+    np.testing.assert_allclose(coeff[0], [0.5, -1.5j, 1.0, 0.0])
+    ref = oracle.solve_boundary(m, [2.5], [np.array([0.5, -1.5j, 1.0, 0.0])], wave, 4)
+    assert relerr(phi, ref) < 2e-5
+
+
+def test_far_field_matches_oracle_and_mie(small):
+    m, H = small
+    F = H.FarField([2.0], 1, DIRS, 3, cg_tol=1e-12, gmres_tol=1e-10)
+    ref, _ = oracle.far_field(m, [2.0], DIRS, 3)
+    assert relerr(F, ref) < 2e-5
+
+
+def test_derivatives_match_oracle(small):
+    m, H = small
+    kappa, wcs = [2.0], 3
+    du = H.Derivative_FF_Helper(kappa, 1, DIRS, wcs, cg_tol=1e-12, gmres_tol=1e-10)
+    du_ref = oracle.derivative_ff_helper(m, kappa, DIRS, wcs)
+    assert relerr(du, du_ref) < 2e-5
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((m.nv, 3))
+    Y, du2 = H.Derivative_FF(kappa, 1, DIRS, wcs, X, du_dn=None, cg_tol=1e-12, gmres_tol=1e-10)
+    assert relerr(du2, du_ref) < 2e-5          # du_dn allocated and filled when not given
+    Yref, _ = oracle.derivative_ff(m, kappa, DIRS, wcs, X, du_ref)
+    assert relerr(Y, Yref) < 5e-5
+    g = rng.standard_normal((m.meas.shape[0], 3)) + 1j * rng.standard_normal((m.meas.shape[0], 3))
+    Z, _ = H.AdjointDerivative_FF(kappa, 1, DIRS, wcs, g, du_dn=du, cg_tol=1e-12, gmres_tol=1e-10)
+    Zref, _ = oracle.adjoint_derivative_ff(m, kappa, DIRS, wcs, g, du_ref)
+    assert relerr(Z, Zref) < 5e-5
+
+
+def test_gauss_newton_solves_its_equation(small):
+    """X = (c I + Mass DF* DF)^{-1} B; check the residual with the GPU's own DF, DF*."""
+    m, H = small
+    kappa, wcs = [2.0], 3
+    c = 0.05
+    du = H.Derivative_FF_Helper(kappa, 1, DIRS, wcs, cg_tol=1e-12, gmres_tol=1e-10)
+
+    def Mop(x, y):
+        y[:] = c * x
+
+    def Pop(x, y):
+        y[:] = x / c
+
+    rng = np.random.default_rng(4)
+    B = rng.standard_normal((m.nv, 3))
+    X = np.zeros_like(B)
+    ok, _ = H.GaussNewtonSolve(kappa, 1, DIRS, wcs, Mop, Pop, 1.0, B, 0.0, X, du_dn=du, cg_tol=1e-12,
+                               gmres_tol_inner=1e-10, gmres_tol_outer=1e-7)
+    assert ok
+    DFX, _ = H.Derivative_FF(kappa, 1, DIRS, wcs, X, du_dn=du, cg_tol=1e-12, gmres_tol=1e-10)
+    Z, _ = H.AdjointDerivative_FF(kappa, 1, DIRS, wcs, DFX, du_dn=du, cg_tol=1e-12, gmres_tol=1e-10)
+    res = c * X + m.M @ Z - B
+    print('GN stats', H.SolverStats(), np.linalg.norm(res) / np.linalg.norm(B))
+    assert np.linalg.norm(res) < 1e-4 * np.linalg.norm(B)
+
+
+def test_far_field_mie_refinement():
+    """Sound-soft unit sphere: far field error vs the Mie series shrinks under refinement."""
+    errs = []
+    meas = fibonacci_sphere(128)
+    for L in (3, 4):
+        V, T = icosphere(L)
+        with Helmholtz_HIP(V, T, meas) as H:
+            H.SetPrecision("mixed")
+            F = H.FarField([4.0], 1, DIRS, 3, cg_tol=1e-10, gmres_tol=1e-8)
+        ref = np.stack([oracle.mie_far_field_soft(4.0, meas @ d) for d in DIRS], axis=1)
+        errs.append(relerr(F, ref))
+    print("Mie errors", errs)
+    assert errs[1] < 0.6 * errs[0] and errs[1] < 0.02
+
+
+def test_cpp_dropin_example_matches_python(tmp_path):
+    """The C++ drop-in class (include/Helmholtz_HIP.hpp) through the example
+    program gives the same far-field norm as the Python mirror."""
+    import re
+    import subprocess
+    from pathlib import Path
+
+    exe = Path(__file__).resolve().parents[1] / "baemm_amd" / "examples" / "example_far_field"
+    if not exe.exists():
+        pytest.fail(f"{exe} not built")
+    V, T = icosphere(3)
+    Vm, Tm = icosphere(4)
+    write_mesh_file(tmp_path / "obstacle.txt", V, T)
+    write_mesh_file(tmp_path / "meas.txt", Vm, Tm)
+    out = subprocess.run([str(exe), str(tmp_path / "obstacle.txt"), str(tmp_path / "meas.txt")],
+                         capture_output=True, text=True, timeout=240)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0
+    norm_cpp = float(re.search(r"L\^2-norm of C = (\S+)", out.stdout).group(1))
+    inc = np.array([[0.22663516023574246, 0.4654289185373517, 0.8555772472045235],
+                    [-0.3474510994575331, -0.7864451523456505, -0.5106679506663584],
+                    [-0.9541836471144971, -0.27451273343735705, -0.11906438073591856],
+                    [0.43775029208027166, -0.8876694056416434, 0.1428905457734996],
+                    [-0.44907802104219113, -0.8139693295211834, 0.36848726113078095],
+                    [0.47666884531973414, 0.3192753380458682, -0.8190543757390276],
+                    [-0.02194389227182042, 0.8818743114784301, -0.47097363444932655],
+                    [0.786615210967289, -0.1684579992437276, 0.5940188653281036],
+                    [0.42244725439759406, -0.5580931680624308, -0.7141920841160128],
+                    [0.6792359217126083, 0.7269051927489542, 0.10122945919953094],
+                    [-0.3904321861768485, 0.03454064212408624, -0.919983506394991],
+                    [-0.3009524704632311, 0.8792870632871999, 0.36916374531886265],
+                    [-0.006080463353741372, -0.38604914872328944, 0.9224581739761577],
+                    [0.9742753958166932, -0.0853848807263007, -0.20855904499585115],
+                    [-0.8068074072061289, 0.5272347230598002, -0.2666183686046023],
+                    [-0.7266826985737691, 0.12599246204091846, 0.675320779409617]])
+    with Helmholtz_HIP(V, T, Vm) as H:
+        H.UseDiagonal(True)
+        F = H.FarField([5 * np.pi], 1, inc, 16, cg_tol=1e-5, gmres_tol=1e-4)
+    norm_py = np.sqrt(2 * np.pi / (Vm.shape[0] * 16)) * np.linalg.norm(F)
+    assert abs(norm_cpp - norm_py) < 1e-6 * norm_py
