@@ -7,7 +7,9 @@ utilities.  Importing the package does not load the GPU library; creating a
 :class:`Helmholtz_HIP` does, and fails loudly if it is missing.
 """
 from ._lib import BaemmError, LIB_PATH
-from .helmholtz import Helmholtz_HIP, WaveType
+from .helmholtz import (ApplyBoundaryOperatorsBatch, BatchAutotune, BatchStats, BatchTuning, Helmholtz_HIP,
+                        WaveType)
 from . import meshes
 
-__all__ = ["Helmholtz_HIP", "WaveType", "BaemmError", "LIB_PATH", "meshes"]
+__all__ = ["Helmholtz_HIP", "WaveType", "BaemmError", "LIB_PATH", "meshes", "ApplyBoundaryOperatorsBatch",
+           "BatchAutotune", "BatchTuning", "BatchStats"]

@@ -57,6 +57,11 @@ SIGNATURES = {
     "baemm_create_incident_wave": (_INT, [_P, _P, _P, _I64, _P, _P, _I64, _INT]),
     "baemm_apply_boundary_dev": (_INT, [_P, _P, _P, _I64, _INT, _P, _P, _I64, _INT, _P]),
     "baemm_apply_far_field_dev": (_INT, [_P, _P, _P, _I64, _INT, _P, _P, _I64, _INT, _P]),
+    "baemm_apply_boundary_batch": (_INT, [_INT, _P, _P, _P, _P, _INT, _P, _P, _P, _INT, _P]),
+    "baemm_batch_set_tuning": (_INT, [_INT, _INT, _INT, _INT]),
+    "baemm_batch_get_tuning": (_INT, [ctypes.POINTER(_INT)] * 4),
+    "baemm_batch_get_stats": (_INT, [ctypes.POINTER(_I64)] * 4),
+    "baemm_batch_autotune": (_INT, [_INT, _P, _P, _P, _INT, _P, _P, _INT, _INT, ctypes.POINTER(_D)]),
     "baemm_apply_mass_inverse": (_INT, [_P, _P, _I64, _P, _I64, _I64, _D]),
     "baemm_apply_mass_inverse_complex": (_INT, [_P, _P, _I64, _P, _I64, _I64, _D]),
     "baemm_apply_lumped_mass_inverse": (_INT, [_P, _P, _I64, _P, _I64, _I64]),

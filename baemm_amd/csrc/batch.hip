@@ -1,0 +1,342 @@
+// batch.hip -- boundary-operator applies of several instances (meshes of
+// uneven triangle counts, wave numbers and RHS widths) in ONE persistent
+// launch per arithmetic mode.
+//
+// Each instance is prepared as in the single-instance path (Av gather and,
+// for the MFMA path, the scaled hi/lo operand split), then a work list of
+// items (instance, target tile, source range, chunk group) is built on the
+// host, ordered by decreasing cost, and walked by a persistent grid: every
+// block takes the next item from a device counter (one vector atomic per
+// item) until the list is exhausted, which every block reaches.  The partial
+// sums of an item go to its instance's segment slice; the per-instance vertex
+// epilogues follow.  Results are bit-identical to the single-instance path
+// for the same segment boundaries, since every item computes exactly what the
+// corresponding block of the single launch computes.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "batch.hpp"
+#include "ops.hpp"
+
+namespace baemm {
+
+struct BatchInst {
+    const float4* mid;
+    const float4* nrm;
+    const uint4* bop;     // mixed: split operand
+    const float2* b;      // plain: staged sources
+    float* part;
+    int64_t seg_stride;
+    int ldp, ldb;
+    int nt, nt_pad, nst;
+    int coef_off;         // first chunk coefficient of the instance in the coefficient table
+};
+
+struct BatchItem {
+    int inst;
+    int tile;
+    int s0, s1;           // mixed: 8-source steps; plain: sources (multiples of 64)
+    int q0;
+    int variant;          // kernel variant id (see the switches below)
+    int seg;
+    int pad;
+};
+
+// ----------------------------------------------------------------- MFMA path
+// Variants: (WCSP, KG) in {(16,1),(16,2),(32,1),(32,2),(64,1)} x ADJ.
+static int mixed_variant(int WCSP, int KG, bool adj)
+{
+    int v = -1;
+    if (WCSP == 16) v = KG == 1 ? 0 : (KG == 2 ? 1 : -1);
+    if (WCSP == 32) v = KG == 1 ? 2 : (KG == 2 ? 3 : -1);
+    if (WCSP == 64 && KG == 1) v = 4;
+    if (v < 0) throw ArgError{"internal: no batched mixed variant"};
+    return 2 * v + (adj ? 1 : 0);
+}
+
+// Largest LDS need over the variants (all have 4 column blocks or fewer).
+constexpr int kMixOpMax = MixedLds<32, 2>::OP_ELEMS;
+constexpr int kMixGeoMax = MixedLds<32, 2>::GEO_ELEMS;
+static_assert(MixedLds<64, 1>::OP_ELEMS <= kMixOpMax && MixedLds<16, 2>::OP_ELEMS <= kMixOpMax, "LDS bound");
+
+__global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_batch_kernel(
+    const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items, const ChunkCoef* __restrict__ coefs,
+    int n_items, unsigned* __restrict__ counter)
+{
+    __shared__ uint4 s_op[kMixOpMax];
+    __shared__ float4 s_geo[kMixGeoMax];
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const BatchItem w = items[it];
+        const BatchInst in = insts[w.inst];
+        const ChunkCoef* cf = coefs + in.coef_off + w.q0;
+        float* out = in.part + (int64_t)w.seg * in.seg_stride;
+#define BAEMM_MIXB(ID, WCSP_, KG_, ADJ_)                                                                         \
+    case ID: {                                                                                                  \
+        CoefPack cp;                                                                                            \
+        for (int g = 0; g < KG_; ++g) cp.c[g] = cf[g];                                                          \
+        mixed_block<WCSP_, KG_, ADJ_>(in.mid, in.nrm, in.nt, in.bop, in.nst, cp, out, in.ldp, w.q0, w.tile, w.s0, \
+                                      w.s1, s_op, s_geo);                                                       \
+        break;                                                                                                  \
+    }
+        switch (w.variant) {
+            BAEMM_MIXB(0, 16, 1, false)
+            BAEMM_MIXB(1, 16, 1, true)
+            BAEMM_MIXB(2, 16, 2, false)
+            BAEMM_MIXB(3, 16, 2, true)
+            BAEMM_MIXB(4, 32, 1, false)
+            BAEMM_MIXB(5, 32, 1, true)
+            BAEMM_MIXB(6, 32, 2, false)
+            BAEMM_MIXB(7, 32, 2, true)
+            BAEMM_MIXB(8, 64, 1, false)
+            BAEMM_MIXB(9, 64, 1, true)
+            default: break;
+        }
+#undef BAEMM_MIXB
+    }
+}
+
+// ----------------------------------------------------------------- FP32 path
+// Variants: WCSP in {4, 8, 16, 32, 64} with KG = 1, x ADJ.
+static int plain_variant(int WCSP, bool adj)
+{
+    int v = -1;
+    switch (WCSP) {
+        case 4: v = 0; break;
+        case 8: v = 1; break;
+        case 16: v = 2; break;
+        case 32: v = 3; break;
+        case 64: v = 4; break;
+        default: break;
+    }
+    if (v < 0) throw ArgError{"internal: no batched plain variant"};
+    return 2 * v + (adj ? 1 : 0);
+}
+
+__global__ __launch_bounds__(kTile) void bdr_plain_batch_kernel(const BatchInst* __restrict__ insts,
+                                                                const BatchItem* __restrict__ items,
+                                                                const ChunkCoef* __restrict__ coefs, int n_items,
+                                                                unsigned* __restrict__ counter)
+{
+    __shared__ float4 s_y[kSrcBlock];
+    __shared__ float4 s_n[kSrcBlock];
+    __shared__ float2 s_b[kSrcBlock * 64];
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const BatchItem w = items[it];
+        const BatchInst in = insts[w.inst];
+        CoefPack cp;
+        cp.c[0] = coefs[in.coef_off + w.q0];
+        float* out = in.part + (int64_t)w.seg * in.seg_stride;
+#define BAEMM_PLB(ID, WCSP_, ADJ_)                                                                              \
+    case ID:                                                                                                   \
+        plain_block<OP_BDR, WCSP_, 1, ADJ_>(in.mid, in.nrm, in.nt, in.mid, in.nrm, in.nt, in.b, in.ldb, cp, out, \
+                                            in.ldp, w.q0, w.tile, w.s0, w.s1, s_y, s_n, s_b);                  \
+        break;
+        switch (w.variant) {
+            BAEMM_PLB(0, 4, false)
+            BAEMM_PLB(1, 4, true)
+            BAEMM_PLB(2, 8, false)
+            BAEMM_PLB(3, 8, true)
+            BAEMM_PLB(4, 16, false)
+            BAEMM_PLB(5, 16, true)
+            BAEMM_PLB(6, 32, false)
+            BAEMM_PLB(7, 32, true)
+            BAEMM_PLB(8, 64, false)
+            BAEMM_PLB(9, 64, true)
+            default: break;
+        }
+#undef BAEMM_PLB
+    }
+}
+
+__global__ void batch_zero_kernel(unsigned* p) { p[threadIdx.x] = 0u; }
+
+// ----------------------------------------------------------------- host
+
+namespace {
+
+struct Prepared {
+    BdrPlan pl;
+};
+
+int persistent_grid(const void* fn, int block, int n_items, int grid_mult)
+{
+    int dev = 0, cus = 256, per_cu = 1;
+    BAEMM_HIP(hipGetDevice(&dev));
+    BAEMM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BAEMM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0));
+    per_cu = std::max(1, per_cu);
+    const int mult = grid_mult > 0 ? std::min(grid_mult, per_cu) : per_cu;
+    return std::max(1, std::min(n_items, cus * mult));
+}
+
+}  // namespace
+
+BatchTuning& batch_tuning()
+{
+    static BatchTuning t;
+    return t;
+}
+
+void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipStream_t s, BatchStats* stats)
+{
+    const BatchTuning tune = batch_tuning();
+    const int n = (int)es.size();
+    std::vector<Prepared> prep(n);
+    for (int k = 0; k < n; ++k) bdr_begin(*es[k].h, es[k].a, prep[k].pl, s);
+
+    for (int mode = 0; mode < 2; ++mode) {     // 0: MFMA instances, 1: FP32 instances
+        const bool mixed = (mode == 0);
+        std::vector<BatchInst> insts;
+        std::vector<ChunkCoef> coefs;
+        std::vector<BatchItem> items;
+        std::vector<double> cost;
+        std::vector<int> owner;
+        for (int k = 0; k < n; ++k) {
+            BdrPlan& pl = prep[k].pl;
+            if (!pl.kernel_on || pl.mixed != mixed) continue;
+            Handle& h = *es[k].h;
+            const Params& p = h.prm;
+            BatchInst in{};
+            in.mid = h.d_mid;
+            in.nrm = h.d_nrm;
+            in.bop = pl.bop;
+            in.b = pl.b;
+            in.ldp = pl.ldp;
+            in.ldb = pl.ldb;
+            in.nt = (int)h.nt;
+            in.nt_pad = (int)h.nt_pad;
+            in.nst = (int)(h.nt_pad / 8);
+            in.coef_off = (int)coefs.size();
+            for (int q = 0; q < p.K; ++q)
+                coefs.push_back(make_pack(p, q, 1, mixed ? pl.sA.data() : nullptr).c[0]);
+            // source ranges of equal length across the batch
+            int len, S, tiles, nsrc_units;
+            if (mixed) {
+                len = std::max(8, tune.mixed_steps);
+                nsrc_units = in.nst;
+                tiles = in.nt_pad / kMixTile;
+            } else {
+                len = std::max(1, tune.plain_blocks) * kSrcBlock;
+                nsrc_units = in.nt_pad;
+                tiles = in.nt_pad / kTile;
+            }
+            S = (nsrc_units + len - 1) / len;
+            in.part = bdr_reserve_part(h, pl, S);
+            in.seg_stride = pl.ea.seg_stride;
+            const int inst_id = (int)insts.size();
+            insts.push_back(in);
+            owner.push_back(k);
+            for (int q0 = 0; q0 < p.K;) {
+                int kg = 1;
+                if (mixed) {
+                    const int maxkg = std::max(1, kMaxCols / pl.WCSP);
+                    const int lim = std::min({maxkg, (int)p.K - q0, 2});
+                    while (kg * 2 <= lim) kg *= 2;
+                }
+                const int var = mixed ? mixed_variant(pl.WCSP, kg, pl.adj) : plain_variant(pl.WCSP, pl.adj);
+                for (int seg = 0; seg < S; ++seg) {
+                    const int s0 = seg * len, s1 = std::min(s0 + len, nsrc_units);
+                    // cost model: kernel-value generation per chunk plus contraction per column
+                    const double per = mixed ? (double)(s1 - s0) * 8.0 * (kg * (1.0 + 0.35 * pl.WCSP / 32.0))
+                                             : (double)(s1 - s0) * (kg * (1.0 + 0.12 * pl.WCSP));
+                    for (int t = 0; t < tiles; ++t) {
+                        items.push_back(BatchItem{inst_id, t, s0, s1, q0, var, seg, 0});
+                        cost.push_back(per);
+                    }
+                }
+                q0 += kg;
+            }
+        }
+        if (items.empty()) continue;
+        // Longest items first (a greedy schedule for the dynamic queue); the
+        // order inside equal costs keeps tiles of one source range together.
+        std::vector<int> order(items.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+        std::vector<BatchItem> sorted(items.size());
+        for (size_t k = 0; k < order.size(); ++k) sorted[k] = items[order[k]];
+
+        const size_t b_insts = sizeof(BatchInst) * insts.size();
+        const size_t b_coefs = sizeof(ChunkCoef) * coefs.size();
+        const size_t b_items = sizeof(BatchItem) * sorted.size();
+        const size_t off_coefs = (b_insts + 255) / 256 * 256;
+        const size_t off_items = off_coefs + (b_coefs + 255) / 256 * 256;
+        const size_t off_ctr = off_items + (b_items + 255) / 256 * 256;
+        const size_t total = off_ctr + 256;
+        bw.dev[mode].reserve(total);
+        // pinned staging; wait until the previous upload from it has completed
+        if (bw.host_bytes[mode] < total) {
+            if (bw.host[mode]) {
+                if (bw.ev[mode]) BAEMM_HIP(hipEventSynchronize(bw.ev[mode]));
+                BAEMM_HIP(hipHostFree(bw.host[mode]));
+                bw.host[mode] = nullptr;
+            }
+            BAEMM_HIP(hipHostMalloc(&bw.host[mode], total, hipHostMallocDefault));
+            bw.host_bytes[mode] = total;
+        }
+        if (!bw.ev[mode]) BAEMM_HIP(hipEventCreateWithFlags(&bw.ev[mode], hipEventDisableTiming));
+        BAEMM_HIP(hipEventSynchronize(bw.ev[mode]));
+        char* hp = static_cast<char*>(bw.host[mode]);
+        std::memcpy(hp, insts.data(), b_insts);
+        std::memcpy(hp + off_coefs, coefs.data(), b_coefs);
+        std::memcpy(hp + off_items, sorted.data(), b_items);
+        char* dp = bw.dev[mode].as<char>();
+        BAEMM_HIP(hipMemcpyAsync(dp, hp, off_ctr, hipMemcpyHostToDevice, s));
+        BAEMM_HIP(hipEventRecord(bw.ev[mode], s));
+        unsigned* ctr = reinterpret_cast<unsigned*>(dp + off_ctr);
+        hipLaunchKernelGGL(batch_zero_kernel, dim3(1), dim3(64), 0, s, ctr);
+        BAEMM_HIP(hipGetLastError());
+
+        const void* fn = mixed ? (const void*)bdr_mixed_batch_kernel : (const void*)bdr_plain_batch_kernel;
+        const int block = mixed ? 256 : kTile;
+        const int grid = persistent_grid(fn, block, (int)sorted.size(), mixed ? tune.mixed_grid_mult : tune.plain_grid_mult);
+        Handle& h0 = *es[owner[0]].h;
+        Timed tm(h0, s);
+        if (mixed)
+            hipLaunchKernelGGL(bdr_mixed_batch_kernel, dim3(grid), dim3(block), 0, s,
+                               reinterpret_cast<const BatchInst*>(dp), reinterpret_cast<const BatchItem*>(dp + off_items),
+                               reinterpret_cast<const ChunkCoef*>(dp + off_coefs), (int)sorted.size(), ctr);
+        else
+            hipLaunchKernelGGL(bdr_plain_batch_kernel, dim3(grid), dim3(block), 0, s,
+                               reinterpret_cast<const BatchInst*>(dp), reinterpret_cast<const BatchItem*>(dp + off_items),
+                               reinterpret_cast<const ChunkCoef*>(dp + off_coefs), (int)sorted.size(), ctr);
+        BAEMM_HIP(hipGetLastError());
+        tm.done();
+        if (stats) {
+            stats->items[mode] = (int64_t)sorted.size();
+            stats->grid[mode] = grid;
+            stats->instances[mode] = (int64_t)insts.size();
+        }
+    }
+    for (int k = 0; k < n; ++k) bdr_end(*es[k].h, es[k].a, prep[k].pl, s);
+}
+
+void BatchWorkspace::release()
+{
+    for (int m = 0; m < 2; ++m) {
+        if (ev[m]) {
+            (void)hipEventSynchronize(ev[m]);
+            (void)hipEventDestroy(ev[m]);
+            ev[m] = nullptr;
+        }
+        if (host[m]) (void)hipHostFree(host[m]);
+        host[m] = nullptr;
+        host_bytes[m] = 0;
+        dev[m].release();
+    }
+}
+
+}  // namespace baemm

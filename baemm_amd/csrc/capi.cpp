@@ -11,8 +11,9 @@ using namespace baemm;
 
 namespace {
 
-// Message of the last failed baemm_create* on this thread (there is no handle
-// to carry it); returned by baemm_last_error(NULL).
+// Message of the last failed call on this thread that has no single handle
+// to carry it (baemm_create*, the batched entry points); returned by
+// baemm_last_error(NULL).
 thread_local std::string g_create_error;
 
 template <class I>
@@ -113,6 +114,10 @@ void host_apply(Handle& h, const double* alpha, const double* B, int64_t ldB, in
 }
 
 }  // namespace
+
+namespace baemm {
+void set_thread_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace baemm
 
 extern "C" {
 

@@ -15,6 +15,9 @@ struct StateError {
 
 inline Handle* H(baemm_handle h) { return reinterpret_cast<Handle*>(h); }
 
+// Message for baemm_last_error(NULL) on the calling thread.
+void set_thread_error(const std::string& msg);
+
 template <class F>
 inline int guard(baemm_handle h, F&& f)
 {

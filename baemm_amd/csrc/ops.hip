@@ -4,7 +4,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "pair_kernels.hpp"
+#include "ops.hpp"
 
 namespace baemm {
 
@@ -34,10 +34,10 @@ static inline int next_pow2(int x)
 }
 
 // Padded chunk width of the plain kernels (instantiated 4..64).
-static inline int wcsp_plain(int wcs) { return std::max(4, next_pow2(wcs)); }
+int wcsp_plain(int wcs) { return std::max(4, next_pow2(wcs)); }
 // Padded chunk width of the MFMA kernel (16, 32 or 64: 2*WCSP real columns
 // must fill 32-column MFMA blocks).
-static inline int wcsp_mixed(int wcs) { return std::max(16, next_pow2(wcs)); }
+int wcsp_mixed(int wcs) { return std::max(16, next_pow2(wcs)); }
 
 // --------------------------------------------------------------- staging
 
@@ -211,30 +211,6 @@ __global__ void rmin_kernel(const float4* __restrict__ mid, int nt, unsigned* __
 
 // --------------------------------------------------------------- epilogues
 
-struct EpiArgs {
-    int nv, W, wcs, WCSP, K;
-    int kernel_on, diag_on, mass_on;
-    double ar, ai, br, bi;
-    int S;
-    int64_t seg_stride;
-    int ldp, ldb;
-    const int32_t* vt_ptr;
-    const int32_t* vt_idx;
-    const float* area;
-    const float* sdiag;
-    const float* part;
-    const float2* b;
-    const float* unscale;
-    const int32_t* M_ptr;
-    const int32_t* M_idx;
-    const double* M_val;
-    const void* B;
-    int64_t ldB;
-    void* C;
-    int64_t ldC;
-    float c1r[64], c1i[64], kap[64];
-    double c0r[64], c0i[64];
-};
 
 // C[v][k] = alpha * ( sum_{t ni v} a_t/3 (sum_s part_s[t][k] * unscale + D_t b[t][k])
 //                     + c0_q (M B)[v][k] ) + beta * C[v][k]
@@ -379,7 +355,7 @@ __global__ void incident_kernel(IncArgs a)
 
 // --------------------------------------------------------------- dispatch
 
-static CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk)
+CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk)
 {
     CoefPack cp{};
     for (int g = 0; g < kg; ++g) {
@@ -439,24 +415,6 @@ static int pick_kg(int WCSP, int remaining, bool mixed)
     return kg;
 }
 
-struct Timed {
-    Handle& h;
-    hipStream_t s;
-    Timed(Handle& h_, hipStream_t s_) : h(h_), s(s_)
-    {
-        if (h.timing) BAEMM_HIP(hipEventRecord(h.ev0, s));
-    }
-    void done()
-    {
-        if (!h.timing) return;
-        BAEMM_HIP(hipEventRecord(h.ev1, s));
-        BAEMM_HIP(hipEventSynchronize(h.ev1));
-        float ms = 0.0f;
-        BAEMM_HIP(hipEventElapsedTime(&ms, h.ev0, h.ev1));
-        h.timed_ms += ms;
-        h.timed_launches += 1;
-    }
-};
 
 // Number of source segments: enough work-groups to fill 256 CUs several times.
 static int choose_segments(const Handle& h, int tiles, int max_seg)
@@ -568,99 +526,124 @@ static void launch_av(Handle& h, const void* B, int64_t ldB, int Btype, int WCSP
     BAEMM_HIP(hipGetLastError());
 }
 
-void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
+void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
 {
     const Params& p = h.prm;
     if (!p.loaded) throw ArgError{"operators not loaded"};
-    const bool kernel_on = p.kernel_on();
-    const bool mass_on = p.mass_on();
-    if (!kernel_on && !mass_on) return;   // the reference leaves C_out untouched
-    const bool mixed = (h.precision == BAEMM_PRECISION_MIXED) && kernel_on;
-    const int WCSP = mixed ? wcsp_mixed((int)p.wcs) : wcsp_plain((int)p.wcs);
-    const int ldb = (int)p.K * WCSP;
-    const int ldp = 2 * ldb;
-    EpiArgs ea = base_epi(h, a, WCSP);
-    ea.mass_on = mass_on;
-    ea.kernel_on = kernel_on;
-    ea.diag_on = kernel_on && h.use_diagonal && p.any(1);
-    if (kernel_on) {
-        const bool adj = p.any(3);
-        h.ws_b32.reserve(sizeof(float2) * h.nt_pad * ldb);
-        float2* b = h.ws_b32.as<float2>();
-        launch_av(h, a.B, a.ldB, a.Btype, WCSP, b, ldb, s);
-        if (!mixed) {
-            const int tiles = (int)(h.nt_pad / kTile);
-            const int max_seg = (int)(h.nt_pad / kSrcBlock);
-            const int S0 = choose_segments(h, tiles, max_seg);
-            const int seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
-            const int S = (int)((h.nt_pad + seg_len - 1) / seg_len);
-            const int64_t seg_stride = h.nt_pad * (int64_t)ldp;
-            h.ws_part.reserve(sizeof(float) * seg_stride * S);
-            run_plain(h, OP_BDR, adj, WCSP, h.d_mid, h.d_nrm, (int)h.nt, (int)h.nt_pad, h.d_mid, h.d_nrm,
-                      (int)h.nt, (int)h.nt_pad, b, ldb, h.ws_part.as<float>(), seg_stride, ldp, S, seg_len, s);
-            ea.S = S;
-            ea.seg_stride = seg_stride;
-            ea.unscale = nullptr;
-        } else {
-            // scales: A per chunk from a bound at the minimum centroid distance,
-            // b per column from its maximum.
-            const float rmin = compute_rmin(h, s);
-            std::vector<float> sA(p.K);
-            ChunkScales csc{};
-            for (int q = 0; q < p.K; ++q) {
-                const double c1 = std::abs(std::complex<double>(p.c[4 * q + 1]));
-                const double c23 = std::abs(std::complex<double>(p.c[4 * q + 2])) +
-                                   std::abs(std::complex<double>(p.c[4 * q + 3]));
-                const double kr = (double)std::fabs(p.kappa[q]) * rmin;
-                const double amax = c1 / rmin + c23 * (1.0 + kr) / ((double)rmin * rmin);
-                int e = 0;
-                std::frexp(std::max(amax, 1e-30), &e);
-                sA[q] = (float)std::ldexp(1.0, 14 - e);
-                csc.sA[q] = sA[q];
-            }
-            const int ncols = ldb;
-            h.ws_scale.reserve(sizeof(float) * 3 * ncols);
-            unsigned* colmax = h.ws_scale.as<unsigned>();
-            float* scale = reinterpret_cast<float*>(colmax + ncols);
-            float* unscale = scale + ncols;
-            hipLaunchKernelGGL(zero_u32_kernel, dim3((ncols + 255) / 256), dim3(256), 0, s, colmax, ncols);
-            hipLaunchKernelGGL(colmax_kernel, dim3(256), dim3(256), 0, s, b, (int)h.nt, ldb, colmax);
-            hipLaunchKernelGGL(colscale_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, colmax, ncols, WCSP, csc,
-                               scale, unscale);
-            const int nst = (int)(h.nt_pad / 8);
-            const int NCB = 2 * WCSP / 32;
-            const int64_t nbop = (int64_t)p.K * nst * NCB * 64;
-            h.ws_bop.reserve(sizeof(uint4) * nbop * 2);
-            hipLaunchKernelGGL(split_kernel, dim3((unsigned)((nbop + 255) / 256)), dim3(256), 0, s, b, ldb, nst, WCSP,
-                               (int)p.K, scale, h.ws_bop.as<uint4>());
-            BAEMM_HIP(hipGetLastError());
-            const int tiles = (int)(h.nt_pad / kMixTile);
-            const int S0 = choose_segments(h, tiles, nst / 8);
-            const int steps = (nst + S0 - 1) / S0;
-            const int S = (nst + steps - 1) / steps;
-            const int64_t seg_stride = h.nt_pad * (int64_t)ldp;
-            h.ws_part.reserve(sizeof(float) * seg_stride * S);
-            Timed tm(h, s);
-            for (int q0 = 0; q0 < p.K;) {
-                const int kg = pick_kg(WCSP, (int)p.K - q0, true);
-                const CoefPack cp = make_pack(p, q0, kg, sA.data());
-                MixedFn fn = mixed_fn(adj, WCSP, kg);
-                hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(256), 0, s, h.d_mid, h.d_nrm, (int)h.nt,
-                                   h.ws_bop.as<uint4>(), nst, cp, h.ws_part.as<float>(), seg_stride, ldp, q0, steps);
-                BAEMM_HIP(hipGetLastError());
-                q0 += kg;
-            }
-            tm.done();
-            ea.S = S;
-            ea.seg_stride = seg_stride;
-            ea.unscale = unscale;
-        }
-        ea.part = h.ws_part.as<float>();
-        ea.b = b;
-        ea.ldp = ldp;
-        ea.ldb = ldb;
+    pl.kernel_on = p.kernel_on();
+    pl.mass_on = p.mass_on();
+    if (!pl.kernel_on && !pl.mass_on) return;   // the reference leaves C_out untouched
+    pl.mixed = (h.precision == BAEMM_PRECISION_MIXED) && pl.kernel_on;
+    pl.WCSP = pl.mixed ? wcsp_mixed((int)p.wcs) : wcsp_plain((int)p.wcs);
+    pl.ldb = (int)p.K * pl.WCSP;
+    pl.ldp = 2 * pl.ldb;
+    pl.adj = p.any(3);
+    EpiArgs& ea = pl.ea;
+    ea = base_epi(h, a, pl.WCSP);
+    ea.mass_on = pl.mass_on;
+    ea.kernel_on = pl.kernel_on;
+    ea.diag_on = pl.kernel_on && h.use_diagonal && p.any(1);
+    if (!pl.kernel_on) return;
+    h.ws_b32.reserve(sizeof(float2) * h.nt_pad * pl.ldb);
+    pl.b = h.ws_b32.as<float2>();
+    launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
+    ea.b = pl.b;
+    ea.ldp = pl.ldp;
+    ea.ldb = pl.ldb;
+    ea.unscale = nullptr;
+    if (!pl.mixed) return;
+    // scales: A per chunk from a bound at the minimum centroid distance,
+    // b per column from its maximum.
+    const float rmin = compute_rmin(h, s);
+    pl.sA.assign(p.K, 1.0f);
+    ChunkScales csc{};
+    for (int q = 0; q < p.K; ++q) {
+        const double c1 = std::abs(std::complex<double>(p.c[4 * q + 1]));
+        const double c23 =
+            std::abs(std::complex<double>(p.c[4 * q + 2])) + std::abs(std::complex<double>(p.c[4 * q + 3]));
+        const double kr = (double)std::fabs(p.kappa[q]) * rmin;
+        const double amax = c1 / rmin + c23 * (1.0 + kr) / ((double)rmin * rmin);
+        int e = 0;
+        std::frexp(std::max(amax, 1e-30), &e);
+        pl.sA[q] = (float)std::ldexp(1.0, 14 - e);
+        csc.sA[q] = pl.sA[q];
     }
-    bdr_epilogue(h, ea, a.Btype, a.Ctype, s);
+    const int ncols = pl.ldb;
+    h.ws_scale.reserve(sizeof(float) * 3 * ncols);
+    unsigned* colmax = h.ws_scale.as<unsigned>();
+    float* scale = reinterpret_cast<float*>(colmax + ncols);
+    float* unscale = scale + ncols;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((ncols + 255) / 256), dim3(256), 0, s, colmax, ncols);
+    hipLaunchKernelGGL(colmax_kernel, dim3(256), dim3(256), 0, s, pl.b, (int)h.nt, pl.ldb, colmax);
+    hipLaunchKernelGGL(colscale_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, colmax, ncols, pl.WCSP, csc, scale,
+                       unscale);
+    pl.nst = (int)(h.nt_pad / 8);
+    const int NCB = 2 * pl.WCSP / 32;
+    const int64_t nbop = (int64_t)p.K * pl.nst * NCB * 64;
+    h.ws_bop.reserve(sizeof(uint4) * nbop * 2);
+    hipLaunchKernelGGL(split_kernel, dim3((unsigned)((nbop + 255) / 256)), dim3(256), 0, s, pl.b, pl.ldb, pl.nst,
+                       pl.WCSP, (int)p.K, scale, h.ws_bop.as<uint4>());
+    BAEMM_HIP(hipGetLastError());
+    pl.bop = h.ws_bop.as<uint4>();
+    ea.unscale = unscale;
+}
+
+float* bdr_reserve_part(Handle& h, BdrPlan& pl, int S)
+{
+    const int64_t seg_stride = h.nt_pad * (int64_t)pl.ldp;
+    h.ws_part.reserve(sizeof(float) * seg_stride * S);
+    pl.ea.S = S;
+    pl.ea.seg_stride = seg_stride;
+    pl.ea.part = h.ws_part.as<float>();
+    return h.ws_part.as<float>();
+}
+
+void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
+{
+    if (!pl.kernel_on) return;
+    const Params& p = h.prm;
+    if (!pl.mixed) {
+        const int tiles = (int)(h.nt_pad / kTile);
+        const int max_seg = (int)(h.nt_pad / kSrcBlock);
+        const int S0 = choose_segments(h, tiles, max_seg);
+        const int seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
+        const int S = (int)((h.nt_pad + seg_len - 1) / seg_len);
+        float* part = bdr_reserve_part(h, pl, S);
+        run_plain(h, OP_BDR, pl.adj, pl.WCSP, h.d_mid, h.d_nrm, (int)h.nt, (int)h.nt_pad, h.d_mid, h.d_nrm,
+                  (int)h.nt, (int)h.nt_pad, pl.b, pl.ldb, part, pl.ea.seg_stride, pl.ldp, S, seg_len, s);
+        return;
+    }
+    const int nst = pl.nst;
+    const int tiles = (int)(h.nt_pad / kMixTile);
+    const int S0 = choose_segments(h, tiles, nst / 8);
+    const int steps = (nst + S0 - 1) / S0;
+    const int S = (nst + steps - 1) / steps;
+    float* part = bdr_reserve_part(h, pl, S);
+    Timed tm(h, s);
+    for (int q0 = 0; q0 < p.K;) {
+        const int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
+        const CoefPack cp = make_pack(p, q0, kg, pl.sA.data());
+        MixedFn fn = mixed_fn(pl.adj, pl.WCSP, kg);
+        hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(256), 0, s, h.d_mid, h.d_nrm, (int)h.nt, pl.bop, nst, cp, part,
+                           pl.ea.seg_stride, pl.ldp, q0, steps);
+        BAEMM_HIP(hipGetLastError());
+        q0 += kg;
+    }
+    tm.done();
+}
+
+void bdr_end(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
+{
+    if (!pl.kernel_on && !pl.mass_on) return;
+    bdr_epilogue(h, pl.ea, a.Btype, a.Ctype, s);
+}
+
+void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
+{
+    BdrPlan pl;
+    bdr_begin(h, a, pl, s);
+    bdr_pairs(h, pl, s);
+    bdr_end(h, a, pl, s);
 }
 
 static void rows_epilogue(int nrows, const Params& p, int WCSP, int S, int64_t seg_stride, int ldp, const float* part,

@@ -1,0 +1,83 @@
+// ops.hpp -- the phases of a boundary-operator apply, shared by the
+// single-instance path (ops.hip) and the batched persistent path (batch.hip).
+#pragma once
+
+#include <vector>
+
+#include "pair_kernels.hpp"
+
+namespace baemm {
+
+// Arguments of the vertex epilogue: AvT of the summed pair partials (plus the
+// single-layer diagonal), the mass term and alpha/beta.
+struct EpiArgs {
+    int nv, W, wcs, WCSP, K;
+    int kernel_on, diag_on, mass_on;
+    double ar, ai, br, bi;
+    int S;
+    int64_t seg_stride;
+    int ldp, ldb;
+    const int32_t* vt_ptr;
+    const int32_t* vt_idx;
+    const float* area;
+    const float* sdiag;
+    const float* part;
+    const float2* b;
+    const float* unscale;
+    const int32_t* M_ptr;
+    const int32_t* M_idx;
+    const double* M_val;
+    const void* B;
+    int64_t ldB;
+    void* C;
+    int64_t ldC;
+    float c1r[64], c1i[64], kap[64];
+    double c0r[64], c0i[64];
+};
+
+// HIP events around the dominant pair-kernel launches of an apply, on the
+// stream they run on (baemm_set_timing).
+struct Timed {
+    Handle& h;
+    hipStream_t s;
+    Timed(Handle& h_, hipStream_t s_) : h(h_), s(s_)
+    {
+        if (h.timing) BAEMM_HIP(hipEventRecord(h.ev0, s));
+    }
+    void done()
+    {
+        if (!h.timing) return;
+        BAEMM_HIP(hipEventRecord(h.ev1, s));
+        BAEMM_HIP(hipEventSynchronize(h.ev1));
+        float ms = 0.0f;
+        BAEMM_HIP(hipEventElapsedTime(&ms, h.ev0, h.ev1));
+        h.timed_ms += ms;
+        h.timed_launches += 1;
+    }
+};
+
+// State of one apply between its phases.
+struct BdrPlan {
+    bool kernel_on = false, mass_on = false, mixed = false, adj = false;
+    int WCSP = 0, ldb = 0, ldp = 0;
+    int nst = 0;                 // 8-source steps (mixed) of the padded source range
+    std::vector<float> sA;       // per-chunk power-of-two scale of A (mixed)
+    float2* b = nullptr;         // staged sources b = Av B
+    const uint4* bop = nullptr;  // split MFMA operand (mixed)
+    EpiArgs ea{};
+};
+
+// Phase 1: parameters, Av gather and (mixed) the scaled hi/lo operand split.
+void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s);
+// Reserves the partial buffer for S segment slices and records it in pl.ea.
+float* bdr_reserve_part(Handle& h, BdrPlan& pl, int S);
+// Phase 2 (single instance): the pair kernel launches.
+void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s);
+// Phase 3: the vertex epilogue.
+void bdr_end(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s);
+
+int wcsp_plain(int wcs);
+int wcsp_mixed(int wcs);
+CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk);
+
+}  // namespace baemm

@@ -98,20 +98,21 @@ __device__ __forceinline__ void plane_value(float uy, float un, const ChunkCoef&
 // Plain FP32 pair kernel: one target per thread, WCSP*KG complex accumulators,
 // sources staged per 64 in LDS and broadcast to the whole block.
 // ---------------------------------------------------------------------------
+// One block of the plain kernel: targets [tile*256, tile*256+256) against the
+// sources [s0, s1) (multiples of 64) of chunks q0..q0+KG-1; partial sums go to
+// out[i*ldp + ...] (one segment slice).  s_y, s_n hold kSrcBlock float4 each,
+// s_b kSrcBlock*WCSP*KG float2.  Ends with a barrier so the caller may reuse
+// the LDS.
 template <int OP, int WCSP, int KG, bool ADJ>
-__global__ __launch_bounds__(kTile) void pair_plain_kernel(
-    const float4* __restrict__ tx, const float4* __restrict__ tn, int ntgt,
-    const float4* __restrict__ sy, const float4* __restrict__ sn_, int nsrc,
-    const float2* __restrict__ b, int ldb, CoefPack cp,
-    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int seg_len, int nsrc_pad)
+__device__ __forceinline__ void plain_block(const float4* __restrict__ tx, const float4* __restrict__ tn, int ntgt,
+                                            const float4* __restrict__ sy, const float4* __restrict__ sn_, int nsrc,
+                                            const float2* __restrict__ b, int ldb, const CoefPack& cp,
+                                            float* __restrict__ outp, int ldp, int q0, int tile, int s0, int s1,
+                                            float4* s_y, float4* s_n, float2* s_b)
 {
     constexpr int NC = WCSP * KG;
-    __shared__ float4 s_y[kSrcBlock];
-    __shared__ float4 s_n[kSrcBlock];
-    __shared__ float2 s_b[kSrcBlock * NC];
-
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * kTile + tid;
+    const int i = tile * kTile + tid;
     const float4 x = tx[i];
     float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
     if constexpr (OP == OP_BDR || OP == OP_HERG) ni = tn[i];
@@ -120,8 +121,6 @@ __global__ __launch_bounds__(kTile) void pair_plain_kernel(
 #pragma unroll
     for (int k = 0; k < NC; ++k) accr[k] = acci[k] = 0.0f;
 
-    const int s0 = blockIdx.y * seg_len;
-    const int s1 = min(s0 + seg_len, nsrc_pad);
     for (int jb = s0; jb < s1; jb += kSrcBlock) {
         __syncthreads();
         if (tid < kSrcBlock) s_y[tid] = sy[jb + tid];
@@ -177,16 +176,34 @@ __global__ __launch_bounds__(kTile) void pair_plain_kernel(
             }
         }
     }
-    if (i >= ntgt) return;
-    float* out = part + (int64_t)blockIdx.y * seg_stride + (int64_t)i * ldp + q0 * 2 * WCSP;
+    if (i < ntgt) {
+        float* out = outp + (int64_t)i * ldp + q0 * 2 * WCSP;
 #pragma unroll
-    for (int q = 0; q < KG; ++q) {
+        for (int q = 0; q < KG; ++q) {
 #pragma unroll
-        for (int k = 0; k < WCSP; ++k) {
-            out[q * 2 * WCSP + k] = accr[q * WCSP + k];
-            out[q * 2 * WCSP + WCSP + k] = acci[q * WCSP + k];
+            for (int k = 0; k < WCSP; ++k) {
+                out[q * 2 * WCSP + k] = accr[q * WCSP + k];
+                out[q * 2 * WCSP + WCSP + k] = acci[q * WCSP + k];
+            }
         }
     }
+    __syncthreads();
+}
+
+template <int OP, int WCSP, int KG, bool ADJ>
+__global__ __launch_bounds__(kTile) void pair_plain_kernel(
+    const float4* __restrict__ tx, const float4* __restrict__ tn, int ntgt,
+    const float4* __restrict__ sy, const float4* __restrict__ sn_, int nsrc,
+    const float2* __restrict__ b, int ldb, CoefPack cp,
+    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int seg_len, int nsrc_pad)
+{
+    __shared__ float4 s_y[kSrcBlock];
+    __shared__ float4 s_n[kSrcBlock];
+    __shared__ float2 s_b[kSrcBlock * WCSP * KG];
+    const int s0 = blockIdx.y * seg_len;
+    const int s1 = min(s0 + seg_len, nsrc_pad);
+    plain_block<OP, WCSP, KG, ADJ>(tx, tn, ntgt, sy, sn_, nsrc, b, ldb, cp, part + (int64_t)blockIdx.y * seg_stride,
+                                   ldp, q0, blockIdx.x, s0, s1, s_y, s_n, s_b);
 }
 
 // ---------------------------------------------------------------------------
@@ -264,26 +281,36 @@ __device__ __forceinline__ void mixed_gen(const float4 x, const float4 ni, const
 #ifndef BAEMM_MIX_MINB
 #define BAEMM_MIX_MINB 3
 #endif
-template <int WCSP, int KG, bool ADJ>
-__global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
-    const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
-    const uint4* __restrict__ bop, int nst, CoefPack cp,
-    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
-{
-    constexpr int NCB = 2 * WCSP / 32;
-    constexpr int OPS = KG * NCB * 2;       // 1-KB operand tiles per step (chunk x col-block x hi/lo)
-    constexpr int NS = BAEMM_MIX_NS;        // steps (8 sources each) per LDS stage
-    constexpr int TILES = NS * OPS;
-    // Operand tiles and source geometry of one stage, double buffered; filled by
-    // global_load_lds (16 bytes per lane straight into LDS, no VGPR staging).
-    __shared__ uint4 s_op[2][TILES][64];
-    constexpr int GEO = 2 * NS * 8;         // float4 per stage: NS*8 centroids, then NS*8 normals
-    __shared__ float4 s_geo[2][GEO];
+// LDS footprint of one mixed block: operand tiles (uint4) and source geometry
+// (float4) of NS steps, double buffered.
+template <int WCSP, int KG>
+struct MixedLds {
+    static constexpr int NCB = 2 * WCSP / 32;
+    static constexpr int OPS = KG * NCB * 2;     // 1-KB operand tiles per step (chunk x col-block x hi/lo)
+    static constexpr int NS = BAEMM_MIX_NS;      // steps (8 sources each) per LDS stage
+    static constexpr int TILES = NS * OPS;
+    static constexpr int GEO = 2 * NS * 8;       // float4 per stage: NS*8 centroids, then NS*8 normals
+    static constexpr int OP_ELEMS = 2 * TILES * 64;
+    static constexpr int GEO_ELEMS = 2 * GEO;
+};
 
+// One block of the mixed kernel: targets [tile*128, tile*128+128) against the
+// source steps [st0, st1) of chunks q0..q0+KG-1, partial sums written to
+// part[row*ldp + ...] (one segment slice).  s_op / s_geo point to LDS of at
+// least MixedLds<WCSP,KG>::OP_ELEMS / GEO_ELEMS elements.  Called by whole
+// blocks of 256 threads; ends with a barrier so the caller may reuse the LDS.
+template <int WCSP, int KG, bool ADJ>
+__device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
+                                            const uint4* __restrict__ bop, int nst, const CoefPack& cp,
+                                            float* __restrict__ out, int ldp, int q0, int tile, int st0, int st1,
+                                            uint4* s_op, float4* s_geo)
+{
+    using L = MixedLds<WCSP, KG>;
+    constexpr int NCB = L::NCB, OPS = L::OPS, NS = L::NS, TILES = L::TILES, GEO = L::GEO;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 5;
-    const int ibase = blockIdx.x * kMixTile + wave * 32;
+    const int ibase = tile * kMixTile + wave * 32;
     const int i = ibase + (lane & 31);
     const float4 x = mid[i];
     float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -297,8 +324,6 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[q][cb][r] = 0.0f;
 
-    const int st0 = blockIdx.y * steps_per_seg;
-    const int st1 = min(st0 + steps_per_seg, nst);
     const int nt_pad = nst * 8;
 
     auto issue_stage = [&](int sbase, int buf) {
@@ -308,7 +333,8 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
             const int o = t % OPS;
             const int q = o / (NCB * 2), cb = (o / 2) % NCB, pt = o % 2;
             const uint4* src = bop + ((((int64_t)(q0 + q) * nst + st) * NCB + cb) * 2 + pt) * 64 + lane;
-            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_op[buf][t][0], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_op[(buf * TILES + t) * 64],
+                                             16, 0, 0);
         }
         if (wave == 0) {
             // lane e of each 64-entry piece writes s_geo[buf][e0 + e]
@@ -319,8 +345,8 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
                     const int which = e / (NS * 8), k = e % (NS * 8);
                     const int j = min(sbase * 8 + k, nt_pad - 1);
                     const float4* src = which ? (nrm + j) : (mid + j);
-                    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_geo[buf][e0], 16,
-                                                     0, 0);
+                    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_geo[buf * GEO + e0],
+                                                     16, 0, 0);
                 }
             }
         }
@@ -338,8 +364,8 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
             float4 y[4], nj[4];
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) {
-                y[jj] = s_geo[buf][s * 8 + 4 * g + jj];
-                nj[jj] = s_geo[buf][NS * 8 + s * 8 + 4 * g + jj];
+                y[jj] = s_geo[buf * GEO + s * 8 + 4 * g + jj];
+                nj[jj] = s_geo[buf * GEO + NS * 8 + s * 8 + 4 * g + jj];
             }
             uint32_t ah[KG][4], al[KG][4];
             // Sources of this step overlap the wave's targets or the padding: mask.
@@ -354,8 +380,8 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
 #pragma unroll
                 for (int cb = 0; cb < NCB; ++cb) {
                     const int o = (q * NCB + cb) * 2;
-                    b_hi[q][cb] = __builtin_bit_cast(half8_t, s_op[buf][s * OPS + o][lane]);
-                    b_lo[q][cb] = __builtin_bit_cast(half8_t, s_op[buf][s * OPS + o + 1][lane]);
+                    b_hi[q][cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o) * 64 + lane]);
+                    b_lo[q][cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o + 1) * 64 + lane]);
                 }
             }
             // Product-major order: consecutive MFMAs write different accumulators.
@@ -376,7 +402,6 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
                     acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo[q], b_hi[q][cb], acc[q][cb], 0, 0, 0);
         }
     }
-    float* out = part + (int64_t)blockIdx.y * seg_stride;
 #pragma unroll
     for (int q = 0; q < KG; ++q)
 #pragma unroll
@@ -386,6 +411,22 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
                 const int row = ibase + (r >> 2) * 8 + g * 4 + (r & 3);
                 out[(int64_t)row * ldp + (q0 + q) * 2 * WCSP + cb * 32 + (lane & 31)] = acc[q][cb][r];
             }
+    __syncthreads();
+}
+
+template <int WCSP, int KG, bool ADJ>
+__global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
+    const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
+    const uint4* __restrict__ bop, int nst, CoefPack cp,
+    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
+{
+    using L = MixedLds<WCSP, KG>;
+    __shared__ uint4 s_op[L::OP_ELEMS];
+    __shared__ float4 s_geo[L::GEO_ELEMS];
+    const int st0 = blockIdx.y * steps_per_seg;
+    const int st1 = min(st0 + steps_per_seg, nst);
+    mixed_block<WCSP, KG, ADJ>(mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
+                               blockIdx.x, st0, st1, s_op, s_geo);
 }
 
 }  // namespace baemm

@@ -455,3 +455,133 @@ class _SolverMixin:
 
 class Helmholtz_HIP(_HelmholtzBase, _SolverMixin):
     """Drop-in for ``BAEMM::Helmholtz_OpenCL`` (Helmholtz_OpenCL.hpp:34-399)."""
+
+
+class _DevArray:
+    """Device buffer of a complex128 host array (row-major), freed on close."""
+
+    def __init__(self, L, host):
+        self.L = L
+        self.nbytes = host.nbytes
+        self.ptr = ctypes.c_void_p()
+        _lib.check(L.baemm_dev_alloc(ctypes.byref(self.ptr), max(16, self.nbytes)))
+        _lib.check(L.baemm_dev_upload(self.ptr, _ptr(host), self.nbytes))
+
+    def download(self, host):
+        _lib.check(self.L.baemm_dev_download(_ptr(host), self.ptr, self.nbytes))
+
+    def close(self):
+        if self.ptr:
+            self.L.baemm_dev_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def _batch_arrays(instances, B_list, C_list):
+    n = len(instances)
+    if n == 0 or len(B_list) != n:
+        raise ValueError("one B per instance")
+    Bs, Cs = [], []
+    for k, H in enumerate(instances):
+        W = H.GetWaveCount()
+        Bs.append(np.ascontiguousarray(_c128_in(B_list[k], "B_in", H.VertexCount())))
+        if Bs[-1].shape[1] != W:
+            raise ValueError(f"instance {k}: B has {Bs[-1].shape[1]} columns, the loaded wave count is {W}")
+        if C_list is not None and C_list[k] is not None:
+            Cs.append(np.ascontiguousarray(C_list[k], dtype=np.complex128))
+        else:
+            Cs.append(np.zeros((H.VertexCount(), W), np.complex128))
+    return Bs, Cs
+
+
+def ApplyBoundaryOperatorsBatch(instances, B_list, C_list=None, alpha=None, beta=None):
+    """C_k = alpha_k A_k B_k + beta_k C_k for several Helmholtz_HIP instances in one
+    persistent launch per precision (baemm_apply_boundary_batch).
+
+    Each instance has its operators loaded (LoadBoundaryOperators_PL) and its
+    own precision.  Returns the list of C arrays (new arrays; C_list is read
+    only when beta is given).  No reference counterpart.
+    """
+    L = _lib.load()
+    n = len(instances)
+    Bs, Cs = _batch_arrays(instances, B_list, C_list if beta is not None else None)
+    dB = [_DevArray(L, b) for b in Bs]
+    dC = [_DevArray(L, c) for c in Cs]
+    try:
+        hs = (ctypes.c_void_p * n)(*[H._h.value for H in instances])
+        pB = (ctypes.c_void_p * n)(*[d.ptr.value for d in dB])
+        pC = (ctypes.c_void_p * n)(*[d.ptr.value for d in dC])
+        ldB = (ctypes.c_int64 * n)(*[b.shape[1] for b in Bs])
+        ldC = (ctypes.c_int64 * n)(*[c.shape[1] for c in Cs])
+
+        def scal(v):
+            if v is None:
+                return None
+            v = np.broadcast_to(np.asarray(v, dtype=np.complex128), (n,))
+            return (ctypes.c_double * (2 * n))(*np.column_stack([v.real, v.imag]).ravel())
+
+        st = L.baemm_apply_boundary_batch(n, hs, scal(alpha), pB, ldB, _lib.C128, scal(beta), pC, ldC,
+                                          _lib.C128, None)
+        if st != _lib.OK:
+            raw = L.baemm_last_error(None)
+            raise BaemmError(st, raw.decode() if raw else "")
+        _lib.check(L.baemm_dev_synchronize())
+        out = []
+        for d, c in zip(dC, Cs):
+            r = np.empty_like(c)
+            d.download(r)
+            out.append(r)
+        return out
+    finally:
+        for d in dB + dC:
+            d.close()
+
+
+def BatchAutotune(instances, B_list, repeats=3):
+    """Times the persistent launch's tile shapes on this batch and keeps the
+    fastest process-wide; returns (tuning dict, best milliseconds)."""
+    L = _lib.load()
+    n = len(instances)
+    Bs, Cs = _batch_arrays(instances, B_list, None)
+    dB = [_DevArray(L, b) for b in Bs]
+    dC = [_DevArray(L, c) for c in Cs]
+    try:
+        hs = (ctypes.c_void_p * n)(*[H._h.value for H in instances])
+        pB = (ctypes.c_void_p * n)(*[d.ptr.value for d in dB])
+        pC = (ctypes.c_void_p * n)(*[d.ptr.value for d in dC])
+        ldB = (ctypes.c_int64 * n)(*[b.shape[1] for b in Bs])
+        ldC = (ctypes.c_int64 * n)(*[c.shape[1] for c in Cs])
+        best = ctypes.c_double()
+        st = L.baemm_batch_autotune(n, hs, pB, ldB, _lib.C128, pC, ldC, _lib.C128, int(repeats), ctypes.byref(best))
+        if st != _lib.OK:
+            raw = L.baemm_last_error(None)
+            raise BaemmError(st, raw.decode() if raw else "")
+        return BatchTuning(), best.value
+    finally:
+        for d in dB + dC:
+            d.close()
+
+
+def BatchTuning(mixed_steps=None, plain_blocks=None, mixed_grid_mult=None, plain_grid_mult=None):
+    """Reads (and with arguments, sets) the process-wide tile shapes of the
+    persistent batched launch."""
+    L = _lib.load()
+    v = [ctypes.c_int() for _ in range(4)]
+    _lib.check(L.baemm_batch_get_tuning(*[ctypes.byref(x) for x in v]))
+    cur = dict(zip(("mixed_steps", "plain_blocks", "mixed_grid_mult", "plain_grid_mult"), [x.value for x in v]))
+    new = dict(cur)
+    for k, val in (("mixed_steps", mixed_steps), ("plain_blocks", plain_blocks),
+                   ("mixed_grid_mult", mixed_grid_mult), ("plain_grid_mult", plain_grid_mult)):
+        if val is not None:
+            new[k] = int(val)
+    if new != cur:
+        _lib.check(L.baemm_batch_set_tuning(new["mixed_steps"], new["plain_blocks"], new["mixed_grid_mult"],
+                                            new["plain_grid_mult"]))
+    return new
+
+
+def BatchStats():
+    """Work items and persistent grid size of the last batched call."""
+    L = _lib.load()
+    v = [ctypes.c_int64() for _ in range(4)]
+    _lib.check(L.baemm_batch_get_stats(*[ctypes.byref(x) for x in v]))
+    return dict(zip(("items_mixed", "items_fp32", "grid_mixed", "grid_fp32"), [x.value for x in v]))

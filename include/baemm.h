@@ -187,6 +187,33 @@ int baemm_apply_boundary_dev(baemm_handle h, const double* alpha, const void* B,
 int baemm_apply_far_field_dev(baemm_handle h, const double* alpha, const void* B, int64_t ldB, int B_type,
                               const double* beta, void* F, int64_t ldF, int F_type, void* stream);
 
+/* ------------------------------------------------------------ batching
+ * No reference counterpart.  Applies the loaded boundary operators of n
+ * handles (different meshes, wave numbers and W) in one persistent launch per
+ * arithmetic mode: instances whose precision is BAEMM_PRECISION_MIXED share
+ * one MFMA launch, FP32 instances one FP32 launch.  Device pointers;
+ * alpha/beta are 2n doubles (re, im per instance) or NULL (1 and 0).  Each
+ * handle may appear once; all on one device; the call is asynchronous on
+ * `stream` (NULL: the first handle's stream).  An instance's result does not
+ * depend on the other instances of the batch. */
+int baemm_apply_boundary_batch(int n, const baemm_handle* handles, const double* alpha, const void* const* B,
+                               const int64_t* ldB, int B_type, const double* beta, void* const* C,
+                               const int64_t* ldC, int C_type, void* stream);
+
+/* Tile shapes of the persistent launches: source range per work item (MFMA:
+ * 8-source steps, >= 8; FP32: 64-source blocks, >= 1) and blocks per CU of
+ * the persistent grid (0: occupancy limit).  Process-wide. */
+int baemm_batch_set_tuning(int mixed_steps, int plain_blocks, int mixed_grid_mult, int plain_grid_mult);
+int baemm_batch_get_tuning(int* mixed_steps, int* plain_blocks, int* mixed_grid_mult, int* plain_grid_mult);
+
+/* Work items and persistent grid size of the last batched call. */
+int baemm_batch_get_stats(int64_t* items_mixed, int64_t* items_fp32, int64_t* grid_mixed, int64_t* grid_fp32);
+
+/* Autotuner: times the candidate tile shapes on this batch (C is overwritten)
+ * and keeps the fastest as the process-wide tuning; *best_ms receives its time. */
+int baemm_batch_autotune(int n, const baemm_handle* handles, const void* const* B, const int64_t* ldB, int B_type,
+                         void* const* C, const int64_t* ldC, int C_type, int repeats, double* best_ms);
+
 /* ----------------------------------------------------------- mass inverse */
 
 /* ApplyMassInverse<NRHS>(X, ldX, Y, ldY, cg_tol, nrhs), ApplyMassInverse.hpp:6-156:

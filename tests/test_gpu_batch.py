@@ -1,0 +1,102 @@
+"""Batched persistent apply (baemm_apply_boundary_batch): uneven meshes,
+wave numbers and RHS widths in one launch per precision, against the FP64
+oracle and the single-instance apply; an instance's result is the same alone
+and in the mix."""
+import numpy as np
+import pytest
+
+import oracle
+from baemm_amd import (ApplyBoundaryOperatorsBatch, BaemmError, BatchAutotune, BatchStats, BatchTuning,
+                       Helmholtz_HIP)
+from baemm_amd.meshes import perturbed_sphere
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+# (level, W, wcs, adjoint-double-layer on, precision)
+SPECS = [(2, 1, 1, False, "mixed"), (3, 2, 2, True, "fp32"), (2, 4, 4, False, "fp32"), (3, 8, 8, True, "mixed"),
+         (2, 16, 16, True, "mixed"), (3, 32, 32, False, "mixed"), (2, 64, 32, True, "mixed"),
+         (3, 3, 3, False, "fp32"), (4, 64, 64, False, "mixed"), (3, 16, 8, True, "fp32")]
+
+
+def _instance(k, level, W, wcs, adj, prec, rng):
+    V, T = perturbed_sphere(level, seed=10 + k)
+    K = W // wcs
+    kappa = list(rng.uniform(1.0, 16.0, K))
+    coeff = []
+    for _ in range(K):
+        row = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+        if not adj:
+            row[3] = 0
+        coeff.extend(row)
+    B = rng.standard_normal((V.shape[0], W)) + 1j * rng.standard_normal((V.shape[0], W))
+    return V, T, kappa, np.array(coeff), B
+
+
+@pytest.fixture(scope="module")
+def batch():
+    rng = np.random.default_rng(3)
+    items = []
+    for k, (level, W, wcs, adj, prec) in enumerate(SPECS):
+        V, T, kappa, coeff, B = _instance(k, level, W, wcs, adj, prec, rng)
+        H = Helmholtz_HIP(V, T)
+        H.SetPrecision(prec)
+        H.LoadBoundaryOperators_PL(kappa, coeff, W, wcs)
+        items.append(dict(H=H, V=V, T=T, kappa=kappa, coeff=coeff, B=B, W=W, wcs=wcs, prec=prec))
+    yield items
+    for it in items:
+        it["H"].close()
+
+
+def test_batch_matches_oracle_and_single(batch):
+    Hs = [it["H"] for it in batch]
+    out = ApplyBoundaryOperatorsBatch(Hs, [it["B"] for it in batch])
+    st = BatchStats()
+    print("batch stats", st)
+    assert st["items_mixed"] > 0 and st["items_fp32"] > 0
+    for k, (it, C) in enumerate(zip(batch, out)):
+        m = oracle.Mesh(it["V"], it["T"])
+        ref = oracle.apply_boundary(m, 1.0, it["B"], 0.0, np.zeros_like(it["B"]), it["kappa"], it["coeff"], it["W"],
+                                    it["wcs"])
+        err = relerr(C, ref)
+        single = np.zeros_like(it["B"])
+        it["H"].ApplyBoundaryOperators_PL(1.0, it["B"], 0.0, single)
+        print(k, it["prec"], it["W"], "oracle", err, "single", relerr(C, single))
+        assert err < 1e-5, k
+        assert relerr(C, single) < 2e-6, k
+
+
+def test_batch_result_independent_of_mix(batch):
+    Hs = [it["H"] for it in batch]
+    Bs = [it["B"] for it in batch]
+    alpha = [0.5 + 0.25j] * len(batch)
+    beta = [0.0] * len(batch)
+    full = ApplyBoundaryOperatorsBatch(Hs, Bs, alpha=alpha)
+    for k in (0, 3, 8, 9):
+        alone = ApplyBoundaryOperatorsBatch([Hs[k]], [Bs[k]], alpha=[alpha[k]])
+        assert np.array_equal(alone[0], full[k]), k
+    # alpha / beta per instance
+    C0 = [np.ones_like(b) for b in Bs]
+    ab = ApplyBoundaryOperatorsBatch(Hs, Bs, C_list=C0, alpha=[1.0] * len(batch), beta=[2.0] * len(batch))
+    for k in range(len(batch)):
+        assert relerr(ab[k], 2.0 * full[k] + 2.0) < 1e-12
+
+
+def test_batch_rejects_duplicate_handles(batch):
+    H = batch[0]["H"]
+    with pytest.raises(BaemmError):
+        ApplyBoundaryOperatorsBatch([H, H], [batch[0]["B"], batch[0]["B"]])
+
+
+def test_batch_autotune_keeps_results(batch):
+    Hs = [it["H"] for it in batch]
+    Bs = [it["B"] for it in batch]
+    before = ApplyBoundaryOperatorsBatch(Hs, Bs)
+    saved = BatchTuning()
+    tuning, ms = BatchAutotune(Hs, Bs, repeats=2)
+    print("autotuned", tuning, ms)
+    assert ms > 0 and tuning["mixed_steps"] >= 8 and tuning["plain_blocks"] >= 1
+    after = ApplyBoundaryOperatorsBatch(Hs, Bs)
+    for k in range(len(batch)):
+        assert relerr(after[k], before[k]) < 2e-6
+    BatchTuning(**saved)
