@@ -10,6 +10,7 @@ from ._lib import BaemmError, LIB_PATH
 from .helmholtz import (ApplyBoundaryOperatorsBatch, BatchAutotune, BatchStats, BatchTuning, Helmholtz_HIP,
                         WaveType)
 from . import meshes
+from .serving import GNServer
 
 __all__ = ["Helmholtz_HIP", "WaveType", "BaemmError", "LIB_PATH", "meshes", "ApplyBoundaryOperatorsBatch",
-           "BatchAutotune", "BatchTuning", "BatchStats"]
+           "BatchAutotune", "BatchTuning", "BatchStats", "GNServer"]

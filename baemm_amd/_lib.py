@@ -62,6 +62,12 @@ SIGNATURES = {
     "baemm_batch_get_tuning": (_INT, [ctypes.POINTER(_INT)] * 4),
     "baemm_batch_get_stats": (_INT, [ctypes.POINTER(_I64)] * 4),
     "baemm_batch_autotune": (_INT, [_INT, _P, _P, _P, _INT, _P, _P, _INT, _INT, ctypes.POINTER(_D)]),
+    "baemm_server_create": (_INT, [_INT, _INT, _I64, ctypes.POINTER(_P)]),
+    "baemm_server_destroy": (_INT, [_P]),
+    "baemm_server_submit_gn": (_INT, [_P, _P, _P, _P, ctypes.POINTER(_I64)]),
+    "baemm_server_wait": (_INT, [_P, _I64, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
+    "baemm_server_stats": (_INT, [_P, _P]),
+    "baemm_server_request_bytes": (_I64, [_P]),
     "baemm_apply_mass_inverse": (_INT, [_P, _P, _I64, _P, _I64, _I64, _D]),
     "baemm_apply_mass_inverse_complex": (_INT, [_P, _P, _I64, _P, _I64, _I64, _D]),
     "baemm_apply_lumped_mass_inverse": (_INT, [_P, _P, _I64, _P, _I64, _I64]),

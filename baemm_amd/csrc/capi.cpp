@@ -55,8 +55,9 @@ int create_impl(const double* coords, int64_t nv, const I* tris, int64_t nt, con
         h->coords.assign(coords, coords + 3 * nv);
         if (m > 0) h->meas.assign(meas, meas + 3 * m);
         build_host_geometry(*h);
+        // Non-blocking: the handle's work never waits for the legacy null stream.
+        BAEMM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         upload_geometry(*h);
-        BAEMM_HIP(hipStreamCreate(&h->stream));
         BAEMM_HIP(hipEventCreate(&h->ev0));
         BAEMM_HIP(hipEventCreate(&h->ev1));
     } catch (const ArgError& e) {
@@ -74,6 +75,9 @@ int create_impl(const double* coords, int64_t nv, const I* tris, int64_t nt, con
     }
     if (st != BAEMM_OK) {
         free_device(*h);
+        if (h->ev0) (void)hipEventDestroy(h->ev0);
+        if (h->ev1) (void)hipEventDestroy(h->ev1);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return st;
     }

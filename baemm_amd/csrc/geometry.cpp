@@ -122,36 +122,42 @@ void build_host_geometry(Handle& h)
     }
 }
 
+// Copies on the handle's own (non-blocking) stream, so that creating a handle
+// does not wait for work other handles have queued on the device.
 template <class T>
-static T* upload(const void* src, size_t bytes)
+static T* upload(const void* src, size_t bytes, hipStream_t s)
 {
     void* p = nullptr;
     BAEMM_HIP(hipMalloc(&p, bytes > 0 ? bytes : 16));
-    if (bytes) BAEMM_HIP(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    if (bytes) {
+        BAEMM_HIP(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s));
+        BAEMM_HIP(hipStreamSynchronize(s));
+    }
     return static_cast<T*>(p);
 }
 
 void upload_geometry(Handle& h)
 {
     BAEMM_HIP(hipSetDevice(h.device));
-    h.d_mid = upload<float4>(h.h_mid4.data(), h.h_mid4.size() * sizeof(float));
-    h.d_nrm = upload<float4>(h.h_nrm4.data(), h.h_nrm4.size() * sizeof(float));
-    h.d_area = upload<float>(h.h_area.data(), h.h_area.size() * sizeof(float));
-    h.d_sdiag = upload<float>(h.h_sdiag.data(), h.h_sdiag.size() * sizeof(float));
+    const hipStream_t s = h.stream;
+    h.d_mid = upload<float4>(h.h_mid4.data(), h.h_mid4.size() * sizeof(float), s);
+    h.d_nrm = upload<float4>(h.h_nrm4.data(), h.h_nrm4.size() * sizeof(float), s);
+    h.d_area = upload<float>(h.h_area.data(), h.h_area.size() * sizeof(float), s);
+    h.d_sdiag = upload<float>(h.h_sdiag.data(), h.h_sdiag.size() * sizeof(float), s);
     std::vector<int32_t> t4(4 * h.nt, 0);
     for (int64_t t = 0; t < h.nt; ++t)
         for (int k = 0; k < 3; ++k) t4[4 * t + k] = h.tris[3 * t + k];
-    h.d_tris = upload<int4>(t4.data(), t4.size() * sizeof(int32_t));
-    h.d_vt_ptr = upload<int32_t>(h.h_vt_ptr.data(), h.h_vt_ptr.size() * sizeof(int32_t));
-    h.d_vt_idx = upload<int32_t>(h.h_vt_idx.data(), h.h_vt_idx.size() * sizeof(int32_t));
-    h.d_M_ptr = upload<int32_t>(h.h_M_ptr.data(), h.h_M_ptr.size() * sizeof(int32_t));
-    h.d_M_idx = upload<int32_t>(h.h_M_idx.data(), h.h_M_idx.size() * sizeof(int32_t));
-    h.d_M_val = upload<double>(h.h_M_val.data(), h.h_M_val.size() * sizeof(double));
-    h.d_lumped_inv = upload<double>(h.h_lumped_inv.data(), h.h_lumped_inv.size() * sizeof(double));
+    h.d_tris = upload<int4>(t4.data(), t4.size() * sizeof(int32_t), s);
+    h.d_vt_ptr = upload<int32_t>(h.h_vt_ptr.data(), h.h_vt_ptr.size() * sizeof(int32_t), s);
+    h.d_vt_idx = upload<int32_t>(h.h_vt_idx.data(), h.h_vt_idx.size() * sizeof(int32_t), s);
+    h.d_M_ptr = upload<int32_t>(h.h_M_ptr.data(), h.h_M_ptr.size() * sizeof(int32_t), s);
+    h.d_M_idx = upload<int32_t>(h.h_M_idx.data(), h.h_M_idx.size() * sizeof(int32_t), s);
+    h.d_M_val = upload<double>(h.h_M_val.data(), h.h_M_val.size() * sizeof(double), s);
+    h.d_lumped_inv = upload<double>(h.h_lumped_inv.data(), h.h_lumped_inv.size() * sizeof(double), s);
     std::vector<float> m4(4 * h.m_pad, 0.0f);
     for (int64_t i = 0; i < h.m; ++i)
         for (int d = 0; d < 3; ++d) m4[4 * i + d] = (float)h.meas[3 * i + d];
-    h.d_meas = upload<float4>(m4.data(), m4.size() * sizeof(float));
+    h.d_meas = upload<float4>(m4.data(), m4.size() * sizeof(float), s);
 }
 
 void free_device(Handle& h)
@@ -165,6 +171,22 @@ void free_device(Handle& h)
                       &h.sws.t, &h.sws.x, &h.sws.r, &h.sws.hc, &h.sws.a, &h.sws.b, &h.sws.c,
                       &h.sws.d, &h.sws.e};
     for (DevBuf* b : bufs) b->release();
+    for (DevBuf& b : h.io) b.release();
+}
+
+// Device bytes a handle holds: geometry plus its grow-only workspaces.
+size_t handle_device_bytes(const Handle& h)
+{
+    size_t g = (size_t)h.nt_pad * (2 * sizeof(float4)) + (size_t)h.nt * (2 * sizeof(float) + sizeof(int4)) +
+               h.h_vt_ptr.size() * 4 + h.h_vt_idx.size() * 4 + h.h_M_ptr.size() * 4 + h.h_M_idx.size() * 4 +
+               h.h_M_val.size() * 8 + h.h_lumped_inv.size() * 8 + (size_t)h.m_pad * sizeof(float4);
+    const DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_scale, &h.ws_misc,
+                            &h.ws_pts, &h.ws_g, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
+                            &h.sws.t, &h.sws.x, &h.sws.r, &h.sws.hc, &h.sws.a, &h.sws.b, &h.sws.c,
+                            &h.sws.d, &h.sws.e};
+    for (const DevBuf* b : bufs) g += b->bytes;
+    for (const DevBuf& b : h.io) g += b.bytes;
+    return g;
 }
 
 // LoadParameters.hpp:198-259; coefficients scaled in FP64 then rounded.

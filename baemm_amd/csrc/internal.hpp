@@ -144,6 +144,7 @@ struct Handle {
     Params prm;
     // workspaces
     DevBuf ws_B, ws_out, ws_b32, ws_bop, ws_part, ws_scale, ws_misc, ws_pts, ws_g;
+    DevBuf io[6];                        // device copies of the host arrays of the solver entry points
     Workspace sws;
     SolverStats stats;
     // profiling of the dominant pair kernel

@@ -358,10 +358,15 @@ void check_common(Handle& h, const double* kappa, int64_t K, int64_t wcs)
     if (K > 64) throw ArgError{"at most 64 wave-number chunks are supported"};
 }
 
+// Device array of an entry point, held in the handle's grow-only I/O slots:
+// no allocation (and no device-wide synchronisation of hipFree) per call.
 struct DevArray {
     void* p = nullptr;
-    explicit DevArray(size_t bytes) { BAEMM_HIP(hipMalloc(&p, bytes > 0 ? bytes : 16)); }
-    ~DevArray() { if (p) (void)hipFree(p); }
+    DevArray(Handle& h, int slot, size_t bytes)
+    {
+        h.io[slot].reserve(bytes > 0 ? bytes : 16);
+        p = h.io[slot].ptr;
+    }
     template <class T>
     T* as() const { return static_cast<T*>(p); }
 };
@@ -409,7 +414,7 @@ int baemm_apply_mass_inverse(baemm_handle h, const double* X, int64_t ldX, doubl
     return guard(h, [&](Handle& hh) {
         if (!X || !Y || nrhs < 1 || ldX < nrhs || ldY < nrhs) throw ArgError{"invalid mass-inverse arguments"};
         hipStream_t s = hh.stream;
-        DevArray dx(sizeof(double) * hh.nv * nrhs), dy(sizeof(double) * hh.nv * nrhs);
+        DevArray dx(hh, 0, sizeof(double) * hh.nv * nrhs), dy(hh, 1, sizeof(double) * hh.nv * nrhs);
         BAEMM_HIP(hipMemcpy2DAsync(dx.p, sizeof(double) * nrhs, X, sizeof(double) * ldX, sizeof(double) * nrhs, hh.nv,
                                    hipMemcpyHostToDevice, s));
         bool conv = false;
@@ -448,7 +453,7 @@ int baemm_boundary_potential(baemm_handle h, const double* kappa, int64_t K, con
         if (!wave || !phi) throw ArgError{"wave and phi must not be NULL"};
         const int64_t NW = hh.nv * K * wcs;
         hipStream_t s = hh.stream;
-        DevArray dw(sizeof(double2) * NW), dp(sizeof(double2) * NW);
+        DevArray dw(hh, 0, sizeof(double2) * NW), dp(hh, 1, sizeof(double2) * NW);
         upload(dw.p, wave, sizeof(double2) * NW, s);
         const GmresResult r = boundary_potential_dev(hh, kappa, eta ? eta : kappa, K, wcs, dw.as<double2>(),
                                                      dp.as<double2>(), cg_tol, gmres_tol, s, coeff);
@@ -465,7 +470,7 @@ int baemm_dirichlet_to_neumann(baemm_handle h, const double* kappa, int64_t K, c
         if (!wave || !neumann) throw ArgError{"wave and neumann_trace must not be NULL"};
         const int64_t NW = hh.nv * K * wcs;
         hipStream_t s = hh.stream;
-        DevArray dw(sizeof(double2) * NW), dp(sizeof(double2) * NW);
+        DevArray dw(hh, 0, sizeof(double2) * NW), dp(hh, 1, sizeof(double2) * NW);
         upload(dw.p, wave, sizeof(double2) * NW, s);
         const GmresResult r = dtn_dev(hh, kappa, K, wcs, dw.as<double2>(), dp.as<double2>(), cg_tol, gmres_tol, s);
         download(neumann, dp.p, sizeof(double2) * NW, s);
@@ -482,7 +487,9 @@ int baemm_far_field_parameters(baemm_handle h, const double* kappa, int64_t K, c
         if (hh.m < 1) throw ArgError{"no measurement directions"};
         const int64_t W = K * wcs, NW = hh.nv * W;
         hipStream_t s = hh.stream;
-        DevArray dwave(sizeof(double2) * NW), dphi(sizeof(double2) * NW), dF(sizeof(double2) * hh.m * W);
+        DevArray dwave(hh, 0, sizeof(double2) * NW);
+        DevArray dphi(hh, 1, sizeof(double2) * NW);
+        DevArray dF(hh, 2, sizeof(double2) * hh.m * W);
         std::vector<cdouble> c12(2 * K, cdouble(0.0, 0.0));
         for (int64_t q = 0; q < K; ++q) c12[2 * q] = -1.0;     // (0, -1, 0, 0): FarField.hpp:120-126
         incident_rows_dev(hh, kappa, K, wcs, dirs, wave_type, c12, dwave.as<double2>(), s);
@@ -509,7 +516,7 @@ int baemm_derivative_ff_helper(baemm_handle h, const double* kappa, int64_t K, c
         if (!dirs || !du_dn) throw ArgError{"directions and du_dn must not be NULL"};
         const int64_t NW = hh.nv * K * wcs;
         hipStream_t s = hh.stream;
-        DevArray ddu(sizeof(double2) * NW);
+        DevArray ddu(hh, 0, sizeof(double2) * NW);
         hh.stats.converged = true;
         du_dn_dev(hh, kappa, K, wcs, dirs, wave_type, ddu.as<double2>(), cg_tol, gmres_tol, s);
         download(du_dn, ddu.p, sizeof(double2) * NW, s);
@@ -528,7 +535,9 @@ int baemm_derivative_ff(baemm_handle h, const double* kappa, int64_t K, const do
         const int64_t W = K * wcs, NW = hh.nv * W;
         hipStream_t s = hh.stream;
         hh.stats.converged = true;
-        DevArray ddu(sizeof(double2) * NW), dX(sizeof(double) * 3 * hh.nv), dF(sizeof(double2) * hh.m * W);
+        DevArray ddu(hh, 0, sizeof(double2) * NW);
+        DevArray dX(hh, 1, sizeof(double) * 3 * hh.nv);
+        DevArray dF(hh, 2, sizeof(double2) * hh.m * W);
         const double2* du = ensure_du_dn(hh, du_dn, kappa, K, wcs, dirs, wave_type, cg_tol, gmres_tol, ddu, s);
         upload(dX.p, X_in, sizeof(double) * 3 * hh.nv, s);
         derivative_dev(hh, kappa, K, wcs, dX.as<double>(), du, dF.as<double2>(), cg_tol, gmres_tol, s);
@@ -548,7 +557,9 @@ int baemm_adjoint_derivative_ff(baemm_handle h, const double* kappa, int64_t K, 
         const int64_t W = K * wcs, NW = hh.nv * W;
         hipStream_t s = hh.stream;
         hh.stats.converged = true;
-        DevArray ddu(sizeof(double2) * NW), dg(sizeof(double2) * hh.m * W), dY(sizeof(double) * 3 * hh.nv);
+        DevArray ddu(hh, 0, sizeof(double2) * NW);
+        DevArray dg(hh, 1, sizeof(double2) * hh.m * W);
+        DevArray dY(hh, 2, sizeof(double) * 3 * hh.nv);
         const double2* du = ensure_du_dn(hh, du_dn, kappa, K, wcs, dirs, wave_type, cg_tol, gmres_tol, ddu, s);
         upload(dg.p, g, sizeof(double2) * hh.m * W, s);
         adjoint_dev(hh, kappa, K, wcs, dg.as<double2>(), du, dY.as<double>(), cg_tol, gmres_tol, s);
@@ -570,8 +581,10 @@ int baemm_gauss_newton_solve(baemm_handle h, const double* kappa, int64_t K, con
         const int64_t W = K * wcs, NW = hh.nv * W, n = hh.nv, N = 3 * n;
         hipStream_t s = hh.stream;
         hh.stats.converged = true;
-        DevArray ddu(sizeof(double2) * NW), dX(sizeof(double) * N), dF(sizeof(double2) * hh.m * W),
-            dY(sizeof(double) * N);
+        DevArray ddu(hh, 0, sizeof(double2) * NW);
+        DevArray dX(hh, 1, sizeof(double) * N);
+        DevArray dF(hh, 2, sizeof(double2) * hh.m * W);
+        DevArray dY(hh, 3, sizeof(double) * N);
         const double2* du = ensure_du_dn(hh, du_dn, kappa, K, wcs, dirs, wave_type, cg_tol, gmres_tol_inner, ddu, s);
         std::vector<double> tmp(N), ys(N);
         // A x = M x + Mass DF* DF x  (FarField.hpp:551-572)

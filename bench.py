@@ -105,6 +105,70 @@ def run_gpu(args):
     return nt, nv, W, results, rel_mixed_vs_fp32
 
 
+def serving_workload(n, max_level, seed=7):
+    """Gauss-Newton-step requests: perturbed icospheres with triangle counts
+    log-uniform over 1280 * 4^(0..max_level-3) (levels 3..max_level), a few
+    distinct meshes per level (mesh ids, so geometry can stay resident),
+    kappa uniform in [1, min(32, 2 pi / (10 h))] (>= 10 points per
+    wavelength, h the mean edge length), W = 2^p RHS with p uniform in 0..6,
+    256 Fibonacci measurement directions, seeded synthetic far-field data."""
+    from baemm_amd.meshes import fibonacci_sphere, perturbed_sphere, random_directions
+
+    rng = np.random.default_rng(seed)
+    meas = fibonacci_sphere(256)
+    meshes = {}
+    reqs = []
+    for _ in range(n):
+        level = int(rng.integers(3, max_level + 1))
+        variant = int(rng.integers(0, 2))
+        key = (level, variant)
+        if key not in meshes:
+            meshes[key] = perturbed_sphere(level, seed=100 + 10 * level + variant)
+        V, T = meshes[key]
+        e = V[T[:, 1]] - V[T[:, 0]]
+        h = float(np.mean(np.linalg.norm(e, axis=1)))
+        kmax = min(32.0, 2.0 * math.pi / (10.0 * h))
+        kappa = float(rng.uniform(1.0, max(1.0, kmax)))
+        W = 2 ** int(rng.integers(0, 7))
+        dirs = random_directions(W, seed=int(rng.integers(1 << 30)))
+        fobs = 0.05 * (rng.standard_normal((meas.shape[0], W)) + 1j * rng.standard_normal((meas.shape[0], W)))
+        reqs.append(dict(V=V, T=T, meas=meas, kappa=[kappa], dirs=dirs, fobs=fobs, mesh_id=100 * level + variant,
+                         nt=T.shape[0], W=W))
+    return reqs
+
+
+def run_serving(reqs, rate, slots, precision, seed=11):
+    """Submits the requests at Poisson arrival times (rate per second) from a
+    driver thread and waits for all; returns the server statistics."""
+    import threading
+
+    from baemm_amd.serving import GNServer
+
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    ids = [None] * len(reqs)
+    with GNServer(slots=slots, hbm_budget_bytes=200 << 30) as srv:
+        t0 = time.perf_counter()
+
+        def driver():
+            for k, r in enumerate(reqs):
+                dt = arrivals[k] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids[k] = srv.submit(r["V"], r["T"], r["meas"], r["kappa"], r["dirs"], far_field_data=r["fobs"],
+                                    mesh_id=r["mesh_id"], precision=precision, cg_tol=1e-10, gmres_tol=1e-6)
+
+        th = threading.Thread(target=driver)
+        th.start()
+        th.join()
+        statuses = [srv.wait(i)[0] for i in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    st["wall_s"] = wall
+    st["not_converged"] = int(sum(1 for x in statuses if x != 0))
+    return st
+
+
 def cpu_baseline(rows_target_seconds=12.0):
     """Times the FP32 restatement of BoundaryOperatorKernel.cl (oracle) on the
     host cores for a bounded sample of target rows of the same workload."""
@@ -150,6 +214,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--serve-max-level", type=int, default=6,
+                    help="largest icosphere level of the serving leg (6: 81920, 7: 327680 triangles)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -218,6 +287,21 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.serve_requests > 0:
+        reqs = serving_workload(args.serve_requests, args.serve_max_level)
+        mixed_st = run_serving(reqs, args.serve_rate, args.serve_slots, "mixed")
+        seq_st = run_serving(reqs, args.serve_rate, 1, "fp32")
+        keys = ("completed", "failed", "not_converged", "solves_per_s", "p50_ms", "p99_ms", "mean_ms", "max_in_flight",
+                "peak_bytes", "gmres_iterations", "wall_s")
+        line["serving"] = {
+            "workload": f"{args.serve_requests} Gauss-Newton-step requests (forward solve + far field, Neumann data, "
+                        f"adjoint derivative of the misfit), GMRES 1e-6, Poisson arrivals {args.serve_rate}/s, "
+                        f"perturbed icospheres 1280..{1280 * 4 ** (args.serve_max_level - 3)} triangles "
+                        f"(log-uniform), W = 1..64, 256 far-field directions",
+            "continuous_mixed": dict({k: mixed_st[k] for k in keys}, slots=args.serve_slots),
+            "sequential_fp32": dict({k: seq_st[k] for k in keys}, slots=1),
+            "speedup_solves_per_s": mixed_st["solves_per_s"] / max(seq_st["solves_per_s"], 1e-12),
+        }
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline()

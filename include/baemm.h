@@ -300,6 +300,61 @@ int baemm_get_solver_stats(baemm_handle h, int64_t* gmres_iterations, int64_t* g
                            int64_t* operator_applies);
 int baemm_reset_solver_stats(baemm_handle h);
 
+/* ------------------------------------------------------------- serving
+ * No reference counterpart.  A request loop for Gauss-Newton-step requests:
+ * forward solve + far field (FarField), Neumann data (Derivative_FF_Helper)
+ * and the adjoint derivative of the misfit g = F - F_obs
+ * (AdjointDerivative_FF).  `slots` worker threads run requests concurrently,
+ * each on its own handle and HIP stream; a queued request starts when a slot
+ * is free and its estimated device memory fits into `hbm_budget_bytes` next
+ * to what is resident.  Geometry and solver workspaces of meshes with a
+ * mesh_id >= 0 stay resident for later requests on the same mesh (evicted
+ * least-recently-used when memory is needed).  A request's results do not
+ * depend on the other requests in flight. */
+typedef struct baemm_server_s* baemm_server;
+
+typedef struct {
+    const double* coords;          /* vertex_count x 3 */
+    int64_t vertex_count;
+    const int64_t* triangles;      /* simplex_count x 3 */
+    int64_t simplex_count;
+    int64_t mesh_id;               /* >= 0: geometry may be cached under this id */
+    const double* meas_directions; /* meas_count x 3 */
+    int64_t meas_count;
+    const double* kappa;           /* kappa_count wave numbers */
+    int64_t kappa_count;
+    const double* inc_directions;  /* wave_chunk_size x 3 */
+    int64_t wave_chunk_size;       /* W = kappa_count * wave_chunk_size <= 64 */
+    const double* far_field_data;  /* meas_count x W complex, or NULL (zero data) */
+    int wave_type;                 /* BAEMM_WAVE_PLANE (0 means plane) or BAEMM_WAVE_RADIAL */
+    int precision;                 /* BAEMM_PRECISION_FP32 or BAEMM_PRECISION_MIXED */
+    double cg_tol;                 /* <= 0: 1e-10 */
+    double gmres_tol;              /* <= 0: 1e-6 */
+} baemm_gn_request;
+
+typedef struct {
+    int64_t completed, failed, queued, running, max_in_flight;
+    int64_t resident_bytes, peak_bytes, gmres_iterations;
+    double p50_ms, p99_ms, mean_ms;   /* latency from submission to completion */
+    double solves_per_s;              /* completed / (last completion - first submission) */
+} baemm_server_stats_t;
+
+int baemm_server_create(int device, int slots, int64_t hbm_budget_bytes, baemm_server* out);
+/* Waits for every submitted request, then releases the cached handles. */
+int baemm_server_destroy(baemm_server srv);
+/* Inputs are copied; far_field_out (meas_count x W complex) and gradient_out
+ * (vertex_count x 3) must stay valid until baemm_server_wait returns. */
+int baemm_server_submit_gn(baemm_server srv, const baemm_gn_request* request, double* far_field_out,
+                           double* gradient_out, int64_t* request_id);
+/* Blocks until the request is done; returns its status (BAEMM_ERR_NOCONV:
+ * results written, a solve stopped above its tolerance).  Each id is waited
+ * for once. */
+int baemm_server_wait(baemm_server srv, int64_t request_id, double* latency_ms, int64_t* gmres_iterations);
+int baemm_server_stats(baemm_server srv, baemm_server_stats_t* stats);
+/* The device-memory estimate admission uses for a request on a fresh handle
+ * (bytes; -1 for an invalid request). */
+int64_t baemm_server_request_bytes(const baemm_gn_request* request);
+
 /* ------------------------------------------------------------- timing */
 
 /* Records HIP events around every launch of the dominant pair kernel on the

@@ -1,0 +1,96 @@
+"""Request loop (baemm_server_*): Gauss-Newton-step requests run concurrently
+give the same results as the same requests run alone, and the same as the
+Python mirror's FarField / Derivative_FF_Helper / AdjointDerivative_FF calls;
+admission respects the device-memory budget."""
+import numpy as np
+import pytest
+
+from baemm_amd import BaemmError, Helmholtz_HIP
+from baemm_amd.meshes import fibonacci_sphere, perturbed_sphere, random_directions
+from baemm_amd.serving import GNServer
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+MEAS = fibonacci_sphere(64)
+
+
+def _requests():
+    rng = np.random.default_rng(5)
+    out = []
+    specs = [(2, 0, [2.0], 4, "fp32"), (3, 1, [3.0, 4.0], 8, "mixed"), (2, 2, [1.5], 16, "mixed"),
+             (3, 1, [5.0], 2, "fp32"), (2, 0, [2.5], 1, "mixed"), (3, 3, [4.0], 32, "mixed")]
+    for level, mesh_seed, kappa, wcs, prec in specs:
+        V, T = perturbed_sphere(level, seed=20 + mesh_seed)
+        dirs = random_directions(wcs, seed=int(rng.integers(1000)))
+        W = len(kappa) * wcs
+        fobs = 0.1 * (rng.standard_normal((MEAS.shape[0], W)) + 1j * rng.standard_normal((MEAS.shape[0], W)))
+        out.append(dict(V=V, T=T, kappa=kappa, dirs=dirs, fobs=fobs, mesh_id=100 * level + mesh_seed, prec=prec))
+    return out
+
+
+def _run(server, reqs):
+    ids = [server.submit(r["V"], r["T"], MEAS, r["kappa"], r["dirs"], far_field_data=r["fobs"],
+                         mesh_id=r["mesh_id"], precision=r["prec"], cg_tol=1e-10, gmres_tol=1e-6) for r in reqs]
+    return [server.wait(i) for i in ids]
+
+
+@pytest.fixture(scope="module")
+def reqs():
+    return _requests()
+
+
+def test_results_same_alone_and_in_the_mix(reqs):
+    alone = []
+    with GNServer(slots=1) as srv:
+        for r in reqs:
+            alone.extend(_run(srv, [r]))
+    with GNServer(slots=4) as srv:
+        mixed = _run(srv, reqs)
+        st = srv.stats()
+    print("mix stats", st)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["max_in_flight"] >= 2
+    assert st["p50_ms"] <= st["p99_ms"] and st["solves_per_s"] > 0
+    for k, (a, b) in enumerate(zip(alone, mixed)):
+        assert a[0] == 0 and b[0] == 0, (k, a[0], b[0])
+        # identical kernels on identical inputs: bit-identical results
+        assert np.array_equal(a[1], b[1]), k
+        assert np.array_equal(a[2], b[2]), k
+
+
+def test_request_matches_python_mirror(reqs):
+    r = reqs[1]
+    with GNServer(slots=2) as srv:
+        st, F, G, lat, its = _run(srv, [r])[0]
+    K = len(r["kappa"])
+    wcs = r["dirs"].shape[0]
+    with Helmholtz_HIP(r["V"], r["T"], MEAS) as H:
+        H.SetPrecision(r["prec"])
+        Fm = H.FarField(r["kappa"], K, r["dirs"], wcs, cg_tol=1e-10, gmres_tol=1e-6)
+        du = H.Derivative_FF_Helper(r["kappa"], K, r["dirs"], wcs, cg_tol=1e-10, gmres_tol=1e-6)
+        Gm, _ = H.AdjointDerivative_FF(r["kappa"], K, r["dirs"], wcs, Fm - r["fobs"], du_dn=du, cg_tol=1e-10,
+                                       gmres_tol=1e-6)
+    assert relerr(F, Fm) < 1e-12
+    assert relerr(G, Gm) < 1e-12
+    assert its > 0
+
+
+def test_admission_by_memory(reqs):
+    # budget for one request at a time: never two in flight
+    small = [reqs[0], reqs[2], reqs[4]]
+    need = [GNServer.request_bytes(r["V"].shape[0], r["T"].shape[0], MEAS.shape[0], len(r["kappa"]),
+                                   r["dirs"].shape[0], r["prec"]) for r in small]
+    budget = int(1.5 * max(need))
+    assert 2 * min(need) > budget
+    with GNServer(slots=4, hbm_budget_bytes=budget) as srv:
+        res = _run(srv, small)
+        st = srv.stats()
+    print("budget stats", st)
+    assert all(x[0] == 0 for x in res)
+    assert st["max_in_flight"] == 1
+    # a request larger than the whole budget fails instead of waiting forever
+    with GNServer(slots=2, hbm_budget_bytes=1 << 20) as srv:
+        rid = srv.submit(reqs[0]["V"], reqs[0]["T"], MEAS, reqs[0]["kappa"], reqs[0]["dirs"])
+        with pytest.raises(BaemmError):
+            srv.wait(rid)
