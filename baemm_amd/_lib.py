@@ -42,6 +42,8 @@ SIGNATURES = {
     "baemm_get_precision": (_INT, [_P]),
     "baemm_set_segments": (_INT, [_P, _INT]),
     "baemm_set_gmres_limits": (_INT, [_P, _INT, _INT]),
+    "baemm_set_graphs": (_INT, [_P, _INT]),
+    "baemm_get_graph_stats": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "baemm_get_triangle_data": (_INT, [_P, _P, _P, _P, _P]),
     "baemm_mass_nnz": (_I64, [_P]),
     "baemm_get_mass_matrix": (_INT, [_P, _P, _P, _P]),

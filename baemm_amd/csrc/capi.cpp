@@ -227,6 +227,23 @@ int baemm_set_gmres_limits(baemm_handle h, int max_iter, int max_restarts)
     });
 }
 
+int baemm_set_graphs(baemm_handle h, int on)
+{
+    return guard(h, [&](Handle& hh) {
+        hh.use_graphs = on != 0;
+        return BAEMM_OK;
+    });
+}
+
+int baemm_get_graph_stats(baemm_handle h, int64_t* captures, int64_t* launches)
+{
+    return guard(h, [&](Handle& hh) {
+        if (captures) *captures = hh.graph_captures;
+        if (launches) *launches = hh.graph_launches;
+        return BAEMM_OK;
+    });
+}
+
 int baemm_get_triangle_data(baemm_handle h, float* mid4, float* nrm4, float* areas, float* sdiag)
 {
     return guard(h, [&](Handle& hh) {

@@ -93,6 +93,8 @@ struct Workspace {
     DevBuf red, scal, cg;                 // reduction partials, per-column scalars, CG vectors
     DevBuf V, w, t, x, r, hc;             // GMRES basis and vectors, Gram-Schmidt coefficients
     DevBuf a, b, c, d, e;                 // composition temporaries
+    DevBuf cgctr;                         // PCG counters: done, iterations, total
+    DevBuf sc;                            // per-column scales of the Krylov vectors
 };
 
 struct SolverStats {
@@ -102,6 +104,12 @@ struct SolverStats {
     int64_t cg_iterations = 0;
     int64_t operator_applies = 0;
     bool converged = true;
+};
+
+// A captured Arnoldi step (one per Krylov index j) and the key it was captured for.
+struct GraphSlot {
+    std::vector<char> key;
+    hipGraphExec_t exec = nullptr;
 };
 
 struct Handle {
@@ -147,6 +155,11 @@ struct Handle {
     DevBuf io[6];                        // device copies of the host arrays of the solver entry points
     Workspace sws;
     SolverStats stats;
+    // hipGraph capture of the GMRES Arnoldi steps
+    bool use_graphs = false;
+    bool capturing = false;              // a capture is being recorded on the handle's stream
+    std::vector<GraphSlot> graphs;
+    int64_t graph_captures = 0, graph_launches = 0;
     // profiling of the dominant pair kernel
     bool timing = false;
     double timed_ms = 0.0;

@@ -42,11 +42,11 @@ struct Timed {
     hipStream_t s;
     Timed(Handle& h_, hipStream_t s_) : h(h_), s(s_)
     {
-        if (h.timing) BAEMM_HIP(hipEventRecord(h.ev0, s));
+        if (h.timing && !h.capturing) BAEMM_HIP(hipEventRecord(h.ev0, s));
     }
     void done()
     {
-        if (!h.timing) return;
+        if (!h.timing || h.capturing) return;
         BAEMM_HIP(hipEventRecord(h.ev1, s));
         BAEMM_HIP(hipEventSynchronize(h.ev1));
         float ms = 0.0f;

@@ -17,10 +17,15 @@ namespace baemm {
 // ---------------------------------------------------------------- sparse
 
 // Y = alpha M X + beta Y, real columns (MassOp.Dot).  Thread per (row, col).
+// skip (may be NULL): the kernel does nothing once *skip is set (device-side
+// loop exit of the PCG, whose kernel sequence is fixed so that it can be
+// captured in a graph).
 __global__ void mass_spmm_kernel(int nv, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                                  const double* __restrict__ val, const double* __restrict__ X, int64_t ldX,
-                                 double* __restrict__ Y, int64_t ldY, int ncols, double alpha, double beta)
+                                 double* __restrict__ Y, int64_t ldY, int ncols, double alpha, double beta,
+                                 const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nv * ncols) return;
     const int v = (int)(t / ncols), c = (int)(t % ncols);
@@ -32,8 +37,9 @@ __global__ void mass_spmm_kernel(int nv, const int32_t* __restrict__ ptr, const 
 
 // Y = diag(d) X (ApplyLumpedMassInverse, ApplyMassInverse.hpp:178-194).
 __global__ void diag_scale_kernel(int nv, const double* __restrict__ d, const double* __restrict__ X, int64_t ldX,
-                                  double* __restrict__ Y, int64_t ldY, int ncols)
+                                  double* __restrict__ Y, int64_t ldY, int ncols, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nv * ncols) return;
     const int v = (int)(t / ncols), c = (int)(t % ncols);
@@ -124,8 +130,9 @@ __global__ void row_product_sum_kernel(int nv, int W, const double2* __restrict_
 // part[b][c] = sum over this block's rows of X[r][c] * Y[r][c]  (real columns).
 __global__ __launch_bounds__(256) void coldot_part_kernel(int n, int ncols, const double* __restrict__ X,
                                                           int64_t ldX, const double* __restrict__ Y, int64_t ldY,
-                                                          double* __restrict__ part)
+                                                          double* __restrict__ part, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ double sm[4][64];
     const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
     for (int c0 = 0; c0 < ncols; c0 += 64) {
@@ -141,8 +148,10 @@ __global__ __launch_bounds__(256) void coldot_part_kernel(int n, int ncols, cons
 }
 
 // out[c] = sum_b part[b][c]
-__global__ void colsum_kernel(int nblocks, int ncols, const double* __restrict__ part, double* __restrict__ out)
+__global__ void colsum_kernel(int nblocks, int ncols, const double* __restrict__ part, double* __restrict__ out,
+                              const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncols) return;
     double s = 0.0;
@@ -228,11 +237,33 @@ __global__ void scale_cols_kernel(int n, int W, const double2* __restrict__ src,
 }
 
 // ---------------------------------------------------------------- CG steps (real columns)
+// State per column: active (not yet converged), and one flag for the whole
+// solve (all columns converged or the iteration limit reached).
+
+// active = bb > 0; done = no column active; iterations = 0
+__global__ void cg_init_kernel(int ncols, const double* __restrict__ bb, int* __restrict__ active,
+                               int* __restrict__ done, int* __restrict__ iters)
+{
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+        const int a = bb[c] > 0.0;
+        active[c] = a;
+        if (a) atomicOr(&any, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *done = !any;
+        *iters = 0;
+    }
+}
 
 // alpha = rz / pq per column (0 when a column is frozen)
 __global__ void cg_alpha_kernel(int ncols, const double* __restrict__ rz, const double* __restrict__ pq,
-                                const int* __restrict__ active, double* __restrict__ alpha)
+                                const int* __restrict__ active, double* __restrict__ alpha, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncols) return;
     alpha[c] = (active[c] && pq[c] != 0.0) ? rz[c] / pq[c] : 0.0;
@@ -241,8 +272,10 @@ __global__ void cg_alpha_kernel(int ncols, const double* __restrict__ rz, const 
 // x += alpha p; r -= alpha q
 __global__ void cg_update_xr_kernel(int64_t n, int ncols, const double* __restrict__ alpha,
                                     const double* __restrict__ p, const double* __restrict__ q,
-                                    double* __restrict__ x, int64_t ldx, double* __restrict__ r)
+                                    double* __restrict__ x, int64_t ldx, double* __restrict__ r,
+                                    const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * ncols) return;
     const int64_t v = t / ncols;
@@ -252,22 +285,72 @@ __global__ void cg_update_xr_kernel(int64_t n, int ncols, const double* __restri
     r[t] -= a * q[t];
 }
 
-// beta = rz_new / rz; rz = rz_new; p = z + beta p
-__global__ void cg_beta_kernel(int ncols, double* __restrict__ rz, const double* __restrict__ rz_new,
-                               double* __restrict__ beta)
+// Column c converges when ||r_c||^2 <= tol^2 ||X_c||^2 (then it is frozen);
+// the solve is done when no column is active or after max_iter iterations.
+// One block; counts the iterations done.
+__global__ void cg_check_kernel(int ncols, const double* __restrict__ rr, const double* __restrict__ bb, double tol2,
+                                int max_iter, int* __restrict__ active, int* __restrict__ done,
+                                int* __restrict__ iters, int* __restrict__ total)
 {
+    if (*done) return;
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+        if (!active[c]) continue;
+        if (rr[c] <= tol2 * bb[c]) active[c] = 0;
+        else atomicOr(&any, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int it = *iters + 1;
+        *iters = it;
+        *total += 1;
+        if (!any || it >= max_iter) *done = 1;
+    }
+}
+
+// beta = rz_new / rz; rz = rz_new
+__global__ void cg_beta_kernel(int ncols, double* __restrict__ rz, const double* __restrict__ rz_new,
+                               double* __restrict__ beta, const int* __restrict__ skip)
+{
+    if (skip && *skip) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncols) return;
     beta[c] = (rz[c] != 0.0) ? rz_new[c] / rz[c] : 0.0;
     rz[c] = rz_new[c];
 }
 
+// p = z + beta p
 __global__ void cg_update_p_kernel(int64_t n, int ncols, const double* __restrict__ beta, const double* __restrict__ z,
-                                   double* __restrict__ p)
+                                   double* __restrict__ p, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * ncols) return;
     p[t] = z[t] + beta[t % ncols] * p[t];
+}
+
+// dst (n x ncols, ld ncols) = src (ld lds); zero (n x ncols, ld ldz) = 0
+__global__ void cg_start_kernel(int64_t n, int ncols, const double* __restrict__ src, int64_t lds,
+                                double* __restrict__ dst, double* __restrict__ zero, int64_t ldz)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * ncols) return;
+    const int64_t v = t / ncols;
+    const int c = (int)(t % ncols);
+    dst[t] = src[v * lds + c];
+    zero[v * ldz + c] = 0.0;
+}
+
+// s[k] = 1 / sqrt(Re hn[k]) (0 for a zero norm): column scale of the next
+// Krylov vector, from the norms computed on the device.
+__global__ void inv_norm_kernel(int W, const double2* __restrict__ hn, double* __restrict__ s)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= W) return;
+    const double v = hn[k].x;
+    s[k] = v > 0.0 ? 1.0 / sqrt(v) : 0.0;
 }
 
 // ---------------------------------------------------------------- host wrappers
@@ -275,17 +358,18 @@ __global__ void cg_update_p_kernel(int64_t n, int ncols, const double* __restric
 static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 void dev_mass(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
-              hipStream_t s)
+              hipStream_t s, const int* skip)
 {
     hipLaunchKernelGGL(mass_spmm_kernel, grid_for(h.nv * ncols), dim3(256), 0, s, (int)h.nv, h.d_M_ptr, h.d_M_idx,
-                       h.d_M_val, X, ldX, Y, ldY, ncols, alpha, beta);
+                       h.d_M_val, X, ldX, Y, ldY, ncols, alpha, beta, skip);
     BAEMM_HIP(hipGetLastError());
 }
 
-void dev_lumped(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s)
+void dev_lumped(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s,
+                const int* skip)
 {
     hipLaunchKernelGGL(diag_scale_kernel, grid_for(h.nv * ncols), dim3(256), 0, s, (int)h.nv, h.d_lumped_inv, X, ldX,
-                       Y, ldY, ncols);
+                       Y, ldY, ncols, skip);
     BAEMM_HIP(hipGetLastError());
 }
 
@@ -329,11 +413,12 @@ void dev_row_product_sum(Handle& h, int W, const double2* a, const double2* b, d
 }
 
 void dev_coldot(Workspace& ws, int64_t n, int ncols, const double* X, int64_t ldX, const double* Y, int64_t ldY,
-                double* out, hipStream_t s)
+                double* out, hipStream_t s, const int* skip)
 {
     double* part = ws.red.as<double>();
-    hipLaunchKernelGGL(coldot_part_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, ncols, X, ldX, Y, ldY, part);
-    hipLaunchKernelGGL(colsum_kernel, grid_for(ncols), dim3(256), 0, s, kRedBlocks, ncols, part, out);
+    hipLaunchKernelGGL(coldot_part_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, ncols, X, ldX, Y, ldY, part,
+                       skip);
+    hipLaunchKernelGGL(colsum_kernel, grid_for(ncols), dim3(256), 0, s, kRedBlocks, ncols, part, out, skip);
     BAEMM_HIP(hipGetLastError());
 }
 
@@ -353,6 +438,12 @@ void dev_gs_update(int64_t n, int W, int nb, const double2* V, int64_t vstride, 
     BAEMM_HIP(hipGetLastError());
 }
 
+void dev_inv_norm(int W, const double2* hn, double* sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(inv_norm_kernel, grid_for(W), dim3(256), 0, s, W, hn, sc);
+    BAEMM_HIP(hipGetLastError());
+}
+
 void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, double2* dst, hipStream_t s)
 {
     hipLaunchKernelGGL(scale_cols_kernel, grid_for(n * W), dim3(256), 0, s, (int)n, W, src, sc, dst);
@@ -362,15 +453,21 @@ void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, doub
 // PCG for M Y = X on real columns (ApplyMassInverse.hpp:6-156): zero initial
 // guess, lumped-diagonal preconditioner, at most max_iter iterations, per
 // column relative residual ||r|| <= tol ||X||.  X: n x ncols (ldX), Y: (ldY).
-// Returns the number of iterations; *converged tells whether all columns met tol.
-int dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
-                int max_iter, bool* converged, hipStream_t s)
+// Convergence is decided on the device: the sequence of launches is the same
+// for every call (no host synchronisation, so it can be captured in a graph);
+// once every column has converged the remaining launches return at once, and
+// a frozen column is no longer updated, so the result equals stopping there.
+// Counters (ws.cgctr, int[4]): done flag, iterations of the last call, total
+// iterations since the buffer was created.
+void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
+                 int max_iter, hipStream_t s)
 {
     const int64_t n = h.nv;
     const int64_t N = n * ncols;
     ws.cg.reserve(sizeof(double) * (4 * N));
     ws.red.reserve(sizeof(double) * kRedBlocks * ncols);
     ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
+    int* ctr = dev_cg_counters(ws, s);
     double* r = ws.cg.as<double>();
     double* z = r + N;
     double* p = z + N;
@@ -383,61 +480,69 @@ int dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* 
     double* rr = beta + ncols;
     double* bb = rr + ncols;
     int* active = reinterpret_cast<int*>(bb + 2 * ncols);
-    // r = X (contiguous copy), Y = 0
-    BAEMM_HIP(hipMemcpy2DAsync(r, sizeof(double) * ncols, X, sizeof(double) * ldX, sizeof(double) * ncols, n,
-                               hipMemcpyDeviceToDevice, s));
-    BAEMM_HIP(hipMemset2DAsync(Y, sizeof(double) * ldY, 0, sizeof(double) * ncols, n, s));
+    int* done = ctr;
+    int* iters = ctr + 1;
+    int* total = ctr + 2;
+    // r = X (contiguous), Y = 0
+    hipLaunchKernelGGL(cg_start_kernel, grid_for(N), dim3(256), 0, s, n, ncols, X, ldX, r, Y, ldY);
     dev_coldot(ws, n, ncols, r, ncols, r, ncols, bb, s);
-    std::vector<double> h_bb(ncols), h_rr(ncols);
-    BAEMM_HIP(hipMemcpyAsync(h_bb.data(), bb, sizeof(double) * ncols, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(256), 0, s, ncols, bb, active, done, iters);
+    dev_lumped(h, r, ncols, p, ncols, ncols, s, done);      // p = z = D r
+    dev_coldot(ws, n, ncols, r, ncols, p, ncols, rz, s, done);
+    for (int it = 0; it < max_iter; ++it) {
+        dev_mass(h, 1.0, p, ncols, 0.0, q, ncols, ncols, s, done);
+        dev_coldot(ws, n, ncols, p, ncols, q, ncols, pq, s, done);
+        hipLaunchKernelGGL(cg_alpha_kernel, grid_for(ncols), dim3(256), 0, s, ncols, rz, pq, active, alpha, done);
+        hipLaunchKernelGGL(cg_update_xr_kernel, grid_for(N), dim3(256), 0, s, n, ncols, alpha, p, q, Y, ldY, r, done);
+        dev_coldot(ws, n, ncols, r, ncols, r, ncols, rr, s, done);
+        hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(256), 0, s, ncols, rr, bb, tol * tol, max_iter, active, done,
+                           iters, total);
+        if (it + 1 == max_iter) break;
+        dev_lumped(h, r, ncols, z, ncols, ncols, s, done);
+        dev_coldot(ws, n, ncols, r, ncols, z, ncols, rz_new, s, done);
+        hipLaunchKernelGGL(cg_beta_kernel, grid_for(ncols), dim3(256), 0, s, ncols, rz, rz_new, beta, done);
+        hipLaunchKernelGGL(cg_update_p_kernel, grid_for(N), dim3(256), 0, s, n, ncols, beta, z, p, done);
+        BAEMM_HIP(hipGetLastError());
+    }
+    BAEMM_HIP(hipGetLastError());
+}
+
+// The PCG counters, created zeroed on first use (outside any graph capture:
+// the first PCG of a handle always runs eagerly).
+int* dev_cg_counters(Workspace& ws, hipStream_t s)
+{
+    if (ws.cgctr.bytes < 4 * sizeof(int)) {
+        ws.cgctr.reserve(4 * sizeof(int));
+        BAEMM_HIP(hipMemsetAsync(ws.cgctr.ptr, 0, 4 * sizeof(int), s));
+    }
+    return ws.cgctr.as<int>();
+}
+
+// Reads the iteration count and the convergence of the last dev_mass_cg
+// (synchronises the stream).
+int dev_mass_cg_result(Workspace& ws, int ncols, bool* converged, hipStream_t s)
+{
+    const double* bb = ws.scal.as<double>() + 6 * ncols;
+    const int* active = reinterpret_cast<const int*>(bb + 2 * ncols);
+    std::vector<int> a(ncols);
+    int ctr[4] = {0, 0, 0, 0};
+    BAEMM_HIP(hipMemcpyAsync(a.data(), active, sizeof(int) * ncols, hipMemcpyDeviceToHost, s));
+    BAEMM_HIP(hipMemcpyAsync(ctr, ws.cgctr.ptr, sizeof(ctr), hipMemcpyDeviceToHost, s));
     BAEMM_HIP(hipStreamSynchronize(s));
-    std::vector<int> h_act(ncols);
-    int nact = 0;
-    for (int c = 0; c < ncols; ++c) {
-        h_act[c] = h_bb[c] > 0.0;
-        nact += h_act[c];
-    }
-    int it = 0;
-    if (nact == 0) {
-        *converged = true;
-        return 0;
-    }
-    BAEMM_HIP(hipMemcpyAsync(active, h_act.data(), sizeof(int) * ncols, hipMemcpyHostToDevice, s));
-    dev_lumped(h, r, ncols, z, ncols, ncols, s);
-    BAEMM_HIP(hipMemcpyAsync(p, z, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-    dev_coldot(ws, n, ncols, r, ncols, z, ncols, rz, s);
-    bool done = false;
-    while (it < max_iter) {
-        ++it;
-        dev_mass(h, 1.0, p, ncols, 0.0, q, ncols, ncols, s);
-        dev_coldot(ws, n, ncols, p, ncols, q, ncols, pq, s);
-        hipLaunchKernelGGL(cg_alpha_kernel, grid_for(ncols), dim3(256), 0, s, ncols, rz, pq, active, alpha);
-        hipLaunchKernelGGL(cg_update_xr_kernel, grid_for(N), dim3(256), 0, s, n, ncols, alpha, p, q, Y, ldY, r);
-        BAEMM_HIP(hipGetLastError());
-        dev_coldot(ws, n, ncols, r, ncols, r, ncols, rr, s);
-        BAEMM_HIP(hipMemcpyAsync(h_rr.data(), rr, sizeof(double) * ncols, hipMemcpyDeviceToHost, s));
-        BAEMM_HIP(hipStreamSynchronize(s));
-        done = true;
-        bool changed = false;
-        for (int c = 0; c < ncols; ++c) {
-            if (!h_act[c]) continue;
-            if (h_rr[c] <= tol * tol * h_bb[c]) {
-                h_act[c] = 0;
-                changed = true;
-            } else {
-                done = false;
-            }
-        }
-        if (done) break;
-        if (changed) BAEMM_HIP(hipMemcpyAsync(active, h_act.data(), sizeof(int) * ncols, hipMemcpyHostToDevice, s));
-        dev_lumped(h, r, ncols, z, ncols, ncols, s);
-        dev_coldot(ws, n, ncols, r, ncols, z, ncols, rz_new, s);
-        hipLaunchKernelGGL(cg_beta_kernel, grid_for(ncols), dim3(256), 0, s, ncols, rz, rz_new, beta);
-        hipLaunchKernelGGL(cg_update_p_kernel, grid_for(N), dim3(256), 0, s, n, ncols, beta, z, p);
-        BAEMM_HIP(hipGetLastError());
-    }
-    *converged = done;
-    return it;
+    bool conv = true;
+    for (int c = 0; c < ncols; ++c) conv = conv && !a[c];
+    if (converged) *converged = conv;
+    return ctr[1];
+}
+
+// Total PCG iterations of the handle so far (synchronises the stream).
+int64_t dev_cg_total(Workspace& ws, hipStream_t s)
+{
+    const int* ctr = dev_cg_counters(ws, s);
+    int h[4];
+    BAEMM_HIP(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, s));
+    BAEMM_HIP(hipStreamSynchronize(s));
+    return h[2];
 }
 
 }  // namespace baemm

@@ -12,8 +12,9 @@ namespace baemm {
 constexpr int kRedBlocks = 240;   // blocks of the first pass of column reductions
 
 void dev_mass(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
-              hipStream_t s);
-void dev_lumped(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s);
+              hipStream_t s, const int* skip = nullptr);
+void dev_lumped(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s,
+                const int* skip = nullptr);
 void dev_av(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, hipStream_t s);
 void dev_avt(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
              hipStream_t s);
@@ -22,14 +23,18 @@ void dev_mul_normals(Handle& h, const double2* x, double* Y, hipStream_t s);
 void dev_scale_rows_neg(Handle& h, int W, const double* sv, const double2* du, double2* out, hipStream_t s);
 void dev_row_product_sum(Handle& h, int W, const double2* a, const double2* b, double2* p, hipStream_t s);
 void dev_coldot(Workspace& ws, int64_t n, int ncols, const double* X, int64_t ldX, const double* Y, int64_t ldY,
-                double* out, hipStream_t s);
+                double* out, hipStream_t s, const int* skip = nullptr);
 void dev_gs_dots(Workspace& ws, int64_t n, int W, int nb, const double2* V, int64_t vstride, const double2* w,
                  double2* out, hipStream_t s);
 void dev_gs_update(int64_t n, int W, int nb, const double2* V, int64_t vstride, const double2* hcoef, double2* w,
                    double sign, hipStream_t s);
 void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, double2* dst, hipStream_t s);
+void dev_inv_norm(int W, const double2* hn, double* sc, hipStream_t s);
 void dev_sub_c(int64_t N, const double2* b, const double2* t, double2* r, hipStream_t s);
-int dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
-                int max_iter, bool* converged, hipStream_t s);
+void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
+                 int max_iter, hipStream_t s);
+int dev_mass_cg_result(Workspace& ws, int ncols, bool* converged, hipStream_t s);
+int* dev_cg_counters(Workspace& ws, hipStream_t s);
+int64_t dev_cg_total(Workspace& ws, hipStream_t s);
 
 }  // namespace baemm

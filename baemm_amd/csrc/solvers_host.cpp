@@ -34,16 +34,87 @@ void apply_op(Handle& h, const double2* in, double2* out, hipStream_t s)
 {
     ApplyArgs a{cdouble(1.0, 0.0), cdouble(0.0, 0.0), in, h.prm.W, kC128, out, h.prm.W, kC128};
     apply_boundary_dev(h, a, s);
-    h.stats.operator_applies++;
+    if (!h.capturing) h.stats.operator_applies++;
 }
 
 void apply_prec(Handle& h, const double2* in, double2* out, double cg_tol, hipStream_t s)
 {
     const int W = (int)h.prm.W;
-    bool conv = false;
-    const int it = dev_mass_cg(h, h.sws, reinterpret_cast<const double*>(in), 2 * W, reinterpret_cast<double*>(out),
-                               2 * W, 2 * W, cg_tol, 20, &conv, s);
-    h.stats.cg_iterations += it;
+    dev_mass_cg(h, h.sws, reinterpret_cast<const double*>(in), 2 * W, reinterpret_cast<double*>(out), 2 * W, 2 * W,
+                cg_tol, 20, s);
+}
+
+// Everything an Arnoldi step's captured graph depends on: the device
+// buffers it touches, the loaded operator (wave numbers and coefficients are
+// baked into the kernel arguments), precision and shape.  A graph is
+// re-captured when any of it changes; solves that reload the same operator
+// (every DtN solve at the same wave numbers, for instance) replay it.
+struct GraphKeyHead {
+    const void* ptr[16];
+    int64_t val[12];
+    double tol;
+};
+
+std::vector<char> graph_key(Handle& h, int j, double cg_tol)
+{
+    GraphKeyHead k;
+    std::memset(&k, 0, sizeof(k));
+    const Workspace& ws = h.sws;
+    const void* p[] = {ws.V.ptr, ws.w.ptr, ws.t.ptr, ws.hc.ptr, ws.sc.ptr, ws.red.ptr, ws.cg.ptr, ws.scal.ptr,
+                       ws.cgctr.ptr, h.ws_b32.ptr, h.ws_bop.ptr, h.ws_part.ptr, h.ws_scale.ptr, h.ws_misc.ptr};
+    static_assert(sizeof(p) / sizeof(p[0]) <= 16, "key size");
+    for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); ++i) k.ptr[i] = p[i];
+    int rmin_bits = 0;
+    std::memcpy(&rmin_bits, &h.rmin, sizeof(float));
+    const int64_t v[] = {j, h.nv, h.prm.W, h.prm.wcs, h.prm.K, h.precision, h.use_diagonal ? 1 : 0, h.segments,
+                         h.gmres_max_iter, rmin_bits};
+    for (size_t i = 0; i < sizeof(v) / sizeof(v[0]); ++i) k.val[i] = v[i];
+    k.tol = cg_tol;
+    std::vector<char> key(reinterpret_cast<const char*>(&k), reinterpret_cast<const char*>(&k) + sizeof(k));
+    const auto add = [&](const void* d, size_t n) {
+        key.insert(key.end(), static_cast<const char*>(d), static_cast<const char*>(d) + n);
+    };
+    add(h.prm.kappa.data(), sizeof(float) * h.prm.kappa.size());
+    add(h.prm.c.data(), sizeof(cfloat) * h.prm.c.size());
+    for (int q = 0; q < 4; ++q) {
+        const char f[2] = {(char)h.prm.re[q], (char)h.prm.im[q]};
+        add(f, 2);
+    }
+    return key;
+}
+
+template <class F>
+void run_graph(Handle& h, int j, double cg_tol, hipStream_t s, F&& enqueue)
+{
+    if ((int)h.graphs.size() <= j) h.graphs.resize(j + 1);
+    GraphSlot& g = h.graphs[j];
+    std::vector<char> key = graph_key(h, j, cg_tol);
+    if (!g.exec || g.key != key) {
+        if (g.exec) {
+            BAEMM_HIP(hipGraphExecDestroy(g.exec));
+            g.exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        BAEMM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        h.capturing = true;
+        try {
+            enqueue();
+        } catch (...) {
+            h.capturing = false;
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        h.capturing = false;
+        BAEMM_HIP(hipStreamEndCapture(s, &graph));
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        BAEMM_HIP(e);
+        g.key.swap(key);
+        h.graph_captures++;
+    }
+    BAEMM_HIP(hipGraphLaunch(g.exec, s));
+    h.graph_launches++;
 }
 
 // Solves (P A) X = P B for the loaded boundary operator A on W columns.
@@ -57,14 +128,16 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
     ws.V.reserve(sizeof(double2) * NW * (m + 1));
     ws.w.reserve(sizeof(double2) * NW);
     ws.t.reserve(sizeof(double2) * NW);
-    ws.hc.reserve(sizeof(double2) * (2 * (m + 2) * W) + sizeof(double) * W);
+    ws.hc.reserve(sizeof(double2) * (2 * (m + 2) * W));
+    ws.sc.reserve(sizeof(double) * W);
     ws.red.reserve(sizeof(double2) * kRedBlocks * (m + 2) * W);
     double2* V = ws.V.as<double2>();
     double2* w = ws.w.as<double2>();
     double2* t = ws.t.as<double2>();
     double2* hc1 = ws.hc.as<double2>();
     double2* hc2 = hc1 + (m + 2) * W;
-    double* sc = reinterpret_cast<double*>(hc2 + (m + 2) * W);
+    double* sc = ws.sc.as<double>();
+    const int64_t cg0 = dev_cg_total(ws, s);
 
     BAEMM_HIP(hipMemsetAsync(X, 0, sizeof(double2) * NW, s));
     std::vector<double> bnorm(W, 0.0), beta(W, 0.0), res(W, 0.0), scale(W);
@@ -109,16 +182,30 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
         for (int k = 0; k < W; ++k) g[(size_t)k * (m + 1)] = beta[k];
         int jdone = 0;
         for (int j = 0; j < m; ++j) {
-            double2* Vj = V + (int64_t)j * NW;
-            double2* Vn = V + (int64_t)(j + 1) * NW;
-            apply_op(h, Vj, t, s);
-            apply_prec(h, t, w, cg_tol, s);
-            // classical Gram-Schmidt, twice
-            dev_gs_dots(ws, n, W, j + 1, V, NW, w, hc1, s);
-            dev_gs_update(n, W, j + 1, V, NW, hc1, w, -1.0, s);
-            dev_gs_dots(ws, n, W, j + 1, V, NW, w, hc2, s);
-            dev_gs_update(n, W, j + 1, V, NW, hc2, w, -1.0, s);
-            dev_gs_dots(ws, n, W, 1, w, NW, w, hc2 + (j + 1) * W, s);
+            // One Arnoldi step on the device: V_{j+1} = normalised (P A V_j
+            // minus its projections), projections in hc1 + hc2, norm^2 in
+            // hc2[j+1].  The first step of a solve runs eagerly (it sizes every
+            // workspace); later steps replay a captured graph when enabled.
+            auto step = [&]() {
+                double2* Vj = V + (int64_t)j * NW;
+                double2* Vn = V + (int64_t)(j + 1) * NW;
+                apply_op(h, Vj, t, s);
+                apply_prec(h, t, w, cg_tol, s);
+                // classical Gram-Schmidt, twice
+                dev_gs_dots(ws, n, W, j + 1, V, NW, w, hc1, s);
+                dev_gs_update(n, W, j + 1, V, NW, hc1, w, -1.0, s);
+                dev_gs_dots(ws, n, W, j + 1, V, NW, w, hc2, s);
+                dev_gs_update(n, W, j + 1, V, NW, hc2, w, -1.0, s);
+                dev_gs_dots(ws, n, W, 1, w, NW, w, hc2 + (j + 1) * W, s);
+                dev_inv_norm(W, hc2 + (j + 1) * W, sc, s);
+                dev_scale_cols(n, W, w, sc, Vn, s);
+            };
+            if (h.use_graphs && !(cyc == 0 && j == 0)) {
+                run_graph(h, j, cg_tol, s, step);
+                h.stats.operator_applies++;
+            } else {
+                step();
+            }
             BAEMM_HIP(hipMemcpyAsync(hh1.data(), hc1, sizeof(double2) * (j + 1) * W, hipMemcpyDeviceToHost, s));
             BAEMM_HIP(hipMemcpyAsync(hh2.data(), hc2, sizeof(double2) * (j + 2) * W, hipMemcpyDeviceToHost, s));
             BAEMM_HIP(hipStreamSynchronize(s));
@@ -126,7 +213,6 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
             for (int k = 0; k < W; ++k) {
                 for (int i = 0; i <= j; ++i) Hk(k, i, j) = hh1[(size_t)i * W + k] + hh2[(size_t)i * W + k];
                 const double hn = std::sqrt(std::max(0.0, hh2[(size_t)(j + 1) * W + k].real()));
-                scale[k] = hn > 0.0 ? 1.0 / hn : 0.0;
                 // previous rotations
                 for (int i = 0; i < j; ++i) {
                     const cdouble c = cs_[(size_t)k * m + i], sn = sn_[(size_t)k * m + i];
@@ -158,8 +244,6 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
                 res[k] = bnorm[k] > 0.0 ? std::abs(g[(size_t)k * (m + 1) + j + 1]) / bnorm[k] : 0.0;
                 worst = std::max(worst, res[k]);
             }
-            BAEMM_HIP(hipMemcpyAsync(sc, scale.data(), sizeof(double) * W, hipMemcpyHostToDevice, s));
-            dev_scale_cols(n, W, w, sc, Vn, s);
             out.iterations++;
             h.stats.gmres_iterations++;
             jdone = j + 1;
@@ -179,6 +263,7 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
         dev_gs_update(n, W, jdone, V, NW, hc1, X, 1.0, s);
         BAEMM_HIP(hipStreamSynchronize(s));
     }
+    h.stats.cg_iterations += dev_cg_total(ws, s) - cg0;
     h.stats.gmres_cycles += out.cycles;
     h.stats.gmres_rel_residual = out.rel_residual;
     h.stats.converged = h.stats.converged && out.converged;
@@ -276,8 +361,8 @@ void du_dn_dev(Handle& h, const double* kappa, int64_t K, int64_t wcs, const dou
 void mass_inverse_dev(Handle& h, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
                       hipStream_t s)
 {
-    bool conv = false;
-    h.stats.cg_iterations += dev_mass_cg(h, h.sws, X, ldX, Y, ldY, ncols, tol, 20, &conv, s);
+    dev_mass_cg(h, h.sws, X, ldX, Y, ldY, ncols, tol, 20, s);
+    h.stats.cg_iterations += dev_mass_cg_result(h.sws, ncols, nullptr, s);
 }
 
 // DotWithNormals_PL (FarField.hpp:956-1022): Xn (device, V) = M^{-1} AvT ((n . Av X)/a).
@@ -417,9 +502,8 @@ int baemm_apply_mass_inverse(baemm_handle h, const double* X, int64_t ldX, doubl
         DevArray dx(hh, 0, sizeof(double) * hh.nv * nrhs), dy(hh, 1, sizeof(double) * hh.nv * nrhs);
         BAEMM_HIP(hipMemcpy2DAsync(dx.p, sizeof(double) * nrhs, X, sizeof(double) * ldX, sizeof(double) * nrhs, hh.nv,
                                    hipMemcpyHostToDevice, s));
-        bool conv = false;
-        hh.stats.cg_iterations += dev_mass_cg(hh, hh.sws, dx.as<double>(), nrhs, dy.as<double>(), nrhs, (int)nrhs,
-                                              cg_tol, 20, &conv, s);
+        dev_mass_cg(hh, hh.sws, dx.as<double>(), nrhs, dy.as<double>(), nrhs, (int)nrhs, cg_tol, 20, s);
+        hh.stats.cg_iterations += dev_mass_cg_result(hh.sws, (int)nrhs, nullptr, s);
         BAEMM_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldY, dy.p, sizeof(double) * nrhs, sizeof(double) * nrhs, hh.nv,
                                    hipMemcpyDeviceToHost, s));
         BAEMM_HIP(hipStreamSynchronize(s));

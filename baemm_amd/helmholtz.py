@@ -286,6 +286,15 @@ def _libc_free(ptr):
 class _SolverMixin:
     """Solver compositions of FarField.hpp on the device (see include/baemm.h)."""
 
+    def SetGraphs(self, on: bool):
+        """Replays each GMRES Arnoldi step from a captured hipGraph (no reference counterpart)."""
+        self._check(self._L.baemm_set_graphs(self._h, 1 if on else 0))
+
+    def GraphStats(self):
+        c, l = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._L.baemm_get_graph_stats(self._h, ctypes.byref(c), ctypes.byref(l)))
+        return {"captures": c.value, "launches": l.value}
+
     def SetGmresLimits(self, max_iter=30, max_restarts=3):
         """gmres_max_iter / gmres_max_restarts (MemberVariables.hpp:64-65)."""
         self._check(self._L.baemm_set_gmres_limits(self._h, int(max_iter), int(max_restarts)))

@@ -115,6 +115,14 @@ int baemm_set_gmres_limits(baemm_handle h, int max_iter, int max_restarts);
 /* TriangleMidpoints / TriangleNormals (float4 padded, 4*T floats),
  * TriangleAreas, SingleLayerDiagonal (T floats), GetSetters.hpp:95-113.
  * Any pointer may be NULL. */
+/* No reference counterpart: capture each GMRES Arnoldi step (operator apply,
+ * PCG preconditioner, Gram-Schmidt, normalisation) in a hipGraph and replay
+ * it (on != 0), instead of launching its ~100 kernels one by one.  A graph is
+ * kept per Krylov index and re-captured when the loaded operator, precision
+ * or workspaces change.  Results are identical either way.  Default off. */
+int baemm_set_graphs(baemm_handle h, int on);
+int baemm_get_graph_stats(baemm_handle h, int64_t* captures, int64_t* launches);
+
 int baemm_get_triangle_data(baemm_handle h, float* midpoints4, float* normals4,
                             float* areas, float* single_diag);
 

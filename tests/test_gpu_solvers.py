@@ -154,3 +154,38 @@ def test_cpp_dropin_example_matches_python(tmp_path):
         F = H.FarField([5 * np.pi], 1, inc, 16, cg_tol=1e-5, gmres_tol=1e-4)
     norm_py = np.sqrt(2 * np.pi / (Vm.shape[0] * 16)) * np.linalg.norm(F)
     assert abs(norm_cpp - norm_py) < 1e-6 * norm_py
+
+
+def test_graph_captured_arnoldi_steps_match_eager(small):
+    """GMRES with each Arnoldi step replayed from a captured hipGraph gives the
+    eager result exactly, on every replay (eager work and synchronisation in
+    between), in both precisions."""
+    m, H = small
+    rng = np.random.default_rng(9)
+    W, wcs = 6, 3
+    kappa = [2.0, 3.0]
+    wave = rng.standard_normal((m.nv, W)) + 1j * rng.standard_normal((m.nv, W))
+    for mode in ("fp32", "mixed"):
+        H.SetPrecision(mode)
+        H.SetGraphs(False)
+        eager, ok = H.DirichletToNeumann(kappa, wave, wcs, 1e-12, 1e-10)
+        assert ok
+        st0 = H.GraphStats()
+        H.SetGraphs(True)
+        for rep in range(3):
+            got, ok = H.DirichletToNeumann(kappa, wave, wcs, 1e-12, 1e-10)
+            assert ok
+            assert np.array_equal(got, eager), (mode, rep)
+            # eager work between the replays
+            H.SetGraphs(False)
+            again, _ = H.DirichletToNeumann(kappa, wave, wcs, 1e-12, 1e-10)
+            assert np.array_equal(again, eager), (mode, rep)
+            H.SetGraphs(True)
+        st = H.GraphStats()
+        print(mode, "graph stats", st)
+        assert st["launches"] - st0["launches"] > 0
+        assert st["captures"] - st0["captures"] > 0
+        # replays reuse the captured graphs: far fewer captures than launches
+        assert st["captures"] - st0["captures"] < st["launches"] - st0["launches"]
+    H.SetGraphs(False)
+    H.SetPrecision("fp32")
