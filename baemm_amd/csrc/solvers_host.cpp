@@ -145,16 +145,27 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
     std::vector<cdouble> hh1((size_t)(m + 2) * W), hh2((size_t)(m + 2) * W), y((size_t)(m + 1) * W);
     auto Hk = [&](int k, int i, int j) -> cdouble& { return H[((size_t)k * (m + 1) + i) * m + j]; };
 
-    // Each cycle starts from the true preconditioned residual P (B - A X), with
-    // X accumulated in FP64 (iterative refinement around GMRES(m)); a solve is
-    // converged only when that residual, not the Givens estimate, meets tol.
-    // The pass after the last cycle only measures the residual.
+    // Iterative refinement around GMRES(m): every restart cycle starts from the
+    // true preconditioned residual P (B - A X), accumulated in FP64 from the
+    // FP64 iterate X; with the MFMA operator the residual is taken with the
+    // FP32 operator, so the cycles correct towards the FP32 operator's
+    // solution while their Krylov steps use the faster one.  A solve converges
+    // when the Givens estimate of a cycle, or the residual a restart measures,
+    // meets tol.
     GmresResult out;
-    for (int cyc = 0; cyc <= cycles; ++cyc) {
+    for (int cyc = 0; cyc < cycles; ++cyc) {
         if (cyc == 0) {
             apply_prec(h, B, V, cg_tol, s);
         } else {
-            apply_op(h, X, t, s);
+            const int prec = h.precision;
+            h.precision = BAEMM_PRECISION_FP32;
+            try {
+                apply_op(h, X, t, s);
+            } catch (...) {
+                h.precision = prec;
+                throw;
+            }
+            h.precision = prec;
             dev_sub_c(NW, B, t, w, s);
             apply_prec(h, w, V, cg_tol, s);
         }
@@ -173,7 +184,6 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
             out.converged = true;
             break;
         }
-        if (cyc == cycles) break;
         out.cycles = cyc + 1;
         for (int k = 0; k < W; ++k) scale[k] = beta[k] > 0.0 ? 1.0 / beta[k] : 0.0;
         BAEMM_HIP(hipMemcpyAsync(sc, scale.data(), sizeof(double) * W, hipMemcpyHostToDevice, s));
@@ -262,6 +272,10 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
         BAEMM_HIP(hipMemcpyAsync(hc1, y.data(), sizeof(double2) * jdone * W, hipMemcpyHostToDevice, s));
         dev_gs_update(n, W, jdone, V, NW, hc1, X, 1.0, s);
         BAEMM_HIP(hipStreamSynchronize(s));
+        if (out.rel_residual <= tol) {
+            out.converged = true;
+            break;
+        }
     }
     h.stats.cg_iterations += dev_cg_total(ws, s) - cg0;
     h.stats.gmres_cycles += out.cycles;

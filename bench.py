@@ -37,6 +37,25 @@ PEAK_FP32 = 157.3        # TFLOP/s, FP32 vector, MI355X
 G_SLDL = 43              # algorithmic flops per pair per wave number, SL+DL (BoundaryOperatorKernel.cl:104-170)
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "bdr_mixed_kernel_icosphere81920_w64_traffic.txt"
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the MFMA pair kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch bdr_mixed_kernel:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def workload():
     from baemm_amd.meshes import icosphere
 
@@ -75,6 +94,7 @@ def run_gpu(args):
 
     results = {}
     for mode in ("mixed", "fp32"):
+        log(f"{mode} apply: {args.warmup} warm-up + {args.steps} timed steps")
         H.SetPrecision(mode)
         for _ in range(args.warmup):
             step()
@@ -161,7 +181,10 @@ def run_serving(reqs, rate, slots, precision, seed=11):
         th = threading.Thread(target=driver)
         th.start()
         th.join()
-        statuses = [srv.wait(i)[0] for i in ids]
+        statuses = []
+        for k, i in enumerate(ids):
+            statuses.append(srv.wait(i)[0])
+            log(f"serving ({precision}, {slots} slots): request {k + 1}/{len(ids)} done")
         wall = time.perf_counter() - t0
         st = srv.stats()
     st["wall_s"] = wall
@@ -263,7 +286,7 @@ def main():
             "peak": PEAK_MFMA_F16,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_MFMA_F16,
-            "traffic": None,
+            "traffic": profiled_traffic(),
             "kernel": "bdr_mixed_kernel",
             "kernel_ms": mixed["kernel_ms"],
             "flops_per_launch": flops,
@@ -288,6 +311,7 @@ def main():
         "cpu_baseline": None,
     }
     if args.serve_requests > 0:
+        log("serving leg: building requests")
         reqs = serving_workload(args.serve_requests, args.serve_max_level)
         mixed_st = run_serving(reqs, args.serve_rate, args.serve_slots, "mixed")
         seq_st = run_serving(reqs, args.serve_rate, 1, "fp32")
@@ -303,6 +327,7 @@ def main():
             "speedup_solves_per_s": mixed_st["solves_per_s"] / max(seq_st["solves_per_s"], 1e-12),
         }
     if not args.no_cpu_baseline:
+        log("cpu baseline (bounded sample)")
         try:
             line["cpu_baseline"] = cpu_baseline()
         except Exception as e:  # the baseline must not hide the GPU result

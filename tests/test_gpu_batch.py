@@ -79,7 +79,7 @@ def test_batch_result_independent_of_mix(batch):
     C0 = [np.ones_like(b) for b in Bs]
     ab = ApplyBoundaryOperatorsBatch(Hs, Bs, C_list=C0, alpha=[1.0] * len(batch), beta=[2.0] * len(batch))
     for k in range(len(batch)):
-        assert relerr(ab[k], 2.0 * full[k] + 2.0) < 1e-12
+        assert relerr(ab[k], full[k] / alpha[k] + 2.0) < 1e-12
 
 
 def test_batch_rejects_duplicate_handles(batch):
