@@ -41,6 +41,7 @@ SIGNATURES = {
     "baemm_set_precision": (_INT, [_P, _INT]),
     "baemm_get_precision": (_INT, [_P]),
     "baemm_set_segments": (_INT, [_P, _INT]),
+    "baemm_set_mixed_kernel": (_INT, [_P, _INT]),
     "baemm_set_gmres_limits": (_INT, [_P, _INT, _INT]),
     "baemm_set_graphs": (_INT, [_P, _INT]),
     "baemm_get_graph_stats": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),

@@ -217,6 +217,15 @@ int baemm_set_segments(baemm_handle h, int segments)
     });
 }
 
+int baemm_set_mixed_kernel(baemm_handle h, int variant)
+{
+    return guard(h, [&](Handle& hh) {
+        if (variant < 0 || variant > 1) throw ArgError{"mixed kernel variant must be 0 or 1"};
+        hh.mixed_kernel = variant;
+        return BAEMM_OK;
+    });
+}
+
 int baemm_set_gmres_limits(baemm_handle h, int max_iter, int max_restarts)
 {
     return guard(h, [&](Handle& hh) {

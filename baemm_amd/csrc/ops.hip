@@ -392,6 +392,20 @@ static PlainFn plain_fn(int op, bool adj, int WCSP, int KG)
 using MixedFn = void (*)(const float4*, const float4*, int, const uint4*, int, CoefPack, float*, int64_t, int, int,
                          int);
 
+static MixedFn mixed_ws_fn(bool adj, int WCSP, int KG)
+{
+#define MIXWS(W_, K_)                                                                   \
+    if (WCSP == W_ && KG == K_)                                                         \
+        return adj ? &bdr_mixed_ws_kernel<W_, K_, true> : &bdr_mixed_ws_kernel<W_, K_, false>;
+    MIXWS(16, 1)
+    MIXWS(16, 2)
+    MIXWS(32, 1)
+    MIXWS(32, 2)
+    MIXWS(64, 1)
+#undef MIXWS
+    throw ArgError{"internal: no wave-specialised mixed kernel instance"};
+}
+
 static MixedFn mixed_fn(bool adj, int WCSP, int KG)
 {
 #define MIX(W_, K_)                                                                 \
@@ -620,12 +634,14 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
     const int S = (nst + steps - 1) / steps;
     float* part = bdr_reserve_part(h, pl, S);
     Timed tm(h, s);
+    const bool ws = h.mixed_kernel == 1;
     for (int q0 = 0; q0 < p.K;) {
-        const int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
+        int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
+        if (ws) kg = std::min(kg, 2);
         const CoefPack cp = make_pack(p, q0, kg, pl.sA.data());
-        MixedFn fn = mixed_fn(pl.adj, pl.WCSP, kg);
-        hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(256), 0, s, h.d_mid, h.d_nrm, (int)h.nt, pl.bop, nst, cp, part,
-                           pl.ea.seg_stride, pl.ldp, q0, steps);
+        MixedFn fn = ws ? mixed_ws_fn(pl.adj, pl.WCSP, kg) : mixed_fn(pl.adj, pl.WCSP, kg);
+        hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(ws ? 512 : 256), 0, s, h.d_mid, h.d_nrm, (int)h.nt, pl.bop, nst,
+                           cp, part, pl.ea.seg_stride, pl.ldp, q0, steps);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;
     }

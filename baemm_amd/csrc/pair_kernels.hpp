@@ -429,4 +429,170 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
                                blockIdx.x, st0, st1, s_op, s_geo);
 }
 
+// ---------------------------------------------------------------------------
+// Wave-specialised variant of the mixed kernel: 8 waves per block.  Waves 0-3
+// (producers) generate the FP16 hi/lo A operands of their 32 targets exactly
+// as bdr_mixed_kernel does and write them to LDS; waves 4-7 (consumers) read
+// them with the staged B operands and issue the MFMAs, one 8-source step
+// behind.  Kernel-value generation (VALU and transcendental units) and the
+// contraction (matrix cores) then run in different waves of the same SIMD and
+// overlap, instead of alternating inside one wave.
+//
+// Per step k (st0 + k), separated by block barriers:
+//   producers: A(k) from geometry(k)            -> s_a[k % 2]
+//   consumers: MFMA with A(k-1), B(k-1)          <- s_a[(k-1) % 2], s_op[(k-1) % 2]
+//   loads:     geometry(k+1) -> s_geo[(k+1) % 2] (producer wave 0),
+//              B(k)          -> s_op[k % 2]      (consumer waves)
+// Every buffer is rewritten only after the barrier that ends its last read.
+// ---------------------------------------------------------------------------
+template <int WCSP, int KG>
+struct WsLds {
+    static constexpr int NCB = 2 * WCSP / 32;
+    static constexpr int OPS = KG * NCB * 2;          // B tiles per step
+    static constexpr int B_ELEMS = 2 * OPS * 64;      // uint4, double buffered
+    static constexpr int A_ELEMS = 2 * 4 * KG * 2 * 64;
+    static constexpr int GEO = 16;                    // 8 centroids + 8 normals per step
+    static constexpr int GEO_ELEMS = 2 * GEO;
+};
+
+template <int WCSP, int KG, bool ADJ>
+__global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
+    const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
+    const uint4* __restrict__ bop, int nst, CoefPack cp,
+    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
+{
+    using L = WsLds<WCSP, KG>;
+    constexpr int NCB = L::NCB, OPS = L::OPS, GEO = L::GEO;
+    __shared__ uint4 s_op[L::B_ELEMS];
+    __shared__ uint4 s_a[L::A_ELEMS];
+    __shared__ float4 s_geo[L::GEO_ELEMS];
+
+    const int lane = threadIdx.x & 63;
+    // wave-uniform role (scalar register), so the branches below are uniform
+    // and every wave executes the same number of barriers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool producer = wave < 4;
+    const int w = wave & 3;
+    const int g = lane >> 5;
+    const int ibase = blockIdx.x * kMixTile + w * 32;
+    const int i = ibase + (lane & 31);
+    const int st0 = blockIdx.y * steps_per_seg;
+    const int st1 = min(st0 + steps_per_seg, nst);
+    const int T = max(0, st1 - st0);
+    const int nt_pad = nst * 8;
+
+    auto issue_geo = [&](int k) {
+        const int st = st0 + k;
+        if (lane < GEO) {
+            const int which = lane >> 3, kk = lane & 7;
+            const int j = min(st * 8 + kk, nt_pad - 1);
+            const float4* src = which ? (nrm + j) : (mid + j);
+            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_geo[(k & 1) * GEO], 16,
+                                             0, 0);
+        }
+    };
+    auto issue_b = [&](int k) {
+        const int st = st0 + k;
+        for (int t = w; t < OPS; t += 4) {
+            const int q = t / (NCB * 2), cb = (t / 2) % NCB, pt = t % 2;
+            const uint4* src = bop + ((((int64_t)(q0 + q) * nst + st) * NCB + cb) * 2 + pt) * 64 + lane;
+            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_op[((k & 1) * OPS + t) * 64],
+                                             16, 0, 0);
+        }
+    };
+
+    if (producer) {
+        const float4 x = mid[i];
+        float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
+        if constexpr (ADJ) ni = nrm[i];
+        if (w == 0 && T > 0) issue_geo(0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (int k = 0; k <= T; ++k) {
+            if (w == 0 && k + 1 < T) issue_geo(k + 1);
+            if (k < T) {
+                const int st = st0 + k;
+                float4 y[4], nj[4];
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    y[jj] = s_geo[(k & 1) * GEO + 4 * g + jj];
+                    nj[jj] = s_geo[(k & 1) * GEO + 8 + 4 * g + jj];
+                }
+                uint32_t ah[KG][4], al[KG][4];
+                const bool masked = (8 * st < ibase + 32 && 8 * st + 8 > ibase) || (8 * st + 8 > nt);
+                if (masked) mixed_gen<KG, ADJ, true>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
+                else mixed_gen<KG, ADJ, false>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
+#pragma unroll
+                for (int q = 0; q < KG; ++q) {
+                    const int base = (((k & 1) * 4 + w) * KG + q) * 2;
+                    s_a[base * 64 + lane] = make_uint4(ah[q][0], ah[q][1], ah[q][2], ah[q][3]);
+                    s_a[(base + 1) * 64 + lane] = make_uint4(al[q][0], al[q][1], al[q][2], al[q][3]);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+    } else {
+        float16_t acc[KG][NCB];
+#pragma unroll
+        for (int q = 0; q < KG; ++q)
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[q][cb][r] = 0.0f;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (int k = 0; k <= T; ++k) {
+            if (k < T) issue_b(k);
+            if (k >= 1) {
+                const int kb = (k - 1) & 1;
+                half8_t a_hi[KG], a_lo[KG];
+#pragma unroll
+                for (int q = 0; q < KG; ++q) {
+                    const int base = ((kb * 4 + w) * KG + q) * 2;
+                    a_hi[q] = __builtin_bit_cast(half8_t, s_a[base * 64 + lane]);
+                    a_lo[q] = __builtin_bit_cast(half8_t, s_a[(base + 1) * 64 + lane]);
+                }
+                half8_t b_hi[KG][NCB], b_lo[KG][NCB];
+#pragma unroll
+                for (int q = 0; q < KG; ++q)
+#pragma unroll
+                    for (int cb = 0; cb < NCB; ++cb) {
+                        const int o = (q * NCB + cb) * 2;
+                        b_hi[q][cb] = __builtin_bit_cast(half8_t, s_op[(kb * OPS + o) * 64 + lane]);
+                        b_lo[q][cb] = __builtin_bit_cast(half8_t, s_op[(kb * OPS + o + 1) * 64 + lane]);
+                    }
+                // product-major: consecutive MFMAs write different accumulators
+#pragma unroll
+                for (int q = 0; q < KG; ++q)
+#pragma unroll
+                    for (int cb = 0; cb < NCB; ++cb)
+                        acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi[q], b_hi[q][cb], acc[q][cb], 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < KG; ++q)
+#pragma unroll
+                    for (int cb = 0; cb < NCB; ++cb)
+                        acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi[q], b_lo[q][cb], acc[q][cb], 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < KG; ++q)
+#pragma unroll
+                    for (int cb = 0; cb < NCB; ++cb)
+                        acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo[q], b_hi[q][cb], acc[q][cb], 0, 0, 0);
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+        float* out = part + (int64_t)blockIdx.y * seg_stride;
+#pragma unroll
+        for (int q = 0; q < KG; ++q)
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int row = ibase + (r >> 2) * 8 + g * 4 + (r & 3);
+                    out[(int64_t)row * ldp + (q0 + q) * 2 * WCSP + cb * 32 + (lane & 31)] = acc[q][cb][r];
+                }
+    }
+}
+
 }  // namespace baemm

@@ -126,6 +126,10 @@ class _HelmholtzBase:
         m = {"fp32": _lib.PRECISION_FP32, "mixed": _lib.PRECISION_MIXED}.get(mode, mode)
         self._check(self._L.baemm_set_precision(self._h, int(m)))
 
+    def SetMixedKernel(self, variant):
+        """MFMA kernel variant of the mixed precision: 0 single-wave, 1 wave-specialised."""
+        self._check(self._L.baemm_set_mixed_kernel(self._h, int(variant)))
+
     def SetSegments(self, n):
         self._check(self._L.baemm_set_segments(self._h, int(n)))
 

@@ -162,3 +162,26 @@ def test_c2_size_sampled_rows(mode):
         a = (v[:, 0] @ Y[:, 0])
         b = (u[:, 0] @ Y[:, 1])
         assert abs(a - b) < 1e-5 * abs(a)
+
+
+@pytest.mark.parametrize("W,wcs,kappa,c3", [(64, 32, [8.0, 16.0], 0.0), (64, 64, [6.0], 0.4 - 0.2j),
+                                            (32, 16, [3.0, 5.0], 0.0), (16, 16, [4.0], 0.3j), (3, 1, [1., 2., 3.], 0.)])
+def test_wave_specialised_mixed_kernel_is_identical(sphere3, W, wcs, kappa, c3):
+    """The wave-specialised MFMA kernel (generation waves feeding MFMA waves
+    through LDS) computes exactly the values of the single-wave kernel."""
+    m, H = sphere3
+    rng = np.random.default_rng(W + wcs)
+    B = _rand(rng, m.nv, W)
+    coeff = np.array([0.25, 1.0 - 0.5j, -0.75 + 1j, c3] * (W // wcs))
+    H.SetPrecision("mixed")
+    outs = []
+    for variant in (0, 1):
+        H.SetMixedKernel(variant)
+        C = np.zeros((m.nv, W), np.complex128)
+        H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)
+        outs.append(C)
+    H.SetMixedKernel(0)
+    H.SetPrecision("fp32")
+    assert np.array_equal(outs[0], outs[1])
+    ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
+    assert relerr(outs[1], ref) < TOL
