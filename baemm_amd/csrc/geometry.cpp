@@ -3,7 +3,7 @@
 // Geometry follows Initialize.hpp:57-305 (centroid, area-weighted normal,
 // area, unit normal, single-layer self term by the edge formula) but is
 // evaluated in FP64 and rounded once to FP32 for the device (the reference
-// evaluates it in FP32, Q8 of the survey; the difference is ~1e-7 relative).
+// evaluates it in FP32, the difference is ~1e-7 relative).
 #include <algorithm>
 #include <cmath>
 #include <numeric>

@@ -19,6 +19,10 @@ Generators (all closed, outward-oriented triangle meshes):
 * :func:`perturbed_sphere` -- icosphere with radius 1 + eps * sum of real
   spherical harmonics of degree <= lmax with seeded N(0,1)/(l+1)^2 weights.
 * :func:`fibonacci_sphere` -- m quasi-uniform unit directions.
+
+STL (ASCII or binary) is read by :func:`read_stl`, the input path of
+``ReadMeshFromSTL.hpp:5-44``: coordinates in single precision, coincident
+corners merged into one vertex.
 """
 from __future__ import annotations
 
@@ -35,6 +39,8 @@ __all__ = [
     "random_directions",
     "read_mesh_file",
     "write_mesh_file",
+    "read_stl",
+    "write_stl",
     "signed_volume",
     "real_sph_harm",
 ]
@@ -228,3 +234,64 @@ def write_mesh_file(path, V: np.ndarray, T: np.ndarray):
     lines += [" ".join(repr(float(x)) for x in row) for row in V]
     lines += [" ".join(str(int(x)) for x in row) for row in T]
     Path(path).write_text("\n".join(lines) + "\n")
+
+
+def read_stl(path):
+    """Read an ASCII or binary STL file (``ReadMeshFromSTL.hpp:5-44``).
+
+    Corners with identical single-precision coordinates become one vertex
+    (numbered in order of first appearance); returns ``(V, T)`` with V float64
+    (values of the file's float32 coordinates) and T int64.  Raises ValueError
+    for a file that is neither form.
+    """
+    data = Path(path).read_bytes()
+    corners = None
+    if len(data) >= 84:
+        n = int.from_bytes(data[80:84], "little")
+        if len(data) == 84 + 50 * n:
+            rec = np.frombuffer(data, dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]),
+                                count=n, offset=84)
+            corners = np.ascontiguousarray(rec["v"].reshape(-1, 3))
+    if corners is None:
+        text = data.decode("ascii", errors="replace")
+        if not text.lstrip().lower().startswith("solid"):
+            raise ValueError(f"{path}: not an STL file")
+        vals = []
+        for line in text.splitlines():
+            tok = line.split()
+            if tok and tok[0] == "vertex":
+                if len(tok) != 4:
+                    raise ValueError(f"{path}: malformed vertex line {line!r}")
+                vals.append([float(t) for t in tok[1:4]])
+        if len(vals) % 3:
+            raise ValueError(f"{path}: vertex count is not a multiple of 3")
+        corners = np.array(vals, dtype=np.float32).reshape(-1, 3)
+    uniq, first, inv = np.unique(corners, axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first)                 # number vertices by first appearance
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    V = uniq[order].astype(np.float64)
+    T = rank[inv.reshape(-1)].reshape(-1, 3).astype(np.int64)
+    return np.ascontiguousarray(V), np.ascontiguousarray(T)
+
+
+def write_stl(path, V: np.ndarray, T: np.ndarray, binary: bool = True):
+    """Write a triangle mesh as STL (facet normals from the corner order)."""
+    P = np.asarray(V, np.float64)[np.asarray(T)]
+    nrm = np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0])
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-300)
+    if binary:
+        rec = np.zeros(P.shape[0], dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]))
+        rec["n"] = nrm
+        rec["v"] = P
+        Path(path).write_bytes(b"baemm".ljust(80, b" ") + np.uint32(P.shape[0]).tobytes() + rec.tobytes())
+    else:
+        out = ["solid baemm"]
+        for n, tri in zip(nrm, P):
+            out.append(f"  facet normal {n[0]:.9e} {n[1]:.9e} {n[2]:.9e}")
+            out.append("    outer loop")
+            out += [f"      vertex {x:.9e} {y:.9e} {z:.9e}" for x, y, z in tri.astype(np.float32)]
+            out.append("    endloop")
+            out.append("  endfacet")
+        out.append("endsolid baemm")
+        Path(path).write_text("\n".join(out) + "\n")

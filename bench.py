@@ -193,6 +193,41 @@ def run_serving(reqs, rate, slots, precision, seed=11):
     return st
 
 
+def run_solve_leg(repeats=2):
+    """Full sound-soft solve: torus R=1, r=0.4 on a 240 x 80 grid (38,400
+    triangles), kappa = 10, 32 seeded plane waves, FarField (incident wave,
+    GMRES(30) to 1e-6 with the PCG mass-inverse preconditioner, far field at 256
+    Fibonacci directions).  Seconds per solve in FP32, mixed, and mixed with
+    hipGraph-replayed Arnoldi steps."""
+    from baemm_amd import Helmholtz_HIP
+    from baemm_amd.meshes import fibonacci_sphere, random_directions, torus
+
+    V, T = torus(1.0, 0.4, 240, 80)
+    meas = fibonacci_sphere(256)
+    dirs = random_directions(32, seed=2)
+    out = {}
+    ref = None
+    with Helmholtz_HIP(V, T, meas) as H:
+        for leg, prec, graphs in (("fp32", "fp32", False), ("mixed", "mixed", False), ("mixed_graphs", "mixed", True)):
+            log(f"solve leg {leg}")
+            H.SetPrecision(prec)
+            H.SetGraphs(graphs)
+            H.FarField([10.0], 1, dirs, 32, cg_tol=1e-10, gmres_tol=1e-6)    # warm-up (workspaces, graphs)
+            H.ResetSolverStats()
+            t0 = time.perf_counter()
+            for _ in range(repeats):
+                F = H.FarField([10.0], 1, dirs, 32, cg_tol=1e-10, gmres_tol=1e-6)
+            dt = (time.perf_counter() - t0) / repeats
+            st = H.SolverStats()
+            if ref is None:
+                ref = F
+            out[leg] = {"s_per_solve": dt, "gmres_iterations_per_solve": st["gmres_iterations"] / repeats,
+                        "rel_residual": st["rel_residual"],
+                        "far_field_rel_diff_vs_fp32": float(np.linalg.norm(F - ref) / np.linalg.norm(ref))}
+        H.SetGraphs(False)
+    return out
+
+
 def cpu_baseline(rows_target_seconds=12.0):
     """Times the FP32 restatement of BoundaryOperatorKernel.cl (oracle) on the
     host cores for a bounded sample of target rows of the same workload."""
@@ -238,6 +273,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="skip the torus solve leg")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--serve-max-level", type=int, default=6,
@@ -246,7 +282,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "the apply shards by target rows or by request, see DESIGN.md.", file=sys.stderr)
+              "the apply would shard by target rows (one all-reduce of vertex sums) or by request.", file=sys.stderr)
         sys.exit(2)
     nt, nv, W, res, rel = run_gpu(args)
     pairs_rhs = float(nt) * nt * W
@@ -314,6 +350,9 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_solve:
+        line["solve"] = dict(workload="sound-soft torus 38400 triangles, kappa 10, 32 plane waves, FarField, "
+                                      "GMRES(30) 1e-6, 256 far-field directions", **run_solve_leg())
     if args.serve_requests > 0:
         log("serving leg: building requests")
         reqs = serving_workload(args.serve_requests, args.serve_max_level)

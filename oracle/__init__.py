@@ -589,7 +589,7 @@ def gmres_dense(A, b, tol=1e-10):
 
 
 # --------------------------------------------------------------------------
-# Closed forms (Appendix C of the survey; pure mathematics)
+# Closed forms of the unit sphere (pure mathematics: spherical Bessel / Hankel series)
 # --------------------------------------------------------------------------
 
 def sphere_eigenvalues(l, kappa):
@@ -618,7 +618,7 @@ def mie_far_field_soft(kappa, cos_theta, lmax=None):
     """Far field of the sound-soft unit sphere for the incident wave e^{i k x.d}.
 
     u_inf(xhat) = (i/k) sum_l (2l+1) j_l(k)/h_l(k) P_l(xhat.d) with the
-    1/(4 pi) normalisation of G (survey Appendix C; sign fixed by the
+    1/(4 pi) normalisation of G = e^{i k r} / (4 pi r) (sign fixed by the
     exp(-i k xhat.y) far-field kernel of FarFieldOperatorKernel.cl:91-101).
     """
     from scipy.special import eval_legendre, spherical_jn, spherical_yn

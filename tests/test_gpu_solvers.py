@@ -189,3 +189,58 @@ def test_graph_captured_arnoldi_steps_match_eager(small):
         assert st["captures"] - st0["captures"] < st["launches"] - st0["launches"]
     H.SetGraphs(False)
     H.SetPrecision("fp32")
+
+
+def test_c3_torus_small_matches_dense_fp64_solve():
+    """Sound-soft torus (R=1, r=0.4), far field from GMRES(30) at 1e-6 against
+    the oracle's dense FP64 LU solve (torus small enough for LU)."""
+    m = oracle_mesh("torus", 10, 256)
+    dirs = np.array([[0.0, 0.0, 1.0], [0.6, 0.8, 0.0], [0.0, 0.6, -0.8], [-0.48, 0.6, 0.64]])
+    ref, _ = oracle.far_field(m, [4.0], dirs, 4)
+    with Helmholtz_HIP(m.V, m.T, m.meas) as H:
+        for mode in ("fp32", "mixed"):
+            H.SetPrecision(mode)
+            F = H.FarField([4.0], 1, dirs, 4, cg_tol=1e-10, gmres_tol=1e-6)
+            err = relerr(F, ref)
+            print("torus", mode, err, H.SolverStats())
+            assert err < 2e-5, mode
+
+
+def test_c3_torus_full_size_mixed_vs_fp32():
+    """The 38,400-triangle torus at kappa = 10 with 32 seeded plane waves and 256
+    Fibonacci far-field directions: the mixed-precision solve (MFMA operator,
+    FP32-operator refinement at restarts) agrees with the plain FP32 solve;
+    both converge to 1e-6."""
+    from baemm_amd.meshes import random_directions, torus
+
+    V, T = torus(1.0, 0.4, 240, 80)
+    assert T.shape[0] == 38400
+    meas = fibonacci_sphere(256)
+    dirs = random_directions(32, seed=2)
+    out = {}
+    with Helmholtz_HIP(V, T, meas) as H:
+        for mode in ("fp32", "mixed"):
+            H.SetPrecision(mode)
+            H.ResetSolverStats()
+            out[mode] = H.FarField([10.0], 1, dirs, 32, cg_tol=1e-10, gmres_tol=1e-6)
+            st = H.SolverStats()
+            print("torus 38400", mode, st)
+            assert st["rel_residual"] <= 1e-6
+    d = relerr(out["mixed"], out["fp32"])
+    print("mixed vs fp32 far field", d)
+    assert d < 1e-4
+
+
+def test_c3b_icosphere_81920_against_mie():
+    """Sound-soft unit sphere with 81,920 triangles at kappa = 10: far field of
+    the mixed-precision solve against the Mie series."""
+    V, T = icosphere(6)
+    meas = fibonacci_sphere(256)
+    with Helmholtz_HIP(V, T, meas) as H:
+        H.SetPrecision("mixed")
+        F = H.FarField([10.0], 1, DIRS, 3, cg_tol=1e-10, gmres_tol=1e-6)
+        print("stats", H.SolverStats())
+    ref = np.stack([oracle.mie_far_field_soft(10.0, meas @ d) for d in DIRS], axis=1)
+    err = relerr(F, ref)
+    print("Mie error 81920", err)
+    assert err < 5e-3

@@ -131,3 +131,23 @@ def test_reference_mesh_files_match_generated_sizes():
         mg = oracle.Mesh(V, T)
         assert mr.area.sum() == pytest.approx(mg.area.sum(), rel=1e-4)
         assert signed_volume(Vr, Tr) > 0
+
+
+@pytest.mark.parametrize("binary", [True, False])
+def test_stl_round_trip(tmp_path, binary):
+    """read_stl (ReadMeshFromSTL.hpp input path) recovers the mesh written as
+    STL: same triangles up to float32 rounding, corners merged into vertices."""
+    from baemm_amd.meshes import icosphere, read_stl, write_stl
+
+    V, T = icosphere(2)
+    f = tmp_path / ("m.stl" if binary else "m_ascii.stl")
+    write_stl(f, V, T, binary=binary)
+    V2, T2 = read_stl(f)
+    assert V2.shape == V.shape and T2.shape == T.shape
+    assert np.allclose(V2[T2], V[T], atol=1e-6)
+    # first-appearance numbering: the first triangle's corners are 0, 1, 2
+    assert list(T2[0]) == [0, 1, 2]
+    with pytest.raises(ValueError):
+        bad = tmp_path / "bad.stl"
+        bad.write_text("not a mesh")
+        read_stl(bad)
