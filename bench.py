@@ -228,6 +228,92 @@ def run_solve_leg(repeats=2):
     return out
 
 
+def run_batch_leg(repeats=5, seed=3):
+    """Batched uneven instances: 24 perturbed icospheres (radius 1 + 0.1 sum of
+    real spherical harmonics of degree <= 4, seeded N(0,1)/(l+1)^2 weights) of
+    1,280, 5,120 or 20,480 triangles, kappa uniform in [1, 16], W = 2^p RHS with
+    p uniform in 0..6 (one wave-number chunk each), SL+DL with complex
+    coefficients, mixed precision.  Times one persistent launch per apply
+    (autotuned tile shapes) against the 24 single-instance applies."""
+    import ctypes
+
+    from baemm_amd import ApplyBoundaryOperatorsBatch, BatchAutotune, BatchStats, Helmholtz_HIP, _lib
+    from baemm_amd.meshes import perturbed_sphere
+
+    rng = np.random.default_rng(seed)
+    inst, Bs = [], []
+    pairs_rhs = 0.0
+    for k in range(24):
+        level = int(rng.integers(3, 6))
+        V, T = perturbed_sphere(level, seed=1000 + k)
+        W = 2 ** int(rng.integers(0, 7))
+        kappa = [float(rng.uniform(1.0, 16.0))]
+        coeff = np.array([0.0, 1.0 + 0.2j, 1.0 - 0.1j, 0.0])
+        H = Helmholtz_HIP(V, T)
+        H.SetPrecision("mixed")
+        H.LoadBoundaryOperators_PL(kappa, coeff, W, W)
+        inst.append(H)
+        Bs.append(rng.standard_normal((V.shape[0], W)) + 1j * rng.standard_normal((V.shape[0], W)))
+        pairs_rhs += float(T.shape[0]) ** 2 * W
+    L = _lib.load()
+    # device-resident inputs and outputs for the timed loops
+    dB, dC = [], []
+    for b in Bs:
+        for lst in (dB, dC):
+            p = ctypes.c_void_p()
+            _lib.check(L.baemm_dev_alloc(ctypes.byref(p), b.nbytes))
+            lst.append(p)
+        _lib.check(L.baemm_dev_upload(dB[-1], b.ctypes.data_as(ctypes.c_void_p), b.nbytes))
+    n = len(inst)
+    hs = (ctypes.c_void_p * n)(*[H._h.value for H in inst])
+    pB = (ctypes.c_void_p * n)(*[d.value for d in dB])
+    pC = (ctypes.c_void_p * n)(*[d.value for d in dC])
+    ld = (ctypes.c_int64 * n)(*[b.shape[1] for b in Bs])
+    log("batch leg: autotune")
+    tuning, best = BatchAutotune(inst, Bs, repeats=2)
+
+    def batched():
+        _lib.check(L.baemm_apply_boundary_batch(n, hs, None, pB, ld, _lib.C128, None, pC, ld, _lib.C128, None))
+
+    one = (ctypes.c_double * 2)(1.0, 0.0)
+    zero = (ctypes.c_double * 2)(0.0, 0.0)
+
+    def single():
+        for k, H in enumerate(inst):
+            _lib.check(L.baemm_apply_boundary_dev(H._h, one, dB[k], ld[k], _lib.C128, zero, dC[k], ld[k], _lib.C128,
+                                                  None), H._h)
+
+    times = {}
+    for name, fn in (("batched", batched), ("single", single)):
+        fn()
+        _lib.check(L.baemm_dev_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            fn()
+        _lib.check(L.baemm_dev_synchronize())
+        times[name] = (time.perf_counter() - t0) / repeats
+    batched()
+    _lib.check(L.baemm_dev_synchronize())
+    stats = BatchStats()
+    for d in dB + dC:
+        L.baemm_dev_free(d)
+    for H in inst:
+        H.close()
+    return {
+        "workload": "24 perturbed icospheres of 1280/5120/20480 triangles, kappa in [1,16], W = 2^p (p in 0..6), "
+                    "SL+DL, mixed precision",
+        "pair_rhs_per_apply": pairs_rhs,
+        "batched_ms": times["batched"] * 1e3,
+        "single_ms": times["single"] * 1e3,
+        "batched_pair_rhs_per_s": pairs_rhs / times["batched"],
+        "single_pair_rhs_per_s": pairs_rhs / times["single"],
+        "speedup": times["single"] / times["batched"],
+        "tuning": tuning,
+        "work_items": stats["items_mixed"],
+        "persistent_grid": stats["grid_mixed"],
+    }
+
+
 def cpu_baseline(rows_target_seconds=12.0):
     """Times the FP32 restatement of BoundaryOperatorKernel.cl (oracle) on the
     host cores for a bounded sample of target rows of the same workload."""
@@ -274,6 +360,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the torus solve leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the serving leg (1/s)")
     ap.add_argument("--serve-max-level", type=int, default=6,
@@ -350,6 +437,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_batch:
+        line["batch"] = run_batch_leg()
     if not args.no_solve:
         line["solve"] = dict(workload="sound-soft torus 38400 triangles, kappa 10, 32 plane waves, FarField, "
                                       "GMRES(30) 1e-6, 256 far-field directions", **run_solve_leg())
