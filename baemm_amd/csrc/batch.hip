@@ -221,7 +221,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
             in.nst = (int)(h.nt_pad / 8);
             in.coef_off = (int)coefs.size();
             for (int q = 0; q < p.K; ++q)
-                coefs.push_back(make_pack(p, q, 1, mixed ? pl.sA.data() : nullptr).c[0]);
+                coefs.push_back(make_pack(p, q, 1, mixed ? pl.sA.data() : nullptr, mixed ? pl.rot.data() : nullptr).c[0]);
             // source ranges of equal length across the batch
             int len, S, tiles, nsrc_units;
             if (mixed) {

@@ -224,28 +224,46 @@ void load_params(Handle& h, const double* kappa, const double* coeff, int64_t W,
     p.loaded = true;
 }
 
-std::vector<ChunkCoef> chunk_coefs(const Params& p, int q0, int kg, float scale)
+std::vector<ChunkCoef> chunk_coefs(const Params& p, int q0, int kg, float scale, const cfloat* rot)
 {
     std::vector<ChunkCoef> out(kg);
     for (int g = 0; g < kg; ++g) {
         const int q = q0 + g;
+        // coefficients times conj(rot[q]) (a unit phase the caller multiplies back)
+        const cdouble ph = rot ? std::conj(cdouble(rot[q])) : cdouble(1.0, 0.0);
+        const cdouble c1 = cdouble(p.c[4 * q + 1]) * ph * (double)scale;
+        const cdouble c2 = cdouble(p.c[4 * q + 2]) * ph * (double)scale;
+        const cdouble c3 = cdouble(p.c[4 * q + 3]) * ph * (double)scale;
         ChunkCoef cc{};
         cc.kap = p.kappa[q];
         cc.kap_rev = (float)((double)p.kappa[q] / (2.0 * M_PI));
-        cc.c1r = p.c[4 * q + 1].real() * scale;
-        cc.c1i = p.c[4 * q + 1].imag() * scale;
-        cc.c2r = p.c[4 * q + 2].real() * scale;
-        cc.c2i = p.c[4 * q + 2].imag() * scale;
-        cc.c3r = p.c[4 * q + 3].real() * scale;
-        cc.c3i = p.c[4 * q + 3].imag() * scale;
+        cc.c1r = (float)c1.real();
+        cc.c1i = (float)c1.imag();
+        cc.c2r = (float)c2.real();
+        cc.c2i = (float)c2.imag();
+        cc.c3r = (float)c3.real();
+        cc.c3i = (float)c3.imag();
         const double k = (double)p.kappa[q];
-        cc.kc2r = (float)(k * p.c[4 * q + 2].real() * scale);
-        cc.kc2i = (float)(k * p.c[4 * q + 2].imag() * scale);
-        cc.kc3r = (float)(k * p.c[4 * q + 3].real() * scale);
-        cc.kc3i = (float)(k * p.c[4 * q + 3].imag() * scale);
+        cc.kc2r = (float)(k * c2.real());
+        cc.kc2i = (float)(k * c2.imag());
+        cc.kc3r = (float)(k * c3.real());
+        cc.kc3i = (float)(k * c3.imag());
         out[g] = cc;
     }
     return out;
+}
+
+// Unit phase per chunk that makes the double-layer coefficient real:
+// c2 / |c2| (1 when c2 = 0).
+std::vector<cfloat> chunk_phases(const Params& p)
+{
+    std::vector<cfloat> rot(p.K, cfloat(1.0f, 0.0f));
+    for (int q = 0; q < p.K; ++q) {
+        const cdouble c2(p.c[4 * q + 2]);
+        const double a = std::abs(c2);
+        if (a > 0.0) rot[q] = cfloat((float)(c2.real() / a), (float)(c2.imag() / a));
+    }
+    return rot;
 }
 
 }  // namespace baemm

@@ -176,7 +176,10 @@ void free_device(Handle& h);
 
 // ---- parameters ----
 void load_params(Handle& h, const double* kappa, const double* coeff, int64_t W, int64_t wcs, double factor);
-std::vector<ChunkCoef> chunk_coefs(const Params& p, int q0, int kg, float scale);
+// Coefficients of chunks q0..q0+kg-1 times `scale` and, if rot is given,
+// times conj(rot[q]).
+std::vector<ChunkCoef> chunk_coefs(const Params& p, int q0, int kg, float scale, const cfloat* rot = nullptr);
+std::vector<cfloat> chunk_phases(const Params& p);
 
 // ---- device ops (ops.hip) ----
 // Complex element types of external device arrays.

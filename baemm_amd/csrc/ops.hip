@@ -241,6 +241,12 @@ __global__ __launch_bounds__(256) void bdr_epilogue_kernel(EpiArgs a)
                 }
                 pr *= us;
                 pi *= us;
+                {
+                    const double rr = a.rotr[q], ri = a.roti[q];
+                    const double tr = pr * rr - pi * ri;
+                    pi = pr * ri + pi * rr;
+                    pr = tr;
+                }
                 if (a.diag_on) {
                     const float2 bv = a.b[(int64_t)t * a.ldb + pc];
                     const double fr = (double)a.c1r[q] * a.sdiag[t] - (double)a.c1i[q] * a.kap[q];
@@ -355,12 +361,12 @@ __global__ void incident_kernel(IncArgs a)
 
 // --------------------------------------------------------------- dispatch
 
-CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk)
+CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk, const cfloat* rot)
 {
     CoefPack cp{};
     for (int g = 0; g < kg; ++g) {
         const float s = scale_per_chunk ? scale_per_chunk[q0 + g] : 1.0f;
-        cp.c[g] = chunk_coefs(p, q0 + g, 1, s)[0];
+        cp.c[g] = chunk_coefs(p, q0 + g, 1, s, rot)[0];
     }
     return cp;
 }
@@ -517,6 +523,8 @@ static EpiArgs base_epi(Handle& h, const ApplyArgs& a, int WCSP)
     ea.C = a.C;
     ea.ldC = a.ldC;
     for (int q = 0; q < p.K; ++q) {
+        ea.rotr[q] = 1.0f;
+        ea.roti[q] = 0.0f;
         ea.c0r[q] = p.c[4 * q].real();
         ea.c0i[q] = p.c[4 * q].imag();
         ea.c1r[q] = p.c[4 * q + 1].real();
@@ -600,6 +608,13 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     BAEMM_HIP(hipGetLastError());
     pl.bop = h.ws_bop.as<uint4>();
     ea.unscale = unscale;
+    // The MFMA kernels take each chunk's coefficients times conj(phase of c2)
+    // (c2 real: fewer operations per pair); the epilogue multiplies the phase back.
+    pl.rot = chunk_phases(p);
+    for (int q = 0; q < p.K; ++q) {
+        ea.rotr[q] = pl.rot[q].real();
+        ea.roti[q] = pl.rot[q].imag();
+    }
 }
 
 float* bdr_reserve_part(Handle& h, BdrPlan& pl, int S)
@@ -638,7 +653,7 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
     for (int q0 = 0; q0 < p.K;) {
         int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
         if (ws) kg = std::min(kg, 2);
-        const CoefPack cp = make_pack(p, q0, kg, pl.sA.data());
+        const CoefPack cp = make_pack(p, q0, kg, pl.sA.data(), pl.rot.data());
         MixedFn fn = ws ? mixed_ws_fn(pl.adj, pl.WCSP, kg) : mixed_fn(pl.adj, pl.WCSP, kg);
         hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(ws ? 512 : 256), 0, s, h.d_mid, h.d_nrm, (int)h.nt, pl.bop, nst,
                            cp, part, pl.ea.seg_stride, pl.ldp, q0, steps);

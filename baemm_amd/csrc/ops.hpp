@@ -32,6 +32,7 @@ struct EpiArgs {
     void* C;
     int64_t ldC;
     float c1r[64], c1i[64], kap[64];
+    float rotr[64], roti[64];   // phase per chunk multiplied into the summed partials
     double c0r[64], c0i[64];
 };
 
@@ -62,6 +63,7 @@ struct BdrPlan {
     int WCSP = 0, ldb = 0, ldp = 0;
     int nst = 0;                 // 8-source steps (mixed) of the padded source range
     std::vector<float> sA;       // per-chunk power-of-two scale of A (mixed)
+    std::vector<cfloat> rot;     // per-chunk phase making c2 real (mixed)
     float2* b = nullptr;         // staged sources b = Av B
     const uint4* bop = nullptr;  // split MFMA operand (mixed)
     EpiArgs ea{};
@@ -78,6 +80,6 @@ void bdr_end(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s);
 
 int wcsp_plain(int wcs);
 int wcsp_mixed(int wcs);
-CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk);
+CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk, const cfloat* rot = nullptr);
 
 }  // namespace baemm
