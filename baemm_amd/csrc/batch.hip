@@ -56,16 +56,21 @@ static int mixed_variant(int WCSP, int KG, bool adj)
 }
 
 // Largest LDS need over the variants (all have 4 column blocks or fewer).
-constexpr int kMixOpMax = MixedLds<32, 2>::OP_ELEMS;
-constexpr int kMixGeoMax = MixedLds<32, 2>::GEO_ELEMS;
-static_assert(MixedLds<64, 1>::OP_ELEMS <= kMixOpMax && MixedLds<16, 2>::OP_ELEMS <= kMixOpMax, "LDS bound");
+constexpr int kWsOpMax = WsLds<32, 2>::B_ELEMS;
+constexpr int kWsAMax = WsLds<32, 2>::A_ELEMS;
+constexpr int kWsGeoMax = WsLds<32, 2>::GEO_ELEMS;
+static_assert(WsLds<64, 1>::B_ELEMS <= kWsOpMax && WsLds<64, 1>::A_ELEMS <= kWsAMax, "LDS bound");
+static_assert(WsLds<16, 2>::B_ELEMS <= kWsOpMax && WsLds<16, 2>::A_ELEMS <= kWsAMax, "LDS bound");
 
-__global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_batch_kernel(
+// Wave-specialised blocks of 8 waves (generation waves feed MFMA waves), as
+// the single-instance MFMA kernel.
+__global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
     const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items, const ChunkCoef* __restrict__ coefs,
     int n_items, unsigned* __restrict__ counter)
 {
-    __shared__ uint4 s_op[kMixOpMax];
-    __shared__ float4 s_geo[kMixGeoMax];
+    __shared__ uint4 s_op[kWsOpMax];
+    __shared__ uint4 s_a[kWsAMax];
+    __shared__ float4 s_geo[kWsGeoMax];
     __shared__ int s_item;
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
@@ -77,13 +82,13 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_batch_kernel(
         const BatchInst in = insts[w.inst];
         const ChunkCoef* cf = coefs + in.coef_off + w.q0;
         float* out = in.part + (int64_t)w.seg * in.seg_stride;
-#define BAEMM_MIXB(ID, WCSP_, KG_, ADJ_)                                                                         \
-    case ID: {                                                                                                  \
-        CoefPack cp;                                                                                            \
-        for (int g = 0; g < KG_; ++g) cp.c[g] = cf[g];                                                          \
-        mixed_block<WCSP_, KG_, ADJ_>(in.mid, in.nrm, in.nt, in.bop, in.nst, cp, out, in.ldp, w.q0, w.tile, w.s0, \
-                                      w.s1, s_op, s_geo);                                                       \
-        break;                                                                                                  \
+#define BAEMM_MIXB(ID, WCSP_, KG_, ADJ_)                                                                          \
+    case ID: {                                                                                                   \
+        CoefPack cp;                                                                                             \
+        for (int g = 0; g < KG_; ++g) cp.c[g] = cf[g];                                                           \
+        mixed_ws_block<WCSP_, KG_, ADJ_>(in.mid, in.nrm, in.nt, in.bop, in.nst, cp, out, in.ldp, w.q0, w.tile,   \
+                                         w.s0, w.s1, s_op, s_a, s_geo);                                          \
+        break;                                                                                                   \
     }
         switch (w.variant) {
             BAEMM_MIXB(0, 16, 1, false)
@@ -301,7 +306,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
         BAEMM_HIP(hipGetLastError());
 
         const void* fn = mixed ? (const void*)bdr_mixed_batch_kernel : (const void*)bdr_plain_batch_kernel;
-        const int block = mixed ? 256 : kTile;
+        const int block = mixed ? 512 : kTile;
         const int grid = persistent_grid(fn, block, (int)sorted.size(), mixed ? tune.mixed_grid_mult : tune.plain_grid_mult);
         Handle& h0 = *es[owner[0]].h;
         Timed tm(h0, s);

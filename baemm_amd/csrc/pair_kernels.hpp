@@ -472,17 +472,19 @@ struct WsLds {
     static constexpr int GEO_ELEMS = 2 * GEO;
 };
 
+// One block (8 waves) of the wave-specialised kernel: targets
+// [tile*128, tile*128+128) against source steps [st0, st1) of chunks
+// q0..q0+KG-1, partial sums to out[row*ldp + ...].  s_op, s_a, s_geo: LDS of at
+// least WsLds<WCSP,KG> B_ELEMS, A_ELEMS, GEO_ELEMS elements.  Ends with every
+// wave past a common barrier, so the caller may reuse the LDS.
 template <int WCSP, int KG, bool ADJ>
-__global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
-    const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
-    const uint4* __restrict__ bop, int nst, CoefPack cp,
-    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
+__device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
+                                               const uint4* __restrict__ bop, int nst, const CoefPack& cp,
+                                               float* __restrict__ out, int ldp, int q0, int tile, int st0, int st1,
+                                               uint4* s_op, uint4* s_a, float4* s_geo)
 {
     using L = WsLds<WCSP, KG>;
     constexpr int NCB = L::NCB, OPS = L::OPS, GEO = L::GEO;
-    __shared__ uint4 s_op[L::B_ELEMS];
-    __shared__ uint4 s_a[L::A_ELEMS];
-    __shared__ float4 s_geo[L::GEO_ELEMS];
 
     const int lane = threadIdx.x & 63;
     // wave-uniform role (scalar register), so the branches below are uniform
@@ -491,10 +493,8 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
     const bool producer = wave < 4;
     const int w = wave & 3;
     const int g = lane >> 5;
-    const int ibase = blockIdx.x * kMixTile + w * 32;
+    const int ibase = tile * kMixTile + w * 32;
     const int i = ibase + (lane & 31);
-    const int st0 = blockIdx.y * steps_per_seg;
-    const int st1 = min(st0 + steps_per_seg, nst);
     const int T = max(0, st1 - st0);
     const int nt_pad = nst * 8;
 
@@ -599,7 +599,6 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
         }
-        float* out = part + (int64_t)blockIdx.y * seg_stride;
 #pragma unroll
         for (int q = 0; q < KG; ++q)
 #pragma unroll
@@ -610,6 +609,22 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
                     out[(int64_t)row * ldp + (q0 + q) * 2 * WCSP + cb * 32 + (lane & 31)] = acc[q][cb][r];
                 }
     }
+}
+
+template <int WCSP, int KG, bool ADJ>
+__global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
+    const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
+    const uint4* __restrict__ bop, int nst, CoefPack cp,
+    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
+{
+    using L = WsLds<WCSP, KG>;
+    __shared__ uint4 s_op[L::B_ELEMS];
+    __shared__ uint4 s_a[L::A_ELEMS];
+    __shared__ float4 s_geo[L::GEO_ELEMS];
+    const int st0 = blockIdx.y * steps_per_seg;
+    const int st1 = min(st0 + steps_per_seg, nst);
+    mixed_ws_block<WCSP, KG, ADJ>(mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
+                                  blockIdx.x, st0, st1, s_op, s_a, s_geo);
 }
 
 }  // namespace baemm

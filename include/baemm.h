@@ -112,8 +112,8 @@ int baemm_set_segments(baemm_handle h, int segments);
 /* gmres_max_iter / gmres_max_restarts public members, MemberVariables.hpp:64-65. */
 /* No reference counterpart: the MFMA kernel variant of the mixed precision.
  * 0 = one wave generates its kernel values and issues their MFMAs;
- * 1 = wave-specialised blocks (generation waves feed MFMA waves through LDS).
- * Both give identical results. */
+ * 1 = wave-specialised blocks (generation waves feed MFMA waves through LDS),
+ * the default.  Both give identical results. */
 int baemm_set_mixed_kernel(baemm_handle h, int variant);
 
 int baemm_set_gmres_limits(baemm_handle h, int max_iter, int max_restarts);
