@@ -58,9 +58,12 @@ static int mixed_variant(int WCSP, int KG, bool adj)
 // Largest LDS need over the variants (all have 4 column blocks or fewer).
 constexpr int kWsOpMax = WsLds<32, 2>::B_ELEMS;
 constexpr int kWsAMax = WsLds<32, 2>::A_ELEMS;
-constexpr int kWsGeoMax = WsLds<32, 2>::GEO_ELEMS;
+constexpr int kWsGeoMax = (WsLds<32, 2>::GEO_ELEMS > 2 * MixedLds<16, 1>::GEO_ELEMS) ? WsLds<32, 2>::GEO_ELEMS
+                                                                                       : 2 * MixedLds<16, 1>::GEO_ELEMS;
 static_assert(WsLds<64, 1>::B_ELEMS <= kWsOpMax && WsLds<64, 1>::A_ELEMS <= kWsAMax, "LDS bound");
 static_assert(WsLds<16, 2>::B_ELEMS <= kWsOpMax && WsLds<16, 2>::A_ELEMS <= kWsAMax, "LDS bound");
+static_assert(2 * MixedLds<16, 1>::OP_ELEMS <= kWsOpMax + kWsAMax, "LDS bound");
+static_assert(2 * MixedLds<16, 1>::GEO_ELEMS <= kWsGeoMax, "LDS bound");
 
 // Wave-specialised blocks of 8 waves (generation waves feed MFMA waves), as
 // the single-instance MFMA kernel.
@@ -68,10 +71,11 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
     const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items, const ChunkCoef* __restrict__ coefs,
     int n_items, unsigned* __restrict__ counter)
 {
-    __shared__ uint4 s_op[kWsOpMax];
-    __shared__ uint4 s_a[kWsAMax];
+    __shared__ uint4 s_opa[kWsOpMax + kWsAMax];
     __shared__ float4 s_geo[kWsGeoMax];
     __shared__ int s_item;
+    uint4* const s_op = s_opa;
+    uint4* const s_a = s_opa + kWsOpMax;
     for (;;) {
         if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
         __syncthreads();
@@ -90,8 +94,25 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
                                          w.s0, w.s1, s_op, s_a, s_geo);                                          \
         break;                                                                                                   \
     }
+        // 16-column single chunks without the adjoint double layer: the
+        // single-wave block body on two target tiles at
+        // once (waves 0-3 tile 2t, waves 4-7 tile 2t+1): every wave generates
+        // kernel values, which these generation-bound items need.
+#define BAEMM_MIXB2(ID, KG_, ADJ_)                                                                               \
+    case ID: {                                                                                                   \
+        CoefPack cp;                                                                                             \
+        for (int g = 0; g < KG_; ++g) cp.c[g] = cf[g];                                                           \
+        using LL = MixedLds<16, KG_>;                                                                            \
+        const int half = (threadIdx.x >> 6) >= 4;                                                                \
+        const int t2 = 2 * w.tile + half;                                                                        \
+        const bool real_tile = t2 < in.nt_pad / kMixTile;                                                        \
+        mixed_block<16, KG_, ADJ_>(in.mid, in.nrm, in.nt, in.bop, in.nst, cp, real_tile ? out : nullptr, in.ldp,   \
+                                   w.q0, real_tile ? t2 : 2 * w.tile, w.s0, w.s1, s_op + half * LL::OP_ELEMS,       \
+                                   s_geo + half * LL::GEO_ELEMS);                                                 \
+        break;                                                                                                   \
+    }
         switch (w.variant) {
-            BAEMM_MIXB(0, 16, 1, false)
+            BAEMM_MIXB2(0, 1, false)
             BAEMM_MIXB(1, 16, 1, true)
             BAEMM_MIXB(2, 16, 2, false)
             BAEMM_MIXB(3, 16, 2, true)
@@ -104,6 +125,7 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
             default: break;
         }
 #undef BAEMM_MIXB
+#undef BAEMM_MIXB2
     }
 }
 
@@ -252,12 +274,16 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
                     while (kg * 2 <= lim) kg *= 2;
                 }
                 const int var = mixed ? mixed_variant(pl.WCSP, kg, pl.adj) : plain_variant(pl.WCSP, pl.adj);
+                // single 16-column chunks: items of two 128-target tiles
+                const bool two = mixed && pl.WCSP == 16 && kg == 1 && !pl.adj;
+                const int item_tiles = two ? (tiles + 1) / 2 : tiles;
                 for (int seg = 0; seg < S; ++seg) {
                     const int s0 = seg * len, s1 = std::min(s0 + len, nsrc_units);
                     // cost model: kernel-value generation per chunk plus contraction per column
-                    const double per = mixed ? (double)(s1 - s0) * 8.0 * (kg * (1.0 + 0.35 * pl.WCSP / 32.0))
-                                             : (double)(s1 - s0) * (kg * (1.0 + 0.12 * pl.WCSP));
-                    for (int t = 0; t < tiles; ++t) {
+                    double per = mixed ? (double)(s1 - s0) * 8.0 * (kg * (1.0 + 0.35 * pl.WCSP / 32.0))
+                                       : (double)(s1 - s0) * (kg * (1.0 + 0.12 * pl.WCSP));
+                    if (two) per *= 2.0;
+                    for (int t = 0; t < item_tiles; ++t) {
                         items.push_back(BatchItem{inst_id, t, s0, s1, q0, var, seg, 0});
                         cost.push_back(per);
                     }

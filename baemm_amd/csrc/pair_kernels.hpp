@@ -325,7 +325,8 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
     using L = MixedLds<WCSP, KG>;
     constexpr int NCB = L::NCB, OPS = L::OPS, NS = L::NS, TILES = L::TILES, GEO = L::GEO;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // 4 waves per tile; a 512-thread block may run two tiles side by side
+    const int wave = (threadIdx.x >> 6) & 3;
     const int g = lane >> 5;
     const int ibase = tile * kMixTile + wave * 32;
     const int i = ibase + (lane & 31);
@@ -419,15 +420,17 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
                     acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo[q], b_hi[q][cb], acc[q][cb], 0, 0, 0);
         }
     }
+    if (out) {   // NULL: a tile computed only to keep the block's barriers in step
 #pragma unroll
-    for (int q = 0; q < KG; ++q)
+        for (int q = 0; q < KG; ++q)
 #pragma unroll
-        for (int cb = 0; cb < NCB; ++cb)
+            for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = ibase + (r >> 2) * 8 + g * 4 + (r & 3);
-                out[(int64_t)row * ldp + (q0 + q) * 2 * WCSP + cb * 32 + (lane & 31)] = acc[q][cb][r];
-            }
+                for (int r = 0; r < 16; ++r) {
+                    const int row = ibase + (r >> 2) * 8 + g * 4 + (r & 3);
+                    out[(int64_t)row * ldp + (q0 + q) * 2 * WCSP + cb * 32 + (lane & 31)] = acc[q][cb][r];
+                }
+    }
     __syncthreads();
 }
 
