@@ -37,7 +37,7 @@ PEAK_FP32 = 157.3        # TFLOP/s, FP32 vector, MI355X
 G_SLDL = 43              # algorithmic flops per pair per wave number, SL+DL (BoundaryOperatorKernel.cl:104-170)
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "bdr_mixed_kernel_icosphere81920_w64_traffic.txt"
+TRAFFIC_PROFILE = ROOT / "profiles" / "bdr_mixed_ws_kernel_icosphere81920_w64_traffic.txt"
 
 
 def log(msg):
@@ -49,7 +49,7 @@ def profiled_traffic():
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
     try:
         for line in TRAFFIC_PROFILE.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch bdr_mixed_kernel:"):
+            if line.startswith("traffic_bytes_per_launch bdr_mixed_ws_kernel:"):
                 return float(line.split(":")[1])
     except OSError:
         pass
