@@ -94,6 +94,7 @@ struct Workspace {
     DevBuf V, w, t, x, r, hc;             // GMRES basis and vectors, Gram-Schmidt coefficients
     DevBuf a, b, c, d, e;                 // composition temporaries
     DevBuf cgctr;                         // PCG counters: done, iterations, total
+    bool cg_block = false;                // the last PCG ran as one block launch
     DevBuf sc;                            // per-column scales of the Krylov vectors
 };
 

@@ -450,6 +450,159 @@ void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, doub
     BAEMM_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- block PCG
+// The whole PCG of a group of CGC columns in one work-group (each block owns
+// its columns for all rows, so every reduction is a block reduction and there
+// is no grid-wide synchronisation): one launch instead of ~13 per iteration.
+// Same algorithm and stopping rule as the multi-kernel PCG below; used when
+// the mesh is small enough that launches, not memory traffic, dominate.
+// work: per block 3 * n * CGC doubles (r, p, q).  ctr[1]: max iterations over
+// blocks (zeroed before the launch), ctr[2] += iterations by pcg_total_kernel.
+template <int CGC>
+__global__ __launch_bounds__(256) void pcg_block_kernel(int n, const int32_t* __restrict__ M_ptr,
+                                                        const int32_t* __restrict__ M_idx,
+                                                        const double* __restrict__ M_val,
+                                                        const double* __restrict__ Dinv, const double* __restrict__ X,
+                                                        int64_t ldX, double* __restrict__ Y, int64_t ldY, int ncols,
+                                                        double tol2, int max_iter, double* __restrict__ work,
+                                                        int* __restrict__ ctr)
+{
+    __shared__ double red[3][CGC][8];
+    __shared__ double s_rz[CGC], s_bb[CGC];
+    __shared__ int s_active[CGC];
+    const int c0 = blockIdx.x * CGC;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double* r = work + (int64_t)blockIdx.x * 3 * n * CGC;
+    double* p = r + (int64_t)n * CGC;
+    double* q = p + (int64_t)n * CGC;
+
+    // block sum of per-thread values v[k][c] for k < K (waves reduce by shuffles, then LDS)
+    auto block_sum = [&](double (&v)[3][CGC], int K) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= K) break;
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) {
+                double x = v[k][c];
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+                if (lane == 0) red[k][c][wv] = x;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= K) break;
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) v[k][c] = red[k][c][0] + red[k][c][1] + red[k][c][2] + red[k][c][3];
+        }
+        __syncthreads();
+    };
+
+    // r = X, Y = 0, p = z = D r; bb = r.r, rz = r.z
+    double acc[3][CGC];
+#pragma unroll
+    for (int c = 0; c < CGC; ++c) acc[0][c] = acc[1][c] = acc[2][c] = 0.0;
+    for (int v = tid; v < n; v += 256) {
+        const double d = Dinv[v];
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) {
+            const bool ok = c0 + c < ncols;
+            const double x = ok ? X[(int64_t)v * ldX + c0 + c] : 0.0;
+            if (ok) Y[(int64_t)v * ldY + c0 + c] = 0.0;
+            r[(int64_t)v * CGC + c] = x;
+            p[(int64_t)v * CGC + c] = d * x;
+            acc[0][c] += x * x;
+            acc[1][c] += x * d * x;
+        }
+    }
+    block_sum(acc, 2);
+    if (tid == 0) {
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) {
+            s_bb[c] = acc[0][c];
+            s_rz[c] = acc[1][c];
+            s_active[c] = (c0 + c < ncols) && acc[0][c] > 0.0;
+        }
+    }
+    __syncthreads();
+    int it = 0;
+    for (;;) {
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) any = any || s_active[c];
+        if (!any || it >= max_iter) break;
+        ++it;
+        // q = M p, pq
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) acc[0][c] = 0.0;
+        for (int v = tid; v < n; v += 256) {
+            double s[CGC];
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) s[c] = 0.0;
+            for (int e = M_ptr[v]; e < M_ptr[v + 1]; ++e) {
+                const double m = M_val[e];
+                const int64_t u = (int64_t)M_idx[e] * CGC;
+#pragma unroll
+                for (int c = 0; c < CGC; ++c) s[c] += m * p[u + c];
+            }
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) {
+                q[(int64_t)v * CGC + c] = s[c];
+                acc[0][c] += p[(int64_t)v * CGC + c] * s[c];
+            }
+        }
+        block_sum(acc, 1);
+        double alpha[CGC];
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) alpha[c] = (s_active[c] && acc[0][c] != 0.0) ? s_rz[c] / acc[0][c] : 0.0;
+        // Y += alpha p, r -= alpha q; rr, rz_new with z = D r
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) acc[0][c] = acc[1][c] = 0.0;
+        for (int v = tid; v < n; v += 256) {
+            const double d = Dinv[v];
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) {
+                const int64_t k = (int64_t)v * CGC + c;
+                if (c0 + c < ncols) Y[(int64_t)v * ldY + c0 + c] += alpha[c] * p[k];
+                const double rv = r[k] - alpha[c] * q[k];
+                r[k] = rv;
+                acc[0][c] += rv * rv;
+                acc[1][c] += rv * d * rv;
+            }
+        }
+        block_sum(acc, 2);
+        double beta[CGC];
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) {
+            beta[c] = (s_rz[c] != 0.0) ? acc[1][c] / s_rz[c] : 0.0;
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) {
+                if (s_active[c] && acc[0][c] <= tol2 * s_bb[c]) s_active[c] = 0;
+                s_rz[c] = acc[1][c];
+            }
+        }
+        // p = z + beta p
+        for (int v = tid; v < n; v += 256) {
+            const double d = Dinv[v];
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) {
+                const int64_t k = (int64_t)v * CGC + c;
+                p[k] = d * r[k] + beta[c] * p[k];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) atomicMax(&ctr[1], it);
+}
+
+// ctr[2] += ctr[1] (iterations of the block PCG just run); ctr[1] = 0 before it.
+__global__ void pcg_reset_kernel(int* __restrict__ ctr) { ctr[1] = 0; }
+__global__ void pcg_total_kernel(int* __restrict__ ctr) { ctr[2] += ctr[1]; }
+
 // PCG for M Y = X on real columns (ApplyMassInverse.hpp:6-156): zero initial
 // guess, lumped-diagonal preconditioner, at most max_iter iterations, per
 // column relative residual ||r|| <= tol ||X||.  X: n x ncols (ldX), Y: (ldY).
@@ -464,6 +617,28 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
 {
     const int64_t n = h.nv;
     const int64_t N = n * ncols;
+    if (n <= kPcgBlockMaxRows) {
+        int* ctr = dev_cg_counters(ws, s);
+        const int CGC = ncols >= 64 ? 4 : (ncols >= 16 ? 2 : 1);
+        const int nb = (ncols + CGC - 1) / CGC;
+        ws.cg.reserve(sizeof(double) * 3 * n * (int64_t)CGC * nb);
+        ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
+        // the convergence flags read by dev_mass_cg_result: all columns converged
+        // or stopped at max_iter; the block version reports via the iteration count
+        hipLaunchKernelGGL(pcg_reset_kernel, dim3(1), dim3(1), 0, s, ctr);
+#define BAEMM_PCGB(C_)                                                                                          \
+    hipLaunchKernelGGL(pcg_block_kernel<C_>, dim3(nb), dim3(256), 0, s, (int)n, h.d_M_ptr, h.d_M_idx, h.d_M_val,   \
+                       h.d_lumped_inv, X, ldX, Y, ldY, ncols, tol * tol, max_iter, ws.cg.as<double>(), ctr)
+        if (CGC == 4) BAEMM_PCGB(4);
+        else if (CGC == 2) BAEMM_PCGB(2);
+        else BAEMM_PCGB(1);
+#undef BAEMM_PCGB
+        hipLaunchKernelGGL(pcg_total_kernel, dim3(1), dim3(1), 0, s, ctr);
+        BAEMM_HIP(hipGetLastError());
+        ws.cg_block = true;
+        return;
+    }
+    ws.cg_block = false;
     ws.cg.reserve(sizeof(double) * (4 * N));
     ws.red.reserve(sizeof(double) * kRedBlocks * ncols);
     ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
@@ -522,6 +697,13 @@ int* dev_cg_counters(Workspace& ws, hipStream_t s)
 // (synchronises the stream).
 int dev_mass_cg_result(Workspace& ws, int ncols, bool* converged, hipStream_t s)
 {
+    if (ws.cg_block) {
+        int ctr[4] = {0, 0, 0, 0};
+        BAEMM_HIP(hipMemcpyAsync(ctr, ws.cgctr.ptr, sizeof(ctr), hipMemcpyDeviceToHost, s));
+        BAEMM_HIP(hipStreamSynchronize(s));
+        if (converged) *converged = ctr[1] < 20;
+        return ctr[1];
+    }
     const double* bb = ws.scal.as<double>() + 6 * ncols;
     const int* active = reinterpret_cast<const int*>(bb + 2 * ncols);
     std::vector<int> a(ncols);

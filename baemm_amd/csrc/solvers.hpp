@@ -10,6 +10,7 @@
 namespace baemm {
 
 constexpr int kRedBlocks = 240;   // blocks of the first pass of column reductions
+constexpr int kPcgBlockMaxRows = 16384;   // meshes up to this many vertices run the one-launch block PCG
 
 void dev_mass(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
               hipStream_t s, const int* skip = nullptr);
