@@ -58,6 +58,7 @@ SIGNATURES = {
     "baemm_apply_near_field": (_INT, [_P, _P, _P, _I64, _P, _P, _I64, _P, _I64]),
     "baemm_create_herglotz_wave": (_INT, [_P, _P, _P, _I64, _P, _P, _I64]),
     "baemm_create_incident_wave": (_INT, [_P, _P, _P, _I64, _P, _P, _I64, _INT]),
+    "baemm_autotune_segments": (_INT, [_P, _INT, ctypes.POINTER(_INT), ctypes.POINTER(_D)]),
     "baemm_apply_boundary_dev": (_INT, [_P, _P, _P, _I64, _INT, _P, _P, _I64, _INT, _P]),
     "baemm_apply_far_field_dev": (_INT, [_P, _P, _P, _I64, _INT, _P, _P, _I64, _INT, _P]),
     "baemm_apply_boundary_batch": (_INT, [_INT, _P, _P, _P, _P, _INT, _P, _P, _P, _INT, _P]),

@@ -1,4 +1,5 @@
 // capi.cpp -- extern "C" entry points of libbaemm_hip (declared in include/baemm.h).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -414,6 +415,65 @@ int baemm_create_incident_wave(baemm_handle h, const double* alpha, const double
             BAEMM_HIP(hipMemcpy2DAsync(C, sizeof(cdouble) * ldC, hh.ws_out.ptr, sizeof(cdouble) * W,
                                        sizeof(cdouble) * W, hh.nv, hipMemcpyDeviceToHost, s));
         BAEMM_HIP(hipStreamSynchronize(s));
+        return BAEMM_OK;
+    });
+}
+
+int baemm_autotune_segments(baemm_handle h, int repeats, int* best_segments, double* best_ms)
+{
+    return guard(h, [&](Handle& hh) {
+        if (!hh.prm.loaded) throw StateError{"operators are not loaded"};
+        const int64_t W = hh.prm.W;
+        const size_t bytes = sizeof(cdouble) * hh.nv * W;
+        hh.io[4].reserve(bytes);
+        hh.io[5].reserve(bytes);
+        std::vector<cdouble> Bh((size_t)hh.nv * W);
+        for (size_t k = 0; k < Bh.size(); ++k) Bh[k] = cdouble(std::sin(0.37 * k), std::cos(0.11 * k));
+        hipStream_t s = hh.stream;
+        BAEMM_HIP(hipMemcpyAsync(hh.io[4].ptr, Bh.data(), bytes, hipMemcpyHostToDevice, s));
+        BAEMM_HIP(hipStreamSynchronize(s));
+        ApplyArgs a{cdouble(1.0, 0.0), cdouble(0.0, 0.0), hh.io[4].ptr, W, kC128, hh.io[5].ptr, W, kC128};
+        const int saved = hh.segments;
+        const int reps = std::max(1, repeats);
+        const bool mixed = hh.precision == BAEMM_PRECISION_MIXED;
+        const int64_t tiles = hh.nt_pad / (mixed ? kMixTile : kTile);
+        const int64_t max_seg = mixed ? hh.nt_pad / 64 : hh.nt_pad / kSrcBlock;
+        hipEvent_t e0, e1;
+        BAEMM_HIP(hipEventCreate(&e0));
+        BAEMM_HIP(hipEventCreate(&e1));
+        int best = 0;
+        double bt = 1e300;
+        std::vector<int> tried;
+        try {
+            // candidate source-segment counts: launches of about 512 .. 8192 blocks
+            for (int64_t blocks = 512; blocks <= 8192; blocks *= 2) {
+                const int S = (int)std::max<int64_t>(1, std::min<int64_t>(max_seg, (blocks + tiles - 1) / tiles));
+                if (std::find(tried.begin(), tried.end(), S) != tried.end()) continue;
+                tried.push_back(S);
+                hh.segments = S;
+                apply_boundary_dev(hh, a, s);                  // sizes the partial buffer
+                BAEMM_HIP(hipEventRecord(e0, s));
+                for (int r = 0; r < reps; ++r) apply_boundary_dev(hh, a, s);
+                BAEMM_HIP(hipEventRecord(e1, s));
+                BAEMM_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                BAEMM_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms / reps < bt) {
+                    bt = ms / reps;
+                    best = S;
+                }
+            }
+        } catch (...) {
+            hh.segments = saved;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        hh.segments = best;
+        if (best_segments) *best_segments = best;
+        if (best_ms) *best_ms = bt;
         return BAEMM_OK;
     });
 }

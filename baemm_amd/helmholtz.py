@@ -130,6 +130,13 @@ class _HelmholtzBase:
         """MFMA kernel variant of the mixed precision: 0 single-wave, 1 wave-specialised."""
         self._check(self._L.baemm_set_mixed_kernel(self._h, int(variant)))
 
+    def AutotuneSegments(self, repeats=3):
+        """Times source-segment counts of the loaded operator and keeps the fastest;
+        returns (segments, milliseconds per apply).  No reference counterpart."""
+        seg, ms = ctypes.c_int(), ctypes.c_double()
+        self._check(self._L.baemm_autotune_segments(self._h, int(repeats), ctypes.byref(seg), ctypes.byref(ms)))
+        return seg.value, ms.value
+
     def SetSegments(self, n):
         self._check(self._L.baemm_set_segments(self._h, int(n)))
 

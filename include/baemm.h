@@ -195,6 +195,13 @@ int baemm_create_incident_wave(baemm_handle h, const double* alpha, const double
                                int64_t inc_count, const double* beta, double* C, int64_t ldC,
                                int wave_type);
 
+/* No reference counterpart: autotuner of the single-instance apply.  Times the
+ * loaded boundary operator (current precision) with source-segment counts
+ * giving launches of about 512..8192 blocks, on synthetic data in the handle's
+ * own buffers, and keeps the fastest (baemm_set_segments); later applies and
+ * GMRES solves with this handle use it. */
+int baemm_autotune_segments(baemm_handle h, int repeats, int* best_segments, double* best_ms);
+
 /* Device-pointer variants.  B_type / C_type: BAEMM_C128 or BAEMM_C64. */
 int baemm_apply_boundary_dev(baemm_handle h, const double* alpha, const void* B, int64_t ldB, int B_type,
                              const double* beta, void* C, int64_t ldC, int C_type, void* stream);

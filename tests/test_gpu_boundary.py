@@ -185,3 +185,25 @@ def test_wave_specialised_mixed_kernel_is_identical(sphere3, W, wcs, kappa, c3):
     assert np.array_equal(outs[0], outs[1])
     ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
     assert relerr(outs[1], ref) < TOL
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+def test_segment_autotuner_keeps_results(sphere3, mode):
+    """baemm_autotune_segments picks a segment count; applies after tuning still
+    match the FP64 oracle."""
+    m, H = sphere3
+    rng = np.random.default_rng(17)
+    W, wcs, kappa = 16, 8, [2.0, 5.0]
+    coeff = np.array(FULL_COEFF * 2)
+    H.SetPrecision(mode)
+    H.LoadBoundaryOperators_PL(kappa, coeff, W, wcs)
+    seg, ms = H.AutotuneSegments(repeats=2)
+    print(mode, "autotuned segments", seg, ms)
+    assert seg >= 1 and ms > 0
+    B = _rand(rng, m.nv, W)
+    C = np.zeros((m.nv, W), np.complex128)
+    H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C)
+    ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
+    assert relerr(C, ref) < TOL
+    H.SetSegments(0)
+    H.SetPrecision("fp32")
