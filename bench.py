@@ -362,7 +362,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the torus solve leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=6.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--serve-rate", type=float, default=40.0,
+                    help="Poisson arrival rate of the serving leg (1/s); the default saturates the server")
     ap.add_argument("--serve-max-level", type=int, default=6,
                     help="largest icosphere level of the serving leg (6: 81920, 7: 327680 triangles)")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -454,6 +455,7 @@ def main():
                         f"adjoint derivative of the misfit), GMRES 1e-6, Poisson arrivals {args.serve_rate}/s, "
                         f"perturbed icospheres 1280..{1280 * 4 ** (args.serve_max_level - 3)} triangles "
                         f"(log-uniform), W = 1..64, 256 far-field directions",
+            "offered_rate_per_s": args.serve_rate,
             "continuous_mixed": dict({k: mixed_st[k] for k in keys}, slots=args.serve_slots),
             "sequential_fp32": dict({k: seq_st[k] for k in keys}, slots=1),
             "speedup_solves_per_s": mixed_st["solves_per_s"] / max(seq_st["solves_per_s"], 1e-12),
