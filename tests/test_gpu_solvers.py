@@ -244,3 +244,20 @@ def test_c3b_icosphere_81920_against_mie():
     err = relerr(F, ref)
     print("Mie error 81920", err)
     assert err < 5e-3
+
+
+def test_cpp_dropin_surface_checks():
+    """Every method of the C++ drop-in class on a small sphere: overloads that
+    must agree, M (M^-1 X) = X, the solvers' equations, the Gauss-Newton
+    residual (baemm_amd/examples/dropin_check.cpp)."""
+    import subprocess
+    from pathlib import Path
+
+    exe = Path(__file__).resolve().parents[1] / "baemm_amd" / "examples" / "dropin_check"
+    if not exe.exists():
+        pytest.fail(f"{exe} not built")
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=240)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0
+    assert "FAIL" not in out.stdout
+    assert out.stdout.count("PASS") >= 20
