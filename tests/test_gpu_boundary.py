@@ -207,3 +207,25 @@ def test_segment_autotuner_keeps_results(sphere3, mode):
     assert relerr(C, ref) < TOL
     H.SetSegments(0)
     H.SetPrecision("fp32")
+
+
+def test_largest_mesh_327680_triangles_sampled_rows():
+    """Largest mesh of the serving range: icosphere with 327,680 triangles
+    (163,842 vertices), kappa = 32, 64 RHS, SL+DL in both precisions against
+    the FP64 oracle on 16 sampled vertex rows."""
+    V, T = icosphere(7)
+    m = oracle.Mesh(V, T)
+    rng = np.random.default_rng(7)
+    W, wcs = 64, 64
+    B = _rand(rng, m.nv, W)
+    coeff = np.array([0.5, -32j, 1, 0], np.complex128)
+    rows = np.sort(rng.choice(m.nv, 16, replace=False))
+    ref = oracle.apply_boundary_rows(m, 1.0, B, 0.0, np.zeros((16, W)), [32.0], coeff, W, wcs, rows)
+    with Helmholtz_HIP(V, T) as H:
+        for mode in ("mixed", "fp32"):
+            H.SetPrecision(mode)
+            C = np.zeros_like(B)
+            H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [32.0], coeff, W, wcs)
+            e = relerr(C[rows], ref)
+            print(f"327680 {mode}: sampled-row rel. error vs FP64 = {e:.3e}")
+            assert e < TOL
