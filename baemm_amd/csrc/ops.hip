@@ -436,12 +436,15 @@ static int pick_kg(int WCSP, int remaining, bool mixed)
 }
 
 
-// Number of source segments: enough work-groups to fill 256 CUs several times.
-static int choose_segments(const Handle& h, int tiles, int max_seg)
+// Number of source segments: enough work-groups to fill 256 CUs several times,
+// and at most kMaxSegSources sources per segment, which bounds the length of
+// each FP32 accumulation (the segment partials are summed in FP64).
+constexpr int kMaxSegSources = 32768;
+static int choose_segments(const Handle& h, int tiles, int max_seg, int64_t nsrc)
 {
-    if (h.segments > 0) return std::min(h.segments, max_seg);
-    const int want = (2048 + tiles - 1) / tiles;
-    return std::max(1, std::min(want, max_seg));
+    const int min_seg = (int)((nsrc + kMaxSegSources - 1) / kMaxSegSources);
+    const int want = h.segments > 0 ? h.segments : (2048 + tiles - 1) / tiles;
+    return std::max(1, std::min(std::max(want, min_seg), max_seg));
 }
 
 // Runs a plain pair kernel over all chunk groups: part has ntgt_pad rows.
@@ -634,7 +637,7 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
     if (!pl.mixed) {
         const int tiles = (int)(h.nt_pad / kTile);
         const int max_seg = (int)(h.nt_pad / kSrcBlock);
-        const int S0 = choose_segments(h, tiles, max_seg);
+        const int S0 = choose_segments(h, tiles, max_seg, h.nt);
         const int seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
         const int S = (int)((h.nt_pad + seg_len - 1) / seg_len);
         float* part = bdr_reserve_part(h, pl, S);
@@ -644,7 +647,7 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
     }
     const int nst = pl.nst;
     const int tiles = (int)(h.nt_pad / kMixTile);
-    const int S0 = choose_segments(h, tiles, nst / 8);
+    const int S0 = choose_segments(h, tiles, nst / 8, h.nt);
     const int steps = (nst + S0 - 1) / S0;
     const int S = (nst + steps - 1) / steps;
     float* part = bdr_reserve_part(h, pl, S);
@@ -707,7 +710,7 @@ static void apply_rows_op(Handle& h, int op, const ApplyArgs& a, const float4* t
     float2* b = h.ws_b32.as<float2>();
     launch_av(h, a.B, a.ldB, a.Btype, WCSP, b, ldb, s);
     const int tiles = ntgt_pad / kTile;
-    const int S0 = choose_segments(h, tiles, (int)(h.nt_pad / kSrcBlock));
+    const int S0 = choose_segments(h, tiles, (int)(h.nt_pad / kSrcBlock), h.nt);
     const int seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
     const int S = (int)((h.nt_pad + seg_len - 1) / seg_len);
     const int64_t seg_stride = (int64_t)ntgt_pad * ldp;
@@ -751,7 +754,7 @@ void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
                            WCSP, ldb, 1, gb);
     BAEMM_HIP(hipGetLastError());
     const int tiles = (int)(h.nt_pad / kTile);
-    const int S0 = choose_segments(h, tiles, (int)(h.m_pad / kSrcBlock));
+    const int S0 = choose_segments(h, tiles, (int)(h.m_pad / kSrcBlock), h.m);
     const int seg_len = (int)(((h.m_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
     const int S = (int)((h.m_pad + seg_len - 1) / seg_len);
     const int64_t seg_stride = h.nt_pad * (int64_t)ldp;

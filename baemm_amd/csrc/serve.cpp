@@ -248,7 +248,15 @@ void Server::worker()
         peak = std::max(peak, resident + reserved);
         rq->t_start = Clock::now();
         lk.unlock();
-        run(*rq, *c);
+        try {
+            run(*rq, *c);
+        } catch (const std::exception& e) {   // e.g. host allocation: fail the request, keep the worker
+            rq->status = BAEMM_ERR_INTERNAL;
+            rq->error = e.what();
+        } catch (...) {
+            rq->status = BAEMM_ERR_INTERNAL;
+            rq->error = "unknown error";
+        }
         lk.lock();
         --running;
         reserved -= rq->need;
