@@ -229,3 +229,21 @@ def test_largest_mesh_327680_triangles_sampled_rows():
             e = relerr(C[rows], ref)
             print(f"327680 {mode}: sampled-row rel. error vs FP64 = {e:.3e}")
             assert e < TOL
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+def test_smallest_mesh_icosahedron(mode):
+    """20 triangles (one padded tile of 256 rows): padding rows and masked
+    steps only, against the FP64 oracle."""
+    V, T = icosphere(0)
+    m = oracle.Mesh(V, T)
+    rng = np.random.default_rng(2)
+    W, wcs = 4, 2
+    B = _rand(rng, m.nv, W)
+    coeff = np.array(FULL_COEFF * 2)
+    ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), [1.0, 3.0], coeff, W, wcs)
+    with Helmholtz_HIP(V, T) as H:
+        H.SetPrecision(mode)
+        C = np.zeros_like(B)
+        H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [1.0, 3.0], coeff, W, wcs)
+    assert relerr(C, ref) < TOL
