@@ -391,7 +391,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 kernel values, f16 hi+lo MFMA contraction, f32 accumulate",
+        "dtype": "f32",
+        "arithmetic": "kernel values generated in f32; complex contraction as f16 hi+lo products on MFMA with f32 "
+                      "accumulation (split FP32); segment partials summed in f64",
         "data": "synthetic",
         "config": {
             "workload": "SL+DL boundary-operator apply, icosphere 81920 triangles / 40962 vertices, "
