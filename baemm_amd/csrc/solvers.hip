@@ -259,67 +259,9 @@ __global__ void cg_init_kernel(int ncols, const double* __restrict__ bb, int* __
     }
 }
 
-// alpha = rz / pq per column (0 when a column is frozen)
-__global__ void cg_alpha_kernel(int ncols, const double* __restrict__ rz, const double* __restrict__ pq,
-                                const int* __restrict__ active, double* __restrict__ alpha, const int* __restrict__ skip)
-{
-    if (skip && *skip) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    alpha[c] = (active[c] && pq[c] != 0.0) ? rz[c] / pq[c] : 0.0;
-}
 
-// x += alpha p; r -= alpha q
-__global__ void cg_update_xr_kernel(int64_t n, int ncols, const double* __restrict__ alpha,
-                                    const double* __restrict__ p, const double* __restrict__ q,
-                                    double* __restrict__ x, int64_t ldx, double* __restrict__ r,
-                                    const int* __restrict__ skip)
-{
-    if (skip && *skip) return;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * ncols) return;
-    const int64_t v = t / ncols;
-    const int c = (int)(t % ncols);
-    const double a = alpha[c];
-    x[v * ldx + c] += a * p[t];
-    r[t] -= a * q[t];
-}
 
-// Column c converges when ||r_c||^2 <= tol^2 ||X_c||^2 (then it is frozen);
-// the solve is done when no column is active or after max_iter iterations.
-// One block; counts the iterations done.
-__global__ void cg_check_kernel(int ncols, const double* __restrict__ rr, const double* __restrict__ bb, double tol2,
-                                int max_iter, int* __restrict__ active, int* __restrict__ done,
-                                int* __restrict__ iters, int* __restrict__ total)
-{
-    if (*done) return;
-    __shared__ int any;
-    if (threadIdx.x == 0) any = 0;
-    __syncthreads();
-    for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
-        if (!active[c]) continue;
-        if (rr[c] <= tol2 * bb[c]) active[c] = 0;
-        else atomicOr(&any, 1);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int it = *iters + 1;
-        *iters = it;
-        *total += 1;
-        if (!any || it >= max_iter) *done = 1;
-    }
-}
 
-// beta = rz_new / rz; rz = rz_new
-__global__ void cg_beta_kernel(int ncols, double* __restrict__ rz, const double* __restrict__ rz_new,
-                               double* __restrict__ beta, const int* __restrict__ skip)
-{
-    if (skip && *skip) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    beta[c] = (rz[c] != 0.0) ? rz_new[c] / rz[c] : 0.0;
-    rz[c] = rz_new[c];
-}
 
 // p = z + beta p
 __global__ void cg_update_p_kernel(int64_t n, int ncols, const double* __restrict__ beta, const double* __restrict__ z,
@@ -448,6 +390,123 @@ void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, doub
 {
     hipLaunchKernelGGL(scale_cols_kernel, grid_for(n * W), dim3(256), 0, s, (int)n, W, src, sc, dst);
     BAEMM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- fused PCG steps
+// Row-blocked like coldot_part_kernel (block b: rows b*4+rl, stride grid*4;
+// 64 column lanes), deterministic two-pass reductions.
+
+// q = M p and part[b][c] = sum over the block's rows of p q
+__global__ __launch_bounds__(256) void cg_spmv_dot_kernel(int n, int ncols, const int32_t* __restrict__ ptr,
+                                                          const int32_t* __restrict__ idx,
+                                                          const double* __restrict__ val, const double* __restrict__ p,
+                                                          double* __restrict__ q, double* __restrict__ part,
+                                                          const int* __restrict__ skip)
+{
+    if (*skip) return;
+    __shared__ double sm[4][64];
+    const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < ncols; c0 += 64) {
+        double acc = 0.0;
+        if (c0 + c < ncols)
+            for (int r = blockIdx.x * 4 + rl; r < n; r += gridDim.x * 4) {
+                double sv = 0.0;
+                for (int e = ptr[r]; e < ptr[r + 1]; ++e) sv += val[e] * p[(int64_t)idx[e] * ncols + c0 + c];
+                q[(int64_t)r * ncols + c0 + c] = sv;
+                acc += p[(int64_t)r * ncols + c0 + c] * sv;
+            }
+        sm[rl][c] = acc;
+        __syncthreads();
+        if (rl == 0 && c0 + c < ncols)
+            part[(int64_t)blockIdx.x * ncols + c0 + c] = sm[0][c] + sm[1][c] + sm[2][c] + sm[3][c];
+        __syncthreads();
+    }
+}
+
+// pq = sum_b part[b]; alpha = active ? rz / pq : 0
+__global__ void cg_alpha_sum_kernel(int nblocks, int ncols, const double* __restrict__ part,
+                                    const double* __restrict__ rz, const int* __restrict__ active,
+                                    double* __restrict__ alpha, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    double pq = 0.0;
+    for (int b = 0; b < nblocks; ++b) pq += part[(int64_t)b * ncols + c];
+    alpha[c] = (active[c] && pq != 0.0) ? rz[c] / pq : 0.0;
+}
+
+// Y += alpha p; r -= alpha q; z = D r; part1 = r.r, part2 = r.z (block partials)
+__global__ __launch_bounds__(256) void cg_update_dots_kernel(int n, int ncols, const double* __restrict__ alpha,
+                                                             const double* __restrict__ p,
+                                                             const double* __restrict__ q,
+                                                             const double* __restrict__ Dinv, double* __restrict__ Y,
+                                                             int64_t ldY, double* __restrict__ r,
+                                                             double* __restrict__ z, double* __restrict__ part1,
+                                                             double* __restrict__ part2, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    __shared__ double sm1[4][64], sm2[4][64];
+    const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < ncols; c0 += 64) {
+        double a1 = 0.0, a2 = 0.0;
+        if (c0 + c < ncols) {
+            const double al = alpha[c0 + c];
+            for (int v = blockIdx.x * 4 + rl; v < n; v += gridDim.x * 4) {
+                const int64_t k = (int64_t)v * ncols + c0 + c;
+                Y[(int64_t)v * ldY + c0 + c] += al * p[k];
+                const double rv = r[k] - al * q[k];
+                r[k] = rv;
+                const double zv = Dinv[v] * rv;
+                z[k] = zv;
+                a1 += rv * rv;
+                a2 += rv * zv;
+            }
+        }
+        sm1[rl][c] = a1;
+        sm2[rl][c] = a2;
+        __syncthreads();
+        if (rl == 0 && c0 + c < ncols) {
+            part1[(int64_t)blockIdx.x * ncols + c0 + c] = sm1[0][c] + sm1[1][c] + sm1[2][c] + sm1[3][c];
+            part2[(int64_t)blockIdx.x * ncols + c0 + c] = sm2[0][c] + sm2[1][c] + sm2[2][c] + sm2[3][c];
+        }
+        __syncthreads();
+    }
+}
+
+// rr, rz_new from the partials; freeze converged columns; beta; rz = rz_new;
+// the solve is done when no column is active or after max_iter iterations.
+// One block.
+__global__ void cg_check_beta_kernel(int nblocks, int ncols, const double* __restrict__ part1,
+                                     const double* __restrict__ part2, const double* __restrict__ bb, double tol2,
+                                     int max_iter, int* __restrict__ active, double* __restrict__ rz,
+                                     double* __restrict__ beta, int* __restrict__ done, int* __restrict__ iters,
+                                     int* __restrict__ total)
+{
+    if (*done) return;
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+        double rr = 0.0, rzn = 0.0;
+        for (int b = 0; b < nblocks; ++b) {
+            rr += part1[(int64_t)b * ncols + c];
+            rzn += part2[(int64_t)b * ncols + c];
+        }
+        if (active[c]) {
+            if (rr <= tol2 * bb[c]) active[c] = 0;
+            else atomicOr(&any, 1);
+        }
+        beta[c] = (rz[c] != 0.0) ? rzn / rz[c] : 0.0;
+        rz[c] = rzn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int it = *iters + 1;
+        *iters = it;
+        *total += 1;
+        if (!any || it >= max_iter) *done = 1;
+    }
 }
 
 // ---------------------------------------------------------------- block PCG
@@ -640,7 +699,7 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     }
     ws.cg_block = false;
     ws.cg.reserve(sizeof(double) * (4 * N));
-    ws.red.reserve(sizeof(double) * kRedBlocks * ncols);
+    ws.red.reserve(sizeof(double) * 2 * kRedBlocks * ncols);
     ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
     int* ctr = dev_cg_counters(ws, s);
     double* r = ws.cg.as<double>();
@@ -664,18 +723,20 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(256), 0, s, ncols, bb, active, done, iters);
     dev_lumped(h, r, ncols, p, ncols, ncols, s, done);      // p = z = D r
     dev_coldot(ws, n, ncols, r, ncols, p, ncols, rz, s, done);
+    // five launches per iteration: q = M p with p.q partials; alpha; updates of
+    // Y, r, z with r.r and r.z partials; convergence and beta; p = z + beta p
+    double* part1 = ws.red.as<double>();
+    double* part2 = part1 + (int64_t)kRedBlocks * ncols;
     for (int it = 0; it < max_iter; ++it) {
-        dev_mass(h, 1.0, p, ncols, 0.0, q, ncols, ncols, s, done);
-        dev_coldot(ws, n, ncols, p, ncols, q, ncols, pq, s, done);
-        hipLaunchKernelGGL(cg_alpha_kernel, grid_for(ncols), dim3(256), 0, s, ncols, rz, pq, active, alpha, done);
-        hipLaunchKernelGGL(cg_update_xr_kernel, grid_for(N), dim3(256), 0, s, n, ncols, alpha, p, q, Y, ldY, r, done);
-        dev_coldot(ws, n, ncols, r, ncols, r, ncols, rr, s, done);
-        hipLaunchKernelGGL(cg_check_kernel, dim3(1), dim3(256), 0, s, ncols, rr, bb, tol * tol, max_iter, active, done,
-                           iters, total);
+        hipLaunchKernelGGL(cg_spmv_dot_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, ncols, h.d_M_ptr,
+                           h.d_M_idx, h.d_M_val, p, q, part1, done);
+        hipLaunchKernelGGL(cg_alpha_sum_kernel, grid_for(ncols), dim3(256), 0, s, kRedBlocks, ncols, part1, rz, active,
+                           alpha, done);
+        hipLaunchKernelGGL(cg_update_dots_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, ncols, alpha, p, q,
+                           h.d_lumped_inv, Y, ldY, r, z, part1, part2, done);
+        hipLaunchKernelGGL(cg_check_beta_kernel, dim3(1), dim3(256), 0, s, kRedBlocks, ncols, part1, part2, bb,
+                           tol * tol, max_iter, active, rz, beta, done, iters, total);
         if (it + 1 == max_iter) break;
-        dev_lumped(h, r, ncols, z, ncols, ncols, s, done);
-        dev_coldot(ws, n, ncols, r, ncols, z, ncols, rz_new, s, done);
-        hipLaunchKernelGGL(cg_beta_kernel, grid_for(ncols), dim3(256), 0, s, ncols, rz, rz_new, beta, done);
         hipLaunchKernelGGL(cg_update_p_kernel, grid_for(N), dim3(256), 0, s, n, ncols, beta, z, p, done);
         BAEMM_HIP(hipGetLastError());
     }
