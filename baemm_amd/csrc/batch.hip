@@ -238,7 +238,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
             const Params& p = h.prm;
             BatchInst in{};
             in.mid = h.d_mid;
-            in.nrm = h.d_nrm;
+            in.nrm = pl.nrm;
             in.bop = pl.bop;
             in.b = pl.b;
             in.ldp = pl.ldp;

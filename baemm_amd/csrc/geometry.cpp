@@ -167,7 +167,7 @@ void free_device(Handle& h)
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_scale, &h.ws_misc,
-                      &h.ws_pts, &h.ws_g, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
+                      &h.ws_pts, &h.ws_g, &h.ws_nrm, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
                       &h.sws.t, &h.sws.x, &h.sws.r, &h.sws.hc, &h.sws.a, &h.sws.b, &h.sws.c,
                       &h.sws.d, &h.sws.e, &h.sws.cgctr, &h.sws.sc};
     for (DevBuf* b : bufs) b->release();
@@ -184,7 +184,7 @@ size_t handle_device_bytes(const Handle& h)
                h.h_vt_ptr.size() * 4 + h.h_vt_idx.size() * 4 + h.h_M_ptr.size() * 4 + h.h_M_idx.size() * 4 +
                h.h_M_val.size() * 8 + h.h_lumped_inv.size() * 8 + (size_t)h.m_pad * sizeof(float4);
     const DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_scale, &h.ws_misc,
-                            &h.ws_pts, &h.ws_g, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
+                            &h.ws_pts, &h.ws_g, &h.ws_nrm, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
                             &h.sws.t, &h.sws.x, &h.sws.r, &h.sws.hc, &h.sws.a, &h.sws.b, &h.sws.c,
                             &h.sws.d, &h.sws.e, &h.sws.cgctr, &h.sws.sc};
     for (const DevBuf* b : bufs) g += b->bytes;
@@ -229,11 +229,11 @@ std::vector<ChunkCoef> chunk_coefs(const Params& p, int q0, int kg, float scale,
     std::vector<ChunkCoef> out(kg);
     for (int g = 0; g < kg; ++g) {
         const int q = q0 + g;
-        // coefficients times conj(rot[q]) (a unit phase the caller multiplies back)
-        const cdouble ph = rot ? std::conj(cdouble(rot[q])) : cdouble(1.0, 0.0);
-        const cdouble c1 = cdouble(p.c[4 * q + 1]) * ph * (double)scale;
-        const cdouble c2 = cdouble(p.c[4 * q + 2]) * ph * (double)scale;
-        const cdouble c3 = cdouble(p.c[4 * q + 3]) * ph * (double)scale;
+        // coefficients times scale / rot[q] (a factor the caller multiplies back)
+        const cdouble f = (double)scale / (rot ? cdouble(rot[q]) : cdouble(1.0, 0.0));
+        const cdouble c1 = cdouble(p.c[4 * q + 1]) * f;
+        const cdouble c2 = cdouble(p.c[4 * q + 2]) * f;
+        const cdouble c3 = cdouble(p.c[4 * q + 3]) * f;
         ChunkCoef cc{};
         cc.kap = p.kappa[q];
         cc.kap_rev = (float)((double)p.kappa[q] / (2.0 * M_PI));
@@ -251,19 +251,6 @@ std::vector<ChunkCoef> chunk_coefs(const Params& p, int q0, int kg, float scale,
         out[g] = cc;
     }
     return out;
-}
-
-// Unit phase per chunk that makes the double-layer coefficient real:
-// c2 / |c2| (1 when c2 = 0).
-std::vector<cfloat> chunk_phases(const Params& p)
-{
-    std::vector<cfloat> rot(p.K, cfloat(1.0f, 0.0f));
-    for (int q = 0; q < p.K; ++q) {
-        const cdouble c2(p.c[4 * q + 2]);
-        const double a = std::abs(c2);
-        if (a > 0.0) rot[q] = cfloat((float)(c2.real() / a), (float)(c2.imag() / a));
-    }
-    return rot;
 }
 
 }  // namespace baemm

@@ -153,7 +153,7 @@ struct Handle {
     int gmres_max_restarts = 3;
     Params prm;
     // workspaces
-    DevBuf ws_B, ws_out, ws_b32, ws_bop, ws_part, ws_scale, ws_misc, ws_pts, ws_g;
+    DevBuf ws_B, ws_out, ws_b32, ws_bop, ws_part, ws_scale, ws_misc, ws_pts, ws_g, ws_nrm;
     DevBuf io[6];                        // device copies of the host arrays of the solver entry points
     Workspace sws;
     SolverStats stats;
@@ -178,9 +178,8 @@ void free_device(Handle& h);
 // ---- parameters ----
 void load_params(Handle& h, const double* kappa, const double* coeff, int64_t W, int64_t wcs, double factor);
 // Coefficients of chunks q0..q0+kg-1 times `scale` and, if rot is given,
-// times conj(rot[q]).
+// divided by rot[q].
 std::vector<ChunkCoef> chunk_coefs(const Params& p, int q0, int kg, float scale, const cfloat* rot = nullptr);
-std::vector<cfloat> chunk_phases(const Params& p);
 
 // ---- device ops (ops.hip) ----
 // Complex element types of external device arrays.

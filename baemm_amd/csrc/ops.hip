@@ -184,6 +184,16 @@ __global__ void split_kernel(const float2* __restrict__ b, int ldb, int nst, int
     bop[base + 64] = __builtin_bit_cast(uint4, lo);
 }
 
+// out[j] = s * nrm[j]: source normals carrying the common scale of the
+// non-adjoint MFMA kernel values.
+__global__ void scale_nrm_kernel(const float4* __restrict__ nrm, int n, float s, float4* __restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float4 v = nrm[j];
+    out[j] = make_float4(s * v.x, s * v.y, s * v.z, v.w);
+}
+
 // Minimum squared distance between distinct centroids (sets the A scale of
 // the MFMA path).  One target per thread, all sources.
 __global__ void rmin_kernel(const float4* __restrict__ mid, int nt, unsigned* __restrict__ out)
@@ -563,6 +573,12 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     pl.ldb = (int)p.K * pl.WCSP;
     pl.ldp = 2 * pl.ldb;
     pl.adj = p.any(3);
+    // The MFMA kernels' non-adjoint form divides each chunk by its c2; chunks
+    // without a double layer take the general (adjoint) form.
+    if (pl.mixed)
+        for (int q = 0; q < p.K; ++q)
+            if (p.c[4 * q + 2] == cfloat(0.0f, 0.0f)) pl.adj = true;
+    pl.nrm = h.d_nrm;
     EpiArgs& ea = pl.ea;
     ea = base_epi(h, a, pl.WCSP);
     ea.mass_on = pl.mass_on;
@@ -581,17 +597,41 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     // b per column from its maximum.
     const float rmin = compute_rmin(h, s);
     pl.sA.assign(p.K, 1.0f);
+    pl.rot.assign(p.K, cfloat(1.0f, 0.0f));
     ChunkScales csc{};
-    for (int q = 0; q < p.K; ++q) {
-        const double c1 = std::abs(std::complex<double>(p.c[4 * q + 1]));
-        const double c23 =
-            std::abs(std::complex<double>(p.c[4 * q + 2])) + std::abs(std::complex<double>(p.c[4 * q + 3]));
-        const double kr = (double)std::fabs(p.kappa[q]) * rmin;
-        const double amax = c1 / rmin + c23 * (1.0 + kr) / ((double)rmin * rmin);
+    if (!pl.adj) {
+        // Every chunk has c2 != 0: the kernel values are taken as A' = s A / c2
+        // with one scale s for all chunks, carried by the source normals
+        // (Q' = c1/c2 s / r - s (z.n)/r^3 + i kappa s (z.n)/r^2); the epilogue
+        // multiplies c2 / s back.
+        double amax = 0.0;
+        for (int q = 0; q < p.K; ++q) {
+            const std::complex<double> c2(p.c[4 * q + 2]);
+            const double rho = std::abs(std::complex<double>(p.c[4 * q + 1]) / c2);
+            const double kr = (double)std::fabs(p.kappa[q]) * rmin;
+            amax = std::max(amax, rho / rmin + (1.0 + kr) / ((double)rmin * rmin));
+            pl.rot[q] = cfloat(p.c[4 * q + 2]);
+        }
         int e = 0;
         std::frexp(std::max(amax, 1e-30), &e);
-        pl.sA[q] = (float)std::ldexp(1.0, 14 - e);
-        csc.sA[q] = pl.sA[q];
+        const float sig = (float)std::ldexp(1.0, 14 - e);
+        for (int q = 0; q < p.K; ++q) pl.sA[q] = csc.sA[q] = sig;
+        h.ws_nrm.reserve(sizeof(float4) * h.nt_pad);
+        hipLaunchKernelGGL(scale_nrm_kernel, dim3((unsigned)((h.nt_pad + 255) / 256)), dim3(256), 0, s, h.d_nrm,
+                           (int)h.nt_pad, sig, h.ws_nrm.as<float4>());
+        pl.nrm = h.ws_nrm.as<float4>();
+    } else {
+        for (int q = 0; q < p.K; ++q) {
+            const double c1 = std::abs(std::complex<double>(p.c[4 * q + 1]));
+            const double c23 =
+                std::abs(std::complex<double>(p.c[4 * q + 2])) + std::abs(std::complex<double>(p.c[4 * q + 3]));
+            const double kr = (double)std::fabs(p.kappa[q]) * rmin;
+            const double amax = c1 / rmin + c23 * (1.0 + kr) / ((double)rmin * rmin);
+            int e = 0;
+            std::frexp(std::max(amax, 1e-30), &e);
+            pl.sA[q] = (float)std::ldexp(1.0, 14 - e);
+            csc.sA[q] = pl.sA[q];
+        }
     }
     const int ncols = pl.ldb;
     h.ws_scale.reserve(sizeof(float) * 3 * ncols);
@@ -611,9 +651,6 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     BAEMM_HIP(hipGetLastError());
     pl.bop = h.ws_bop.as<uint4>();
     ea.unscale = unscale;
-    // The MFMA kernels take each chunk's coefficients times conj(phase of c2)
-    // (c2 real: fewer operations per pair); the epilogue multiplies the phase back.
-    pl.rot = chunk_phases(p);
     for (int q = 0; q < p.K; ++q) {
         ea.rotr[q] = pl.rot[q].real();
         ea.roti[q] = pl.rot[q].imag();
@@ -658,7 +695,7 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
         if (ws) kg = std::min(kg, 2);
         const CoefPack cp = make_pack(p, q0, kg, pl.sA.data(), pl.rot.data());
         MixedFn fn = ws ? mixed_ws_fn(pl.adj, pl.WCSP, kg) : mixed_fn(pl.adj, pl.WCSP, kg);
-        hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(ws ? 512 : 256), 0, s, h.d_mid, h.d_nrm, (int)h.nt, pl.bop, nst,
+        hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(ws ? 512 : 256), 0, s, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst,
                            cp, part, pl.ea.seg_stride, pl.ldp, q0, steps);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;

@@ -63,7 +63,8 @@ struct BdrPlan {
     int WCSP = 0, ldb = 0, ldp = 0;
     int nst = 0;                 // 8-source steps (mixed) of the padded source range
     std::vector<float> sA;       // per-chunk power-of-two scale of A (mixed)
-    std::vector<cfloat> rot;     // per-chunk phase making c2 real (mixed)
+    std::vector<cfloat> rot;     // per-chunk factor the kernel values are divided by (mixed: c2, or 1)
+    const float4* nrm = nullptr; // source normals of the pair kernel (scaled in the non-adjoint mixed form)
     float2* b = nullptr;         // staged sources b = Av B
     const uint4* bop = nullptr;  // split MFMA operand (mixed)
     EpiArgs ea{};

@@ -227,23 +227,18 @@ __device__ __forceinline__ uint32_t pack_rtz(float a, float b)
     return __builtin_bit_cast(uint32_t, h);
 }
 
-// x - (float)half(h, HI): one v_fma_mix_f32 (-h * 1.0 + x, the f16 operand
-// read from the low or high half of the packed register).
-template <int HI>
-__device__ __forceinline__ float sub_packed_half(uint32_t h, float x)
-{
-    float r;
-    if constexpr (HI) asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
-    else asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
-    return r;
-}
-
 // hi = fp16(a), fp16(b) rounded toward zero; lo = fp16 of the remainders
 // a - hi, b - hi (exact in FP32: hi is a truncated to 11 significant bits).
+// Each remainder is one v_fma_mix (-hi * 1.0 + a, the f16 operand read from
+// the low or high half of hi) that rounds its result to f16 and writes it
+// into the low (mixlo) or high (mixhi) half of lo, keeping the other half.
 __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t& lo)
 {
     hi = pack_rtz(a, b);
-    lo = pack_rtz(sub_packed_half<0>(hi, a), sub_packed_half<1>(hi, b));
+    uint32_t l;
+    asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(hi), "v"(a));
+    asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(l) : "v"(hi), "v"(b));
+    lo = l;
 }
 
 template <int KG, bool ADJ, bool MASK>
@@ -280,10 +275,12 @@ __device__ __forceinline__ void mixed_gen(const float4 x, const float4 ni, const
                 Qr = fmaf(-c.c3r, g.hi, fmaf(-c.kc3i, g.gi, Qr));
                 Qi = fmaf(-c.c3i, g.hi, fmaf(c.kc3r, g.gi, Qi));
             } else {
-                // c2 is real here (the host rotates each chunk by the phase of
-                // c2 and the epilogue rotates back): Q' = (c1 - c2 g) / r + i kappa c2 g
-                Qr = g.rinv * fmaf(-c.c2r, g.gj, c.c1r);
-                Qi = fmaf(c.kc2r, g.gj, c.c1i * g.rinv);
+                // The host divides each chunk's coefficients by c2 / s (s the
+                // chunk scale; the epilogue multiplies c2 / s back) and the
+                // source normals carry the factor s, so gj = s (z.n_j)/r^2,
+                // hj = s (z.n_j)/r^3 and Q' = c1' / r - hj + i kappa gj.
+                Qr = fmaf(c.c1r, g.rinv, -g.hj);
+                Qi = fmaf(c.kap, g.gj, c.c1i * g.rinv);
             }
             const float Ar = fmaf(cs[jj], Qr, -sn[jj] * Qi);
             const float Ai = fmaf(cs[jj], Qi, sn[jj] * Qr);
