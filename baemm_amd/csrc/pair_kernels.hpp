@@ -497,6 +497,10 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
     const int i = ibase + (lane & 31);
     const int T = max(0, st1 - st0);
     const int nt_pad = nst * 8;
+    // Steps to mask (wave-uniform, kept in scalar registers): the 4 steps whose
+    // sources are this wave's targets, and steps reaching past nt.
+    const int st_diag = __builtin_amdgcn_readfirstlane(ibase >> 3);
+    const int st_tail = nt >> 3;
 
     auto issue_geo = [&](int k) {
         const int st = st0 + k;
@@ -536,7 +540,7 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
                     nj[jj] = s_geo[(k & 1) * GEO + 8 + 4 * g + jj];
                 }
                 uint32_t ah[KG][4], al[KG][4];
-                const bool masked = (8 * st < ibase + 32 && 8 * st + 8 > ibase) || (8 * st + 8 > nt);
+                const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
                 if (masked) mixed_gen<KG, ADJ, true>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
                 else mixed_gen<KG, ADJ, false>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
 #pragma unroll
