@@ -222,7 +222,23 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
     const BatchTuning tune = batch_tuning();
     const int n = (int)es.size();
     std::vector<Prepared> prep(n);
-    for (int k = 0; k < n; ++k) bdr_begin(*es[k].h, es[k].a, prep[k].pl, s);
+    // Per-instance preparation and epilogues are short launches; they run on
+    // each handle's own stream so that those of different instances overlap.
+    // Events: [0] fork, [1] after the persistent launches, then per instance
+    // [2 + k] prepared and [2 + n + k] finished.
+    while ((int)bw.sync.size() < 2 + 2 * n) {
+        hipEvent_t e = nullptr;
+        BAEMM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        bw.sync.push_back(e);
+    }
+    BAEMM_HIP(hipEventRecord(bw.sync[0], s));
+    for (int k = 0; k < n; ++k) {
+        hipStream_t sk = es[k].h->stream;
+        BAEMM_HIP(hipStreamWaitEvent(sk, bw.sync[0], 0));
+        bdr_begin(*es[k].h, es[k].a, prep[k].pl, sk);
+        BAEMM_HIP(hipEventRecord(bw.sync[2 + k], sk));
+    }
+    for (int k = 0; k < n; ++k) BAEMM_HIP(hipStreamWaitEvent(s, bw.sync[2 + k], 0));
 
     for (int mode = 0; mode < 2; ++mode) {     // 0: MFMA instances, 1: FP32 instances
         const bool mixed = (mode == 0);
@@ -352,7 +368,14 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
             stats->instances[mode] = (int64_t)insts.size();
         }
     }
-    for (int k = 0; k < n; ++k) bdr_end(*es[k].h, es[k].a, prep[k].pl, s);
+    BAEMM_HIP(hipEventRecord(bw.sync[1], s));
+    for (int k = 0; k < n; ++k) {
+        hipStream_t sk = es[k].h->stream;
+        BAEMM_HIP(hipStreamWaitEvent(sk, bw.sync[1], 0));
+        bdr_end(*es[k].h, es[k].a, prep[k].pl, sk);
+        BAEMM_HIP(hipEventRecord(bw.sync[2 + n + k], sk));
+    }
+    for (int k = 0; k < n; ++k) BAEMM_HIP(hipStreamWaitEvent(s, bw.sync[2 + n + k], 0));
 }
 
 void BatchWorkspace::release()
@@ -368,6 +391,8 @@ void BatchWorkspace::release()
         host_bytes[m] = 0;
         dev[m].release();
     }
+    for (hipEvent_t e : sync) (void)hipEventDestroy(e);
+    sync.clear();
 }
 
 }  // namespace baemm

@@ -28,6 +28,7 @@ struct BatchWorkspace {
     void* host[2] = {nullptr, nullptr};
     size_t host_bytes[2] = {0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> sync;   // fork/join events between the call's stream and the handles' streams
     void release();
 };
 
