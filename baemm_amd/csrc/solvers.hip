@@ -147,16 +147,27 @@ __global__ __launch_bounds__(256) void coldot_part_kernel(int n, int ncols, cons
     }
 }
 
-// out[c] = sum_b part[b][c]
-__global__ void colsum_kernel(int nblocks, int ncols, const double* __restrict__ part, double* __restrict__ out,
-                              const int* __restrict__ skip)
+// Sum over b < nblocks of part[b * stride + e], by one wave: lane-strided
+// partial sums, then a butterfly (a fixed order, so results are reproducible).
+// Every lane returns the sum.
+__device__ __forceinline__ double wave_colsum(const double* __restrict__ part, int nblocks, int64_t stride, int64_t e)
+{
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int b = lane; b < nblocks; b += 64) s += part[(int64_t)b * stride + e];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+// out[c] = sum_b part[b][c]; one 64-thread block per column c
+__global__ __launch_bounds__(64) void colsum_kernel(int nblocks, int ncols, const double* __restrict__ part,
+                                                    double* __restrict__ out, const int* __restrict__ skip)
 {
     if (skip && *skip) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * ncols + c];
-    out[c] = s;
+    const int c = blockIdx.x;
+    const double v = wave_colsum(part, nblocks, ncols, c);
+    if (threadIdx.x == 0) out[c] = v;
 }
 
 // Complex Gram-Schmidt dots: part[b][i][k] = sum_rows conj(V_i[r][k]) w[r][k], i < nb.
@@ -192,17 +203,15 @@ __global__ __launch_bounds__(256) void gs_dots_part_kernel(int n, int W, int nb,
     }
 }
 
-__global__ void gs_dots_sum_kernel(int nblocks, int nbW, const double2* __restrict__ part, double2* __restrict__ out)
+// out[e] = sum_b part[b][e]; one 64-thread block per element e
+__global__ __launch_bounds__(64) void gs_dots_sum_kernel(int nblocks, int nbW, const double2* __restrict__ part,
+                                                         double2* __restrict__ out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nbW) return;
-    double sr = 0.0, si = 0.0;
-    for (int b = 0; b < nblocks; ++b) {
-        const double2 p = part[(int64_t)b * nbW + e];
-        sr += p.x;
-        si += p.y;
-    }
-    out[e] = make_double2(sr, si);
+    const int e = blockIdx.x;
+    const double* pd = reinterpret_cast<const double*>(part);
+    const double sr = wave_colsum(pd, nblocks, 2 * (int64_t)nbW, 2 * (int64_t)e);
+    const double si = wave_colsum(pd, nblocks, 2 * (int64_t)nbW, 2 * (int64_t)e + 1);
+    if (threadIdx.x == 0) out[e] = make_double2(sr, si);
 }
 
 // w[r][k] -= sum_{i<nb} h[i][k] V_i[r][k]   (or x += ... with sign = +1)
@@ -256,6 +265,8 @@ __global__ void cg_init_kernel(int ncols, const double* __restrict__ bb, int* __
     if (threadIdx.x == 0) {
         *done = !any;
         *iters = 0;
+        done[3] = 0;   // ticket and active count of cg_check_beta_kernel
+        done[4] = 0;
     }
 }
 
@@ -360,7 +371,7 @@ void dev_coldot(Workspace& ws, int64_t n, int ncols, const double* X, int64_t ld
     double* part = ws.red.as<double>();
     hipLaunchKernelGGL(coldot_part_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, ncols, X, ldX, Y, ldY, part,
                        skip);
-    hipLaunchKernelGGL(colsum_kernel, grid_for(ncols), dim3(256), 0, s, kRedBlocks, ncols, part, out, skip);
+    hipLaunchKernelGGL(colsum_kernel, dim3(ncols), dim3(64), 0, s, kRedBlocks, ncols, part, out, skip);
     BAEMM_HIP(hipGetLastError());
 }
 
@@ -369,7 +380,7 @@ void dev_gs_dots(Workspace& ws, int64_t n, int W, int nb, const double2* V, int6
 {
     double2* part = ws.red.as<double2>();
     hipLaunchKernelGGL(gs_dots_part_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, W, nb, V, vstride, w, part);
-    hipLaunchKernelGGL(gs_dots_sum_kernel, grid_for((int64_t)nb * W), dim3(256), 0, s, kRedBlocks, nb * W, part, out);
+    hipLaunchKernelGGL(gs_dots_sum_kernel, dim3(nb * W), dim3(64), 0, s, kRedBlocks, nb * W, part, out);
     BAEMM_HIP(hipGetLastError());
 }
 
@@ -423,17 +434,16 @@ __global__ __launch_bounds__(256) void cg_spmv_dot_kernel(int n, int ncols, cons
     }
 }
 
-// pq = sum_b part[b]; alpha = active ? rz / pq : 0
-__global__ void cg_alpha_sum_kernel(int nblocks, int ncols, const double* __restrict__ part,
-                                    const double* __restrict__ rz, const int* __restrict__ active,
-                                    double* __restrict__ alpha, const int* __restrict__ skip)
+// pq = sum_b part[b]; alpha = active ? rz / pq : 0; one 64-thread block per column
+__global__ __launch_bounds__(64) void cg_alpha_sum_kernel(int nblocks, int ncols, const double* __restrict__ part,
+                                                          const double* __restrict__ rz,
+                                                          const int* __restrict__ active, double* __restrict__ alpha,
+                                                          const int* __restrict__ skip)
 {
     if (*skip) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    double pq = 0.0;
-    for (int b = 0; b < nblocks; ++b) pq += part[(int64_t)b * ncols + c];
-    alpha[c] = (active[c] && pq != 0.0) ? rz[c] / pq : 0.0;
+    const int c = blockIdx.x;
+    const double pq = wave_colsum(part, nblocks, ncols, c);
+    if (threadIdx.x == 0) alpha[c] = (active[c] && pq != 0.0) ? rz[c] / pq : 0.0;
 }
 
 // Y += alpha p; r -= alpha q; z = D r; part1 = r.r, part2 = r.z (block partials)
@@ -474,38 +484,40 @@ __global__ __launch_bounds__(256) void cg_update_dots_kernel(int n, int ncols, c
     }
 }
 
-// rr, rz_new from the partials; freeze converged columns; beta; rz = rz_new;
-// the solve is done when no column is active or after max_iter iterations.
-// One block.
-__global__ void cg_check_beta_kernel(int nblocks, int ncols, const double* __restrict__ part1,
-                                     const double* __restrict__ part2, const double* __restrict__ bb, double tol2,
-                                     int max_iter, int* __restrict__ active, double* __restrict__ rz,
-                                     double* __restrict__ beta, int* __restrict__ done, int* __restrict__ iters,
-                                     int* __restrict__ total)
+// Column c = blockIdx.x (64 threads): rr = sum part1, rz_new = sum part2;
+// freeze the column once rr <= tol2 bb; beta = rz_new / rz.  The last block to
+// finish (ticket ctr[3]) reads the count of still-active columns (ctr[4]),
+// advances the iteration counters and sets the done flag, then clears ctr[3..4]
+// for the next iteration.  ctr: done, iterations, total, ticket, active count.
+__global__ __launch_bounds__(64) void cg_check_beta_kernel(int nblocks, int ncols, const double* __restrict__ part1,
+                                                           const double* __restrict__ part2,
+                                                           const double* __restrict__ bb, double tol2, int max_iter,
+                                                           int* __restrict__ active, double* __restrict__ rz,
+                                                           double* __restrict__ beta, int* __restrict__ ctr)
 {
-    if (*done) return;
-    __shared__ int any;
-    if (threadIdx.x == 0) any = 0;
-    __syncthreads();
-    for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
-        double rr = 0.0, rzn = 0.0;
-        for (int b = 0; b < nblocks; ++b) {
-            rr += part1[(int64_t)b * ncols + c];
-            rzn += part2[(int64_t)b * ncols + c];
-        }
-        if (active[c]) {
-            if (rr <= tol2 * bb[c]) active[c] = 0;
-            else atomicOr(&any, 1);
-        }
-        beta[c] = (rz[c] != 0.0) ? rzn / rz[c] : 0.0;
-        rz[c] = rzn;
+    if (ctr[0]) return;
+    const int c = blockIdx.x;
+    const double rr = wave_colsum(part1, nblocks, ncols, c);
+    const double rzn = wave_colsum(part2, nblocks, ncols, c);
+    if (threadIdx.x != 0) return;
+    int still = 0;
+    if (active[c]) {
+        if (rr <= tol2 * bb[c]) active[c] = 0;
+        else still = 1;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int it = *iters + 1;
-        *iters = it;
-        *total += 1;
-        if (!any || it >= max_iter) *done = 1;
+    beta[c] = (rz[c] != 0.0) ? rzn / rz[c] : 0.0;
+    rz[c] = rzn;
+    if (still) atomicAdd(&ctr[4], 1);
+    __threadfence();
+    if (atomicAdd(&ctr[3], 1) == (int)gridDim.x - 1) {
+        __threadfence();
+        const int any = atomicAdd(&ctr[4], 0);
+        const int it = ctr[1] + 1;
+        ctr[1] = it;
+        ctr[2] += 1;
+        if (!any || it >= max_iter) ctr[0] = 1;
+        atomicExch(&ctr[4], 0);
+        atomicExch(&ctr[3], 0);
     }
 }
 
@@ -669,8 +681,9 @@ __global__ void pcg_total_kernel(int* __restrict__ ctr) { ctr[2] += ctr[1]; }
 // for every call (no host synchronisation, so it can be captured in a graph);
 // once every column has converged the remaining launches return at once, and
 // a frozen column is no longer updated, so the result equals stopping there.
-// Counters (ws.cgctr, int[4]): done flag, iterations of the last call, total
-// iterations since the buffer was created.
+// Counters (ws.cgctr, int[8]): done flag, iterations of the last call, total
+// iterations since the buffer was created, then the ticket and active-column
+// count of cg_check_beta_kernel.
 void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
                  int max_iter, hipStream_t s)
 {
@@ -699,7 +712,11 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     }
     ws.cg_block = false;
     ws.cg.reserve(sizeof(double) * (4 * N));
-    ws.red.reserve(sizeof(double) * 2 * kRedBlocks * ncols);
+    // partial-sum blocks of the SpMV and update passes: about 16 rows per block
+    // (4 per wave), so the row loops stay short; the column sums read them with
+    // one wave per column
+    const int nred = (int)std::min<int64_t>(4096, std::max<int64_t>(kRedBlocks, (n + 15) / 16));
+    ws.red.reserve(sizeof(double) * 2 * std::max(nred, kRedBlocks) * ncols);
     ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
     int* ctr = dev_cg_counters(ws, s);
     double* r = ws.cg.as<double>();
@@ -716,7 +733,6 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     int* active = reinterpret_cast<int*>(bb + 2 * ncols);
     int* done = ctr;
     int* iters = ctr + 1;
-    int* total = ctr + 2;
     // r = X (contiguous), Y = 0
     hipLaunchKernelGGL(cg_start_kernel, grid_for(N), dim3(256), 0, s, n, ncols, X, ldX, r, Y, ldY);
     dev_coldot(ws, n, ncols, r, ncols, r, ncols, bb, s);
@@ -726,16 +742,16 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     // five launches per iteration: q = M p with p.q partials; alpha; updates of
     // Y, r, z with r.r and r.z partials; convergence and beta; p = z + beta p
     double* part1 = ws.red.as<double>();
-    double* part2 = part1 + (int64_t)kRedBlocks * ncols;
+    double* part2 = part1 + (int64_t)nred * ncols;
     for (int it = 0; it < max_iter; ++it) {
-        hipLaunchKernelGGL(cg_spmv_dot_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, ncols, h.d_M_ptr,
-                           h.d_M_idx, h.d_M_val, p, q, part1, done);
-        hipLaunchKernelGGL(cg_alpha_sum_kernel, grid_for(ncols), dim3(256), 0, s, kRedBlocks, ncols, part1, rz, active,
-                           alpha, done);
-        hipLaunchKernelGGL(cg_update_dots_kernel, dim3(kRedBlocks), dim3(256), 0, s, (int)n, ncols, alpha, p, q,
+        hipLaunchKernelGGL(cg_spmv_dot_kernel, dim3(nred), dim3(256), 0, s, (int)n, ncols, h.d_M_ptr, h.d_M_idx,
+                           h.d_M_val, p, q, part1, done);
+        hipLaunchKernelGGL(cg_alpha_sum_kernel, dim3(ncols), dim3(64), 0, s, nred, ncols, part1, rz, active, alpha,
+                           done);
+        hipLaunchKernelGGL(cg_update_dots_kernel, dim3(nred), dim3(256), 0, s, (int)n, ncols, alpha, p, q,
                            h.d_lumped_inv, Y, ldY, r, z, part1, part2, done);
-        hipLaunchKernelGGL(cg_check_beta_kernel, dim3(1), dim3(256), 0, s, kRedBlocks, ncols, part1, part2, bb,
-                           tol * tol, max_iter, active, rz, beta, done, iters, total);
+        hipLaunchKernelGGL(cg_check_beta_kernel, dim3(ncols), dim3(64), 0, s, nred, ncols, part1, part2, bb,
+                           tol * tol, max_iter, active, rz, beta, ctr);
         if (it + 1 == max_iter) break;
         hipLaunchKernelGGL(cg_update_p_kernel, grid_for(N), dim3(256), 0, s, n, ncols, beta, z, p, done);
         BAEMM_HIP(hipGetLastError());
@@ -747,9 +763,9 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
 // the first PCG of a handle always runs eagerly).
 int* dev_cg_counters(Workspace& ws, hipStream_t s)
 {
-    if (ws.cgctr.bytes < 4 * sizeof(int)) {
-        ws.cgctr.reserve(4 * sizeof(int));
-        BAEMM_HIP(hipMemsetAsync(ws.cgctr.ptr, 0, 4 * sizeof(int), s));
+    if (ws.cgctr.bytes < 8 * sizeof(int)) {
+        ws.cgctr.reserve(8 * sizeof(int));
+        BAEMM_HIP(hipMemsetAsync(ws.cgctr.ptr, 0, 8 * sizeof(int), s));
     }
     return ws.cgctr.as<int>();
 }

@@ -61,7 +61,8 @@ std::vector<char> graph_key(Handle& h, int j, double cg_tol)
     std::memset(&k, 0, sizeof(k));
     const Workspace& ws = h.sws;
     const void* p[] = {ws.V.ptr, ws.w.ptr, ws.t.ptr, ws.hc.ptr, ws.sc.ptr, ws.red.ptr, ws.cg.ptr, ws.scal.ptr,
-                       ws.cgctr.ptr, h.ws_b32.ptr, h.ws_bop.ptr, h.ws_part.ptr, h.ws_scale.ptr, h.ws_misc.ptr};
+                       ws.cgctr.ptr, h.ws_b32.ptr, h.ws_bop.ptr, h.ws_part.ptr, h.ws_scale.ptr, h.ws_misc.ptr,
+                       h.ws_nrm.ptr};
     static_assert(sizeof(p) / sizeof(p[0]) <= 16, "key size");
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); ++i) k.ptr[i] = p[i];
     int rmin_bits = 0;
@@ -91,6 +92,8 @@ void run_graph(Handle& h, int j, double cg_tol, hipStream_t s, F&& enqueue)
     std::vector<char> key = graph_key(h, j, cg_tol);
     if (!g.exec || g.key != key) {
         if (g.exec) {
+            // the previous launch of this graph may still run on s
+            BAEMM_HIP(hipStreamSynchronize(s));
             BAEMM_HIP(hipGraphExecDestroy(g.exec));
             g.exec = nullptr;
         }

@@ -193,7 +193,7 @@ def run_serving(reqs, rate, slots, precision, seed=11):
     return st
 
 
-def run_solve_leg(repeats=2):
+def run_solve_leg(repeats=2, graphs=True):
     """Full sound-soft solve: torus R=1, r=0.4 on a 240 x 80 grid (38,400
     triangles), kappa = 10, 32 seeded plane waves, FarField (incident wave,
     GMRES(30) to 1e-6 with the PCG mass-inverse preconditioner, far field at 256
@@ -208,10 +208,13 @@ def run_solve_leg(repeats=2):
     out = {}
     ref = None
     with Helmholtz_HIP(V, T, meas) as H:
-        for leg, prec, graphs in (("fp32", "fp32", False), ("mixed", "mixed", False), ("mixed_graphs", "mixed", True)):
+        legs = [("fp32", "fp32", False), ("mixed", "mixed", False)]
+        if graphs:
+            legs.append(("mixed_graphs", "mixed", True))
+        for leg, prec, use_graphs in legs:
             log(f"solve leg {leg}")
             H.SetPrecision(prec)
-            H.SetGraphs(graphs)
+            H.SetGraphs(use_graphs)
             H.FarField([10.0], 1, dirs, 32, cg_tol=1e-10, gmres_tol=1e-6)    # warm-up (workspaces, graphs)
             H.ResetSolverStats()
             t0 = time.perf_counter()
@@ -361,6 +364,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the torus solve leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
+    ap.add_argument("--no-graphs", action="store_true", help="skip the hipGraph-replay solve of the solve leg")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0,
                     help="Poisson arrival rate of the serving leg (1/s); the default saturates the server")
@@ -444,7 +448,7 @@ def main():
         line["batch"] = run_batch_leg()
     if not args.no_solve:
         line["solve"] = dict(workload="sound-soft torus 38400 triangles, kappa 10, 32 plane waves, FarField, "
-                                      "GMRES(30) 1e-6, 256 far-field directions", **run_solve_leg())
+                                      "GMRES(30) 1e-6, 256 far-field directions", **run_solve_leg(graphs=not args.no_graphs))
     if args.serve_requests > 0:
         log("serving leg: building requests")
         reqs = serving_workload(args.serve_requests, args.serve_max_level)
