@@ -58,8 +58,7 @@ static int mixed_variant(int WCSP, int KG, bool adj)
 // Largest LDS need over the variants (all have 4 column blocks or fewer).
 constexpr int kWsOpMax = WsLds<32, 2>::B_ELEMS;
 constexpr int kWsAMax = WsLds<32, 2>::A_ELEMS;
-constexpr int kWsGeoMax = (WsLds<32, 2>::GEO_ELEMS > 2 * MixedLds<16, 1>::GEO_ELEMS) ? WsLds<32, 2>::GEO_ELEMS
-                                                                                       : 2 * MixedLds<16, 1>::GEO_ELEMS;
+constexpr int kWsGeoMax = 2 * MixedLds<16, 1>::GEO_ELEMS;   // geometry of the two-tile single-wave items
 static_assert(WsLds<64, 1>::B_ELEMS <= kWsOpMax && WsLds<64, 1>::A_ELEMS <= kWsAMax, "LDS bound");
 static_assert(WsLds<16, 2>::B_ELEMS <= kWsOpMax && WsLds<16, 2>::A_ELEMS <= kWsAMax, "LDS bound");
 static_assert(2 * MixedLds<16, 1>::OP_ELEMS <= kWsOpMax + kWsAMax, "LDS bound");
@@ -91,7 +90,7 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
         CoefPack cp;                                                                                             \
         for (int g = 0; g < KG_; ++g) cp.c[g] = cf[g];                                                           \
         mixed_ws_block<WCSP_, KG_, ADJ_>(in.mid, in.nrm, in.nt, in.bop, in.nst, cp, out, in.ldp, w.q0, w.tile,   \
-                                         w.s0, w.s1, s_op, s_a, s_geo);                                          \
+                                         w.s0, w.s1, s_op, s_a);                                                 \
         break;                                                                                                   \
     }
         // 16-column single chunks without the adjoint double layer: the

@@ -241,20 +241,14 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t&
     lo = l;
 }
 
-template <int KG, bool ADJ, bool MASK>
-__device__ __forceinline__ void mixed_gen(const float4 x, const float4 ni, const float4 (&y)[4], const float4 (&nj)[4],
-                                          int jbase, int i, int nt, const CoefPack& cp, uint32_t (&ah)[KG][4],
-                                          uint32_t (&al)[KG][4])
+// FP16 hi/lo A operands of the 4 pairs of one lane for chunks q0..q0+KG-1.
+// Written stage by stage across the 4 sources so that the independent
+// dependency chains interleave (the kernel is bound by VALU latency
+// otherwise).
+template <int KG, bool ADJ>
+__device__ __forceinline__ void mixed_vals(const BdrGeo (&geo)[4], const CoefPack& cp, uint32_t (&ah)[KG][4],
+                                           uint32_t (&al)[KG][4])
 {
-    // Written stage by stage across the 4 sources so that the independent
-    // dependency chains interleave (the kernel is bound by VALU latency
-    // otherwise).
-    BdrGeo geo[4];
-#pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-        const int j = jbase + jj;
-        geo[jj] = bdr_geo<ADJ, MASK>(x, ni, y[jj], nj[jj], (j != i) && (j < nt));
-    }
 #pragma unroll
     for (int q = 0; q < KG; ++q) {
         float cs[4], sn[4];
@@ -286,6 +280,90 @@ __device__ __forceinline__ void mixed_gen(const float4 x, const float4 ni, const
             const float Ai = fmaf(cs[jj], Qi, sn[jj] * Qr);
             split2(Ar, Ai, ah[q][jj], al[q][jj]);
         }
+    }
+}
+
+template <int KG, bool ADJ, bool MASK>
+__device__ __forceinline__ void mixed_gen(const float4 x, const float4 ni, const float4 (&y)[4], const float4 (&nj)[4],
+                                          int jbase, int i, int nt, const CoefPack& cp, uint32_t (&ah)[KG][4],
+                                          uint32_t (&al)[KG][4])
+{
+    BdrGeo geo[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const int j = jbase + jj;
+        geo[jj] = bdr_geo<ADJ, MASK>(x, ni, y[jj], nj[jj], (j != i) && (j < nt));
+    }
+    mixed_vals<KG, ADJ>(geo, cp, ah, al);
+}
+
+// Geometry of the 4 (target i, source jbase + jj) pairs of one lane when the
+// sources are spread over the lanes instead of read by every lane: lane
+// 16 r + jj (jj < 4) of each 16-lane row r holds source jj of its half-wave
+// (centroid yv, normal nv); the other lanes hold copies.  Each difference
+// z = y - x and each term of z . n_j takes the source value from lane jj of the
+// row through a row_newbcast:jj DPP operand, so the broadcast costs no
+// instruction: the same 24 VALU operations as bdr_geo, and the same rounding.
+// The DPP source registers come from memory loads; s_nop 4 gives the wait
+// states a DPP read needs after a VALU or EXEC write, in case the compiler
+// moved a value just before.
+template <bool ADJ, bool MASK>
+__device__ __forceinline__ void mixed_geo_dpp(const float4 x, const float4 ni, const float4 yv, const float4 nv,
+                                              int jbase, int i, int nt, BdrGeo (&geo)[4])
+{
+    float z0[4], z1[4], z2[4], dn[4];
+    asm("s_nop 4\n\t"
+        "v_sub_f32_dpp %0, %12, %15 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %1, %13, %16 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %2, %14, %17 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %3, %12, %15 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %4, %13, %16 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %5, %14, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %6, %12, %15 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %7, %13, %16 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %8, %14, %17 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %9, %12, %15 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %10, %13, %16 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+        "v_sub_f32_dpp %11, %14, %17 row_newbcast:3 row_mask:0xf bank_mask:0xf"
+        : "=&v"(z0[0]), "=&v"(z1[0]), "=&v"(z2[0]), "=&v"(z0[1]), "=&v"(z1[1]), "=&v"(z2[1]), "=&v"(z0[2]),
+          "=&v"(z1[2]), "=&v"(z2[2]), "=&v"(z0[3]), "=&v"(z1[3]), "=&v"(z2[3])
+        : "v"(yv.x), "v"(yv.y), "v"(yv.z), "v"(x.x), "v"(x.y), "v"(x.z));
+    // z . n_j = (z2 n_z + z1 n_y) + z0 n_x, in the order of bdr_geo's fmaf chain
+    asm("s_nop 4\n\t"
+        "v_mul_f32_dpp %0, %6, %9 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %6, %12 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %2, %6, %15 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %3, %6, %18 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %0, %5, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %1, %5, %11 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %2, %5, %14 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %3, %5, %17 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %0, %4, %7 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %1, %4, %10 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %2, %4, %13 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_fmac_f32_dpp %3, %4, %16 row_newbcast:3 row_mask:0xf bank_mask:0xf"
+        : "=&v"(dn[0]), "=&v"(dn[1]), "=&v"(dn[2]), "=&v"(dn[3])
+        : "v"(nv.x), "v"(nv.y), "v"(nv.z), "v"(z0[0]), "v"(z1[0]), "v"(z2[0]), "v"(z0[1]), "v"(z1[1]),
+          "v"(z2[1]), "v"(z0[2]), "v"(z1[2]), "v"(z2[2]), "v"(z0[3]), "v"(z1[3]), "v"(z2[3]));
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const int j = jbase + jj;
+        const float r2 = fmaf(z0[jj], z0[jj], fmaf(z1[jj], z1[jj], z2[jj] * z2[jj]));
+        float rinv = __builtin_amdgcn_rsqf(r2);
+        if constexpr (MASK) rinv = ((j != i) && (j < nt)) ? rinv : 0.0f;
+        BdrGeo g;
+        g.rinv = rinv;
+        g.r = r2 * rinv;
+        const float rinv2 = rinv * rinv;
+        g.gj = dn[jj] * rinv2;
+        g.hj = g.gj * rinv;
+        if constexpr (ADJ) {
+            g.gi = -fmaf(z0[jj], ni.x, fmaf(z1[jj], ni.y, z2[jj] * ni.z)) * rinv2;
+            g.hi = g.gi * rinv;
+        } else {
+            g.gi = g.hi = 0.0f;
+        }
+        geo[jj] = g;
     }
 }
 
@@ -458,8 +536,9 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
 // Per step k (st0 + k), separated by block barriers:
 //   producers: A(k) from geometry(k)            -> s_a[k % 2]
 //   consumers: MFMA with A(k-1), B(k-1)          <- s_a[(k-1) % 2], s_op[(k-1) % 2]
-//   loads:     geometry(k+1) -> s_geo[(k+1) % 2] (producer wave 0),
-//              B(k)          -> s_op[k % 2]      (consumer waves)
+//   loads:     B(k) -> s_op[k % 2] (consumer waves); each generation lane
+//              loads the one source of step k+1 it holds for mixed_geo_dpp
+//              into registers
 // Every buffer is rewritten only after the barrier that ends its last read.
 // ---------------------------------------------------------------------------
 template <int WCSP, int KG>
@@ -468,23 +547,21 @@ struct WsLds {
     static constexpr int OPS = KG * NCB * 2;          // B tiles per step
     static constexpr int B_ELEMS = 2 * OPS * 64;      // uint4, double buffered
     static constexpr int A_ELEMS = 2 * 4 * KG * 2 * 64;
-    static constexpr int GEO = 16;                    // 8 centroids + 8 normals per step
-    static constexpr int GEO_ELEMS = 2 * GEO;
 };
 
 // One block (8 waves) of the wave-specialised kernel: targets
 // [tile*128, tile*128+128) against source steps [st0, st1) of chunks
-// q0..q0+KG-1, partial sums to out[row*ldp + ...].  s_op, s_a, s_geo: LDS of at
-// least WsLds<WCSP,KG> B_ELEMS, A_ELEMS, GEO_ELEMS elements.  Ends with every
-// wave past a common barrier, so the caller may reuse the LDS.
+// q0..q0+KG-1, partial sums to out[row*ldp + ...].  s_op, s_a: LDS of at least
+// WsLds<WCSP,KG> B_ELEMS, A_ELEMS elements.  Ends with every wave past a common
+// barrier, so the caller may reuse the LDS.
 template <int WCSP, int KG, bool ADJ>
 __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
                                                const uint4* __restrict__ bop, int nst, const CoefPack& cp,
                                                float* __restrict__ out, int ldp, int q0, int tile, int st0, int st1,
-                                               uint4* s_op, uint4* s_a, float4* s_geo)
+                                               uint4* s_op, uint4* s_a)
 {
     using L = WsLds<WCSP, KG>;
-    constexpr int NCB = L::NCB, OPS = L::OPS, GEO = L::GEO;
+    constexpr int NCB = L::NCB, OPS = L::OPS;
 
     const int lane = threadIdx.x & 63;
     // wave-uniform role (scalar register), so the branches below are uniform
@@ -502,16 +579,6 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
     const int st_diag = __builtin_amdgcn_readfirstlane(ibase >> 3);
     const int st_tail = nt >> 3;
 
-    auto issue_geo = [&](int k) {
-        const int st = st0 + k;
-        if (lane < GEO) {
-            const int which = lane >> 3, kk = lane & 7;
-            const int j = min(st * 8 + kk, nt_pad - 1);
-            const float4* src = which ? (nrm + j) : (mid + j);
-            __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_geo[(k & 1) * GEO], 16,
-                                             0, 0);
-        }
-    };
     auto issue_b = [&](int k) {
         const int st = st0 + k;
         for (int t = w; t < OPS; t += 4) {
@@ -526,23 +593,31 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
         const float4 x = mid[i];
         float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
         if constexpr (ADJ) ni = nrm[i];
-        if (w == 0 && T > 0) issue_geo(0);
+        // the source of each step this lane holds for mixed_geo_dpp
+        const int src_lane = 4 * g + (lane & 3);
+        float4 yv = make_float4(0.f, 0.f, 0.f, 0.f), nv = yv;
+        if (T > 0) {
+            const int j = min(st0 * 8 + src_lane, nt_pad - 1);
+            yv = mid[j];
+            nv = nrm[j];
+        }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         for (int k = 0; k <= T; ++k) {
-            if (w == 0 && k + 1 < T) issue_geo(k + 1);
+            float4 yn = yv, nn = nv;
+            if (k + 1 < T) {
+                const int j = min((st0 + k + 1) * 8 + src_lane, nt_pad - 1);
+                yn = mid[j];
+                nn = nrm[j];
+            }
             if (k < T) {
                 const int st = st0 + k;
-                float4 y[4], nj[4];
-#pragma unroll
-                for (int jj = 0; jj < 4; ++jj) {
-                    y[jj] = s_geo[(k & 1) * GEO + 4 * g + jj];
-                    nj[jj] = s_geo[(k & 1) * GEO + 8 + 4 * g + jj];
-                }
                 uint32_t ah[KG][4], al[KG][4];
                 const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
-                if (masked) mixed_gen<KG, ADJ, true>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
-                else mixed_gen<KG, ADJ, false>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
+                BdrGeo geo[4];
+                if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
+                else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
+                mixed_vals<KG, ADJ>(geo, cp, ah, al);
 #pragma unroll
                 for (int q = 0; q < KG; ++q) {
                     const int base = (((k & 1) * 4 + w) * KG + q) * 2;
@@ -550,6 +625,8 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
                     s_a[(base + 1) * 64 + lane] = make_uint4(al[q][0], al[q][1], al[q][2], al[q][3]);
                 }
             }
+            yv = yn;
+            nv = nn;
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
         }
@@ -624,11 +701,10 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
     using L = WsLds<WCSP, KG>;
     __shared__ uint4 s_op[L::B_ELEMS];
     __shared__ uint4 s_a[L::A_ELEMS];
-    __shared__ float4 s_geo[L::GEO_ELEMS];
     const int st0 = blockIdx.y * steps_per_seg;
     const int st1 = min(st0 + steps_per_seg, nst);
     mixed_ws_block<WCSP, KG, ADJ>(mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
-                                  blockIdx.x, st0, st1, s_op, s_a, s_geo);
+                                  blockIdx.x, st0, st1, s_op, s_a);
 }
 
 }  // namespace baemm
