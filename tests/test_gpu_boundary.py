@@ -164,15 +164,24 @@ def test_c2_size_sampled_rows(mode):
         assert abs(a - b) < 1e-5 * abs(a)
 
 
-@pytest.mark.parametrize("W,wcs,kappa,c3", [(64, 32, [8.0, 16.0], 0.0), (64, 64, [6.0], 0.4 - 0.2j),
-                                            (32, 16, [3.0, 5.0], 0.0), (16, 16, [4.0], 0.3j), (3, 1, [1., 2., 3.], 0.)])
-def test_wave_specialised_mixed_kernel_is_identical(sphere3, W, wcs, kappa, c3):
+# every (column width, chunks per launch, adjoint form) of the wave-specialised
+# kernel: (16|32|64 columns) x (1|2 chunks) x (general | c2-normalised form);
+# c2 = 0 (no double layer) takes the general form
+@pytest.mark.parametrize("W,wcs,kappa,c2,c3", [
+    (64, 32, [8.0, 16.0], -0.75 + 1j, 0.0), (64, 32, [8.0, 16.0], -0.75 + 1j, 0.3),
+    (32, 32, [5.0], 1.0, 0.0), (32, 32, [5.0], 1.0, 0.2j),
+    (64, 64, [6.0], 0.5j, 0.0), (64, 64, [6.0], -0.75 + 1j, 0.4 - 0.2j),
+    (32, 16, [3.0, 5.0], -0.75 + 1j, 0.0), (32, 16, [3.0, 5.0], -0.75 + 1j, 0.5j),
+    (16, 16, [4.0], 2.0, 0.0), (16, 16, [4.0], -0.75 + 1j, 0.3j),
+    (3, 1, [1., 2., 3.], -0.75 + 1j, 0.), (32, 32, [5.0], 0.0, 0.0)])
+def test_wave_specialised_mixed_kernel_is_identical(sphere3, W, wcs, kappa, c2, c3):
     """The wave-specialised MFMA kernel (generation waves feeding MFMA waves
-    through LDS) computes exactly the values of the single-wave kernel."""
+    through LDS, source geometry through DPP row broadcasts) computes exactly
+    the values of the single-wave kernel, and matches the FP64 oracle."""
     m, H = sphere3
     rng = np.random.default_rng(W + wcs)
     B = _rand(rng, m.nv, W)
-    coeff = np.array([0.25, 1.0 - 0.5j, -0.75 + 1j, c3] * (W // wcs))
+    coeff = np.array([0.25, 1.0 - 0.5j, c2, c3] * (W // wcs))
     H.SetPrecision("mixed")
     outs = []
     for variant in (0, 1):
