@@ -15,7 +15,7 @@ LIB_PATH = _HERE / "libbaemm_hip.so"
 
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOCONV, ERR_INTERNAL = range(6)
 C128, C64 = 0, 1
-PRECISION_FP32, PRECISION_MIXED = 0, 1
+PRECISION_FP32, PRECISION_MIXED, PRECISION_FP32_MFMA = 0, 1, 2
 WAVE_PLANE, WAVE_RADIAL = 201, 202
 
 _P = ctypes.c_void_p

@@ -220,6 +220,9 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
 {
     const BatchTuning tune = batch_tuning();
     const int n = (int)es.size();
+    for (int k = 0; k < n; ++k)
+        if (es[k].h->precision == BAEMM_PRECISION_FP32_MFMA)
+            throw ArgError{"batched apply: instances must use BAEMM_PRECISION_FP32 or BAEMM_PRECISION_MIXED"};
     std::vector<Prepared> prep(n);
     // Per-instance preparation and epilogues are short launches; they run on
     // each handle's own stream so that those of different instances overlap.

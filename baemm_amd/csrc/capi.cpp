@@ -200,7 +200,8 @@ int baemm_get_wave_chunk_size(baemm_handle h) { return h ? H(h)->wave_chunk_size
 int baemm_set_precision(baemm_handle h, int precision)
 {
     return guard(h, [&](Handle& hh) {
-        if (precision != BAEMM_PRECISION_FP32 && precision != BAEMM_PRECISION_MIXED)
+        if (precision != BAEMM_PRECISION_FP32 && precision != BAEMM_PRECISION_MIXED &&
+            precision != BAEMM_PRECISION_FP32_MFMA)
             throw ArgError{"unknown precision mode"};
         hh.precision = precision;
         return BAEMM_OK;
@@ -435,7 +436,7 @@ int baemm_autotune_segments(baemm_handle h, int repeats, int* best_segments, dou
         ApplyArgs a{cdouble(1.0, 0.0), cdouble(0.0, 0.0), hh.io[4].ptr, W, kC128, hh.io[5].ptr, W, kC128};
         const int saved = hh.segments;
         const int reps = std::max(1, repeats);
-        const bool mixed = hh.precision == BAEMM_PRECISION_MIXED;
+        const bool mixed = hh.precision != BAEMM_PRECISION_FP32;   // the MFMA kernels
         const int64_t tiles = hh.nt_pad / (mixed ? kMixTile : kTile);
         const int64_t max_seg = mixed ? hh.nt_pad / 64 : hh.nt_pad / kSrcBlock;
         hipEvent_t e0, e1;

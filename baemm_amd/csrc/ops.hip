@@ -194,6 +194,41 @@ __global__ void scale_nrm_kernel(const float4* __restrict__ nrm, int n, float s,
     out[j] = make_float4(s * v.x, s * v.y, s * v.z, v.w);
 }
 
+// B-operand of the FP32 MFMA kernel: for (chunk, step, col-block) and lane,
+// the 8 FP32 values of MFMAs m = 0..7 (two 16-byte parts), in the order of the
+// FP16 operand of split_kernel: rows (source 4 (lane / 32) + m / 2, re | im) of
+// [[br, bi], [-bi, br]], unscaled.
+__global__ void stage_f32_kernel(const float2* __restrict__ b, int ldb, int nst, int WCSP, int K,
+                                 uint4* __restrict__ bop)
+{
+    const int NCB = 2 * WCSP / 32;
+    const int64_t total = (int64_t)K * nst * NCB * 64;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int lane = (int)(idx % 64);
+    int64_t rest = idx / 64;
+    const int cb = (int)(rest % NCB);
+    rest /= NCB;
+    const int st = (int)(rest % nst);
+    const int q = (int)(rest / nst);
+    const int g = lane >> 5;
+    const int c = cb * 32 + (lane & 31);
+    const bool imcol = c >= WCSP;
+    const int kk = imcol ? c - WCSP : c;
+    const int pc = q * WCSP + kk;
+    float e[8];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const int j = 8 * st + 4 * g + jj;
+        const float2 v = b[(int64_t)j * ldb + pc];
+        e[2 * jj] = imcol ? v.y : v.x;
+        e[2 * jj + 1] = imcol ? v.x : -v.y;
+    }
+    const int64_t base = ((((int64_t)q * nst + st) * NCB + cb) * 2) * 64 + lane;
+    bop[base] = make_uint4(__float_as_uint(e[0]), __float_as_uint(e[1]), __float_as_uint(e[2]), __float_as_uint(e[3]));
+    bop[base + 64] = make_uint4(__float_as_uint(e[4]), __float_as_uint(e[5]), __float_as_uint(e[6]), __float_as_uint(e[7]));
+}
+
 // Minimum squared distance between distinct centroids (sets the A scale of
 // the MFMA path).  One target per thread, all sources.
 __global__ void rmin_kernel(const float4* __restrict__ mid, int nt, unsigned* __restrict__ out)
@@ -408,11 +443,13 @@ static PlainFn plain_fn(int op, bool adj, int WCSP, int KG)
 using MixedFn = void (*)(const float4*, const float4*, int, const uint4*, int, CoefPack, float*, int64_t, int, int,
                          int);
 
-static MixedFn mixed_ws_fn(bool adj, int WCSP, int KG)
+static MixedFn mixed_ws_fn(bool adj, int WCSP, int KG, bool f32)
 {
-#define MIXWS(W_, K_)                                                                   \
-    if (WCSP == W_ && KG == K_)                                                         \
-        return adj ? &bdr_mixed_ws_kernel<W_, K_, true> : &bdr_mixed_ws_kernel<W_, K_, false>;
+#define MIXWS(W_, K_)                                                                                    \
+    if (WCSP == W_ && KG == K_) {                                                                        \
+        if (f32) return adj ? &bdr_mixed_ws_kernel<W_, K_, true, true> : &bdr_mixed_ws_kernel<W_, K_, false, true>; \
+        return adj ? &bdr_mixed_ws_kernel<W_, K_, true> : &bdr_mixed_ws_kernel<W_, K_, false>;           \
+    }
     MIXWS(16, 1)
     MIXWS(16, 2)
     MIXWS(32, 1)
@@ -568,7 +605,8 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     pl.kernel_on = p.kernel_on();
     pl.mass_on = p.mass_on();
     if (!pl.kernel_on && !pl.mass_on) return;   // the reference leaves C_out untouched
-    pl.mixed = (h.precision == BAEMM_PRECISION_MIXED) && pl.kernel_on;
+    pl.mixed = (h.precision != BAEMM_PRECISION_FP32) && pl.kernel_on;
+    pl.f32 = pl.mixed && h.precision == BAEMM_PRECISION_FP32_MFMA;
     pl.WCSP = pl.mixed ? wcsp_mixed((int)p.wcs) : wcsp_plain((int)p.wcs);
     pl.ldb = (int)p.K * pl.WCSP;
     pl.ldp = 2 * pl.ldb;
@@ -593,6 +631,27 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     ea.ldb = pl.ldb;
     ea.unscale = nullptr;
     if (!pl.mixed) return;
+    if (pl.f32) {
+        // FP32 MFMA: no FP16 range to fit, no scales; the non-adjoint form still
+        // divides each chunk by c2 (the epilogue multiplies it back)
+        pl.sA.assign(p.K, 1.0f);
+        pl.rot.assign(p.K, cfloat(1.0f, 0.0f));
+        if (!pl.adj)
+            for (int q = 0; q < p.K; ++q) pl.rot[q] = cfloat(p.c[4 * q + 2]);
+        pl.nst = (int)(h.nt_pad / 8);
+        const int NCB = 2 * pl.WCSP / 32;
+        const int64_t nbop = (int64_t)p.K * pl.nst * NCB * 64;
+        h.ws_bop.reserve(sizeof(uint4) * nbop * 2);
+        hipLaunchKernelGGL(stage_f32_kernel, dim3((unsigned)((nbop + 255) / 256)), dim3(256), 0, s, pl.b, pl.ldb,
+                           pl.nst, pl.WCSP, (int)p.K, h.ws_bop.as<uint4>());
+        BAEMM_HIP(hipGetLastError());
+        pl.bop = h.ws_bop.as<uint4>();
+        for (int q = 0; q < p.K; ++q) {
+            ea.rotr[q] = pl.rot[q].real();
+            ea.roti[q] = pl.rot[q].imag();
+        }
+        return;
+    }
     // scales: A per chunk from a bound at the minimum centroid distance,
     // b per column from its maximum.
     const float rmin = compute_rmin(h, s);
@@ -689,12 +748,13 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
     const int S = (nst + steps - 1) / steps;
     float* part = bdr_reserve_part(h, pl, S);
     Timed tm(h, s);
-    const bool ws = h.mixed_kernel == 1;
+    // the FP32 MFMA contraction exists in the wave-specialised kernel only
+    const bool ws = h.mixed_kernel == 1 || pl.f32;
     for (int q0 = 0; q0 < p.K;) {
         int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
         if (ws) kg = std::min(kg, 2);
         const CoefPack cp = make_pack(p, q0, kg, pl.sA.data(), pl.rot.data());
-        MixedFn fn = ws ? mixed_ws_fn(pl.adj, pl.WCSP, kg) : mixed_fn(pl.adj, pl.WCSP, kg);
+        MixedFn fn = ws ? mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32) : mixed_fn(pl.adj, pl.WCSP, kg);
         hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(ws ? 512 : 256), 0, s, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst,
                            cp, part, pl.ea.seg_stride, pl.ldp, q0, steps);
         BAEMM_HIP(hipGetLastError());

@@ -60,6 +60,7 @@ struct Timed {
 // State of one apply between its phases.
 struct BdrPlan {
     bool kernel_on = false, mass_on = false, mixed = false, adj = false;
+    bool f32 = false;            // MFMA kernels in FP32 (BAEMM_PRECISION_FP32_MFMA); mixed is set too
     int WCSP = 0, ldb = 0, ldp = 0;
     int nst = 0;                 // 8-source steps (mixed) of the padded source range
     std::vector<float> sA;       // per-chunk power-of-two scale of A (mixed)

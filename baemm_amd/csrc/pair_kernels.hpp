@@ -241,11 +241,11 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t&
     lo = l;
 }
 
-// FP16 hi/lo A operands of the 4 pairs of one lane for chunks q0..q0+KG-1.
-// Written stage by stage across the 4 sources so that the independent
-// dependency chains interleave (the kernel is bound by VALU latency
-// otherwise).
-template <int KG, bool ADJ>
+// FP16 hi/lo A operands of the 4 pairs of one lane for chunks q0..q0+KG-1
+// (with F32: the FP32 values, ah = Re A, al = Im A).  Written stage by stage
+// across the 4 sources so that the independent dependency chains interleave
+// (the kernel is bound by VALU latency otherwise).
+template <int KG, bool ADJ, bool F32 = false>
 __device__ __forceinline__ void mixed_vals(const BdrGeo (&geo)[4], const CoefPack& cp, uint32_t (&ah)[KG][4],
                                            uint32_t (&al)[KG][4])
 {
@@ -278,7 +278,12 @@ __device__ __forceinline__ void mixed_vals(const BdrGeo (&geo)[4], const CoefPac
             }
             const float Ar = fmaf(cs[jj], Qr, -sn[jj] * Qi);
             const float Ai = fmaf(cs[jj], Qi, sn[jj] * Qr);
-            split2(Ar, Ai, ah[q][jj], al[q][jj]);
+            if constexpr (F32) {
+                ah[q][jj] = __float_as_uint(Ar);
+                al[q][jj] = __float_as_uint(Ai);
+            } else {
+                split2(Ar, Ai, ah[q][jj], al[q][jj]);
+            }
         }
     }
 }
@@ -554,7 +559,7 @@ struct WsLds {
 // q0..q0+KG-1, partial sums to out[row*ldp + ...].  s_op, s_a: LDS of at least
 // WsLds<WCSP,KG> B_ELEMS, A_ELEMS elements.  Ends with every wave past a common
 // barrier, so the caller may reuse the LDS.
-template <int WCSP, int KG, bool ADJ>
+template <int WCSP, int KG, bool ADJ, bool F32 = false>
 __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
                                                const uint4* __restrict__ bop, int nst, const CoefPack& cp,
                                                float* __restrict__ out, int ldp, int q0, int tile, int st0, int st1,
@@ -617,12 +622,18 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
                 BdrGeo geo[4];
                 if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
                 else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
-                mixed_vals<KG, ADJ>(geo, cp, ah, al);
+                mixed_vals<KG, ADJ, F32>(geo, cp, ah, al);
 #pragma unroll
                 for (int q = 0; q < KG; ++q) {
                     const int base = (((k & 1) * 4 + w) * KG + q) * 2;
-                    s_a[base * 64 + lane] = make_uint4(ah[q][0], ah[q][1], ah[q][2], ah[q][3]);
-                    s_a[(base + 1) * 64 + lane] = make_uint4(al[q][0], al[q][1], al[q][2], al[q][3]);
+                    if constexpr (F32) {
+                        // operands of the FP32 MFMAs m = 0..7: (Re, Im) of sources 0..3
+                        s_a[base * 64 + lane] = make_uint4(ah[q][0], al[q][0], ah[q][1], al[q][1]);
+                        s_a[(base + 1) * 64 + lane] = make_uint4(ah[q][2], al[q][2], ah[q][3], al[q][3]);
+                    } else {
+                        s_a[base * 64 + lane] = make_uint4(ah[q][0], ah[q][1], ah[q][2], ah[q][3]);
+                        s_a[(base + 1) * 64 + lane] = make_uint4(al[q][0], al[q][1], al[q][2], al[q][3]);
+                    }
                 }
             }
             yv = yn;
@@ -642,7 +653,36 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
         __syncthreads();
         for (int k = 0; k <= T; ++k) {
             if (k < T) issue_b(k);
-            if (k >= 1) {
+            if (F32 && k >= 1) {
+                // FP32 MFMA m (K = 2): lane half h supplies K element h, i.e. the
+                // component (m & 1) of source 4 h + (m >> 1), for A and for B
+                const int kb = (k - 1) & 1;
+                float a[KG][8], bv[KG][NCB][8];
+#pragma unroll
+                for (int q = 0; q < KG; ++q) {
+                    const int base = ((kb * 4 + w) * KG + q) * 2;
+                    const uint4 u0 = s_a[base * 64 + lane], u1 = s_a[(base + 1) * 64 + lane];
+                    const uint32_t e[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) a[q][m] = __uint_as_float(e[m]);
+#pragma unroll
+                    for (int cb = 0; cb < NCB; ++cb) {
+                        const int o = (q * NCB + cb) * 2;
+                        const uint4 v0 = s_op[(kb * OPS + o) * 64 + lane], v1 = s_op[(kb * OPS + o + 1) * 64 + lane];
+                        const uint32_t f[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) bv[q][cb][m] = __uint_as_float(f[m]);
+                    }
+                }
+                // m-major: consecutive MFMAs write different accumulators
+#pragma unroll
+                for (int m = 0; m < 8; ++m)
+#pragma unroll
+                    for (int q = 0; q < KG; ++q)
+#pragma unroll
+                        for (int cb = 0; cb < NCB; ++cb)
+                            acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q][m], bv[q][cb][m], acc[q][cb], 0, 0, 0);
+            } else if (k >= 1) {
                 const int kb = (k - 1) & 1;
                 half8_t a_hi[KG], a_lo[KG];
 #pragma unroll
@@ -692,7 +732,7 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
     }
 }
 
-template <int WCSP, int KG, bool ADJ>
+template <int WCSP, int KG, bool ADJ, bool F32 = false>
 __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
     const uint4* __restrict__ bop, int nst, CoefPack cp,
@@ -703,7 +743,7 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
     __shared__ uint4 s_a[L::A_ELEMS];
     const int st0 = blockIdx.y * steps_per_seg;
     const int st1 = min(st0 + steps_per_seg, nst);
-    mixed_ws_block<WCSP, KG, ADJ>(mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
+    mixed_ws_block<WCSP, KG, ADJ, F32>(mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
                                   blockIdx.x, st0, st1, s_op, s_a);
 }
 

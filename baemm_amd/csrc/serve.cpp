@@ -75,12 +75,13 @@ size_t estimate_bytes(int64_t nv, int64_t nt, int64_t m, int64_t K, int64_t wcs,
     const int64_t nt_pad = (nt + 255) / 256 * 256;
     int64_t wcsp = 4;
     while (wcsp < wcs) wcsp *= 2;
-    if (precision == BAEMM_PRECISION_MIXED) wcsp = std::max<int64_t>(wcsp, 16);
+    const bool mfma = precision != BAEMM_PRECISION_FP32;
+    if (mfma) wcsp = std::max<int64_t>(wcsp, 16);
     const int64_t ldb = K * wcsp;
     size_t geo = (size_t)nt_pad * 56 + (size_t)nt * 12 + (size_t)nv * 8 * 12 + (size_t)(m + 256) * 16;
     size_t b = (size_t)nt_pad * ldb * 8;
-    size_t bop = precision == BAEMM_PRECISION_MIXED ? 2 * b : 0;
-    const int64_t tiles = std::max<int64_t>(1, nt_pad / (precision == BAEMM_PRECISION_MIXED ? 128 : 256));
+    size_t bop = mfma ? 2 * b : 0;
+    const int64_t tiles = std::max<int64_t>(1, nt_pad / (mfma ? 128 : 256));
     const int64_t S = std::max<int64_t>(1, std::min<int64_t>((2048 + tiles - 1) / tiles, nt_pad / 64));
     size_t part = (size_t)S * nt_pad * 2 * ldb * 4;
     const size_t vec = (size_t)nv * W * 16;

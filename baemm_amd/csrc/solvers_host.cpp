@@ -150,9 +150,9 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
 
     // Iterative refinement around GMRES(m): every restart cycle starts from the
     // true preconditioned residual P (B - A X), accumulated in FP64 from the
-    // FP64 iterate X; with the MFMA operator the residual is taken with the
-    // FP32 operator, so the cycles correct towards the FP32 operator's
-    // solution while their Krylov steps use the faster one.  A solve converges
+    // FP64 iterate X; with the mixed-precision operator the residual is taken
+    // with the FP32 operator, so the cycles correct towards the FP32
+    // operator's solution while their Krylov steps use the faster one.  A solve converges
     // when the Givens estimate of a cycle, or the residual a restart measures,
     // meets tol.
     GmresResult out;
@@ -161,7 +161,7 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
             apply_prec(h, B, V, cg_tol, s);
         } else {
             const int prec = h.precision;
-            h.precision = BAEMM_PRECISION_FP32;
+            if (prec == BAEMM_PRECISION_MIXED) h.precision = BAEMM_PRECISION_FP32;
             try {
                 apply_op(h, X, t, s);
             } catch (...) {

@@ -123,7 +123,8 @@ class _HelmholtzBase:
         return int(self._L.baemm_get_wave_chunk_size(self._h))
 
     def SetPrecision(self, mode):
-        m = {"fp32": _lib.PRECISION_FP32, "mixed": _lib.PRECISION_MIXED}.get(mode, mode)
+        m = {"fp32": _lib.PRECISION_FP32, "mixed": _lib.PRECISION_MIXED,
+             "fp32_mfma": _lib.PRECISION_FP32_MFMA}.get(mode, mode)
         self._check(self._L.baemm_set_precision(self._h, int(m)))
 
     def SetMixedKernel(self, variant):

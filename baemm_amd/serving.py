@@ -66,7 +66,8 @@ class GNServer:
     @staticmethod
     def request_bytes(nv, nt, m, K, wcs, precision="fp32"):
         """Device-memory estimate that admission uses for a request on a fresh handle."""
-        prec = {"fp32": _lib.PRECISION_FP32, "mixed": _lib.PRECISION_MIXED}.get(precision, precision)
+        prec = {"fp32": _lib.PRECISION_FP32, "mixed": _lib.PRECISION_MIXED,
+                "fp32_mfma": _lib.PRECISION_FP32_MFMA}.get(precision, precision)
         rq = GNRequest(None, int(nv), None, int(nt), -1, None, int(m), None, int(K), None, int(wcs), None, 0, int(prec),
                        0.0, 0.0)
         return int(_L().baemm_server_request_bytes(ctypes.byref(rq)))
@@ -88,7 +89,8 @@ class GNServer:
             fobs = np.ascontiguousarray(far_field_data, np.complex128)
             if fobs.shape != F.shape:
                 raise ValueError(f"far_field_data must be {F.shape}")
-        prec = {"fp32": _lib.PRECISION_FP32, "mixed": _lib.PRECISION_MIXED}.get(precision, precision)
+        prec = {"fp32": _lib.PRECISION_FP32, "mixed": _lib.PRECISION_MIXED,
+                "fp32_mfma": _lib.PRECISION_FP32_MFMA}.get(precision, precision)
         rq = GNRequest(V.ctypes.data, V.shape[0], T.ctypes.data, T.shape[0], int(mesh_id), meas.ctypes.data,
                        meas.shape[0], kappa.ctypes.data, K, dirs.ctypes.data, wcs,
                        fobs.ctypes.data if fobs is not None else None, int(wave_type), int(prec), float(cg_tol),

@@ -13,6 +13,7 @@ outputs resident in HBM.  Reported:
 
   value              mixed-precision (FP16 hi+lo MFMA) pair*RHS/s = N_t^2 W / t_step
   fp32_value         plain FP32 apply, same workload (the >= 3x target's denominator)
+  fp32_mfma          FP32 kernel values contracted with FP32 MFMA, same workload
   roofline           dominant kernel (MFMA pair kernel): algorithmic FLOP/s vs dense
                      FP16 MFMA peak; roofline_valu: kernel-generation FLOP/s vs FP32 peak
   cpu_baseline       FP32 restatement of the reference OpenCL kernel (oracle) on the
@@ -93,9 +94,9 @@ def run_gpu(args):
         _lib.check(L.baemm_apply_boundary_dev(h, one, dB, W, _lib.C128, zero, dC, W, _lib.C128, None), h)
 
     results = {}
-    for mode in ("mixed", "mixed_single_wave", "fp32"):
+    for mode in ("mixed", "mixed_single_wave", "fp32", "fp32_mfma"):
         log(f"{mode} apply: {args.warmup} warm-up + {args.steps} timed steps")
-        H.SetPrecision("fp32" if mode == "fp32" else "mixed")
+        H.SetPrecision({"fp32": "fp32", "fp32_mfma": "fp32_mfma"}.get(mode, "mixed"))
         H.SetMixedKernel(0 if mode == "mixed_single_wave" else 1)
         for _ in range(args.warmup):
             step()
@@ -414,6 +415,15 @@ def main():
         "fp32_ms_per_step": fp32["ms_per_step"],
         "speedup_mixed_vs_fp32": fp32["ms_per_step"] / mixed["ms_per_step"],
         "rel_diff_mixed_vs_fp32": rel,
+        "fp32_mfma": {
+            "value": pairs_rhs / (res["fp32_mfma"]["ms_per_step"] * 1e-3),
+            "ms_per_step": res["fp32_mfma"]["ms_per_step"],
+            "kernel_ms": res["fp32_mfma"]["kernel_ms"],
+            "speedup_vs_fp32": fp32["ms_per_step"] / res["fp32_mfma"]["ms_per_step"],
+            "rel_diff_vs_fp32": float(np.linalg.norm(res["fp32_mfma"]["out"] - fp32["out"]) /
+                                      np.linalg.norm(fp32["out"])),
+            "arithmetic": "kernel values in f32, contraction on v_mfma_f32_32x32x2_f32 with f32 accumulation",
+        },
         "roofline": {
             "bound": "mfma",
             "achieved": achieved,

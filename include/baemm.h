@@ -50,7 +50,7 @@ enum { BAEMM_C128 = 0, BAEMM_C64 = 1 };
 /* Arithmetic of the triangle-pair contraction.  FP32: kernel values and
  * contraction in FP32 on the vector ALU.  MIXED: kernel values in FP32, the
  * contraction on MFMA as FP16 hi+lo split products with FP32 accumulation. */
-enum { BAEMM_PRECISION_FP32 = 0, BAEMM_PRECISION_MIXED = 1 };
+enum { BAEMM_PRECISION_FP32 = 0, BAEMM_PRECISION_MIXED = 1, BAEMM_PRECISION_FP32_MFMA = 2 };
 
 /* WaveType, BAEMM_Common.hpp:7-11 */
 enum { BAEMM_WAVE_PLANE = 201, BAEMM_WAVE_RADIAL = 202 };
@@ -101,7 +101,16 @@ int baemm_set_wave_chunk_size(baemm_handle h, int wave_chunk_size);
 int baemm_get_wave_chunk_size(baemm_handle h);
 
 /* Pair-contraction arithmetic (no reference counterpart: the reference has
- * only FP32).  Default BAEMM_PRECISION_FP32. */
+ * only FP32).  Kernel values are generated in FP32 in every mode; the
+ * contraction with the staged sources runs
+ *   BAEMM_PRECISION_FP32:      in FP32 on the vector ALUs (the default),
+ *   BAEMM_PRECISION_MIXED:     on MFMA as FP16 hi+lo products with FP32
+ *                              accumulation (within 1e-6 of FP32),
+ *   BAEMM_PRECISION_FP32_MFMA: on MFMA in FP32 (v_mfma_f32_32x32x2_f32) with
+ *                              FP32 accumulation.
+ * The MFMA modes apply to the boundary operators; the far field, potential
+ * and Herglotz kernels stay FP32.  baemm_apply_boundary_batch takes FP32 and
+ * MIXED instances only. */
 int baemm_set_precision(baemm_handle h, int precision);
 int baemm_get_precision(baemm_handle h);
 

@@ -100,3 +100,15 @@ def test_batch_autotune_keeps_results(batch):
     for k in range(len(batch)):
         assert relerr(after[k], before[k]) < 2e-6
     BatchTuning(**saved)
+
+
+def test_batch_rejects_fp32_mfma_instances(batch):
+    """The batched launch has FP32 and mixed-precision bodies only: an instance
+    set to BAEMM_PRECISION_FP32_MFMA is refused with an argument error."""
+    H = batch[0]["H"]
+    H.SetPrecision("fp32_mfma")
+    try:
+        with pytest.raises(BaemmError):
+            ApplyBoundaryOperatorsBatch([H], [batch[0]["B"]])
+    finally:
+        H.SetPrecision(batch[0]["prec"])

@@ -29,7 +29,7 @@ def sphere3():
     H.close()
 
 
-@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+@pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
 @pytest.mark.parametrize(
     "W,wcs,kappa",
     [(16, 16, [4.0]), (8, 4, [4.0, 2.5]), (12, 3, [1.0, 2.0, 3.0, 4.0]), (64, 32, [8.0, 16.0]), (1, 1, [3.0]),
@@ -50,7 +50,7 @@ def test_boundary_apply_matches_oracle(sphere3, mode, W, wcs, kappa):
     assert relerr(C, ref) < TOL
 
 
-@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+@pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
 def test_c1_all_operators_vs_fp32_restatement(mode):
     """Config 1: 5120 triangles, kappa 4, 16 plane-wave RHS, all four operators;
     compared with the FP32 restatement of the reference kernel and the FP64 oracle."""
@@ -119,7 +119,7 @@ def test_argument_errors(sphere3):
         H.SetPrecision(7)
 
 
-@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+@pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
 def test_nonspherical_meshes(mode):
     for V, T in (perturbed_sphere(3, seed=3), torus(1.0, 0.4, 48, 16)):
         m = oracle.Mesh(V, T)
@@ -133,7 +133,7 @@ def test_nonspherical_meshes(mode):
         assert relerr(C, ref) < TOL
 
 
-@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+@pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
 def test_c2_size_sampled_rows(mode):
     """Config 2 size: 81,920 triangles, kappa in {8, 16}, 2 x 32 RHS, SL+DL with
     coefficients (0,1,1,0) and (1/2, -i kappa, 1, 0); 48 sampled vertex rows
@@ -196,7 +196,7 @@ def test_wave_specialised_mixed_kernel_is_identical(sphere3, W, wcs, kappa, c2, 
     assert relerr(outs[1], ref) < TOL
 
 
-@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+@pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
 def test_segment_autotuner_keeps_results(sphere3, mode):
     """baemm_autotune_segments picks a segment count; applies after tuning still
     match the FP64 oracle."""
@@ -231,7 +231,7 @@ def test_largest_mesh_327680_triangles_sampled_rows():
     rows = np.sort(rng.choice(m.nv, 16, replace=False))
     ref = oracle.apply_boundary_rows(m, 1.0, B, 0.0, np.zeros((16, W)), [32.0], coeff, W, wcs, rows)
     with Helmholtz_HIP(V, T) as H:
-        for mode in ("mixed", "fp32"):
+        for mode in ("mixed", "fp32", "fp32_mfma"):
             H.SetPrecision(mode)
             C = np.zeros_like(B)
             H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [32.0], coeff, W, wcs)
@@ -240,7 +240,7 @@ def test_largest_mesh_327680_triangles_sampled_rows():
             assert e < TOL
 
 
-@pytest.mark.parametrize("mode", ["fp32", "mixed"])
+@pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
 def test_smallest_mesh_icosahedron(mode):
     """20 triangles (one padded tile of 256 rows): padding rows and masked
     steps only, against the FP64 oracle."""
@@ -277,7 +277,7 @@ def test_high_wavenumber_sampled_rows():
     for coeff in (np.array([0.5, -100j, 1, 0, 0.5, -250j, 1, 0], np.complex128), np.array(FULL_COEFF * 2)):
         ref = oracle.apply_boundary_rows(m, 1.0, B, 0.0, np.zeros((32, W)), kappa, coeff, W, wcs, rows)
         with Helmholtz_HIP(V, T) as H:
-            for mode in ("mixed", "fp32"):
+            for mode in ("mixed", "fp32", "fp32_mfma"):
                 H.SetPrecision(mode)
                 C = np.zeros_like(B)
                 H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)

@@ -29,7 +29,7 @@ def test_dirichlet_to_neumann_matches_dense_solve(small):
     kappa = [2.0, 3.0]
     wave = rng.standard_normal((m.nv, W)) + 1j * rng.standard_normal((m.nv, W))
     ref = oracle.dtn(m, kappa, wave, wcs)
-    for mode in ("fp32", "mixed"):
+    for mode in ("fp32", "mixed", "fp32_mfma"):
         H.SetPrecision(mode)
         psi, ok = H.DirichletToNeumann(kappa, wave, wcs, 1e-12, 1e-10)
         assert ok
@@ -165,7 +165,7 @@ def test_graph_captured_arnoldi_steps_match_eager(small):
     W, wcs = 6, 3
     kappa = [2.0, 3.0]
     wave = rng.standard_normal((m.nv, W)) + 1j * rng.standard_normal((m.nv, W))
-    for mode in ("fp32", "mixed"):
+    for mode in ("fp32", "mixed", "fp32_mfma"):
         H.SetPrecision(mode)
         H.SetGraphs(False)
         eager, ok = H.DirichletToNeumann(kappa, wave, wcs, 1e-12, 1e-10)
@@ -198,7 +198,7 @@ def test_c3_torus_small_matches_dense_fp64_solve():
     dirs = np.array([[0.0, 0.0, 1.0], [0.6, 0.8, 0.0], [0.0, 0.6, -0.8], [-0.48, 0.6, 0.64]])
     ref, _ = oracle.far_field(m, [4.0], dirs, 4)
     with Helmholtz_HIP(m.V, m.T, m.meas) as H:
-        for mode in ("fp32", "mixed"):
+        for mode in ("fp32", "mixed", "fp32_mfma"):
             H.SetPrecision(mode)
             F = H.FarField([4.0], 1, dirs, 4, cg_tol=1e-10, gmres_tol=1e-6)
             err = relerr(F, ref)
@@ -219,16 +219,17 @@ def test_c3_torus_full_size_mixed_vs_fp32():
     dirs = random_directions(32, seed=2)
     out = {}
     with Helmholtz_HIP(V, T, meas) as H:
-        for mode in ("fp32", "mixed"):
+        for mode in ("fp32", "mixed", "fp32_mfma"):
             H.SetPrecision(mode)
             H.ResetSolverStats()
             out[mode] = H.FarField([10.0], 1, dirs, 32, cg_tol=1e-10, gmres_tol=1e-6)
             st = H.SolverStats()
             print("torus 38400", mode, st)
             assert st["rel_residual"] <= 1e-6
-    d = relerr(out["mixed"], out["fp32"])
-    print("mixed vs fp32 far field", d)
-    assert d < 1e-4
+    for mode in ("mixed", "fp32_mfma"):
+        d = relerr(out[mode], out["fp32"])
+        print(mode, "vs fp32 far field", d)
+        assert d < 1e-4
 
 
 def test_c3b_icosphere_81920_against_mie():
