@@ -89,7 +89,8 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
     case ID: {                                                                                                   \
         CoefPack cp;                                                                                             \
         for (int g = 0; g < KG_; ++g) cp.c[g] = cf[g];                                                           \
-        mixed_ws_block<WCSP_, KG_, ADJ_>(in.mid, in.nrm, in.nt, in.bop, in.nst, cp, out, in.ldp, w.q0, w.tile,   \
+        mixed_ws_block<WCSP_, KG_, ADJ_>(in.mid, in.nt_pad, true, in.mid, in.nrm, in.nt, in.bop, in.nst, cp, out,  \
+                                         in.ldp, w.q0, w.tile,                                                     \
                                          w.s0, w.s1, s_op, s_a);                                                 \
         break;                                                                                                   \
     }

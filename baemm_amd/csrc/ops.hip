@@ -229,13 +229,16 @@ __global__ void stage_f32_kernel(const float2* __restrict__ b, int ldb, int nst,
     bop[base + 64] = make_uint4(__float_as_uint(e[4]), __float_as_uint(e[5]), __float_as_uint(e[6]), __float_as_uint(e[7]));
 }
 
-// Minimum squared distance between distinct centroids (sets the A scale of
-// the MFMA path).  One target per thread, all sources.
-__global__ void rmin_kernel(const float4* __restrict__ mid, int nt, unsigned* __restrict__ out)
+// Minimum squared distance between the targets x_i (i < ntgt) and the
+// centroids y_j (j < nt), leaving out j = i when self (distinct centroids);
+// sets the A scale of the mixed-precision MFMA path.  One target per thread,
+// all centroids; the result (a non-negative float) is atomically min-ed as bits.
+__global__ void rmin_kernel(const float4* __restrict__ tgt, int ntgt, const float4* __restrict__ mid, int nt,
+                            int self, unsigned* __restrict__ out)
 {
     __shared__ float4 s[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const float4 x = mid[min(i, nt - 1)];
+    const float4 x = tgt[min(i, ntgt - 1)];
     float m = 3.0e38f;
     for (int jb = 0; jb < nt; jb += 256) {
         __syncthreads();
@@ -246,12 +249,12 @@ __global__ void rmin_kernel(const float4* __restrict__ mid, int nt, unsigned* __
             const float4 y = s[jj];
             const float d0 = y.x - x.x, d1 = y.y - x.y, d2 = y.z - x.z;
             const float r2 = fmaf(d0, d0, fmaf(d1, d1, d2 * d2));
-            if (jb + jj != i) m = fminf(m, r2);
+            if (!self || jb + jj != i) m = fminf(m, r2);
         }
     }
-    if (i >= nt) m = 3.0e38f;
+    if (i >= ntgt) m = 3.0e38f;
     for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0 && i < nt) atomicMin(out, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0 && i < ntgt) atomicMin(out, __float_as_uint(m));
 }
 
 // --------------------------------------------------------------- epilogues
@@ -329,11 +332,17 @@ __global__ __launch_bounds__(256) void bdr_epilogue_kernel(EpiArgs a)
     }
 }
 
-// F[i][k] = alpha * unscale * sum_s part_s[i][k] + beta * F[i][k]  (far field, potential).
+// Factors per wave-number chunk the MFMA near-field partials are multiplied by.
+struct ChunkRot {
+    float r[64], i[64];
+};
+
+// F[i][k] = alpha * rot_q * unscale_k * sum_s part_s[i][k] + beta * F[i][k]
+// (far field, potential; unscale null and rot = 1 for the FP32 kernels).
 template <int CT>
 __global__ void rows_epilogue_kernel(int nrows, int W, int wcs, int WCSP, int S, int64_t seg_stride, int ldp,
                                      const float* __restrict__ part, double ar, double ai, double br, double bi,
-                                     void* Cv, int64_t ldC)
+                                     void* Cv, int64_t ldC, const float* __restrict__ unscale, ChunkRot rot)
 {
     using TC = typename Elem<CT>::type;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -345,6 +354,17 @@ __global__ void rows_epilogue_kernel(int nrows, int W, int wcs, int WCSP, int S,
     for (int s = 0; s < S; ++s) {
         pr += (double)pp[(int64_t)s * seg_stride];
         pi += (double)pp[(int64_t)s * seg_stride + WCSP];
+    }
+    if (unscale) {
+        const double us = (double)unscale[q * WCSP + kk];
+        pr *= us;
+        pi *= us;
+    }
+    {
+        const double rr = rot.r[q], ri = rot.i[q];
+        const double tr = pr * rr - pi * ri;
+        pi = pr * ri + pi * rr;
+        pr = tr;
     }
     TC* C = static_cast<TC*>(Cv);
     TC& dst = C[(int64_t)r * ldC + k];
@@ -443,7 +463,10 @@ static PlainFn plain_fn(int op, bool adj, int WCSP, int KG)
 using MixedFn = void (*)(const float4*, const float4*, int, const uint4*, int, CoefPack, float*, int64_t, int, int,
                          int);
 
-static MixedFn mixed_ws_fn(bool adj, int WCSP, int KG, bool f32)
+using WsFn = void (*)(const float4*, int, int, const float4*, const float4*, int, const uint4*, int, CoefPack, float*,
+                      int64_t, int, int, int);
+
+static WsFn mixed_ws_fn(bool adj, int WCSP, int KG, bool f32)
 {
 #define MIXWS(W_, K_)                                                                                    \
     if (WCSP == W_ && KG == K_) {                                                                        \
@@ -514,21 +537,27 @@ static void run_plain(Handle& h, int op, bool adj, int WCSP, const float4* tx, c
     tm.done();
 }
 
-float compute_rmin(Handle& h, hipStream_t s)
+// Minimum distance between the targets and the centroids (synchronises s).
+static float min_distance(Handle& h, const float4* tgt, int ntgt, bool self, hipStream_t s)
 {
-    if (h.rmin >= 0.0f) return h.rmin;
     h.ws_misc.reserve(256);
     unsigned init = 0x7f7fffffu;
     BAEMM_HIP(hipMemcpyAsync(h.ws_misc.ptr, &init, sizeof(unsigned), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(rmin_kernel, dim3((h.nt + 255) / 256), dim3(256), 0, s, h.d_mid, (int)h.nt,
-                       h.ws_misc.as<unsigned>());
+    hipLaunchKernelGGL(rmin_kernel, dim3((ntgt + 255) / 256), dim3(256), 0, s, tgt, ntgt, h.d_mid, (int)h.nt,
+                       self ? 1 : 0, h.ws_misc.as<unsigned>());
     BAEMM_HIP(hipGetLastError());
     unsigned out = 0;
     BAEMM_HIP(hipMemcpyAsync(&out, h.ws_misc.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     BAEMM_HIP(hipStreamSynchronize(s));
     float r2;
     memcpy(&r2, &out, sizeof(float));
-    h.rmin = std::sqrt(std::max(r2, 1e-30f));
+    return std::sqrt(std::max(r2, 1e-30f));
+}
+
+float compute_rmin(Handle& h, hipStream_t s)
+{
+    if (h.rmin >= 0.0f) return h.rmin;
+    h.rmin = min_distance(h, h.d_mid, (int)h.nt, true, s);
     return h.rmin;
 }
 
@@ -598,39 +627,14 @@ static void launch_av(Handle& h, const void* B, int64_t ldB, int Btype, int WCSP
     BAEMM_HIP(hipGetLastError());
 }
 
-void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
+// Operands of the MFMA pair kernels for the staged sources pl.b: the B operand
+// (hi/lo split with per-column scales, or FP32), the chunk factors pl.rot and
+// scales pl.sA folded into the coefficients, the source normals pl.nrm the
+// kernel reads, and pl.unscale (per column, null for FP32 MFMA).  rmin: lower
+// bound of the target-centroid distances (mixed precision only).
+static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
 {
     const Params& p = h.prm;
-    if (!p.loaded) throw ArgError{"operators not loaded"};
-    pl.kernel_on = p.kernel_on();
-    pl.mass_on = p.mass_on();
-    if (!pl.kernel_on && !pl.mass_on) return;   // the reference leaves C_out untouched
-    pl.mixed = (h.precision != BAEMM_PRECISION_FP32) && pl.kernel_on;
-    pl.f32 = pl.mixed && h.precision == BAEMM_PRECISION_FP32_MFMA;
-    pl.WCSP = pl.mixed ? wcsp_mixed((int)p.wcs) : wcsp_plain((int)p.wcs);
-    pl.ldb = (int)p.K * pl.WCSP;
-    pl.ldp = 2 * pl.ldb;
-    pl.adj = p.any(3);
-    // The MFMA kernels' non-adjoint form divides each chunk by its c2; chunks
-    // without a double layer take the general (adjoint) form.
-    if (pl.mixed)
-        for (int q = 0; q < p.K; ++q)
-            if (p.c[4 * q + 2] == cfloat(0.0f, 0.0f)) pl.adj = true;
-    pl.nrm = h.d_nrm;
-    EpiArgs& ea = pl.ea;
-    ea = base_epi(h, a, pl.WCSP);
-    ea.mass_on = pl.mass_on;
-    ea.kernel_on = pl.kernel_on;
-    ea.diag_on = pl.kernel_on && h.use_diagonal && p.any(1);
-    if (!pl.kernel_on) return;
-    h.ws_b32.reserve(sizeof(float2) * h.nt_pad * pl.ldb);
-    pl.b = h.ws_b32.as<float2>();
-    launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
-    ea.b = pl.b;
-    ea.ldp = pl.ldp;
-    ea.ldb = pl.ldb;
-    ea.unscale = nullptr;
-    if (!pl.mixed) return;
     if (pl.f32) {
         // FP32 MFMA: no FP16 range to fit, no scales; the non-adjoint form still
         // divides each chunk by c2 (the epilogue multiplies it back)
@@ -646,15 +650,11 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
                            pl.nst, pl.WCSP, (int)p.K, h.ws_bop.as<uint4>());
         BAEMM_HIP(hipGetLastError());
         pl.bop = h.ws_bop.as<uint4>();
-        for (int q = 0; q < p.K; ++q) {
-            ea.rotr[q] = pl.rot[q].real();
-            ea.roti[q] = pl.rot[q].imag();
-        }
+        pl.unscale = nullptr;
         return;
     }
-    // scales: A per chunk from a bound at the minimum centroid distance,
-    // b per column from its maximum.
-    const float rmin = compute_rmin(h, s);
+    // scales: A per chunk from a bound at the minimum target-centroid
+    // distance rmin, b per column from its maximum.
     pl.sA.assign(p.K, 1.0f);
     pl.rot.assign(p.K, cfloat(1.0f, 0.0f));
     ChunkScales csc{};
@@ -709,7 +709,45 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
                        pl.WCSP, (int)p.K, scale, h.ws_bop.as<uint4>());
     BAEMM_HIP(hipGetLastError());
     pl.bop = h.ws_bop.as<uint4>();
-    ea.unscale = unscale;
+    pl.unscale = unscale;
+}
+
+
+void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
+{
+    const Params& p = h.prm;
+    if (!p.loaded) throw ArgError{"operators not loaded"};
+    pl.kernel_on = p.kernel_on();
+    pl.mass_on = p.mass_on();
+    if (!pl.kernel_on && !pl.mass_on) return;   // the reference leaves C_out untouched
+    pl.mixed = (h.precision != BAEMM_PRECISION_FP32) && pl.kernel_on;
+    pl.f32 = pl.mixed && h.precision == BAEMM_PRECISION_FP32_MFMA;
+    pl.WCSP = pl.mixed ? wcsp_mixed((int)p.wcs) : wcsp_plain((int)p.wcs);
+    pl.ldb = (int)p.K * pl.WCSP;
+    pl.ldp = 2 * pl.ldb;
+    pl.adj = p.any(3);
+    // The MFMA kernels' non-adjoint form divides each chunk by its c2; chunks
+    // without a double layer take the general (adjoint) form.
+    if (pl.mixed)
+        for (int q = 0; q < p.K; ++q)
+            if (p.c[4 * q + 2] == cfloat(0.0f, 0.0f)) pl.adj = true;
+    pl.nrm = h.d_nrm;
+    EpiArgs& ea = pl.ea;
+    ea = base_epi(h, a, pl.WCSP);
+    ea.mass_on = pl.mass_on;
+    ea.kernel_on = pl.kernel_on;
+    ea.diag_on = pl.kernel_on && h.use_diagonal && p.any(1);
+    if (!pl.kernel_on) return;
+    h.ws_b32.reserve(sizeof(float2) * h.nt_pad * pl.ldb);
+    pl.b = h.ws_b32.as<float2>();
+    launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
+    ea.b = pl.b;
+    ea.ldp = pl.ldp;
+    ea.ldb = pl.ldb;
+    ea.unscale = nullptr;
+    if (!pl.mixed) return;
+    stage_mfma(h, pl, pl.f32 ? 0.0f : compute_rmin(h, s), s);
+    ea.unscale = pl.unscale;
     for (int q = 0; q < p.K; ++q) {
         ea.rotr[q] = pl.rot[q].real();
         ea.roti[q] = pl.rot[q].imag();
@@ -754,9 +792,13 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
         int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
         if (ws) kg = std::min(kg, 2);
         const CoefPack cp = make_pack(p, q0, kg, pl.sA.data(), pl.rot.data());
-        MixedFn fn = ws ? mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32) : mixed_fn(pl.adj, pl.WCSP, kg);
-        hipLaunchKernelGGL(fn, dim3(tiles, S), dim3(ws ? 512 : 256), 0, s, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst,
-                           cp, part, pl.ea.seg_stride, pl.ldp, q0, steps);
+        if (ws)
+            hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32), dim3(tiles, S), dim3(512), 0, s, h.d_mid,
+                               (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
+                               pl.ldp, q0, steps);
+        else
+            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg), dim3(tiles, S), dim3(256), 0, s, h.d_mid, pl.nrm,
+                               (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride, pl.ldp, q0, steps);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;
     }
@@ -778,18 +820,25 @@ void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
 }
 
 static void rows_epilogue(int nrows, const Params& p, int WCSP, int S, int64_t seg_stride, int ldp, const float* part,
-                          const ApplyArgs& a, hipStream_t s)
+                          const ApplyArgs& a, hipStream_t s, const float* unscale = nullptr,
+                          const std::vector<cfloat>* rotv = nullptr)
 {
+    ChunkRot rot;
+    for (int q = 0; q < 64; ++q) {
+        const cfloat f = (rotv && q < (int)rotv->size()) ? (*rotv)[q] : cfloat(1.0f, 0.0f);
+        rot.r[q] = f.real();
+        rot.i[q] = f.imag();
+    }
     const int64_t total = (int64_t)nrows * p.W;
     const dim3 grid((unsigned)((total + 255) / 256));
     if (a.Ctype == kC128)
         hipLaunchKernelGGL(rows_epilogue_kernel<kC128>, grid, dim3(256), 0, s, nrows, (int)p.W, (int)p.wcs, WCSP, S,
                            seg_stride, ldp, part, a.alpha.real(), a.alpha.imag(), a.beta.real(), a.beta.imag(), a.C,
-                           a.ldC);
+                           a.ldC, unscale, rot);
     else
         hipLaunchKernelGGL(rows_epilogue_kernel<kC64>, grid, dim3(256), 0, s, nrows, (int)p.W, (int)p.wcs, WCSP, S,
                            seg_stride, ldp, part, a.alpha.real(), a.alpha.imag(), a.beta.real(), a.beta.imag(), a.C,
-                           a.ldC);
+                           a.ldC, unscale, rot);
     BAEMM_HIP(hipGetLastError());
 }
 
@@ -823,8 +872,58 @@ void apply_far_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
     apply_rows_op(h, OP_FAR, a, h.d_meas, (int)h.m, (int)h.m_pad, s);
 }
 
+// Potential at P points with the MFMA kernels (precision MIXED or FP32_MFMA):
+// the wave-specialised boundary kernel with the points as targets and no pair
+// left out (NearFieldOperatorKernel.cl uses the same single/double-layer
+// kernel values), then the row epilogue with the scales and chunk factors.
+static void apply_near_mfma(Handle& h, const ApplyArgs& a, const float4* d_pts, int P, hipStream_t s)
+{
+    const Params& p = h.prm;
+    BdrPlan pl;
+    pl.kernel_on = pl.mixed = true;
+    pl.f32 = h.precision == BAEMM_PRECISION_FP32_MFMA;
+    pl.WCSP = wcsp_mixed((int)p.wcs);
+    pl.ldb = (int)p.K * pl.WCSP;
+    pl.ldp = 2 * pl.ldb;
+    // no adjoint double layer for points: the general form only for chunks without c2
+    pl.adj = false;
+    for (int q = 0; q < p.K; ++q)
+        if (p.c[4 * q + 2] == cfloat(0.0f, 0.0f)) pl.adj = true;
+    pl.nrm = h.d_nrm;
+    h.ws_b32.reserve(sizeof(float2) * h.nt_pad * pl.ldb);
+    pl.b = h.ws_b32.as<float2>();
+    launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
+    stage_mfma(h, pl, pl.f32 ? 0.0f : min_distance(h, d_pts, P, false, s), s);
+    const int P_pad = (P + kMixTile - 1) / kMixTile * kMixTile;
+    const int nst = pl.nst;
+    const int tiles = P_pad / kMixTile;
+    const int S0 = choose_segments(h, tiles, nst / 8, h.nt);
+    const int steps = (nst + S0 - 1) / S0;
+    const int S = (nst + steps - 1) / steps;
+    const int64_t seg_stride = (int64_t)P_pad * pl.ldp;
+    h.ws_part.reserve(sizeof(float) * seg_stride * S);
+    float* part = h.ws_part.as<float>();
+    // every pair of the source range is written: no clearing needed
+    Timed tm(h, s);
+    for (int q0 = 0; q0 < p.K;) {
+        const int kg = std::min(pick_kg(pl.WCSP, (int)p.K - q0, true), 2);
+        const CoefPack cp = make_pack(p, q0, kg, pl.sA.data(), pl.rot.data());
+        hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32), dim3(tiles, S), dim3(512), 0, s, d_pts, P, 0,
+                           h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, seg_stride, pl.ldp, q0, steps);
+        BAEMM_HIP(hipGetLastError());
+        q0 += kg;
+    }
+    tm.done();
+    rows_epilogue(P, p, pl.WCSP, S, seg_stride, pl.ldp, part, a, s, pl.unscale, &pl.rot);
+}
+
 void apply_near_dev(Handle& h, const ApplyArgs& a, const float4* d_pts, int64_t P, hipStream_t s)
 {
+    const Params& p = h.prm;
+    if (h.precision != BAEMM_PRECISION_FP32 && p.loaded && (p.any(1) || p.any(2)) && p.K <= 64 && P > 0) {
+        apply_near_mfma(h, a, d_pts, (int)P, s);
+        return;
+    }
     const int P_pad = (int)(((P + kTile - 1) / kTile) * kTile);
     apply_rows_op(h, OP_NEAR, a, d_pts, (int)P, P_pad, s);
 }

@@ -68,6 +68,7 @@ struct BdrPlan {
     const float4* nrm = nullptr; // source normals of the pair kernel (scaled in the non-adjoint mixed form)
     float2* b = nullptr;         // staged sources b = Av B
     const uint4* bop = nullptr;  // split MFMA operand (mixed)
+    float* unscale = nullptr;    // per-column 1 / (A scale x b scale) of the mixed path
     EpiArgs ea{};
 };
 

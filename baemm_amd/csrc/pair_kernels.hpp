@@ -556,11 +556,15 @@ struct WsLds {
 
 // One block (8 waves) of the wave-specialised kernel: targets
 // [tile*128, tile*128+128) against source steps [st0, st1) of chunks
-// q0..q0+KG-1, partial sums to out[row*ldp + ...].  s_op, s_a: LDS of at least
-// WsLds<WCSP,KG> B_ELEMS, A_ELEMS elements.  Ends with every wave past a common
-// barrier, so the caller may reuse the LDS.
+// q0..q0+KG-1, partial sums to out[row*ldp + ...].  The targets are tgt[0 ..
+// ntgt) (reads clamp there): the centroids themselves (self, boundary
+// operators: the pair i = j is left out, the target normals are nrm) or other
+// points (potential: no pair left out, no target normals).  s_op, s_a: LDS of
+// at least WsLds<WCSP,KG> B_ELEMS, A_ELEMS elements.  Ends with every wave past
+// a common barrier, so the caller may reuse the LDS.
 template <int WCSP, int KG, bool ADJ, bool F32 = false>
-__device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
+__device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ tgt, int ntgt, bool self,
+                                               const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
                                                const uint4* __restrict__ bop, int nst, const CoefPack& cp,
                                                float* __restrict__ out, int ldp, int q0, int tile, int st0, int st1,
                                                uint4* s_op, uint4* s_a)
@@ -580,9 +584,10 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
     const int T = max(0, st1 - st0);
     const int nt_pad = nst * 8;
     // Steps to mask (wave-uniform, kept in scalar registers): the 4 steps whose
-    // sources are this wave's targets, and steps reaching past nt.
-    const int st_diag = __builtin_amdgcn_readfirstlane(ibase >> 3);
+    // sources are this wave's targets (self only), and steps reaching past nt.
+    const int st_diag = self ? __builtin_amdgcn_readfirstlane(ibase >> 3) : -(1 << 28);
     const int st_tail = nt >> 3;
+    const int i_mask = self ? i : -1;   // the source index left out
 
     auto issue_b = [&](int k) {
         const int st = st0 + k;
@@ -595,9 +600,10 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
     };
 
     if (producer) {
-        const float4 x = mid[i];
+        const float4 x = tgt[min(i, ntgt - 1)];
         float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
-        if constexpr (ADJ) ni = nrm[i];
+        if constexpr (ADJ)
+            if (self) ni = nrm[i];
         // the source of each step this lane holds for mixed_geo_dpp
         const int src_lane = 4 * g + (lane & 3);
         float4 yv = make_float4(0.f, 0.f, 0.f, 0.f), nv = yv;
@@ -620,8 +626,8 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
                 uint32_t ah[KG][4], al[KG][4];
                 const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
                 BdrGeo geo[4];
-                if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
-                else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
+                if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
+                else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
                 mixed_vals<KG, ADJ, F32>(geo, cp, ah, al);
 #pragma unroll
                 for (int q = 0; q < KG; ++q) {
@@ -734,6 +740,7 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ mid, c
 
 template <int WCSP, int KG, bool ADJ, bool F32 = false>
 __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
+    const float4* __restrict__ tgt, int ntgt, int self,
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
     const uint4* __restrict__ bop, int nst, CoefPack cp,
     float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
@@ -743,7 +750,8 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
     __shared__ uint4 s_a[L::A_ELEMS];
     const int st0 = blockIdx.y * steps_per_seg;
     const int st1 = min(st0 + steps_per_seg, nst);
-    mixed_ws_block<WCSP, KG, ADJ, F32>(mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
+    mixed_ws_block<WCSP, KG, ADJ, F32>(tgt, ntgt, self != 0, mid, nrm, nt, bop, nst, cp,
+                                       part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
                                   blockIdx.x, st0, st1, s_op, s_a);
 }
 

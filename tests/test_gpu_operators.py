@@ -40,18 +40,37 @@ def test_far_field_apply(mesh_h, W, wcs, kappa):
     assert relerr(F, ref) < TOL
 
 
-def test_near_field_apply(mesh_h):
+@pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
+def test_near_field_apply(mesh_h, mode):
+    """Potential at 300 exterior points (radii 1.6-2.6) in each precision; the
+    MFMA modes run the wave-specialised boundary kernel with the points as
+    targets.  A second call with points close to the surface (radius 1.02) and
+    alpha, beta != 0 checks the mixed-precision scale bound and the epilogue."""
     m, H = mesh_h
-    rng = np.random.default_rng(7)
-    W, wcs = 8, 4
-    B = _rand(rng, m.nv, W)
-    P = rng.standard_normal((300, 3))
-    P *= (1.6 + rng.random(300))[:, None] / np.linalg.norm(P, axis=1, keepdims=True)
-    coeff = np.array(CO * 2)
-    ref = oracle.apply_near_field(m, 1.0, B, 0.0, np.zeros((300, W)), [2.0, 4.5], coeff, W, wcs, P)
-    U = np.zeros((300, W), np.complex128)
-    H.ApplyNearFieldOperators_PL(1.0, B, 0.0, U, [2.0, 4.5], coeff, W, wcs, P)
-    assert relerr(U, ref) < TOL
+    H.SetPrecision(mode)
+    try:
+        rng = np.random.default_rng(7)
+        W, wcs = 8, 4
+        B = _rand(rng, m.nv, W)
+        P = rng.standard_normal((300, 3))
+        P *= (1.6 + rng.random(300))[:, None] / np.linalg.norm(P, axis=1, keepdims=True)
+        coeff = np.array(CO * 2)
+        ref = oracle.apply_near_field(m, 1.0, B, 0.0, np.zeros((300, W)), [2.0, 4.5], coeff, W, wcs, P)
+        U = np.zeros((300, W), np.complex128)
+        H.ApplyNearFieldOperators_PL(1.0, B, 0.0, U, [2.0, 4.5], coeff, W, wcs, P)
+        print(mode, "near field", relerr(U, ref))
+        assert relerr(U, ref) < TOL
+        # points near the surface, single- and double-layer rows with c2 = 0 in one chunk
+        P2 = P / np.linalg.norm(P, axis=1, keepdims=True) * 1.02
+        coeff2 = np.array([0, 1.0 - 0.5j, 0.0, 0, 0, 0.3j, 1.0 + 0.2j, 0], np.complex128)
+        U0 = _rand(rng, 300, W)
+        ref2 = oracle.apply_near_field(m, 0.5 - 1j, B, 2.0, U0, [2.0, 4.5], coeff2, W, wcs, P2)
+        U2 = U0.copy()
+        H.ApplyNearFieldOperators_PL(0.5 - 1j, B, 2.0, U2, [2.0, 4.5], coeff2, W, wcs, P2)
+        print(mode, "near field at r = 1.02", relerr(U2, ref2))
+        assert relerr(U2, ref2) < TOL
+    finally:
+        H.SetPrecision("fp32")
 
 
 def test_herglotz_wave(mesh_h):
