@@ -104,16 +104,16 @@ void host_apply(Handle& h, const double* alpha, const double* B, int64_t ldB, in
     hipStream_t s = h.stream;
     h.ws_B.reserve(sizeof(cdouble) * brows * ncols_in);
     h.ws_out.reserve(sizeof(cdouble) * crows * ncols_out);
-    BAEMM_HIP(hipMemcpy2DAsync(h.ws_B.ptr, sizeof(cdouble) * ncols_in, B, sizeof(cdouble) * ldB,
+    BAEMM_HIP(copy_rows_async(h.ws_B.ptr, sizeof(cdouble) * ncols_in, B, sizeof(cdouble) * ldB,
                                sizeof(cdouble) * ncols_in, brows, hipMemcpyHostToDevice, s));
     const cdouble be = cplx(beta, 0.0);
     if (be != cdouble(0.0, 0.0))
-        BAEMM_HIP(hipMemcpy2DAsync(h.ws_out.ptr, sizeof(cdouble) * ncols_out, C, sizeof(cdouble) * ldC,
+        BAEMM_HIP(copy_rows_async(h.ws_out.ptr, sizeof(cdouble) * ncols_out, C, sizeof(cdouble) * ldC,
                                    sizeof(cdouble) * ncols_out, crows, hipMemcpyHostToDevice, s));
     ApplyArgs a{cplx(alpha, 1.0), be, h.ws_B.ptr, ncols_in, kC128, h.ws_out.ptr, ncols_out, kC128};
     const bool touched = dev(a, s);
     if (touched)
-        BAEMM_HIP(hipMemcpy2DAsync(C, sizeof(cdouble) * ldC, h.ws_out.ptr, sizeof(cdouble) * ncols_out,
+        BAEMM_HIP(copy_rows_async(C, sizeof(cdouble) * ldC, h.ws_out.ptr, sizeof(cdouble) * ncols_out,
                                    sizeof(cdouble) * ncols_out, crows, hipMemcpyDeviceToHost, s));
     BAEMM_HIP(hipStreamSynchronize(s));
 }
@@ -408,12 +408,12 @@ int baemm_create_incident_wave(baemm_handle h, const double* alpha, const double
         hh.ws_out.reserve(sizeof(cdouble) * hh.nv * W);
         const cdouble be = cplx(beta, 0.0);
         if (be != cdouble(0.0, 0.0))
-            BAEMM_HIP(hipMemcpy2DAsync(hh.ws_out.ptr, sizeof(cdouble) * W, C, sizeof(cdouble) * ldC,
+            BAEMM_HIP(copy_rows_async(hh.ws_out.ptr, sizeof(cdouble) * W, C, sizeof(cdouble) * ldC,
                                        sizeof(cdouble) * W, hh.nv, hipMemcpyHostToDevice, s));
         ApplyArgs a{cplx(alpha, 1.0), be, nullptr, 0, kC128, hh.ws_out.ptr, W, kC128};
         incident_dev(hh, a, dirs, inc_count, wave_type, s);
         if (p.any(1) || p.any(2))
-            BAEMM_HIP(hipMemcpy2DAsync(C, sizeof(cdouble) * ldC, hh.ws_out.ptr, sizeof(cdouble) * W,
+            BAEMM_HIP(copy_rows_async(C, sizeof(cdouble) * ldC, hh.ws_out.ptr, sizeof(cdouble) * W,
                                        sizeof(cdouble) * W, hh.nv, hipMemcpyDeviceToHost, s));
         BAEMM_HIP(hipStreamSynchronize(s));
         return BAEMM_OK;

@@ -175,6 +175,16 @@ void build_host_geometry(Handle& h);
 void upload_geometry(Handle& h);
 void free_device(Handle& h);
 
+// Strided host/device copy; a single linear copy when both sides are
+// contiguous (pitch = width), which pageable host memory moves far faster than
+// row by row.
+inline hipError_t copy_rows_async(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                                  size_t height, hipMemcpyKind kind, hipStream_t s)
+{
+    if (dpitch == width && spitch == width) return hipMemcpyAsync(dst, src, width * height, kind, s);
+    return hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, kind, s);
+}
+
 // ---- parameters ----
 void load_params(Handle& h, const double* kappa, const double* coeff, int64_t W, int64_t wcs, double factor);
 // Coefficients of chunks q0..q0+kg-1 times `scale` and, if rot is given,

@@ -517,11 +517,11 @@ int baemm_apply_mass_inverse(baemm_handle h, const double* X, int64_t ldX, doubl
         if (!X || !Y || nrhs < 1 || ldX < nrhs || ldY < nrhs) throw ArgError{"invalid mass-inverse arguments"};
         hipStream_t s = hh.stream;
         DevArray dx(hh, 0, sizeof(double) * hh.nv * nrhs), dy(hh, 1, sizeof(double) * hh.nv * nrhs);
-        BAEMM_HIP(hipMemcpy2DAsync(dx.p, sizeof(double) * nrhs, X, sizeof(double) * ldX, sizeof(double) * nrhs, hh.nv,
+        BAEMM_HIP(copy_rows_async(dx.p, sizeof(double) * nrhs, X, sizeof(double) * ldX, sizeof(double) * nrhs, hh.nv,
                                    hipMemcpyHostToDevice, s));
         dev_mass_cg(hh, hh.sws, dx.as<double>(), nrhs, dy.as<double>(), nrhs, (int)nrhs, cg_tol, 20, s);
         hh.stats.cg_iterations += dev_mass_cg_result(hh.sws, (int)nrhs, nullptr, s);
-        BAEMM_HIP(hipMemcpy2DAsync(Y, sizeof(double) * ldY, dy.p, sizeof(double) * nrhs, sizeof(double) * nrhs, hh.nv,
+        BAEMM_HIP(copy_rows_async(Y, sizeof(double) * ldY, dy.p, sizeof(double) * nrhs, sizeof(double) * nrhs, hh.nv,
                                    hipMemcpyDeviceToHost, s));
         BAEMM_HIP(hipStreamSynchronize(s));
         return BAEMM_OK;

@@ -232,6 +232,44 @@ def run_solve_leg(repeats=2, graphs=True):
     return out
 
 
+def run_potential_leg(repeats=3):
+    """Potential (near field) of the same SL+DL operator at 65,536 points on the
+    sphere of radius 1.5 (Fibonacci lattice) around the 81,920-triangle
+    icosphere, the same 64 vertex-value columns and kappa {8, 16}.  Kernel time
+    (HIP events around the pair kernels) and host-path wall time per apply in
+    each precision; the MFMA modes run the wave-specialised boundary kernel with
+    the points as targets."""
+    from baemm_amd import Helmholtz_HIP
+    from baemm_amd.meshes import fibonacci_sphere
+
+    V, T, W, wcs, kappa, coeff, B = workload()
+    P = 65536
+    pts = 1.5 * fibonacci_sphere(P)
+    out, ref = {}, None
+    with Helmholtz_HIP(V, T) as H:
+        H.LoadParameters(kappa, coeff, W, wcs)
+        for mode in ("fp32", "mixed", "fp32_mfma"):
+            log(f"potential leg {mode}")
+            H.SetPrecision(mode)
+            U = np.zeros((P, W), np.complex128)
+            H.ApplyNearFieldOperators_PL(1.0, B, 0.0, U, evaluation_points=pts)   # warm-up
+            H.SetTiming(True)
+            t0 = time.perf_counter()
+            for _ in range(repeats):
+                H.ApplyNearFieldOperators_PL(1.0, B, 0.0, U, evaluation_points=pts)
+            wall = (time.perf_counter() - t0) / repeats
+            kms, _ = H.GetTiming()
+            H.SetTiming(False)
+            if ref is None:
+                ref = U.copy()
+            out[mode] = {"kernel_ms": kms / repeats, "wall_ms": wall * 1e3,
+                         "pair_rhs_per_s_kernel": float(P) * T.shape[0] * W / (kms / repeats * 1e-3),
+                         "rel_diff_vs_fp32": float(np.linalg.norm(U - ref) / np.linalg.norm(ref))}
+    out["workload"] = ("potential at 65536 points (radius 1.5) of the SL+DL operator on the 81920-triangle "
+                       "icosphere, kappa {8,16} x 32 RHS")
+    return out
+
+
 def run_batch_leg(repeats=5, seed=3):
     """Batched uneven instances: 24 perturbed icospheres (radius 1 + 0.1 sum of
     real spherical harmonics of degree <= 4, seeded N(0,1)/(l+1)^2 weights) of
@@ -366,6 +404,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the torus solve leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--no-graphs", action="store_true", help="skip the hipGraph-replay solve of the solve leg")
+    ap.add_argument("--no-potential", action="store_true", help="skip the potential (near-field) leg")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0,
                     help="Poisson arrival rate of the serving leg (1/s); the default saturates the server")
@@ -456,6 +495,8 @@ def main():
     }
     if not args.no_batch:
         line["batch"] = run_batch_leg()
+    if not args.no_potential:
+        line["potential"] = run_potential_leg()
     if not args.no_solve:
         line["solve"] = dict(workload="sound-soft torus 38400 triangles, kappa 10, 32 plane waves, FarField, "
                                       "GMRES(30) 1e-6, 256 far-field directions", **run_solve_leg(graphs=not args.no_graphs))

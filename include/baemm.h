@@ -108,9 +108,9 @@ int baemm_get_wave_chunk_size(baemm_handle h);
  *                              accumulation (within 1e-6 of FP32),
  *   BAEMM_PRECISION_FP32_MFMA: on MFMA in FP32 (v_mfma_f32_32x32x2_f32) with
  *                              FP32 accumulation.
- * The MFMA modes apply to the boundary operators; the far field, potential
- * and Herglotz kernels stay FP32.  baemm_apply_boundary_batch takes FP32 and
- * MIXED instances only. */
+ * The MFMA modes apply to the boundary operators and the potential
+ * (baemm_apply_near_field); the far-field and Herglotz kernels stay FP32.
+ * baemm_apply_boundary_batch takes FP32 and MIXED instances only. */
 int baemm_set_precision(baemm_handle h, int precision);
 int baemm_get_precision(baemm_handle h);
 
