@@ -659,35 +659,37 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ tgt, i
         __syncthreads();
         for (int k = 0; k <= T; ++k) {
             if (k < T) issue_b(k);
-            if (F32 && k >= 1) {
-                // FP32 MFMA m (K = 2): lane half h supplies K element h, i.e. the
-                // component (m & 1) of source 4 h + (m >> 1), for A and for B
-                const int kb = (k - 1) & 1;
-                float a[KG][8], bv[KG][NCB][8];
+            if constexpr (F32) {
+                if (k >= 1) {
+                    // FP32 MFMA m (K = 2): lane half h supplies K element h, i.e. the
+                    // component (m & 1) of source 4 h + (m >> 1), for A and for B
+                    const int kb = (k - 1) & 1;
+                    float a[KG][8], bv[KG][NCB][8];
 #pragma unroll
-                for (int q = 0; q < KG; ++q) {
-                    const int base = ((kb * 4 + w) * KG + q) * 2;
-                    const uint4 u0 = s_a[base * 64 + lane], u1 = s_a[(base + 1) * 64 + lane];
-                    const uint32_t e[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+                    for (int q = 0; q < KG; ++q) {
+                        const int base = ((kb * 4 + w) * KG + q) * 2;
+                        const uint4 u0 = s_a[base * 64 + lane], u1 = s_a[(base + 1) * 64 + lane];
+                        const uint32_t e[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
 #pragma unroll
-                    for (int m = 0; m < 8; ++m) a[q][m] = __uint_as_float(e[m]);
+                        for (int m = 0; m < 8; ++m) a[q][m] = __uint_as_float(e[m]);
 #pragma unroll
-                    for (int cb = 0; cb < NCB; ++cb) {
-                        const int o = (q * NCB + cb) * 2;
-                        const uint4 v0 = s_op[(kb * OPS + o) * 64 + lane], v1 = s_op[(kb * OPS + o + 1) * 64 + lane];
-                        const uint32_t f[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                        for (int cb = 0; cb < NCB; ++cb) {
+                            const int o = (q * NCB + cb) * 2;
+                            const uint4 v0 = s_op[(kb * OPS + o) * 64 + lane], v1 = s_op[(kb * OPS + o + 1) * 64 + lane];
+                            const uint32_t f[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-                        for (int m = 0; m < 8; ++m) bv[q][cb][m] = __uint_as_float(f[m]);
+                            for (int m = 0; m < 8; ++m) bv[q][cb][m] = __uint_as_float(f[m]);
+                        }
                     }
+                    // m-major: consecutive MFMAs write different accumulators
+#pragma unroll
+                    for (int m = 0; m < 8; ++m)
+#pragma unroll
+                        for (int q = 0; q < KG; ++q)
+#pragma unroll
+                            for (int cb = 0; cb < NCB; ++cb)
+                                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q][m], bv[q][cb][m], acc[q][cb], 0, 0, 0);
                 }
-                // m-major: consecutive MFMAs write different accumulators
-#pragma unroll
-                for (int m = 0; m < 8; ++m)
-#pragma unroll
-                    for (int q = 0; q < KG; ++q)
-#pragma unroll
-                        for (int cb = 0; cb < NCB; ++cb)
-                            acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[q][m], bv[q][cb][m], acc[q][cb], 0, 0, 0);
             } else if (k >= 1) {
                 const int kb = (k - 1) & 1;
                 half8_t a_hi[KG], a_lo[KG];
