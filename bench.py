@@ -473,6 +473,11 @@ def main():
             "rel_diff_vs_fp32": float(np.linalg.norm(res["fp32_mfma"]["out"] - fp32["out"]) /
                                       np.linalg.norm(fp32["out"])),
             "arithmetic": "kernel values in f32, contraction on v_mfma_f32_32x32x2_f32 with f32 accumulation",
+            # the contraction (8 flops per pair and RHS column) against the dense FP32 matrix peak
+            "roofline": {"bound": "mfma", "unit": "TFLOP/s", "peak": 157.3,
+                         "achieved": float(nt) * nt * 8 * W / (res["fp32_mfma"]["kernel_ms"] * 1e-3) / 1e12,
+                         "frac": float(nt) * nt * 8 * W / (res["fp32_mfma"]["kernel_ms"] * 1e-3) / 1e12 / 157.3,
+                         "kernel": "bdr_mixed_ws_kernel<32, 2, false, true>"},
         },
         "roofline": {
             "bound": "mfma",
