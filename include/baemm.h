@@ -119,10 +119,12 @@ int baemm_get_precision(baemm_handle h);
 int baemm_set_segments(baemm_handle h, int segments);
 
 /* gmres_max_iter / gmres_max_restarts public members, MemberVariables.hpp:64-65. */
-/* No reference counterpart: the MFMA kernel variant of the mixed precision.
+/* No reference counterpart: the MFMA kernel variant of the mixed precision
+ * for the boundary operators.
  * 0 = one wave generates its kernel values and issues their MFMAs;
  * 1 = wave-specialised blocks (generation waves feed MFMA waves through LDS),
- * the default.  Both give identical results. */
+ * the default.  Both give identical results.  The FP32 MFMA precision and the
+ * potential always use the wave-specialised kernel. */
 int baemm_set_mixed_kernel(baemm_handle h, int variant);
 
 int baemm_set_gmres_limits(baemm_handle h, int max_iter, int max_restarts);
