@@ -127,6 +127,27 @@ def run_gpu(args):
     for _ in range(3):
         H.ApplyBoundaryOperators_PL(1.0, Bc, 0.0, Ch)
     results["host_path_ms"] = (time.perf_counter() - t0) * 1e3 / 3
+    # one wave number with 16 columns (the reference example's shape): the first
+    # 16 columns of the same vertex values, kappa 8, SL+DL row (1/2, -8i, 1, 0)
+    H.LoadBoundaryOperators_PL([8.0], np.array([0.5, -8j, 1.0, 0.0]), 16, 16)
+    small = {}
+    for mode in ("fp32", "mixed", "fp32_mfma"):
+        H.SetPrecision(mode)
+        step16 = lambda: _lib.check(L.baemm_apply_boundary_dev(h, one, dB, W, _lib.C128, zero, dC, W, _lib.C128,
+                                                               None), h)
+        step16()
+        _lib.check(L.baemm_dev_synchronize())
+        H.SetTiming(True)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            step16()
+        _lib.check(L.baemm_dev_synchronize())
+        dt = (time.perf_counter() - t0) / 5
+        kms, _ = H.GetTiming()
+        H.SetTiming(False)
+        small[mode] = {"ms_per_step": dt * 1e3, "kernel_ms": kms / 5,
+                       "pair_rhs_per_s": float(nt) * nt * 16 / dt}
+    results["w16"] = small
     # accuracy of the mixed apply relative to the FP32 apply (same inputs)
     dm = results["mixed"]["out"] - results["fp32"]["out"]
     rel_mixed_vs_fp32 = float(np.linalg.norm(dm) / np.linalg.norm(results["fp32"]["out"]))
@@ -463,6 +484,7 @@ def main():
         "fp32_ms_per_step": fp32["ms_per_step"],
         "speedup_mixed_vs_fp32": fp32["ms_per_step"] / mixed["ms_per_step"],
         "rel_diff_mixed_vs_fp32": rel,
+        "w16": dict(workload="same mesh, one wave number (kappa 8) x 16 RHS, SL+DL", **res["w16"]),
         "host_path": {"ms_per_apply": res["host_path_ms"], "value": pairs_rhs / (res["host_path_ms"] * 1e-3),
                       "note": "baemm_apply_boundary with host arrays: 42 MB in, 42 MB out over PCIe per apply"},
         "fp32_mfma": {
