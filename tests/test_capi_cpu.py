@@ -60,3 +60,17 @@ def test_create_argument_errors_carry_messages_without_gpu():
     st = L.baemm_create(V.ctypes.data, V.shape[0], T.ctypes.data, T.shape[0], None, -1, 0, ctypes.byref(h))
     assert st == _lib.ERR_ARG
     assert b"measurement" in L.baemm_last_error(None)
+
+
+def test_header_constants_match_binding():
+    """The status codes, element types, precision modes and wave types the
+    binding uses are the values include/baemm.h defines."""
+    import re
+    text = (ROOT / "include" / "baemm.h").read_text()
+    consts = {m.group(1): int(m.group(2)) for m in re.finditer(r"\b(BAEMM_[A-Z0-9_]+)\s*=\s*(-?\d+)", text)}
+    assert consts["BAEMM_PRECISION_FP32"] == _lib.PRECISION_FP32
+    assert consts["BAEMM_PRECISION_MIXED"] == _lib.PRECISION_MIXED
+    assert consts["BAEMM_PRECISION_FP32_MFMA"] == _lib.PRECISION_FP32_MFMA
+    assert consts["BAEMM_C128"] == _lib.C128 and consts["BAEMM_C64"] == _lib.C64
+    assert consts["BAEMM_OK"] == _lib.OK and consts["BAEMM_ERR_ARG"] == _lib.ERR_ARG
+    assert consts["BAEMM_ERR_NOCONV"] == _lib.ERR_NOCONV
