@@ -149,12 +149,22 @@ __global__ __launch_bounds__(256) void coldot_part_kernel(int n, int ncols, cons
 
 // Sum over b < nblocks of part[b * stride + e], by one wave: lane-strided
 // partial sums, then a butterfly (a fixed order, so results are reproducible).
+// The loads are issued eight at a time and then added in order, so the wave
+// waits for one memory round trip per 512 partials, not one per 64.
 // Every lane returns the sum.
 __device__ __forceinline__ double wave_colsum(const double* __restrict__ part, int nblocks, int64_t stride, int64_t e)
 {
     const int lane = threadIdx.x & 63;
     double s = 0.0;
-    for (int b = lane; b < nblocks; b += 64) s += part[(int64_t)b * stride + e];
+    int b = lane;
+    for (; b + 7 * 64 < nblocks; b += 8 * 64) {
+        double x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = part[(int64_t)(b + 64 * j) * stride + e];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s += x[j];
+    }
+    for (; b < nblocks; b += 64) s += part[(int64_t)b * stride + e];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     return s;
@@ -268,20 +278,6 @@ __global__ void cg_init_kernel(int ncols, const double* __restrict__ bb, int* __
         done[3] = 0;   // ticket and active count of cg_check_beta_kernel
         done[4] = 0;
     }
-}
-
-
-
-
-
-// p = z + beta p
-__global__ void cg_update_p_kernel(int64_t n, int ncols, const double* __restrict__ beta, const double* __restrict__ z,
-                                   double* __restrict__ p, const int* __restrict__ skip)
-{
-    if (skip && *skip) return;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * ncols) return;
-    p[t] = z[t] + beta[t % ncols] * p[t];
 }
 
 // dst (n x ncols, ld ncols) = src (ld lds); zero (n x ncols, ld ldz) = 0
@@ -405,9 +401,27 @@ void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, doub
 
 // ---------------------------------------------------------------- fused PCG steps
 // Row-blocked like coldot_part_kernel (block b: rows b*4+rl, stride grid*4;
-// 64 column lanes), deterministic two-pass reductions.
+// 64 column lanes), deterministic two-pass reductions.  Block partials are
+// stored column-major, part[c * nblocks + b], so the one-wave column sums read
+// contiguous memory.
 
-// q = M p and part[b][c] = sum over the block's rows of p q
+// Row entries of the mass matrix gathered per batch in the SpMV: the loads of a
+// batch are issued together, then the products are added in entry order.
+constexpr int kSpmvBatch = 8;
+
+// The block's partial of x.y for column c0 + c (lanes rl = 0..3 of the block
+// hold the sums of their rows) goes to part[(c0 + c) * gridDim.x + blockIdx.x].
+__device__ __forceinline__ void cg_store_partial(double (&sm)[4][64], double acc, int c0, int c, int rl, int ncols,
+                                                 double* __restrict__ part)
+{
+    sm[rl][c] = acc;
+    __syncthreads();
+    if (rl == 0 && c0 + c < ncols)
+        part[(int64_t)(c0 + c) * gridDim.x + blockIdx.x] = sm[0][c] + sm[1][c] + sm[2][c] + sm[3][c];
+    __syncthreads();
+}
+
+// First step: q = M p and the block partials of p.q
 __global__ __launch_bounds__(256) void cg_spmv_dot_kernel(int n, int ncols, const int32_t* __restrict__ ptr,
                                                           const int32_t* __restrict__ idx,
                                                           const double* __restrict__ val, const double* __restrict__ p,
@@ -421,20 +435,79 @@ __global__ __launch_bounds__(256) void cg_spmv_dot_kernel(int n, int ncols, cons
         double acc = 0.0;
         if (c0 + c < ncols)
             for (int r = blockIdx.x * 4 + rl; r < n; r += gridDim.x * 4) {
+                const int e0 = ptr[r], e1 = ptr[r + 1];
                 double sv = 0.0;
-                for (int e = ptr[r]; e < ptr[r + 1]; ++e) sv += val[e] * p[(int64_t)idx[e] * ncols + c0 + c];
+                for (int e = e0; e < e1; e += kSpmvBatch) {
+                    double mv[kSpmvBatch], xv[kSpmvBatch];
+#pragma unroll
+                    for (int j = 0; j < kSpmvBatch; ++j) {
+                        const int ej = min(e + j, e1 - 1);   // in the row: no branch between the loads
+                        mv[j] = val[ej];
+                        xv[j] = p[(int64_t)idx[ej] * ncols + c0 + c];
+                    }
+#pragma unroll
+                    for (int j = 0; j < kSpmvBatch; ++j)
+                        if (e + j < e1) sv += mv[j] * xv[j];
+                }
                 q[(int64_t)r * ncols + c0 + c] = sv;
                 acc += p[(int64_t)r * ncols + c0 + c] * sv;
             }
-        sm[rl][c] = acc;
-        __syncthreads();
-        if (rl == 0 && c0 + c < ncols)
-            part[(int64_t)blockIdx.x * ncols + c0 + c] = sm[0][c] + sm[1][c] + sm[2][c] + sm[3][c];
-        __syncthreads();
+        cg_store_partial(sm, acc, c0, c, rl, ncols, part);
     }
 }
 
-// pq = sum_b part[b]; alpha = active ? rz / pq : 0; one 64-thread block per column
+// Later steps: the new direction p = z + beta p_in with z = D r, formed where it
+// is needed, and q = M p.  Each block writes p_out for its own rows and forms
+// the entries of its neighbours' rows again (the same operations, so the same
+// values), so the direction update needs no pass of its own.  p_in is not
+// written in this launch (the directions are double-buffered).  Block partials
+// of p_out.q.
+__global__ __launch_bounds__(256) void cg_spmv_p_dot_kernel(int n, int ncols, const int32_t* __restrict__ ptr,
+                                                            const int32_t* __restrict__ idx,
+                                                            const double* __restrict__ val,
+                                                            const double* __restrict__ Dinv,
+                                                            const double* __restrict__ r,
+                                                            const double* __restrict__ beta,
+                                                            const double* __restrict__ p_in,
+                                                            double* __restrict__ p_out, double* __restrict__ q,
+                                                            double* __restrict__ part, const int* __restrict__ skip)
+{
+    if (*skip) return;
+    __shared__ double sm[4][64];
+    const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < ncols; c0 += 64) {
+        double acc = 0.0;
+        if (c0 + c < ncols) {
+            const double bt = beta[c0 + c];
+            for (int v = blockIdx.x * 4 + rl; v < n; v += gridDim.x * 4) {
+                const int e0 = ptr[v], e1 = ptr[v + 1];
+                double sv = 0.0;
+                for (int e = e0; e < e1; e += kSpmvBatch) {
+                    double mv[kSpmvBatch], xv[kSpmvBatch];
+#pragma unroll
+                    for (int j = 0; j < kSpmvBatch; ++j) {
+                        const int ej = min(e + j, e1 - 1);   // in the row: no branch between the loads
+                        const int u = idx[ej];
+                        const int64_t k = (int64_t)u * ncols + c0 + c;
+                        mv[j] = val[ej];
+                        xv[j] = fma(bt, p_in[k], Dinv[u] * r[k]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < kSpmvBatch; ++j)
+                        if (e + j < e1) sv += mv[j] * xv[j];
+                }
+                const int64_t k = (int64_t)v * ncols + c0 + c;
+                const double pv = fma(bt, p_in[k], Dinv[v] * r[k]);
+                p_out[k] = pv;
+                q[k] = sv;
+                acc += pv * sv;
+            }
+        }
+        cg_store_partial(sm, acc, c0, c, rl, ncols, part);
+    }
+}
+
+// pq = sum_b part[c][b]; alpha = active ? rz / pq : 0; one 64-thread block per column
 __global__ __launch_bounds__(64) void cg_alpha_sum_kernel(int nblocks, int ncols, const double* __restrict__ part,
                                                           const double* __restrict__ rz,
                                                           const int* __restrict__ active, double* __restrict__ alpha,
@@ -442,18 +515,19 @@ __global__ __launch_bounds__(64) void cg_alpha_sum_kernel(int nblocks, int ncols
 {
     if (*skip) return;
     const int c = blockIdx.x;
-    const double pq = wave_colsum(part, nblocks, ncols, c);
+    const double pq = wave_colsum(part, nblocks, 1, (int64_t)c * nblocks);
     if (threadIdx.x == 0) alpha[c] = (active[c] && pq != 0.0) ? rz[c] / pq : 0.0;
 }
 
-// Y += alpha p; r -= alpha q; z = D r; part1 = r.r, part2 = r.z (block partials)
+// Y += alpha p; r -= alpha q; block partials part1 = r.r and part2 = r.z with
+// z = D r (not stored: the next step forms it again)
 __global__ __launch_bounds__(256) void cg_update_dots_kernel(int n, int ncols, const double* __restrict__ alpha,
                                                              const double* __restrict__ p,
                                                              const double* __restrict__ q,
                                                              const double* __restrict__ Dinv, double* __restrict__ Y,
                                                              int64_t ldY, double* __restrict__ r,
-                                                             double* __restrict__ z, double* __restrict__ part1,
-                                                             double* __restrict__ part2, const int* __restrict__ skip)
+                                                             double* __restrict__ part1, double* __restrict__ part2,
+                                                             const int* __restrict__ skip)
 {
     if (*skip) return;
     __shared__ double sm1[4][64], sm2[4][64];
@@ -468,19 +542,12 @@ __global__ __launch_bounds__(256) void cg_update_dots_kernel(int n, int ncols, c
                 const double rv = r[k] - al * q[k];
                 r[k] = rv;
                 const double zv = Dinv[v] * rv;
-                z[k] = zv;
                 a1 += rv * rv;
                 a2 += rv * zv;
             }
         }
-        sm1[rl][c] = a1;
-        sm2[rl][c] = a2;
-        __syncthreads();
-        if (rl == 0 && c0 + c < ncols) {
-            part1[(int64_t)blockIdx.x * ncols + c0 + c] = sm1[0][c] + sm1[1][c] + sm1[2][c] + sm1[3][c];
-            part2[(int64_t)blockIdx.x * ncols + c0 + c] = sm2[0][c] + sm2[1][c] + sm2[2][c] + sm2[3][c];
-        }
-        __syncthreads();
+        cg_store_partial(sm1, a1, c0, c, rl, ncols, part1);
+        cg_store_partial(sm2, a2, c0, c, rl, ncols, part2);
     }
 }
 
@@ -497,8 +564,8 @@ __global__ __launch_bounds__(64) void cg_check_beta_kernel(int nblocks, int ncol
 {
     if (ctr[0]) return;
     const int c = blockIdx.x;
-    const double rr = wave_colsum(part1, nblocks, ncols, c);
-    const double rzn = wave_colsum(part2, nblocks, ncols, c);
+    const double rr = wave_colsum(part1, nblocks, 1, (int64_t)c * nblocks);
+    const double rzn = wave_colsum(part2, nblocks, 1, (int64_t)c * nblocks);
     if (threadIdx.x != 0) return;
     int still = 0;
     if (active[c]) {
@@ -719,10 +786,11 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     ws.red.reserve(sizeof(double) * 2 * std::max(nred, kRedBlocks) * ncols);
     ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
     int* ctr = dev_cg_counters(ws, s);
+    // the search directions are double-buffered: step it uses pa (even it) or pb
     double* r = ws.cg.as<double>();
-    double* z = r + N;
-    double* p = z + N;
-    double* q = p + N;
+    double* pa = r + N;
+    double* pb = pa + N;
+    double* q = pb + N;
     double* rz = ws.scal.as<double>();
     double* rz_new = rz + ncols;
     double* pq = rz_new + ncols;
@@ -737,23 +805,28 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     hipLaunchKernelGGL(cg_start_kernel, grid_for(N), dim3(256), 0, s, n, ncols, X, ldX, r, Y, ldY);
     dev_coldot(ws, n, ncols, r, ncols, r, ncols, bb, s);
     hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(256), 0, s, ncols, bb, active, done, iters);
-    dev_lumped(h, r, ncols, p, ncols, ncols, s, done);      // p = z = D r
-    dev_coldot(ws, n, ncols, r, ncols, p, ncols, rz, s, done);
-    // five launches per iteration: q = M p with p.q partials; alpha; updates of
-    // Y, r, z with r.r and r.z partials; convergence and beta; p = z + beta p
+    dev_lumped(h, r, ncols, pa, ncols, ncols, s, done);      // p = z = D r
+    dev_coldot(ws, n, ncols, r, ncols, pa, ncols, rz, s, done);
+    // four launches per iteration: p = z + beta p (after the first) and q = M p
+    // with p.q partials; alpha; updates of Y and r with r.r and r.z partials;
+    // convergence and beta
     double* part1 = ws.red.as<double>();
     double* part2 = part1 + (int64_t)nred * ncols;
     for (int it = 0; it < max_iter; ++it) {
-        hipLaunchKernelGGL(cg_spmv_dot_kernel, dim3(nred), dim3(256), 0, s, (int)n, ncols, h.d_M_ptr, h.d_M_idx,
-                           h.d_M_val, p, q, part1, done);
+        double* p = (it & 1) ? pb : pa;
+        if (it == 0)
+            hipLaunchKernelGGL(cg_spmv_dot_kernel, dim3(nred), dim3(256), 0, s, (int)n, ncols, h.d_M_ptr, h.d_M_idx,
+                               h.d_M_val, p, q, part1, done);
+        else
+            hipLaunchKernelGGL(cg_spmv_p_dot_kernel, dim3(nred), dim3(256), 0, s, (int)n, ncols, h.d_M_ptr,
+                               h.d_M_idx, h.d_M_val, h.d_lumped_inv, r, beta, (it & 1) ? pa : pb, p, q, part1,
+                               done);
         hipLaunchKernelGGL(cg_alpha_sum_kernel, dim3(ncols), dim3(64), 0, s, nred, ncols, part1, rz, active, alpha,
                            done);
         hipLaunchKernelGGL(cg_update_dots_kernel, dim3(nred), dim3(256), 0, s, (int)n, ncols, alpha, p, q,
-                           h.d_lumped_inv, Y, ldY, r, z, part1, part2, done);
+                           h.d_lumped_inv, Y, ldY, r, part1, part2, done);
         hipLaunchKernelGGL(cg_check_beta_kernel, dim3(ncols), dim3(64), 0, s, nred, ncols, part1, part2, bb,
                            tol * tol, max_iter, active, rz, beta, ctr);
-        if (it + 1 == max_iter) break;
-        hipLaunchKernelGGL(cg_update_p_kernel, grid_for(N), dim3(256), 0, s, n, ncols, beta, z, p, done);
         BAEMM_HIP(hipGetLastError());
     }
     BAEMM_HIP(hipGetLastError());
