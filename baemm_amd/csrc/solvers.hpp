@@ -10,7 +10,15 @@
 namespace baemm {
 
 constexpr int kRedBlocks = 240;   // blocks of the first pass of column reductions
-constexpr int kPcgBlockMaxRows = 16384;   // meshes up to this many vertices run the one-launch block PCG
+// Meshes up to this many vertices run the one-launch block PCG, larger ones the
+// multi-block PCG (a build-time setting: -DBAEMM_PCG_BLOCK_MAX_ROWS=n).  The
+// block PCG runs one work-group per group of up to four columns, so it uses few
+// CUs; FarField solves measured on MI355X put the crossover between 640 and
+// 1,200 vertices.
+#ifndef BAEMM_PCG_BLOCK_MAX_ROWS
+#define BAEMM_PCG_BLOCK_MAX_ROWS 1024
+#endif
+constexpr int kPcgBlockMaxRows = BAEMM_PCG_BLOCK_MAX_ROWS;
 
 void dev_mass(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
               hipStream_t s, const int* skip = nullptr);

@@ -124,7 +124,7 @@ def test_mass_inverse(mesh_h):
 
 
 def test_mass_inverse_multiblock():
-    """Meshes above 16,384 vertices take the multi-block PCG (direction update
+    """Meshes above 1,024 vertices take the multi-block PCG (direction update
     fused into the SpMV, column-major block partials).  The torus has 19,200
     vertices; 80 real columns (40 complex) cover two 64-column chunks.  The
     reference is the exact sparse solve with the device's own mass matrix."""

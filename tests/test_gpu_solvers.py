@@ -159,12 +159,21 @@ def test_cpp_dropin_example_matches_python(tmp_path):
 def test_graph_captured_arnoldi_steps_match_eager(small):
     """GMRES with each Arnoldi step replayed from a captured hipGraph gives the
     eager result exactly, on every replay (eager work and synchronisation in
-    between), in both precisions."""
+    between), in every precision.  The small sphere (162 vertices) runs the
+    one-launch block PCG, the icosphere with 2,562 vertices the multi-block PCG
+    (device-side convergence flag, launches that return at once)."""
     m, H = small
+    _graph_vs_eager(H, m.nv)
+    V, T = icosphere(4)
+    with Helmholtz_HIP(V, T) as H4:
+        _graph_vs_eager(H4, V.shape[0])
+
+
+def _graph_vs_eager(H, nv):
     rng = np.random.default_rng(9)
     W, wcs = 6, 3
     kappa = [2.0, 3.0]
-    wave = rng.standard_normal((m.nv, W)) + 1j * rng.standard_normal((m.nv, W))
+    wave = rng.standard_normal((nv, W)) + 1j * rng.standard_normal((nv, W))
     for mode in ("fp32", "mixed", "fp32_mfma"):
         H.SetPrecision(mode)
         H.SetGraphs(False)
