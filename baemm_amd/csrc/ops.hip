@@ -259,6 +259,9 @@ __global__ void rmin_kernel(const float4* __restrict__ tgt, int ntgt, const floa
 
 // --------------------------------------------------------------- epilogues
 
+// Batch sizes of the epilogue's loads: incident triangles and segment partials.
+constexpr int EPI_TRI = 2;
+constexpr int EPI_SEG = 8;
 
 // C[v][k] = alpha * ( sum_{t ni v} a_t/3 (sum_s part_s[t][k] * unscale + D_t b[t][k])
 //                     + c0_q (M B)[v][k] ) + beta * C[v][k]
@@ -278,43 +281,83 @@ __global__ __launch_bounds__(256) void bdr_epilogue_kernel(EpiArgs a)
         double sr = 0.0, si = 0.0;
         if (a.kernel_on) {
             const double us = a.unscale ? (double)a.unscale[pc] : 1.0;
-            const int e1 = a.vt_ptr[v + 1];
-            for (int e = a.vt_ptr[v]; e < e1; ++e) {
-                const int t = a.vt_idx[e];
-                double pr = 0.0, pi = 0.0;
-                const float* pp = a.part + (int64_t)t * a.ldp + q * 2 * a.WCSP + kk;
-                for (int s = 0; s < a.S; ++s) {
-                    pr += (double)pp[(int64_t)s * a.seg_stride];
-                    pi += (double)pp[(int64_t)s * a.seg_stride + a.WCSP];
+            const int e0 = a.vt_ptr[v], e1 = a.vt_ptr[v + 1];
+            // EPI_TRI triangles and EPI_SEG segments per batch: every load of a
+            // batch is issued before the first is added (indices clamped to the
+            // valid range, so there is no branch between them), and the sums run
+            // in the order of the plain loops (s ascending per triangle, then the
+            // triangles in list order)
+            for (int e = e0; e < e1; e += EPI_TRI) {
+                int t[EPI_TRI];
+                double pr[EPI_TRI], pi[EPI_TRI];
+#pragma unroll
+                for (int u = 0; u < EPI_TRI; ++u) {
+                    t[u] = a.vt_idx[min(e + u, e1 - 1)];
+                    pr[u] = pi[u] = 0.0;
                 }
-                pr *= us;
-                pi *= us;
-                {
-                    const double rr = a.rotr[q], ri = a.roti[q];
-                    const double tr = pr * rr - pi * ri;
-                    pi = pr * ri + pi * rr;
-                    pr = tr;
+                for (int s0 = 0; s0 < a.S; s0 += EPI_SEG) {
+                    float xr[EPI_TRI][EPI_SEG], xi[EPI_TRI][EPI_SEG];
+#pragma unroll
+                    for (int u = 0; u < EPI_TRI; ++u) {
+                        const float* pp = a.part + (int64_t)t[u] * a.ldp + q * 2 * a.WCSP + kk;
+#pragma unroll
+                        for (int j = 0; j < EPI_SEG; ++j) {
+                            const int64_t o = (int64_t)min(s0 + j, a.S - 1) * a.seg_stride;
+                            xr[u][j] = pp[o];
+                            xi[u][j] = pp[o + a.WCSP];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < EPI_TRI; ++u)
+#pragma unroll
+                        for (int j = 0; j < EPI_SEG; ++j)
+                            if (s0 + j < a.S) {
+                                pr[u] += (double)xr[u][j];
+                                pi[u] += (double)xi[u][j];
+                            }
                 }
-                if (a.diag_on) {
-                    const float2 bv = a.b[(int64_t)t * a.ldb + pc];
-                    const double fr = (double)a.c1r[q] * a.sdiag[t] - (double)a.c1i[q] * a.kap[q];
-                    const double fi = (double)a.c1i[q] * a.sdiag[t] + (double)a.c1r[q] * a.kap[q];
-                    pr += fr * bv.x - fi * bv.y;
-                    pi += fr * bv.y + fi * bv.x;
+#pragma unroll
+                for (int u = 0; u < EPI_TRI; ++u) {
+                    if (e + u >= e1) break;
+                    double qr = pr[u] * us, qi = pi[u] * us;
+                    {
+                        const double rr = a.rotr[q], ri = a.roti[q];
+                        const double tr = qr * rr - qi * ri;
+                        qi = qr * ri + qi * rr;
+                        qr = tr;
+                    }
+                    if (a.diag_on) {
+                        const float2 bv = a.b[(int64_t)t[u] * a.ldb + pc];
+                        const double fr = (double)a.c1r[q] * a.sdiag[t[u]] - (double)a.c1i[q] * a.kap[q];
+                        const double fi = (double)a.c1i[q] * a.sdiag[t[u]] + (double)a.c1r[q] * a.kap[q];
+                        qr += fr * bv.x - fi * bv.y;
+                        qi += fr * bv.y + fi * bv.x;
+                    }
+                    const double w = (double)a.area[t[u]] / 3.0;
+                    sr += w * qr;
+                    si += w * qi;
                 }
-                const double w = (double)a.area[t] / 3.0;
-                sr += w * pr;
-                si += w * pi;
             }
         }
         if (a.mass_on) {
             const TB* B = static_cast<const TB*>(a.B);
             double mr = 0.0, mi = 0.0;
-            const int e1 = a.M_ptr[v + 1];
-            for (int e = a.M_ptr[v]; e < e1; ++e) {
-                const double2 bv = to_d2(B[(int64_t)a.M_idx[e] * a.ldB + k]);
-                mr += a.M_val[e] * bv.x;
-                mi += a.M_val[e] * bv.y;
+            const int e0 = a.M_ptr[v], e1 = a.M_ptr[v + 1];
+            for (int e = e0; e < e1; e += EPI_SEG) {
+                double mv[EPI_SEG];
+                double2 bv[EPI_SEG];
+#pragma unroll
+                for (int j = 0; j < EPI_SEG; ++j) {
+                    const int ej = min(e + j, e1 - 1);
+                    mv[j] = a.M_val[ej];
+                    bv[j] = to_d2(B[(int64_t)a.M_idx[ej] * a.ldB + k]);
+                }
+#pragma unroll
+                for (int j = 0; j < EPI_SEG; ++j)
+                    if (e + j < e1) {
+                        mr += mv[j] * bv[j].x;
+                        mi += mv[j] * bv[j].y;
+                    }
             }
             sr += a.c0r[q] * mr - a.c0i[q] * mi;
             si += a.c0r[q] * mi + a.c0i[q] * mr;
@@ -351,9 +394,20 @@ __global__ void rows_epilogue_kernel(int nrows, int W, int wcs, int WCSP, int S,
     const int q = k / wcs, kk = k % wcs;
     const float* pp = part + (int64_t)r * ldp + q * 2 * WCSP + kk;
     double pr = 0.0, pi = 0.0;
-    for (int s = 0; s < S; ++s) {
-        pr += (double)pp[(int64_t)s * seg_stride];
-        pi += (double)pp[(int64_t)s * seg_stride + WCSP];
+    for (int s0 = 0; s0 < S; s0 += EPI_SEG) {
+        float xr[EPI_SEG], xi[EPI_SEG];
+#pragma unroll
+        for (int j = 0; j < EPI_SEG; ++j) {
+            const int64_t o = (int64_t)min(s0 + j, S - 1) * seg_stride;
+            xr[j] = pp[o];
+            xi[j] = pp[o + WCSP];
+        }
+#pragma unroll
+        for (int j = 0; j < EPI_SEG; ++j)
+            if (s0 + j < S) {
+                pr += (double)xr[j];
+                pi += (double)xi[j];
+            }
     }
     if (unscale) {
         const double us = (double)unscale[q * WCSP + kk];

@@ -182,33 +182,50 @@ __global__ __launch_bounds__(64) void colsum_kernel(int nblocks, int ncols, cons
 
 // Complex Gram-Schmidt dots: part[b][i][k] = sum_rows conj(V_i[r][k]) w[r][k], i < nb.
 // V: basis vectors with stride vstride (elements) between vectors, ld W.
+// GS_BATCH basis vectors per pass over the rows: each row's w and the batch's
+// V_i are loaded together before they are used (w once per batch instead of
+// once per vector); every dot still sums its rows in the same order.
+constexpr int GS_BATCH = 8;
+
 __global__ __launch_bounds__(256) void gs_dots_part_kernel(int n, int W, int nb, const double2* __restrict__ V,
                                                            int64_t vstride, const double2* __restrict__ w,
                                                            double2* __restrict__ part)
 {
     __shared__ double2 sm[4][64];
     const int k = threadIdx.x & 63, rl = threadIdx.x >> 6;
-    for (int i = 0; i < nb; ++i) {
+    for (int i0 = 0; i0 < nb; i0 += GS_BATCH) {
         for (int k0 = 0; k0 < W; k0 += 64) {
-            double sr = 0.0, si = 0.0;
+            double sr[GS_BATCH], si[GS_BATCH];
+#pragma unroll
+            for (int j = 0; j < GS_BATCH; ++j) sr[j] = si[j] = 0.0;
             if (k0 + k < W)
                 for (int r = blockIdx.x * 4 + rl; r < n; r += gridDim.x * 4) {
-                    const double2 a = V[(int64_t)i * vstride + (int64_t)r * W + k0 + k];
                     const double2 b = w[(int64_t)r * W + k0 + k];
-                    sr += a.x * b.x + a.y * b.y;
-                    si += a.x * b.y - a.y * b.x;
+                    double2 a[GS_BATCH];
+#pragma unroll
+                    for (int j = 0; j < GS_BATCH; ++j)   // vectors past nb read the last one (not used)
+                        a[j] = V[(int64_t)min(i0 + j, nb - 1) * vstride + (int64_t)r * W + k0 + k];
+#pragma unroll
+                    for (int j = 0; j < GS_BATCH; ++j) {
+                        sr[j] += a[j].x * b.x + a[j].y * b.y;
+                        si[j] += a[j].x * b.y - a[j].y * b.x;
+                    }
                 }
-            sm[rl][k] = make_double2(sr, si);
-            __syncthreads();
-            if (rl == 0 && k0 + k < W) {
-                double2 s = sm[0][k];
-                for (int q = 1; q < 4; ++q) {
-                    s.x += sm[q][k].x;
-                    s.y += sm[q][k].y;
+#pragma unroll
+            for (int j = 0; j < GS_BATCH; ++j) {
+                if (i0 + j >= nb) break;
+                sm[rl][k] = make_double2(sr[j], si[j]);
+                __syncthreads();
+                if (rl == 0 && k0 + k < W) {
+                    double2 s = sm[0][k];
+                    for (int q = 1; q < 4; ++q) {
+                        s.x += sm[q][k].x;
+                        s.y += sm[q][k].y;
+                    }
+                    part[((int64_t)blockIdx.x * nb + i0 + j) * W + k0 + k] = s;
                 }
-                part[((int64_t)blockIdx.x * nb + i) * W + k0 + k] = s;
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
 }
@@ -232,11 +249,20 @@ __global__ void gs_update_kernel(int n, int W, int nb, const double2* __restrict
     if (t >= (int64_t)n * W) return;
     const int k = (int)(t % W);
     double sr = 0.0, si = 0.0;
-    for (int i = 0; i < nb; ++i) {
-        const double2 a = V[(int64_t)i * vstride + t];
-        const double2 c = h[i * W + k];
-        sr += c.x * a.x - c.y * a.y;
-        si += c.x * a.y + c.y * a.x;
+    for (int i0 = 0; i0 < nb; i0 += GS_BATCH) {
+        double2 a[GS_BATCH], c[GS_BATCH];
+#pragma unroll
+        for (int j = 0; j < GS_BATCH; ++j) {
+            const int i = min(i0 + j, nb - 1);
+            a[j] = V[(int64_t)i * vstride + t];
+            c[j] = h[i * W + k];
+        }
+#pragma unroll
+        for (int j = 0; j < GS_BATCH; ++j)
+            if (i0 + j < nb) {
+                sr += c[j].x * a[j].x - c[j].y * a[j].y;
+                si += c[j].x * a[j].y + c[j].y * a[j].x;
+            }
     }
     double2 o = w[t];
     o.x += sign * sr;
