@@ -3,6 +3,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "ops.hpp"
 
@@ -571,6 +574,48 @@ static int choose_segments(const Handle& h, int tiles, int max_seg, int64_t nsrc
     return std::max(1, std::min(std::max(want, min_seg), max_seg));
 }
 
+// Work-groups of kernel fn (bs threads each) resident on the current device at
+// once: the occupancy per CU times the CU count, cached per device and kernel.
+static int resident_blocks(const void* fn, int bs)
+{
+    static std::mutex mtx;
+    static std::map<std::pair<int, const void*>, int> cache;
+    int dev = 0;
+    BAEMM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mtx);
+    auto it = cache.find({dev, fn});
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    BAEMM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bs, 0));
+    BAEMM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int r = std::max(1, per_cu * cus);
+    cache[{dev, fn}] = r;
+    return r;
+}
+
+// Automatic segment count of an MFMA launch of tiles x S work-groups over nst
+// source steps: starting from S0, the smallest count up to 2 S0 whose grid
+// leaves the last round of resident work-groups at least 90 % full (a nearly
+// empty last round costs a whole round of kernel time); otherwise the count
+// with the fullest last round.  Returns the segment count to split nst by.
+static int balance_segments(int tiles, int nst, int S0, int max_seg, int resident)
+{
+    int best = S0;
+    double best_fill = -1.0;
+    for (int c = S0; c <= std::min(2 * S0, std::max(S0, max_seg)); ++c) {
+        const int steps = (nst + c - 1) / c;
+        const int S = (nst + steps - 1) / steps;
+        const double rounds = (double)tiles * S / resident;
+        const double fill = rounds / std::ceil(rounds);
+        if (fill >= 0.9) return c;
+        if (fill > best_fill) {
+            best_fill = fill;
+            best = c;
+        }
+    }
+    return best;
+}
+
 // Runs a plain pair kernel over all chunk groups: part has ntgt_pad rows.
 static void run_plain(Handle& h, int op, bool adj, int WCSP, const float4* tx, const float4* tn, int ntgt,
                       int ntgt_pad, const float4* sy, const float4* sn, int nsrc, int nsrc_pad, const float2* b,
@@ -835,13 +880,19 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
     }
     const int nst = pl.nst;
     const int tiles = (int)(h.nt_pad / kMixTile);
-    const int S0 = choose_segments(h, tiles, nst / 8, h.nt);
+    // the FP32 MFMA contraction exists in the wave-specialised kernel only
+    const bool ws = h.mixed_kernel == 1 || pl.f32;
+    int S0 = choose_segments(h, tiles, nst / 8, h.nt);
+    if (h.segments == 0) {
+        const int kg0 = ws ? std::min(pick_kg(pl.WCSP, (int)p.K, true), 2) : pick_kg(pl.WCSP, (int)p.K, true);
+        const void* fn0 = ws ? (const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32)
+                             : (const void*)mixed_fn(pl.adj, pl.WCSP, kg0);
+        S0 = balance_segments(tiles, nst, S0, nst / 8, resident_blocks(fn0, ws ? 512 : 256));
+    }
     const int steps = (nst + S0 - 1) / S0;
     const int S = (nst + steps - 1) / steps;
     float* part = bdr_reserve_part(h, pl, S);
     Timed tm(h, s);
-    // the FP32 MFMA contraction exists in the wave-specialised kernel only
-    const bool ws = h.mixed_kernel == 1 || pl.f32;
     for (int q0 = 0; q0 < p.K;) {
         int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
         if (ws) kg = std::min(kg, 2);
@@ -951,7 +1002,12 @@ static void apply_near_mfma(Handle& h, const ApplyArgs& a, const float4* d_pts, 
     const int P_pad = (P + kMixTile - 1) / kMixTile * kMixTile;
     const int nst = pl.nst;
     const int tiles = P_pad / kMixTile;
-    const int S0 = choose_segments(h, tiles, nst / 8, h.nt);
+    int S0 = choose_segments(h, tiles, nst / 8, h.nt);
+    if (h.segments == 0) {
+        const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
+        S0 = balance_segments(tiles, nst, S0, nst / 8,
+                              resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512));
+    }
     const int steps = (nst + S0 - 1) / S0;
     const int S = (nst + steps - 1) / steps;
     const int64_t seg_stride = (int64_t)P_pad * pl.ldp;
