@@ -593,11 +593,12 @@ static int resident_blocks(const void* fn, int bs)
     return r;
 }
 
-// Automatic segment count of an MFMA launch of tiles x S work-groups over nst
-// source steps: starting from S0, the smallest count up to 2 S0 whose grid
-// leaves the last round of resident work-groups at least 90 % full (a nearly
-// empty last round costs a whole round of kernel time); otherwise the count
-// with the fullest last round.  Returns the segment count to split nst by.
+// Automatic segment count of a launch of tiles x S work-groups whose segments
+// split nst source units (8-source steps or 64-source blocks): among the
+// counts from S0 to 2 S0 (within max_seg), the one whose grid fills the last
+// round of resident work-groups best (a part-empty last round costs a whole
+// round of kernel time); the first count that fills it to 99.5 %, else the
+// fullest (the smallest on ties).  Returns the count to split nst by.
 static int balance_segments(int tiles, int nst, int S0, int max_seg, int resident)
 {
     int best = S0;
@@ -607,8 +608,8 @@ static int balance_segments(int tiles, int nst, int S0, int max_seg, int residen
         const int S = (nst + steps - 1) / steps;
         const double rounds = (double)tiles * S / resident;
         const double fill = rounds / std::ceil(rounds);
-        if (fill >= 0.9) return c;
-        if (fill > best_fill) {
+        if (fill >= 0.995) return c;
+        if (fill > best_fill + 1e-9) {
             best_fill = fill;
             best = c;
         }
@@ -870,7 +871,12 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
     if (!pl.mixed) {
         const int tiles = (int)(h.nt_pad / kTile);
         const int max_seg = (int)(h.nt_pad / kSrcBlock);
-        const int S0 = choose_segments(h, tiles, max_seg, h.nt);
+        int S0 = choose_segments(h, tiles, max_seg, h.nt);
+        if (h.segments == 0)
+            S0 = balance_segments(tiles, max_seg, S0, max_seg,
+                                  resident_blocks((const void*)plain_fn(OP_BDR, pl.adj, pl.WCSP,
+                                                                        pick_kg(pl.WCSP, (int)p.K, false)),
+                                                  kTile));
         const int seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
         const int S = (int)((h.nt_pad + seg_len - 1) / seg_len);
         float* part = bdr_reserve_part(h, pl, S);

@@ -82,7 +82,9 @@ size_t estimate_bytes(int64_t nv, int64_t nt, int64_t m, int64_t K, int64_t wcs,
     size_t b = (size_t)nt_pad * ldb * 8;
     size_t bop = mfma ? 2 * b : 0;
     const int64_t tiles = std::max<int64_t>(1, nt_pad / (mfma ? 128 : 256));
-    const int64_t S = std::max<int64_t>(1, std::min<int64_t>((2048 + tiles - 1) / tiles, nt_pad / 64));
+    // about 2,048 work-groups, grown up to twice that count to fill the last
+    // round of resident work-groups
+    const int64_t S = 2 * std::max<int64_t>(1, std::min<int64_t>((2048 + tiles - 1) / tiles, nt_pad / 64));
     size_t part = (size_t)S * nt_pad * 2 * ldb * 4;
     const size_t vec = (size_t)nv * W * 16;
     size_t gm = (size_t)(restart + 1) * vec + 9 * vec + (size_t)nv * 2 * W * 8 * 4;
