@@ -967,7 +967,12 @@ static void apply_rows_op(Handle& h, int op, const ApplyArgs& a, const float4* t
     float2* b = h.ws_b32.as<float2>();
     launch_av(h, a.B, a.ldB, a.Btype, WCSP, b, ldb, s);
     const int tiles = ntgt_pad / kTile;
-    const int S0 = choose_segments(h, tiles, (int)(h.nt_pad / kSrcBlock), h.nt);
+    const int nsb = (int)(h.nt_pad / kSrcBlock);
+    int S0 = choose_segments(h, tiles, nsb, h.nt);
+    if (h.segments == 0)
+        S0 = balance_segments(tiles, nsb, S0, nsb,
+                              resident_blocks((const void*)plain_fn(op, false, WCSP, pick_kg(WCSP, (int)p.K, false)),
+                                              kTile));
     const int seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
     const int S = (int)((h.nt_pad + seg_len - 1) / seg_len);
     const int64_t seg_stride = (int64_t)ntgt_pad * ldp;
@@ -1066,7 +1071,13 @@ void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
                            WCSP, ldb, 1, gb);
     BAEMM_HIP(hipGetLastError());
     const int tiles = (int)(h.nt_pad / kTile);
-    const int S0 = choose_segments(h, tiles, (int)(h.m_pad / kSrcBlock), h.m);
+    const int nsb = (int)(h.m_pad / kSrcBlock);
+    int S0 = choose_segments(h, tiles, nsb, h.m);
+    if (h.segments == 0)
+        S0 = balance_segments(tiles, nsb, S0, nsb,
+                              resident_blocks((const void*)plain_fn(OP_HERG, false, WCSP,
+                                                                    pick_kg(WCSP, (int)p.K, false)),
+                                              kTile));
     const int seg_len = (int)(((h.m_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
     const int S = (int)((h.m_pad + seg_len - 1) / seg_len);
     const int64_t seg_stride = h.nt_pad * (int64_t)ldp;
