@@ -269,16 +269,17 @@ constexpr int EPI_SEG = 8;
 // C[v][k] = alpha * ( sum_{t ni v} a_t/3 (sum_s part_s[t][k] * unscale + D_t b[t][k])
 //                     + c0_q (M B)[v][k] ) + beta * C[v][k]
 // (AvOpTransp.Dot ApplyBoundaryOperators.hpp:136-140, ApplySingleLayerDiagonal.hpp:36-50,
-//  MassOp.Dot :150-175).  One 64-lane wave per vertex, lanes over columns.
+//  MassOp.Dot :150-175).  lpv lanes (16, 32 or 64) per vertex, lanes over
+// columns, so narrow right-hand sides leave no lanes idle.
 template <int BT, int CT>
-__global__ __launch_bounds__(256) void bdr_epilogue_kernel(EpiArgs a)
+__global__ __launch_bounds__(256) void bdr_epilogue_kernel(EpiArgs a, int lpv)
 {
     using TB = typename Elem<BT>::type;
     using TC = typename Elem<CT>::type;
-    const int lane = threadIdx.x & 63;
-    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (lpv - 1);
+    const int v = blockIdx.x * (256 / lpv) + threadIdx.x / lpv;
     if (v >= a.nv) return;
-    for (int k = lane; k < a.W; k += 64) {
+    for (int k = lane; k < a.W; k += lpv) {
         const int q = k / a.wcs, kk = k % a.wcs;
         const int pc = q * a.WCSP + kk;
         double sr = 0.0, si = 0.0;
@@ -664,7 +665,9 @@ float compute_rmin(Handle& h, hipStream_t s)
 template <int BT, int CT>
 static void launch_bdr_epilogue(EpiArgs& ea, hipStream_t s)
 {
-    hipLaunchKernelGGL((bdr_epilogue_kernel<BT, CT>), dim3((ea.nv + 3) / 4), dim3(256), 0, s, ea);
+    const int lpv = ea.W <= 16 ? 16 : (ea.W <= 32 ? 32 : 64);
+    const int vpb = 256 / lpv;
+    hipLaunchKernelGGL((bdr_epilogue_kernel<BT, CT>), dim3((ea.nv + vpb - 1) / vpb), dim3(256), 0, s, ea, lpv);
 }
 
 static void bdr_epilogue(Handle& h, EpiArgs& ea, int Btype, int Ctype, hipStream_t s)
