@@ -17,6 +17,7 @@ OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOCONV, ERR_INTERNAL = range(6)
 C128, C64 = 0, 1
 PRECISION_FP32, PRECISION_MIXED, PRECISION_FP32_MFMA = 0, 1, 2
 WAVE_PLANE, WAVE_RADIAL = 201, 202
+SERVE_FIFO, SERVE_SHORTEST_FIRST = 0, 1
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -67,6 +68,7 @@ SIGNATURES = {
     "baemm_batch_get_stats": (_INT, [ctypes.POINTER(_I64)] * 4),
     "baemm_batch_autotune": (_INT, [_INT, _P, _P, _P, _INT, _P, _P, _INT, _INT, ctypes.POINTER(_D)]),
     "baemm_server_create": (_INT, [_INT, _INT, _I64, ctypes.POINTER(_P)]),
+    "baemm_server_set_policy": (_INT, [_P, _INT, _INT]),
     "baemm_server_destroy": (_INT, [_P]),
     "baemm_server_submit_gn": (_INT, [_P, _P, _P, _P, ctypes.POINTER(_I64)]),
     "baemm_server_wait": (_INT, [_P, _I64, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),

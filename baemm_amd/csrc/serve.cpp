@@ -4,11 +4,13 @@
 // the reference runs one solve at a time from the caller's thread.
 //
 // Design:
-//  * `slots` worker threads.  Each takes the oldest queued request whose
-//    device-memory need fits into the budget next to what is resident, so a
-//    request that arrives while others run starts as soon as a slot and
-//    memory are free.  Each request runs on its own handle and HIP stream, so
-//    the kernels of the requests in flight overlap on the device.
+//  * `slots` worker threads.  Each takes the next queued request whose
+//    device-memory need fits into the budget next to what is resident (the
+//    oldest, or with the shortest-first policy the one of least estimated
+//    work, with aging), so a request that arrives while others run starts as
+//    soon as a slot and memory are free.  Each request runs on its own handle
+//    and HIP stream, so the kernels of the requests in flight overlap on the
+//    device.
 //  * Mesh geometry and the grow-only solver workspaces (Krylov basis, CG
 //    vectors, pair-kernel partials) stay resident in HBM on cached handles,
 //    keyed by the caller's mesh id, and are reused by later requests on the
@@ -51,6 +53,8 @@ struct Request {
     double* F_out = nullptr;
     double* grad_out = nullptr;
     size_t need = 0;
+    double work = 0.0;          // cost estimate of the shortest-first policy: N_t^2 * W
+    int skipped = 0;            // admissions of younger requests while this one waited
     Clock::time_point t_submit, t_start, t_done;
     int status = -1;            // -1 queued/running, else a BAEMM status
     std::string error;
@@ -98,6 +102,8 @@ struct Server {
     int device = 0;
     int slots = 1;
     size_t budget = 0;
+    int policy = BAEMM_SERVE_FIFO;
+    int max_skips = 8;
     std::mutex mtx;
     std::condition_variable cv;       // queue changes, completions
     std::deque<std::shared_ptr<Request>> queue;
@@ -121,11 +127,27 @@ struct Server {
     void run(Request& rq, Cached& c);
 };
 
-// Picks the oldest queued request that fits; returns false when none does.
+// Picks the next queued request that fits; returns false when none does.
 // Called with mtx held.
+//  * FIFO: the oldest request; one that does not fit holds back the later ones.
+//  * Shortest first: requests that have been passed over max_skips times come
+//    first, oldest first, and hold back the others while they do not fit (so
+//    no request waits forever); then the rest by estimated work N_t^2 * W
+//    (the oldest on ties), each tried in turn.  Every queued request older
+//    than the admitted one counts one more skip.
 bool Server::admit(std::shared_ptr<Request>& out, Cached*& slot)
 {
-    for (auto it = queue.begin(); it != queue.end(); ++it) {
+    std::vector<std::deque<std::shared_ptr<Request>>::iterator> cand;
+    for (auto it = queue.begin(); it != queue.end(); ++it) cand.push_back(it);
+    size_t n_blocking = cand.size();     // FIFO: every candidate holds back the later ones
+    if (policy == BAEMM_SERVE_SHORTEST_FIRST) {
+        std::stable_partition(cand.begin(), cand.end(), [&](auto it) { return (*it)->skipped >= max_skips; });
+        n_blocking = (size_t)std::count_if(cand.begin(), cand.end(), [&](auto it) { return (*it)->skipped >= max_skips; });
+        std::stable_sort(cand.begin() + n_blocking, cand.end(),
+                         [](auto x, auto y) { return (*x)->work < (*y)->work; });
+    }
+    for (size_t ci = 0; ci < cand.size(); ++ci) {
+        auto it = cand[ci];
         std::shared_ptr<Request> rq = *it;
         // an idle cached handle of the same mesh: only growth beyond its size counts
         Cached* reuse = nullptr;
@@ -144,15 +166,17 @@ bool Server::admit(std::shared_ptr<Request>& out, Cached*& slot)
             if (!victim) break;
             baemm_destroy(victim->h);
             resident -= victim->bytes;
-            for (auto ci = cache.begin(); ci != cache.end(); ++ci)
-                if (&*ci == victim) {
-                    cache.erase(ci);
+            for (auto ci2 = cache.begin(); ci2 != cache.end(); ++ci2)
+                if (&*ci2 == victim) {
+                    cache.erase(ci2);
                     break;
                 }
         }
         if (resident + reserved + extra > budget) {
-            if (running == 0 && it == queue.begin()) {
-                // can never fit: fail it instead of waiting forever
+            if (running == 0) {
+                // nothing runs and every other idle handle is gone: it can never
+                // fit, so fail it instead of waiting forever, then look again
+                // (the erase invalidates the candidate list)
                 rq->status = BAEMM_ERR_ARG;
                 rq->error = "request needs more device memory than the server budget";
                 rq->done = true;
@@ -160,10 +184,11 @@ bool Server::admit(std::shared_ptr<Request>& out, Cached*& slot)
                 ++failed;
                 queue.erase(it);
                 cv.notify_all();
-                return false;
+                return admit(out, slot);
             }
-            // keep the order: a request that does not fit holds back the later ones
-            return false;
+            // keep the order: a blocking request that does not fit holds back the later ones
+            if (ci < n_blocking) return false;
+            continue;
         }
         if (!reuse) {
             cache.push_back(Cached{});
@@ -176,6 +201,7 @@ bool Server::admit(std::shared_ptr<Request>& out, Cached*& slot)
         reuse->busy = true;
         reserved += extra;
         rq->need = extra;   // what this request holds in `reserved` while it runs
+        for (auto older = queue.begin(); older != it; ++older) ++(*older)->skipped;
         queue.erase(it);
         out = rq;
         slot = reuse;
@@ -352,6 +378,20 @@ int baemm_server_destroy(baemm_server srv)
     return BAEMM_OK;
 }
 
+int baemm_server_set_policy(baemm_server srv, int policy, int max_skips)
+{
+    if (!srv || (policy != BAEMM_SERVE_FIFO && policy != BAEMM_SERVE_SHORTEST_FIRST) || max_skips < 0)
+        return BAEMM_ERR_ARG;
+    Server& s = srv->s;
+    {
+        std::lock_guard<std::mutex> lk(s.mtx);
+        s.policy = policy;
+        s.max_skips = max_skips;
+    }
+    s.cv.notify_all();
+    return BAEMM_OK;
+}
+
 int baemm_server_submit_gn(baemm_server srv, const baemm_gn_request* rq_in, double* far_field_out,
                            double* gradient_out, int64_t* request_id)
 {
@@ -384,6 +424,7 @@ int baemm_server_submit_gn(baemm_server srv, const baemm_gn_request* rq_in, doub
     rq->F_out = far_field_out;
     rq->grad_out = gradient_out;
     rq->need = estimate_bytes(rq->nv, rq->nt, rq->m, rq->K, rq->wcs, rq->precision, 30);
+    rq->work = (double)rq->nt * (double)rq->nt * (double)(rq->K * rq->wcs);
     Server& s = srv->s;
     {
         std::lock_guard<std::mutex> lk(s.mtx);

@@ -55,13 +55,23 @@ class GNServer:
     """Request loop on one device: ``slots`` concurrent requests, admitted while
     their estimated device memory fits into ``hbm_budget_bytes``."""
 
-    def __init__(self, slots=4, hbm_budget_bytes=64 << 30, device=0):
+    def __init__(self, slots=4, hbm_budget_bytes=64 << 30, device=0, policy="fifo", max_skips=8):
         self._L = _L()
         self._s = _P()
         st = self._L.baemm_server_create(int(device), int(slots), int(hbm_budget_bytes), ctypes.byref(self._s))
         if st != _lib.OK:
             raise _err(self._L, st)
         self._pending = {}
+        if policy != "fifo":
+            self.set_policy(policy, max_skips)
+
+    def set_policy(self, policy, max_skips=8):
+        """Admission order: "fifo" (oldest first) or "sjf" (least estimated work
+        N_t^2 * W first; a request passed over max_skips times goes first)."""
+        code = {"fifo": _lib.SERVE_FIFO, "sjf": _lib.SERVE_SHORTEST_FIRST}[policy]
+        st = self._L.baemm_server_set_policy(self._s, code, int(max_skips))
+        if st != _lib.OK:
+            raise _err(self._L, st)
 
     @staticmethod
     def request_bytes(nv, nt, m, K, wcs, precision="fp32"):

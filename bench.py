@@ -189,9 +189,11 @@ def serving_workload(n, max_level, seed=7):
     return reqs
 
 
-def run_serving(reqs, rate, slots, precision, seed=11):
+def run_serving(reqs, rate, slots, precision, seed=11, policy="fifo"):
     """Submits the requests at Poisson arrival times (rate per second) from a
-    driver thread and waits for all; returns the server statistics."""
+    driver thread and waits for all; returns the server statistics.  policy:
+    the admission order ("fifo", or "sjf": least estimated work first, with
+    aging)."""
     import threading
 
     from baemm_amd.serving import GNServer
@@ -199,7 +201,7 @@ def run_serving(reqs, rate, slots, precision, seed=11):
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
     ids = [None] * len(reqs)
-    with GNServer(slots=slots, hbm_budget_bytes=200 << 30) as srv:
+    with GNServer(slots=slots, hbm_budget_bytes=200 << 30, policy=policy) as srv:
         t0 = time.perf_counter()
 
         def driver():
@@ -216,7 +218,7 @@ def run_serving(reqs, rate, slots, precision, seed=11):
         statuses = []
         for k, i in enumerate(ids):
             statuses.append(srv.wait(i)[0])
-            log(f"serving ({precision}, {slots} slots): request {k + 1}/{len(ids)} done")
+            log(f"serving ({precision}, {slots} slots, {policy}): request {k + 1}/{len(ids)} done")
         wall = time.perf_counter() - t0
         st = srv.stats()
     st["wall_s"] = wall
@@ -542,6 +544,7 @@ def main():
         log("serving leg: building requests")
         reqs = serving_workload(args.serve_requests, args.serve_max_level)
         mixed_st = run_serving(reqs, args.serve_rate, args.serve_slots, "mixed")
+        sjf_st = run_serving(reqs, args.serve_rate, args.serve_slots, "mixed", policy="sjf")
         seq_st = run_serving(reqs, args.serve_rate, 1, "fp32")
         keys = ("completed", "failed", "not_converged", "solves_per_s", "p50_ms", "p99_ms", "mean_ms", "max_in_flight",
                 "peak_bytes", "gmres_iterations", "wall_s")
@@ -552,6 +555,7 @@ def main():
                         f"(log-uniform), W = 1..64, 256 far-field directions",
             "offered_rate_per_s": args.serve_rate,
             "continuous_mixed": dict({k: mixed_st[k] for k in keys}, slots=args.serve_slots),
+            "continuous_mixed_shortest_first": dict({k: sjf_st[k] for k in keys}, slots=args.serve_slots),
             "sequential_fp32": dict({k: seq_st[k] for k in keys}, slots=1),
             "speedup_solves_per_s": mixed_st["solves_per_s"] / max(seq_st["solves_per_s"], 1e-12),
         }

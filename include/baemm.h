@@ -372,6 +372,14 @@ typedef struct {
 } baemm_server_stats_t;
 
 int baemm_server_create(int device, int slots, int64_t hbm_budget_bytes, baemm_server* out);
+/* Admission order (no reference counterpart).  BAEMM_SERVE_FIFO (the default):
+ * the oldest queued request; one that does not fit holds back the later ones.
+ * BAEMM_SERVE_SHORTEST_FIRST: the queued request with the least estimated work
+ * (N_t^2 * W) that fits, except that a request passed over max_skips times
+ * (>= 0) goes first and holds back the others until it fits, so none waits
+ * forever.  Applies to admissions after the call. */
+enum { BAEMM_SERVE_FIFO = 0, BAEMM_SERVE_SHORTEST_FIRST = 1 };
+int baemm_server_set_policy(baemm_server srv, int policy, int max_skips);
 /* Waits for every submitted request, then releases the cached handles. */
 int baemm_server_destroy(baemm_server srv);
 /* Inputs are copied; far_field_out (meas_count x W complex) and gradient_out

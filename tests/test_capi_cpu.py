@@ -74,3 +74,5 @@ def test_header_constants_match_binding():
     assert consts["BAEMM_C128"] == _lib.C128 and consts["BAEMM_C64"] == _lib.C64
     assert consts["BAEMM_OK"] == _lib.OK and consts["BAEMM_ERR_ARG"] == _lib.ERR_ARG
     assert consts["BAEMM_ERR_NOCONV"] == _lib.ERR_NOCONV
+    assert consts["BAEMM_SERVE_FIFO"] == _lib.SERVE_FIFO
+    assert consts["BAEMM_SERVE_SHORTEST_FIRST"] == _lib.SERVE_SHORTEST_FIRST

@@ -59,6 +59,42 @@ def test_results_same_alone_and_in_the_mix(reqs):
         assert np.array_equal(a[2], b[2]), k
 
 
+def _completion_order(srv, reqs):
+    """Submits all requests at once, waits, returns (results, ids in completion order)."""
+    res = _run(srv, reqs)
+    lat = [r[3] for r in res]
+    return res, sorted(range(len(reqs)), key=lambda k: lat[k])
+
+
+def test_shortest_first_policy(reqs):
+    """Shortest-estimated-work-first admission (with aging) changes only the
+    order: every request completes with the bits of the FIFO run.  With one
+    slot, the requests queued behind the first admitted one run in order of
+    estimated work N_t^2 * W; with max_skips = 0 every request ages at once
+    and the order is the arrival order again."""
+    work = [r["T"].shape[0] ** 2 * len(r["kappa"]) * r["dirs"].shape[0] for r in reqs]
+    assert len(set(work)) == len(work)
+    with GNServer(slots=1) as srv:
+        fifo, order_fifo = _completion_order(srv, reqs)
+    with GNServer(slots=1, policy="sjf", max_skips=100) as srv:
+        sjf, order_sjf = _completion_order(srv, reqs)
+        st = srv.stats()
+    with GNServer(slots=1, policy="sjf", max_skips=0) as srv:
+        aged, order_aged = _completion_order(srv, reqs)
+    print("work", work, "fifo", order_fifo, "sjf", order_sjf, "aged", order_aged)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    for k in range(len(reqs)):
+        assert fifo[k][0] == 0 and sjf[k][0] == 0 and aged[k][0] == 0, k
+        assert np.array_equal(fifo[k][1], sjf[k][1]) and np.array_equal(fifo[k][2], sjf[k][2]), k
+        assert np.array_equal(fifo[k][1], aged[k][1]), k
+    # the first admission may happen before the queue is full; the rest follow the policy
+    assert order_fifo[1:] == sorted(order_fifo[1:])
+    assert [work[k] for k in order_sjf[1:]] == sorted(work[k] for k in order_sjf[1:])
+    assert order_aged[1:] == sorted(order_aged[1:])
+    with pytest.raises(KeyError):
+        GNServer(slots=1, policy="lifo")
+
+
 def test_request_matches_python_mirror(reqs):
     r = reqs[1]
     with GNServer(slots=2) as srv:
