@@ -262,7 +262,8 @@ __global__ void rmin_kernel(const float4* __restrict__ tgt, int ntgt, const floa
 
 // --------------------------------------------------------------- epilogues
 
-// Batch sizes of the epilogue's loads: incident triangles and segment partials.
+// Batch sizes of the epilogue's loads: incident triangles, and segment partials
+// (also mass-matrix entries per row).
 constexpr int EPI_TRI = 2;
 constexpr int EPI_SEG = 8;
 
