@@ -894,10 +894,12 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
     const bool ws = h.mixed_kernel == 1 || pl.f32;
     int S0 = choose_segments(h, tiles, nst / 8, h.nt);
     if (h.segments == 0) {
-        const int kg0 = ws ? std::min(pick_kg(pl.WCSP, (int)p.K, true), 2) : pick_kg(pl.WCSP, (int)p.K, true);
-        const void* fn0 = ws ? (const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32)
-                             : (const void*)mixed_fn(pl.adj, pl.WCSP, kg0);
-        S0 = balance_segments(tiles, nst, S0, nst / 8, resident_blocks(fn0, ws ? 512 : 256));
+        // balanced for the wave-specialised kernel (the default); the
+        // single-wave kernel takes the same segments, so the two stay
+        // bit-identical
+        const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
+        S0 = balance_segments(tiles, nst, S0, nst / 8,
+                              resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512));
     }
     const int steps = (nst + S0 - 1) / S0;
     const int S = (nst + steps - 1) / steps;
