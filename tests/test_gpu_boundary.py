@@ -196,6 +196,27 @@ def test_wave_specialised_mixed_kernel_is_identical(sphere3, W, wcs, kappa, c2, 
     assert relerr(outs[1], ref) < TOL
 
 
+def test_kernel_variants_identical_at_default_segments():
+    """At 20,480 triangles the automatic segment count is balanced against the
+    wave-specialised kernel's occupancy (it differs from the single-wave
+    kernel's); the single-wave kernel takes the same segments, so the two
+    variants stay bit-identical with the default settings too."""
+    V, T = icosphere(5)
+    rng = np.random.default_rng(21)
+    W, wcs = 64, 32
+    B = _rand(rng, V.shape[0], W)
+    coeff = np.array([0, 1, 1, 0, 0.5, -16j, 1, 0], np.complex128)
+    with Helmholtz_HIP(V, T) as H:
+        H.SetPrecision("mixed")
+        outs = []
+        for variant in (1, 0):
+            H.SetMixedKernel(variant)
+            C = np.zeros((V.shape[0], W), np.complex128)
+            H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [8.0, 16.0], coeff, W, wcs)
+            outs.append(C)
+    assert np.array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
 def test_segment_autotuner_keeps_results(sphere3, mode):
     """baemm_autotune_segments picks a segment count; applies after tuning still
