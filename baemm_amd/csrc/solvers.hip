@@ -7,6 +7,8 @@
 // Column reductions are two-pass (block partials, then a per-column sum) and
 // deterministic.
 #include <algorithm>
+#include <mutex>
+#include <set>
 #include <vector>
 
 #include "solvers.hpp"
@@ -763,6 +765,222 @@ __global__ __launch_bounds__(256) void pcg_block_kernel(int n, const int32_t* __
     if (tid == 0) atomicMax(&ctr[1], it);
 }
 
+// ---------------------------------------------------------------- LDS block PCG
+// The block PCG of small meshes with its working set on chip: the CSR mass
+// matrix and the search directions p live in LDS (copied there once), and each
+// thread keeps r, q and Y of its R rows (tid + k * 256) in registers.  The
+// loop touches no global memory; Y is written once at the end.  Same
+// algorithm, stopping rule and counters as pcg_block_kernel.
+// Dynamic LDS: val (nnz doubles), p (n * CGC doubles), ptr (n + 1 ints),
+// idx (nnz ints); pcg_lds_bytes gives the size.
+constexpr int kPcgLdsThreads = 256;
+constexpr size_t kPcgLdsMaxBytes = 150u * 1024u;
+
+static size_t pcg_lds_bytes(int64_t n, int64_t nnz, int CGC)
+{
+    return (size_t)(8 * nnz + 8 * n * CGC + 4 * (n + 1) + 4 * nnz);
+}
+
+template <int CGC, int R>
+__global__ __launch_bounds__(kPcgLdsThreads) void pcg_lds_kernel(int n, int nnz, const int32_t* __restrict__ M_ptr,
+                                                                 const int32_t* __restrict__ M_idx,
+                                                                 const double* __restrict__ M_val,
+                                                                 const double* __restrict__ Dinv,
+                                                                 const double* __restrict__ X, int64_t ldX,
+                                                                 double* __restrict__ Y, int64_t ldY, int ncols,
+                                                                 double tol2, int max_iter, int* __restrict__ ctr)
+{
+    constexpr int NT = kPcgLdsThreads, NW = NT / 64;
+    extern __shared__ double pcg_lds[];
+    double* s_val = pcg_lds;
+    double* s_p = s_val + nnz;
+    int32_t* s_ptr = reinterpret_cast<int32_t*>(s_p + (int64_t)n * CGC);
+    int32_t* s_idx = s_ptr + (n + 1);
+    __shared__ double red[2][CGC][NW];
+    __shared__ double s_rz[CGC], s_bb[CGC];
+    __shared__ int s_active[CGC];
+    const int c0 = blockIdx.x * CGC;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    bool okc[CGC];
+#pragma unroll
+    for (int c = 0; c < CGC; ++c) okc[c] = c0 + c < ncols;
+
+    // block sums of per-thread values v[k][c] for k < K (waves reduce by
+    // shuffles, then LDS in wave order)
+    auto block_sum = [&](double (&v)[2][CGC], int K) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (k >= K) break;
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) {
+                double x = v[k][c];
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+                if (lane == 0) red[k][c][wv] = x;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (k >= K) break;
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) {
+                double x = red[k][c][0];
+#pragma unroll
+                for (int w = 1; w < NW; ++w) x += red[k][c][w];
+                v[k][c] = x;
+            }
+        }
+        __syncthreads();
+    };
+
+    for (int e = tid; e < nnz; e += NT) {
+        s_val[e] = M_val[e];
+        s_idx[e] = M_idx[e];
+    }
+    for (int v = tid; v <= n; v += NT) s_ptr[v] = M_ptr[v];
+
+    // r = X, Y = 0, p = D r; bb = r.r, rz = r.(D r)
+    double r[R][CGC], q[R][CGC], y[R][CGC], d[R];
+    double acc[2][CGC];
+#pragma unroll
+    for (int c = 0; c < CGC; ++c) acc[0][c] = acc[1][c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int v = tid + k * NT;
+        d[k] = v < n ? Dinv[v] : 0.0;
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) {
+            const double x = (v < n && okc[c]) ? X[(int64_t)v * ldX + c0 + c] : 0.0;
+            r[k][c] = x;
+            q[k][c] = 0.0;
+            y[k][c] = 0.0;
+            if (v < n) s_p[(int64_t)v * CGC + c] = d[k] * x;
+            acc[0][c] += x * x;
+            acc[1][c] += x * d[k] * x;
+        }
+    }
+    block_sum(acc, 2);   // its barriers also publish the LDS copies
+    if (tid == 0) {
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) {
+            s_bb[c] = acc[0][c];
+            s_rz[c] = acc[1][c];
+            s_active[c] = okc[c] && acc[0][c] > 0.0;
+        }
+    }
+    __syncthreads();
+    int it = 0;
+    for (;;) {
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) any = any || s_active[c];
+        if (!any || it >= max_iter) break;
+        ++it;
+        // q = M p, pq
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) acc[0][c] = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int v = tid + k * NT;
+            if (v < n) {
+                double sv[CGC];
+#pragma unroll
+                for (int c = 0; c < CGC; ++c) sv[c] = 0.0;
+                for (int e = s_ptr[v]; e < s_ptr[v + 1]; ++e) {
+                    const double m = s_val[e];
+                    const int64_t u = (int64_t)s_idx[e] * CGC;
+#pragma unroll
+                    for (int c = 0; c < CGC; ++c) sv[c] += m * s_p[u + c];
+                }
+#pragma unroll
+                for (int c = 0; c < CGC; ++c) {
+                    q[k][c] = sv[c];
+                    acc[0][c] += s_p[(int64_t)v * CGC + c] * sv[c];
+                }
+            }
+        }
+        block_sum(acc, 1);
+        double alpha[CGC];
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) alpha[c] = (s_active[c] && acc[0][c] != 0.0) ? s_rz[c] / acc[0][c] : 0.0;
+        // Y += alpha p, r -= alpha q; rr and r.(D r)
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) acc[0][c] = acc[1][c] = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int v = tid + k * NT;
+            if (v < n) {
+#pragma unroll
+                for (int c = 0; c < CGC; ++c) {
+                    y[k][c] += alpha[c] * s_p[(int64_t)v * CGC + c];
+                    r[k][c] -= alpha[c] * q[k][c];
+                    acc[0][c] += r[k][c] * r[k][c];
+                    acc[1][c] += r[k][c] * d[k] * r[k][c];
+                }
+            }
+        }
+        block_sum(acc, 2);
+        double beta[CGC];
+#pragma unroll
+        for (int c = 0; c < CGC; ++c) beta[c] = (s_rz[c] != 0.0) ? acc[1][c] / s_rz[c] : 0.0;
+        __syncthreads();   // every thread has read s_rz and s_active
+        if (tid == 0) {
+#pragma unroll
+            for (int c = 0; c < CGC; ++c) {
+                if (s_active[c] && acc[0][c] <= tol2 * s_bb[c]) s_active[c] = 0;
+                s_rz[c] = acc[1][c];
+            }
+        }
+        // p = z + beta p on the own rows (the SpMV reads of p ended at the
+        // barriers above; the next SpMV reads after the barrier below)
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int v = tid + k * NT;
+            if (v < n) {
+#pragma unroll
+                for (int c = 0; c < CGC; ++c) {
+                    const int64_t kk = (int64_t)v * CGC + c;
+                    s_p[kk] = d[k] * r[k][c] + beta[c] * s_p[kk];
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int v = tid + k * NT;
+        if (v < n) {
+#pragma unroll
+            for (int c = 0; c < CGC; ++c)
+                if (okc[c]) Y[(int64_t)v * ldY + c0 + c] = y[k][c];
+        }
+    }
+    if (tid == 0) atomicMax(&ctr[1], it);
+}
+
+// Launches pcg_lds_kernel<CGC, R> on nb work-groups with lds bytes of dynamic
+// LDS, first allowing that much on this device (once per device and kernel: a
+// host-side attribute, no stream work).
+template <int CGC, int R>
+static void launch_pcg_lds(Handle& h, int nb, size_t lds, const double* X, int64_t ldX, double* Y, int64_t ldY,
+                           int ncols, double tol, int max_iter, int* ctr, hipStream_t s)
+{
+    static std::mutex mtx;
+    static std::set<int> allowed;
+    {
+        std::lock_guard<std::mutex> lk(mtx);
+        if (!allowed.count(h.device)) {
+            BAEMM_HIP(hipFuncSetAttribute((const void*)pcg_lds_kernel<CGC, R>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPcgLdsMaxBytes));
+            allowed.insert(h.device);
+        }
+    }
+    hipLaunchKernelGGL((pcg_lds_kernel<CGC, R>), dim3(nb), dim3(kPcgLdsThreads), lds, s, (int)h.nv,
+                       (int)h.h_M_idx.size(), h.d_M_ptr, h.d_M_idx, h.d_M_val, h.d_lumped_inv, X, ldX, Y, ldY, ncols,
+                       tol * tol, max_iter, ctr);
+}
+
 // ctr[2] += ctr[1] (iterations of the block PCG just run); ctr[1] = 0 before it.
 __global__ void pcg_reset_kernel(int* __restrict__ ctr) { ctr[1] = 0; }
 __global__ void pcg_total_kernel(int* __restrict__ ctr) { ctr[2] += ctr[1]; }
@@ -782,10 +1000,37 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
 {
     const int64_t n = h.nv;
     const int64_t N = n * ncols;
-    if (n <= kPcgBlockMaxRows) {
+    const int CGC = ncols >= 64 ? 4 : (ncols >= 16 ? 2 : 1);
+    const int nb = (ncols + CGC - 1) / CGC;
+    const int64_t nnz = (int64_t)h.h_M_idx.size();
+    const size_t lds = pcg_lds_bytes(n, nnz, CGC);
+    if ((n <= 8 * kPcgLdsThreads && lds <= kPcgLdsMaxBytes) || n <= kPcgBlockMaxRows) {
         int* ctr = dev_cg_counters(ws, s);
-        const int CGC = ncols >= 64 ? 4 : (ncols >= 16 ? 2 : 1);
-        const int nb = (ncols + CGC - 1) / CGC;
+        if (n <= 8 * kPcgLdsThreads && lds <= kPcgLdsMaxBytes) {
+            // working set on chip: one launch with no global traffic in the loop
+            ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
+            hipLaunchKernelGGL(pcg_reset_kernel, dim3(1), dim3(1), 0, s, ctr);
+            const int R = n <= kPcgLdsThreads ? 1 : (n <= 2 * kPcgLdsThreads ? 2 : (n <= 4 * kPcgLdsThreads ? 4 : 8));
+            const int C_R = CGC * 8 + R;
+            switch (C_R) {
+                case 1 * 8 + 1: launch_pcg_lds<1, 1>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 1 * 8 + 2: launch_pcg_lds<1, 2>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 1 * 8 + 4: launch_pcg_lds<1, 4>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 2 * 8 + 1: launch_pcg_lds<2, 1>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 2 * 8 + 2: launch_pcg_lds<2, 2>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 2 * 8 + 4: launch_pcg_lds<2, 4>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 4 * 8 + 1: launch_pcg_lds<4, 1>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 4 * 8 + 2: launch_pcg_lds<4, 2>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 4 * 8 + 4: launch_pcg_lds<4, 4>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 1 * 8 + 8: launch_pcg_lds<1, 8>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                case 2 * 8 + 8: launch_pcg_lds<2, 8>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+                default: launch_pcg_lds<4, 8>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
+            }
+            hipLaunchKernelGGL(pcg_total_kernel, dim3(1), dim3(1), 0, s, ctr);
+            BAEMM_HIP(hipGetLastError());
+            ws.cg_block = true;
+            return;
+        }
         ws.cg.reserve(sizeof(double) * 3 * n * (int64_t)CGC * nb);
         ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
         // the convergence flags read by dev_mass_cg_result: all columns converged
