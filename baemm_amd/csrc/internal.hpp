@@ -208,6 +208,4 @@ void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s);
 void incident_dev(Handle& h, const ApplyArgs& a, const double* dirs_host, int64_t inc_count, int wave_type, hipStream_t s);
 float compute_rmin(Handle& h, hipStream_t s);
 
-
-
 }  // namespace baemm

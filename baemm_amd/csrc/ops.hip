@@ -564,7 +564,6 @@ static int pick_kg(int WCSP, int remaining, bool mixed)
     return kg;
 }
 
-
 // Number of source segments: enough work-groups to fill 256 CUs several times,
 // and at most kMaxSegSources sources per segment, which bounds the length of
 // each FP32 accumulation (the segment partials are summed in FP64).
@@ -815,7 +814,6 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
     pl.bop = h.ws_bop.as<uint4>();
     pl.unscale = unscale;
 }
-
 
 void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
 {

@@ -15,7 +15,6 @@
 
 namespace baemm {
 
-
 // ---------------------------------------------------------------- sparse
 
 // Y = alpha M X + beta Y, real columns (MassOp.Dot).  Thread per (row, col).
