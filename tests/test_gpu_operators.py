@@ -123,22 +123,25 @@ def test_mass_inverse(mesh_h):
     assert relerr(Yl, lumped[:, None] * X) < 1e-6
 
 
-def test_mass_inverse_multiblock():
-    """Meshes above 1,024 vertices take the multi-block PCG (direction update
-    fused into the SpMV, column-major block partials).  The torus has 19,200
-    vertices; 80 real columns (40 complex) cover two 64-column chunks.  The
-    reference is the exact sparse solve with the device's own mass matrix."""
+@pytest.mark.parametrize("grid", [(60, 20), (240, 80)])
+def test_mass_inverse_larger_meshes(grid):
+    """The two PCG paths of larger meshes against the exact sparse solve with
+    the device's own mass matrix, with 5, 32 and 80 real columns (one, two and
+    four columns per work-group).  A 60 x 20 torus (1,200 vertices) runs the
+    LDS-resident block PCG with eight rows per thread; a 240 x 80 torus (19,200
+    vertices) the multi-block PCG (direction update fused into the SpMV,
+    column-major block partials; 80 columns cover two 64-column chunks)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
     from baemm_amd.meshes import torus
 
-    V, T = torus(1.0, 0.4, 240, 80)
+    V, T = torus(1.0, 0.4, *grid)
     nv = V.shape[0]
     rng = np.random.default_rng(12)
     with Helmholtz_HIP(V, T) as H:
         ptr, idx, val = H.MassMatrix()
         M = sp.csr_matrix((val, idx, ptr), shape=(nv, nv)).tocsc()
-        for X in (rng.standard_normal((nv, 5)), _rand(rng, nv, 40)):
+        for X in (rng.standard_normal((nv, 5)), _rand(rng, nv, 16), _rand(rng, nv, 40)):
             Y = np.zeros_like(X)
             H.ApplyMassInverse(X, Y, 1e-10)
             assert relerr(Y, spla.spsolve(M, X)) < 1e-6
