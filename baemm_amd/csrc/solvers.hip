@@ -765,36 +765,65 @@ __global__ __launch_bounds__(256) void pcg_block_kernel(int n, const int32_t* __
 }
 
 // ---------------------------------------------------------------- LDS block PCG
-// The block PCG of small meshes with its working set on chip: the CSR mass
-// matrix and the search directions p live in LDS (copied there once), and each
-// thread keeps r, q and Y of its R rows (tid + k * 256) in registers.  The
-// loop touches no global memory; Y is written once at the end.  Same
+// The block PCG of small meshes with its working set on chip: the mass
+// matrix and the search directions p live in LDS (copied there once), and
+// each thread keeps r, q and Y of its R rows (tid + k * 256) in registers.
+// The loop touches no global memory; Y is written once at the end.  Same
 // algorithm, stopping rule and counters as pcg_block_kernel.
-// Dynamic LDS: val (nnz doubles), p (n * CGC doubles), ptr (n + 1 ints),
-// idx (nnz ints); pcg_lds_bytes gives the size.
+// The matrix is held in one of two forms:
+//  * CSR (TRI = false): val (nnz doubles), ptr (n + 1 ints), idx (nnz ints);
+//  * triangles (TRI = true), for meshes whose CSR does not fit: per triangle
+//    its weight a/12 (double, from the same FP32 area as the CSR values) and
+//    its vertices (16-bit), and the vertex-to-triangle incidence (16-bit), so
+//    that (M p)_i = sum over the triangles t of i of w_t (p_a + p_b + p_c + p_i)
+//    (a/6 on the diagonal, a/12 off it, as the CSR assembles it).
+// pcg_lds_bytes gives the dynamic LDS size of either form.
 constexpr int kPcgLdsThreads = 256;
 constexpr size_t kPcgLdsMaxBytes = 150u * 1024u;
+// Columns per work-group of the LDS block PCG (a build-time setting).  One
+// column per work-group measured fastest on MI355X: FarField solves with 32
+// plane waves took 25-40 % less time than with four (more work-groups, fewer
+// rows x columns per thread, and a smaller LDS footprint).
+#ifndef BAEMM_PCG_LDS_CGC_MAX
+#define BAEMM_PCG_LDS_CGC_MAX 1
+#endif
 
-static size_t pcg_lds_bytes(int64_t n, int64_t nnz, int CGC)
+struct PcgMatrix {
+    int n, nnz, nt, nvt;    // nvt: vertex-to-triangle incidences
+    const int32_t* ptr;     // CSR
+    const int32_t* idx;
+    const double* val;
+    const int4* tris;       // triangle form
+    const float* area;
+    const int32_t* vt_ptr;
+    const int32_t* vt_idx;
+};
+
+static size_t pcg_lds_bytes(int64_t n, int64_t nnz, int64_t nt, int64_t nvt, int CGC, bool tri)
 {
-    return (size_t)(8 * nnz + 8 * n * CGC + 4 * (n + 1) + 4 * nnz);
+    if (!tri) return (size_t)(8 * nnz + 8 * n * CGC + 4 * (n + 1) + 4 * nnz);
+    return (size_t)(8 * nt + 8 * n * CGC + 8 * nt + 2 * (n + 1) + 2 * nvt);
 }
 
-template <int CGC, int R>
-__global__ __launch_bounds__(kPcgLdsThreads) void pcg_lds_kernel(int n, int nnz, const int32_t* __restrict__ M_ptr,
-                                                                 const int32_t* __restrict__ M_idx,
-                                                                 const double* __restrict__ M_val,
-                                                                 const double* __restrict__ Dinv,
+template <int CGC, int R, bool TRI>
+__global__ __launch_bounds__(kPcgLdsThreads) void pcg_lds_kernel(PcgMatrix A, const double* __restrict__ Dinv,
                                                                  const double* __restrict__ X, int64_t ldX,
                                                                  double* __restrict__ Y, int64_t ldY, int ncols,
                                                                  double tol2, int max_iter, int* __restrict__ ctr)
 {
     constexpr int NT = kPcgLdsThreads, NW = NT / 64;
+    const int n = A.n, nnz = A.nnz, nt = A.nt;
     extern __shared__ double pcg_lds[];
+    // CSR form
     double* s_val = pcg_lds;
-    double* s_p = s_val + nnz;
+    double* s_p = TRI ? pcg_lds + nt : s_val + nnz;
     int32_t* s_ptr = reinterpret_cast<int32_t*>(s_p + (int64_t)n * CGC);
     int32_t* s_idx = s_ptr + (n + 1);
+    // triangle form
+    double* s_tw = pcg_lds;
+    ushort4* s_tv = reinterpret_cast<ushort4*>(s_p + (int64_t)n * CGC);
+    uint16_t* s_vptr = reinterpret_cast<uint16_t*>(s_tv + nt);
+    uint16_t* s_vidx = s_vptr + (n + 1);
     __shared__ double red[2][CGC][NW];
     __shared__ double s_rz[CGC], s_bb[CGC];
     __shared__ int s_active[CGC];
@@ -833,11 +862,21 @@ __global__ __launch_bounds__(kPcgLdsThreads) void pcg_lds_kernel(int n, int nnz,
         __syncthreads();
     };
 
-    for (int e = tid; e < nnz; e += NT) {
-        s_val[e] = M_val[e];
-        s_idx[e] = M_idx[e];
+    if constexpr (TRI) {
+        for (int t = tid; t < nt; t += NT) {
+            const int4 tv = A.tris[t];
+            s_tv[t] = make_ushort4((unsigned short)tv.x, (unsigned short)tv.y, (unsigned short)tv.z, 0);
+            s_tw[t] = (double)A.area[t] / 12.0;
+        }
+        for (int v = tid; v <= n; v += NT) s_vptr[v] = (uint16_t)A.vt_ptr[v];
+        for (int e = tid; e < A.nvt; e += NT) s_vidx[e] = (uint16_t)A.vt_idx[e];
+    } else {
+        for (int e = tid; e < nnz; e += NT) {
+            s_val[e] = A.val[e];
+            s_idx[e] = A.idx[e];
+        }
+        for (int v = tid; v <= n; v += NT) s_ptr[v] = A.ptr[v];
     }
-    for (int v = tid; v <= n; v += NT) s_ptr[v] = M_ptr[v];
 
     // r = X, Y = 0, p = D r; bb = r.r, rz = r.(D r)
     double r[R][CGC], q[R][CGC], y[R][CGC], d[R];
@@ -886,11 +925,23 @@ __global__ __launch_bounds__(kPcgLdsThreads) void pcg_lds_kernel(int n, int nnz,
                 double sv[CGC];
 #pragma unroll
                 for (int c = 0; c < CGC; ++c) sv[c] = 0.0;
-                for (int e = s_ptr[v]; e < s_ptr[v + 1]; ++e) {
-                    const double m = s_val[e];
-                    const int64_t u = (int64_t)s_idx[e] * CGC;
+                if constexpr (TRI) {
+                    for (int e = s_vptr[v]; e < s_vptr[v + 1]; ++e) {
+                        const int t = s_vidx[e];
+                        const ushort4 tv = s_tv[t];
+                        const double w = s_tw[t];
 #pragma unroll
-                    for (int c = 0; c < CGC; ++c) sv[c] += m * s_p[u + c];
+                        for (int c = 0; c < CGC; ++c)
+                            sv[c] += w * (s_p[tv.x * CGC + c] + s_p[tv.y * CGC + c] + s_p[tv.z * CGC + c] +
+                                          s_p[(int64_t)v * CGC + c]);
+                    }
+                } else {
+                    for (int e = s_ptr[v]; e < s_ptr[v + 1]; ++e) {
+                        const double m = s_val[e];
+                        const int64_t u = (int64_t)s_idx[e] * CGC;
+#pragma unroll
+                        for (int c = 0; c < CGC; ++c) sv[c] += m * s_p[u + c];
+                    }
                 }
 #pragma unroll
                 for (int c = 0; c < CGC; ++c) {
@@ -958,10 +1009,10 @@ __global__ __launch_bounds__(kPcgLdsThreads) void pcg_lds_kernel(int n, int nnz,
     if (tid == 0) atomicMax(&ctr[1], it);
 }
 
-// Launches pcg_lds_kernel<CGC, R> on nb work-groups with lds bytes of dynamic
-// LDS, first allowing that much on this device (once per device and kernel: a
-// host-side attribute, no stream work).
-template <int CGC, int R>
+// Launches pcg_lds_kernel<CGC, R, TRI> on nb work-groups with lds bytes of
+// dynamic LDS, first allowing that much on this device (once per device and
+// kernel: a host-side attribute, no stream work).
+template <int CGC, int R, bool TRI>
 static void launch_pcg_lds(Handle& h, int nb, size_t lds, const double* X, int64_t ldX, double* Y, int64_t ldY,
                            int ncols, double tol, int max_iter, int* ctr, hipStream_t s)
 {
@@ -970,14 +1021,32 @@ static void launch_pcg_lds(Handle& h, int nb, size_t lds, const double* X, int64
     {
         std::lock_guard<std::mutex> lk(mtx);
         if (!allowed.count(h.device)) {
-            BAEMM_HIP(hipFuncSetAttribute((const void*)pcg_lds_kernel<CGC, R>,
+            BAEMM_HIP(hipFuncSetAttribute((const void*)pcg_lds_kernel<CGC, R, TRI>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPcgLdsMaxBytes));
             allowed.insert(h.device);
         }
     }
-    hipLaunchKernelGGL((pcg_lds_kernel<CGC, R>), dim3(nb), dim3(kPcgLdsThreads), lds, s, (int)h.nv,
-                       (int)h.h_M_idx.size(), h.d_M_ptr, h.d_M_idx, h.d_M_val, h.d_lumped_inv, X, ldX, Y, ldY, ncols,
-                       tol * tol, max_iter, ctr);
+    const PcgMatrix A{(int)h.nv, (int)h.h_M_idx.size(), (int)h.nt, (int)h.h_vt_idx.size(), h.d_M_ptr, h.d_M_idx,
+                      h.d_M_val, h.d_tris, h.d_area, h.d_vt_ptr, h.d_vt_idx};
+    hipLaunchKernelGGL((pcg_lds_kernel<CGC, R, TRI>), dim3(nb), dim3(kPcgLdsThreads), lds, s, A, h.d_lumped_inv, X,
+                       ldX, Y, ldY, ncols, tol * tol, max_iter, ctr);
+}
+
+// The LDS block PCG for CGC columns per work-group and R rows per thread.
+template <bool TRI>
+static void launch_pcg_lds_any(int CGC, int R, Handle& h, int nb, size_t lds, const double* X, int64_t ldX,
+                               double* Y, int64_t ldY, int ncols, double tol, int max_iter, int* ctr, hipStream_t s)
+{
+#define BAEMM_PCGL(C_, R_)                                                                        \
+    if (CGC == C_ && R == R_) {                                                                   \
+        launch_pcg_lds<C_, R_, TRI>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s);     \
+        return;                                                                                   \
+    }
+    BAEMM_PCGL(1, 1) BAEMM_PCGL(1, 2) BAEMM_PCGL(1, 4) BAEMM_PCGL(1, 8) BAEMM_PCGL(1, 16)
+    BAEMM_PCGL(2, 1) BAEMM_PCGL(2, 2) BAEMM_PCGL(2, 4) BAEMM_PCGL(2, 8) BAEMM_PCGL(2, 16)
+    BAEMM_PCGL(4, 1) BAEMM_PCGL(4, 2) BAEMM_PCGL(4, 4) BAEMM_PCGL(4, 8) BAEMM_PCGL(4, 16)
+#undef BAEMM_PCGL
+    throw ArgError{"internal: no LDS PCG variant"};
 }
 
 // ctr[2] += ctr[1] (iterations of the block PCG just run); ctr[1] = 0 before it.
@@ -1001,30 +1070,35 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     const int64_t N = n * ncols;
     const int CGC = ncols >= 64 ? 4 : (ncols >= 16 ? 2 : 1);
     const int nb = (ncols + CGC - 1) / CGC;
-    const int64_t nnz = (int64_t)h.h_M_idx.size();
-    const size_t lds = pcg_lds_bytes(n, nnz, CGC);
-    if ((n <= 8 * kPcgLdsThreads && lds <= kPcgLdsMaxBytes) || n <= kPcgBlockMaxRows) {
+    // LDS block PCG: the CSR form when it fits, else the triangle form (16-bit
+    // vertex and triangle indices); fewer columns per work-group (more
+    // work-groups) when that makes it fit
+    const int64_t nnz = (int64_t)h.h_M_idx.size(), nvt = (int64_t)h.h_vt_idx.size();
+    bool fit_csr = false, fit_tri = false;
+    int lcgc = BAEMM_PCG_LDS_CGC_MAX < CGC ? BAEMM_PCG_LDS_CGC_MAX : CGC;
+    size_t lds = 0;
+    for (; lcgc >= 1 && !fit_csr && !fit_tri; lcgc /= 2) {
+        const size_t b_csr = pcg_lds_bytes(n, nnz, h.nt, nvt, lcgc, false);
+        const size_t b_tri = pcg_lds_bytes(n, nnz, h.nt, nvt, lcgc, true);
+        fit_csr = n <= 8 * kPcgLdsThreads && b_csr <= kPcgLdsMaxBytes;
+        fit_tri = !fit_csr && n <= 16 * kPcgLdsThreads && h.nt <= 65535 && nvt <= 65535 &&
+                  b_tri <= kPcgLdsMaxBytes;
+        lds = fit_csr ? b_csr : b_tri;
+        if (fit_csr || fit_tri) break;
+    }
+    if (fit_csr || fit_tri || n <= kPcgBlockMaxRows) {
         int* ctr = dev_cg_counters(ws, s);
-        if (n <= 8 * kPcgLdsThreads && lds <= kPcgLdsMaxBytes) {
+        if (fit_csr || fit_tri) {
             // working set on chip: one launch with no global traffic in the loop
             ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
             hipLaunchKernelGGL(pcg_reset_kernel, dim3(1), dim3(1), 0, s, ctr);
-            const int R = n <= kPcgLdsThreads ? 1 : (n <= 2 * kPcgLdsThreads ? 2 : (n <= 4 * kPcgLdsThreads ? 4 : 8));
-            const int C_R = CGC * 8 + R;
-            switch (C_R) {
-                case 1 * 8 + 1: launch_pcg_lds<1, 1>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 1 * 8 + 2: launch_pcg_lds<1, 2>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 1 * 8 + 4: launch_pcg_lds<1, 4>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 2 * 8 + 1: launch_pcg_lds<2, 1>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 2 * 8 + 2: launch_pcg_lds<2, 2>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 2 * 8 + 4: launch_pcg_lds<2, 4>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 4 * 8 + 1: launch_pcg_lds<4, 1>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 4 * 8 + 2: launch_pcg_lds<4, 2>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 4 * 8 + 4: launch_pcg_lds<4, 4>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 1 * 8 + 8: launch_pcg_lds<1, 8>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                case 2 * 8 + 8: launch_pcg_lds<2, 8>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-                default: launch_pcg_lds<4, 8>(h, nb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s); break;
-            }
+            int R = 1;
+            while (R * kPcgLdsThreads < n) R *= 2;
+            const int lnb = (ncols + lcgc - 1) / lcgc;
+            if (fit_csr)
+                launch_pcg_lds_any<false>(lcgc, R, h, lnb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s);
+            else
+                launch_pcg_lds_any<true>(lcgc, R, h, lnb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s);
             hipLaunchKernelGGL(pcg_total_kernel, dim3(1), dim3(1), 0, s, ctr);
             BAEMM_HIP(hipGetLastError());
             ws.cg_block = true;

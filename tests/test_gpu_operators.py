@@ -123,14 +123,16 @@ def test_mass_inverse(mesh_h):
     assert relerr(Yl, lumped[:, None] * X) < 1e-6
 
 
-@pytest.mark.parametrize("grid", [(60, 20), (240, 80)])
+@pytest.mark.parametrize("grid", [(60, 20), (80, 24), (240, 80)])
 def test_mass_inverse_larger_meshes(grid):
-    """The two PCG paths of larger meshes against the exact sparse solve with
-    the device's own mass matrix, with 5, 32 and 80 real columns (one, two and
+    """The PCG paths of larger meshes against the exact sparse solve with the
+    device's own mass matrix, with 5, 32 and 80 real columns (one, two and
     four columns per work-group).  A 60 x 20 torus (1,200 vertices) runs the
-    LDS-resident block PCG with eight rows per thread; a 240 x 80 torus (19,200
-    vertices) the multi-block PCG (direction update fused into the SpMV,
-    column-major block partials; 80 columns cover two 64-column chunks)."""
+    LDS-resident block PCG on the CSR matrix with eight rows per thread; an
+    80 x 24 torus (1,920 vertices) the LDS-resident block PCG on the triangle
+    form of the matrix; a 240 x 80 torus (19,200 vertices) the multi-block PCG
+    (direction update fused into the SpMV, column-major block partials; 80
+    columns cover two 64-column chunks)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
     from baemm_amd.meshes import torus
