@@ -136,7 +136,7 @@ int baemm_set_gmres_limits(baemm_handle h, int max_iter, int max_restarts);
  * PCG preconditioner, Gram-Schmidt, normalisation) in a hipGraph and replay
  * it (on != 0), instead of launching its ~100 kernels one by one.  A graph is
  * kept per Krylov index and re-captured when the loaded operator, precision
- * or workspaces change.  Results are identical either way.  Default off. */
+ * or workspaces change.  Results are identical either way.  Default on. */
 int baemm_set_graphs(baemm_handle h, int on);
 int baemm_get_graph_stats(baemm_handle h, int64_t* captures, int64_t* launches);
 
