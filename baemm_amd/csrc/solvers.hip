@@ -427,41 +427,52 @@ void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, doub
 }
 
 // ---------------------------------------------------------------- fused PCG steps
-// Row-blocked like coldot_part_kernel (block b: rows b*4+rl, stride grid*4;
+// Row-blocked (block b: rows b*CGW+rl, stride grid*CGW, for its CGW waves;
 // 64 column lanes), deterministic two-pass reductions.  Block partials are
 // stored column-major, part[c * nblocks + b], so the one-wave column sums read
 // contiguous memory.
+
+// Waves per block of the multi-block PCG passes (a build-time setting).
+#ifndef BAEMM_CG_WAVES
+#define BAEMM_CG_WAVES 4
+#endif
+constexpr int CGW = BAEMM_CG_WAVES;
 
 // Row entries of the mass matrix gathered per batch in the SpMV: the loads of a
 // batch are issued together, then the products are added in entry order.
 constexpr int kSpmvBatch = 8;
 
-// The block's partial of x.y for column c0 + c (lanes rl = 0..3 of the block
-// hold the sums of their rows) goes to part[(c0 + c) * gridDim.x + blockIdx.x].
-__device__ __forceinline__ void cg_store_partial(double (&sm)[4][64], double acc, int c0, int c, int rl, int ncols,
+// The block's partial of x.y for column c0 + c (waves rl = 0..CGW-1 of the
+// block hold the sums of their rows, added in wave order) goes to
+// part[(c0 + c) * gridDim.x + blockIdx.x].
+__device__ __forceinline__ void cg_store_partial(double (&sm)[CGW][64], double acc, int c0, int c, int rl, int ncols,
                                                  double* __restrict__ part)
 {
     sm[rl][c] = acc;
     __syncthreads();
-    if (rl == 0 && c0 + c < ncols)
-        part[(int64_t)(c0 + c) * gridDim.x + blockIdx.x] = sm[0][c] + sm[1][c] + sm[2][c] + sm[3][c];
+    if (rl == 0 && c0 + c < ncols) {
+        double x = sm[0][c];
+#pragma unroll
+        for (int w = 1; w < CGW; ++w) x += sm[w][c];
+        part[(int64_t)(c0 + c) * gridDim.x + blockIdx.x] = x;
+    }
     __syncthreads();
 }
 
 // First step: q = M p and the block partials of p.q
-__global__ __launch_bounds__(256) void cg_spmv_dot_kernel(int n, int ncols, const int32_t* __restrict__ ptr,
+__global__ __launch_bounds__(64 * CGW) void cg_spmv_dot_kernel(int n, int ncols, const int32_t* __restrict__ ptr,
                                                           const int32_t* __restrict__ idx,
                                                           const double* __restrict__ val, const double* __restrict__ p,
                                                           double* __restrict__ q, double* __restrict__ part,
                                                           const int* __restrict__ skip)
 {
     if (*skip) return;
-    __shared__ double sm[4][64];
+    __shared__ double sm[CGW][64];
     const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
     for (int c0 = 0; c0 < ncols; c0 += 64) {
         double acc = 0.0;
         if (c0 + c < ncols)
-            for (int r = blockIdx.x * 4 + rl; r < n; r += gridDim.x * 4) {
+            for (int r = blockIdx.x * CGW + rl; r < n; r += gridDim.x * CGW) {
                 const int e0 = ptr[r], e1 = ptr[r + 1];
                 double sv = 0.0;
                 for (int e = e0; e < e1; e += kSpmvBatch) {
@@ -489,7 +500,7 @@ __global__ __launch_bounds__(256) void cg_spmv_dot_kernel(int n, int ncols, cons
 // values), so the direction update needs no pass of its own.  p_in is not
 // written in this launch (the directions are double-buffered).  Block partials
 // of p_out.q.
-__global__ __launch_bounds__(256) void cg_spmv_p_dot_kernel(int n, int ncols, const int32_t* __restrict__ ptr,
+__global__ __launch_bounds__(64 * CGW) void cg_spmv_p_dot_kernel(int n, int ncols, const int32_t* __restrict__ ptr,
                                                             const int32_t* __restrict__ idx,
                                                             const double* __restrict__ val,
                                                             const double* __restrict__ Dinv,
@@ -500,13 +511,13 @@ __global__ __launch_bounds__(256) void cg_spmv_p_dot_kernel(int n, int ncols, co
                                                             double* __restrict__ part, const int* __restrict__ skip)
 {
     if (*skip) return;
-    __shared__ double sm[4][64];
+    __shared__ double sm[CGW][64];
     const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
     for (int c0 = 0; c0 < ncols; c0 += 64) {
         double acc = 0.0;
         if (c0 + c < ncols) {
             const double bt = beta[c0 + c];
-            for (int v = blockIdx.x * 4 + rl; v < n; v += gridDim.x * 4) {
+            for (int v = blockIdx.x * CGW + rl; v < n; v += gridDim.x * CGW) {
                 const int e0 = ptr[v], e1 = ptr[v + 1];
                 double sv = 0.0;
                 for (int e = e0; e < e1; e += kSpmvBatch) {
@@ -548,7 +559,7 @@ __global__ __launch_bounds__(64) void cg_alpha_sum_kernel(int nblocks, int ncols
 
 // Y += alpha p; r -= alpha q; block partials part1 = r.r and part2 = r.z with
 // z = D r (not stored: the next step forms it again)
-__global__ __launch_bounds__(256) void cg_update_dots_kernel(int n, int ncols, const double* __restrict__ alpha,
+__global__ __launch_bounds__(64 * CGW) void cg_update_dots_kernel(int n, int ncols, const double* __restrict__ alpha,
                                                              const double* __restrict__ p,
                                                              const double* __restrict__ q,
                                                              const double* __restrict__ Dinv, double* __restrict__ Y,
@@ -557,13 +568,13 @@ __global__ __launch_bounds__(256) void cg_update_dots_kernel(int n, int ncols, c
                                                              const int* __restrict__ skip)
 {
     if (*skip) return;
-    __shared__ double sm1[4][64], sm2[4][64];
+    __shared__ double sm1[CGW][64], sm2[CGW][64];
     const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
     for (int c0 = 0; c0 < ncols; c0 += 64) {
         double a1 = 0.0, a2 = 0.0;
         if (c0 + c < ncols) {
             const double al = alpha[c0 + c];
-            for (int v = blockIdx.x * 4 + rl; v < n; v += gridDim.x * 4) {
+            for (int v = blockIdx.x * CGW + rl; v < n; v += gridDim.x * CGW) {
                 const int64_t k = (int64_t)v * ncols + c0 + c;
                 Y[(int64_t)v * ldY + c0 + c] += al * p[k];
                 const double rv = r[k] - al * q[k];
@@ -1126,7 +1137,7 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     // partial-sum blocks of the SpMV and update passes: about 16 rows per block
     // (4 per wave), so the row loops stay short; the column sums read them with
     // one wave per column
-    const int nred = (int)std::min<int64_t>(4096, std::max<int64_t>(kRedBlocks, (n + 15) / 16));
+    const int nred = (int)std::min<int64_t>(4096, std::max<int64_t>(kRedBlocks, (n + 4 * CGW - 1) / (4 * CGW)));
     ws.red.reserve(sizeof(double) * 2 * std::max(nred, kRedBlocks) * ncols);
     ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
     int* ctr = dev_cg_counters(ws, s);
@@ -1159,15 +1170,15 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     for (int it = 0; it < max_iter; ++it) {
         double* p = (it & 1) ? pb : pa;
         if (it == 0)
-            hipLaunchKernelGGL(cg_spmv_dot_kernel, dim3(nred), dim3(256), 0, s, (int)n, ncols, h.d_M_ptr, h.d_M_idx,
+            hipLaunchKernelGGL(cg_spmv_dot_kernel, dim3(nred), dim3(64 * CGW), 0, s, (int)n, ncols, h.d_M_ptr, h.d_M_idx,
                                h.d_M_val, p, q, part1, done);
         else
-            hipLaunchKernelGGL(cg_spmv_p_dot_kernel, dim3(nred), dim3(256), 0, s, (int)n, ncols, h.d_M_ptr,
+            hipLaunchKernelGGL(cg_spmv_p_dot_kernel, dim3(nred), dim3(64 * CGW), 0, s, (int)n, ncols, h.d_M_ptr,
                                h.d_M_idx, h.d_M_val, h.d_lumped_inv, r, beta, (it & 1) ? pa : pb, p, q, part1,
                                done);
         hipLaunchKernelGGL(cg_alpha_sum_kernel, dim3(ncols), dim3(64), 0, s, nred, ncols, part1, rz, active, alpha,
                            done);
-        hipLaunchKernelGGL(cg_update_dots_kernel, dim3(nred), dim3(256), 0, s, (int)n, ncols, alpha, p, q,
+        hipLaunchKernelGGL(cg_update_dots_kernel, dim3(nred), dim3(64 * CGW), 0, s, (int)n, ncols, alpha, p, q,
                            h.d_lumped_inv, Y, ldY, r, part1, part2, done);
         hipLaunchKernelGGL(cg_check_beta_kernel, dim3(ncols), dim3(64), 0, s, nred, ncols, part1, part2, bb,
                            tol * tol, max_iter, active, rz, beta, ctr);
