@@ -432,11 +432,16 @@ void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, doub
 // stored column-major, part[c * nblocks + b], so the one-wave column sums read
 // contiguous memory.
 
-// Waves per block of the multi-block PCG passes (a build-time setting).
+// Waves per block of the multi-block PCG passes, and rows per wave (the grid
+// is about n / (CGW * CG_RPW) blocks); build-time settings.
 #ifndef BAEMM_CG_WAVES
 #define BAEMM_CG_WAVES 4
 #endif
+#ifndef BAEMM_CG_ROWS_PER_WAVE
+#define BAEMM_CG_ROWS_PER_WAVE 4
+#endif
 constexpr int CGW = BAEMM_CG_WAVES;
+constexpr int CG_RPW = BAEMM_CG_ROWS_PER_WAVE;
 
 // Row entries of the mass matrix gathered per batch in the SpMV: the loads of a
 // batch are issued together, then the products are added in entry order.
@@ -1137,7 +1142,7 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     // partial-sum blocks of the SpMV and update passes: about 16 rows per block
     // (4 per wave), so the row loops stay short; the column sums read them with
     // one wave per column
-    const int nred = (int)std::min<int64_t>(4096, std::max<int64_t>(kRedBlocks, (n + 4 * CGW - 1) / (4 * CGW)));
+    const int nred = (int)std::min<int64_t>(4096, std::max<int64_t>(kRedBlocks, (n + CG_RPW * CGW - 1) / (CG_RPW * CGW)));
     ws.red.reserve(sizeof(double) * 2 * std::max(nred, kRedBlocks) * ncols);
     ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
     int* ctr = dev_cg_counters(ws, s);
