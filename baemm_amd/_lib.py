@@ -91,6 +91,7 @@ SIGNATURES = {
     "baemm_reset_solver_stats": (_INT, [_P]),
     "baemm_set_timing": (_INT, [_P, _INT]),
     "baemm_get_timing": (_INT, [_P, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
+    "baemm_hw_sincos": (_INT, [_INT, _P, _I64, _P, _P]),
     "baemm_dev_alloc": (_INT, [ctypes.POINTER(_P), ctypes.c_size_t]),
     "baemm_dev_free": (_INT, [_P]),
     "baemm_dev_upload": (_INT, [_P, _P, ctypes.c_size_t]),

@@ -530,6 +530,30 @@ int baemm_get_timing(baemm_handle h, double* total_ms, int64_t* launches)
     });
 }
 
+int baemm_hw_sincos(int device, const float* rev, int64_t n, float* sin_out, float* cos_out)
+{
+    if (n < 0 || (n > 0 && (!rev || !sin_out || !cos_out))) return BAEMM_ERR_ARG;
+    if (n == 0) return BAEMM_OK;
+    void* d = nullptr;
+    int st = BAEMM_OK;
+    try {
+        int ndev = 0;
+        BAEMM_HIP(hipGetDeviceCount(&ndev));
+        BAEMM_HIP(hipSetDevice((device >= 0 && device < ndev) ? device : 0));
+        BAEMM_HIP(hipMalloc(&d, sizeof(float) * 3 * n));
+        float* dr = static_cast<float*>(d);
+        BAEMM_HIP(hipMemcpy(dr, rev, sizeof(float) * n, hipMemcpyHostToDevice));
+        hw_sincos_dev(dr, n, dr + n, dr + 2 * n, nullptr);
+        BAEMM_HIP(hipMemcpy(sin_out, dr + n, sizeof(float) * n, hipMemcpyDeviceToHost));
+        BAEMM_HIP(hipMemcpy(cos_out, dr + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+    } catch (const HipError& e) {
+        set_thread_error(std::string("HIP error '") + hipGetErrorString(e.err) + "' in " + e.what);
+        st = BAEMM_ERR_HIP;
+    }
+    if (d) (void)hipFree(d);
+    return st;
+}
+
 int baemm_dev_alloc(void** ptr, size_t bytes)
 {
     if (!ptr) return BAEMM_ERR_ARG;

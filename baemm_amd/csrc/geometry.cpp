@@ -42,6 +42,22 @@ void build_host_geometry(Handle& h)
     h.h_nrm4.assign(4 * h.nt_pad, 0.0f);
     h.h_area.assign(nt, 0.0f);
     h.h_sdiag.assign(nt, 0.0f);
+    {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, rmax2 = 0.0;
+        for (int64_t v = 0; v < nv; ++v) {
+            double r2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const double x = h.coords[3 * v + d];
+                lo[d] = std::min(lo[d], x);
+                hi[d] = std::max(hi[d], x);
+                r2 += x * x;
+            }
+            rmax2 = std::max(rmax2, r2);
+        }
+        double diag2 = 0.0;
+        for (int d = 0; d < 3; ++d) diag2 += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+        h.extent = std::sqrt(std::max(diag2, rmax2));
+    }
     std::vector<double> area_d(nt);
     for (int64_t t = 0; t < nt; ++t) {
         double x[3][3], c[3];
@@ -198,6 +214,10 @@ void load_params(Handle& h, const double* kappa, const double* coeff, int64_t W,
     if (W < 1 || wcs < 1) throw ArgError{"wave_count and wave_chunk_size must be >= 1"};
     if (W % wcs != 0) throw ArgError{"wave_count must be a multiple of wave_chunk_size"};
     if (!kappa || !coeff) throw ArgError{"kappa and coeff must not be NULL"};
+    for (int64_t q = 0; q < W / wcs; ++q)
+        if (!(std::fabs(kappa[q]) * h.extent / (2.0 * M_PI) <= kMaxPhaseRev))
+            throw ArgError{"kappa times the mesh extent exceeds the FP32 phase range of the pair kernels "
+                           "(65,536 revolutions)"};
     Params& p = h.prm;
     p.W = W;
     p.wcs = wcs;

@@ -94,7 +94,6 @@ struct Workspace {
     DevBuf V, w, t, x, r, hc;             // GMRES basis and vectors, Gram-Schmidt coefficients
     DevBuf a, b, c, d, e;                 // composition temporaries
     DevBuf cgctr;                         // PCG counters: done, iterations, total
-    bool cg_block = false;                // the last PCG ran as one block launch
     DevBuf sc;                            // per-column scales of the Krylov vectors
 };
 
@@ -142,6 +141,7 @@ struct Handle {
     double* d_lumped_inv = nullptr;
     float4* d_meas = nullptr;
     float rmin = -1.0f;                  // min centroid distance (mixed scaling), lazily computed
+    double extent = 0.0;                 // max(bounding-box diagonal, max |x|) of the vertices
     // settings (GetSetters.hpp)
     bool use_diagonal = true;
     int block_size = 64;
@@ -208,5 +208,13 @@ void apply_near_dev(Handle& h, const ApplyArgs& a, const float4* d_pts, int64_t 
 void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s);
 void incident_dev(Handle& h, const ApplyArgs& a, const double* dirs_host, int64_t inc_count, int wave_type, hipStream_t s);
 float compute_rmin(Handle& h, hipStream_t s);
+// Hardware sine and cosine (v_sin_f32 / v_cos_f32, argument in revolutions)
+// of rev[0..n) as the pair kernels evaluate them; device pointers.
+void hw_sincos_dev(const float* rev, int64_t n, float* sin_out, float* cos_out, hipStream_t s);
+
+// Largest phase kappa r / (2 pi) in revolutions the pair kernels accept: the
+// hardware sine and cosine are accurate to about 1e-7 up to there, and the
+// FP32 phase itself resolves 2^-8 revolutions at that size.
+constexpr double kMaxPhaseRev = 65536.0;
 
 }  // namespace baemm

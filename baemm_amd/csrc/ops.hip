@@ -1151,4 +1151,24 @@ void incident_dev(Handle& h, const ApplyArgs& a, const double* dirs, int64_t inc
     bdr_epilogue(h, ea, kC64, a.Ctype, s);
 }
 
+
+// --------------------------------------------------------------- diagnostics
+
+__global__ void hw_sincos_kernel(const float* __restrict__ rev, int64_t n, float* __restrict__ s,
+                                 float* __restrict__ c)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s[i] = hw_sin_rev(rev[i]);
+    c[i] = hw_cos_rev(rev[i]);
+}
+
+void hw_sincos_dev(const float* rev, int64_t n, float* sin_out, float* cos_out, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(hw_sincos_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rev, n, sin_out,
+                       cos_out);
+    BAEMM_HIP(hipGetLastError());
+}
+
 }  // namespace baemm

@@ -2,8 +2,13 @@
 // per-vertex / per-triangle operators used by the solver compositions.
 //
 // All solver vectors are FP64, row-major (vertex-major, column-minor) with a
-// leading dimension; complex vectors are viewed as 2x as many real columns
-// where the arithmetic is real (CG), as ApplyMassInverse.hpp:71-89 does.
+// leading dimension.  The mass-inverse PCG works on real columns: a complex
+// column is two real ones (its real and imaginary parts), each solved as its
+// own real CG with its own step lengths and stopping test.  The reference runs
+// one complex CG per complex column whose operator and preconditioner view the
+// column as two real ones (ApplyMassInverse.hpp:56-104), so its real and
+// imaginary parts share the step lengths; both converge to M^{-1} X and differ
+// only below the CG tolerance.
 // Column reductions are two-pass (block partials, then a per-column sum) and
 // deterministic.
 #include <algorithm>
@@ -631,161 +636,14 @@ __global__ __launch_bounds__(64) void cg_check_beta_kernel(int nblocks, int ncol
     }
 }
 
-// ---------------------------------------------------------------- block PCG
-// The whole PCG of a group of CGC columns in one work-group (each block owns
-// its columns for all rows, so every reduction is a block reduction and there
-// is no grid-wide synchronisation): one launch instead of ~13 per iteration.
-// Same algorithm and stopping rule as the multi-kernel PCG below; used when
-// the mesh is small enough that launches, not memory traffic, dominate.
-// work: per block 3 * n * CGC doubles (r, p, q).  ctr[1]: max iterations over
-// blocks (zeroed before the launch), ctr[2] += iterations by pcg_total_kernel.
-template <int CGC>
-__global__ __launch_bounds__(256) void pcg_block_kernel(int n, const int32_t* __restrict__ M_ptr,
-                                                        const int32_t* __restrict__ M_idx,
-                                                        const double* __restrict__ M_val,
-                                                        const double* __restrict__ Dinv, const double* __restrict__ X,
-                                                        int64_t ldX, double* __restrict__ Y, int64_t ldY, int ncols,
-                                                        double tol2, int max_iter, double* __restrict__ work,
-                                                        int* __restrict__ ctr)
-{
-    __shared__ double red[3][CGC][8];
-    __shared__ double s_rz[CGC], s_bb[CGC];
-    __shared__ int s_active[CGC];
-    const int c0 = blockIdx.x * CGC;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    double* r = work + (int64_t)blockIdx.x * 3 * n * CGC;
-    double* p = r + (int64_t)n * CGC;
-    double* q = p + (int64_t)n * CGC;
-
-    // block sum of per-thread values v[k][c] for k < K (waves reduce by shuffles, then LDS)
-    auto block_sum = [&](double (&v)[3][CGC], int K) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (k >= K) break;
-#pragma unroll
-            for (int c = 0; c < CGC; ++c) {
-                double x = v[k][c];
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-                if (lane == 0) red[k][c][wv] = x;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (k >= K) break;
-#pragma unroll
-            for (int c = 0; c < CGC; ++c) v[k][c] = red[k][c][0] + red[k][c][1] + red[k][c][2] + red[k][c][3];
-        }
-        __syncthreads();
-    };
-
-    // r = X, Y = 0, p = z = D r; bb = r.r, rz = r.z
-    double acc[3][CGC];
-#pragma unroll
-    for (int c = 0; c < CGC; ++c) acc[0][c] = acc[1][c] = acc[2][c] = 0.0;
-    for (int v = tid; v < n; v += 256) {
-        const double d = Dinv[v];
-#pragma unroll
-        for (int c = 0; c < CGC; ++c) {
-            const bool ok = c0 + c < ncols;
-            const double x = ok ? X[(int64_t)v * ldX + c0 + c] : 0.0;
-            if (ok) Y[(int64_t)v * ldY + c0 + c] = 0.0;
-            r[(int64_t)v * CGC + c] = x;
-            p[(int64_t)v * CGC + c] = d * x;
-            acc[0][c] += x * x;
-            acc[1][c] += x * d * x;
-        }
-    }
-    block_sum(acc, 2);
-    if (tid == 0) {
-#pragma unroll
-        for (int c = 0; c < CGC; ++c) {
-            s_bb[c] = acc[0][c];
-            s_rz[c] = acc[1][c];
-            s_active[c] = (c0 + c < ncols) && acc[0][c] > 0.0;
-        }
-    }
-    __syncthreads();
-    int it = 0;
-    for (;;) {
-        bool any = false;
-#pragma unroll
-        for (int c = 0; c < CGC; ++c) any = any || s_active[c];
-        if (!any || it >= max_iter) break;
-        ++it;
-        // q = M p, pq
-#pragma unroll
-        for (int c = 0; c < CGC; ++c) acc[0][c] = 0.0;
-        for (int v = tid; v < n; v += 256) {
-            double s[CGC];
-#pragma unroll
-            for (int c = 0; c < CGC; ++c) s[c] = 0.0;
-            for (int e = M_ptr[v]; e < M_ptr[v + 1]; ++e) {
-                const double m = M_val[e];
-                const int64_t u = (int64_t)M_idx[e] * CGC;
-#pragma unroll
-                for (int c = 0; c < CGC; ++c) s[c] += m * p[u + c];
-            }
-#pragma unroll
-            for (int c = 0; c < CGC; ++c) {
-                q[(int64_t)v * CGC + c] = s[c];
-                acc[0][c] += p[(int64_t)v * CGC + c] * s[c];
-            }
-        }
-        block_sum(acc, 1);
-        double alpha[CGC];
-#pragma unroll
-        for (int c = 0; c < CGC; ++c) alpha[c] = (s_active[c] && acc[0][c] != 0.0) ? s_rz[c] / acc[0][c] : 0.0;
-        // Y += alpha p, r -= alpha q; rr, rz_new with z = D r
-#pragma unroll
-        for (int c = 0; c < CGC; ++c) acc[0][c] = acc[1][c] = 0.0;
-        for (int v = tid; v < n; v += 256) {
-            const double d = Dinv[v];
-#pragma unroll
-            for (int c = 0; c < CGC; ++c) {
-                const int64_t k = (int64_t)v * CGC + c;
-                if (c0 + c < ncols) Y[(int64_t)v * ldY + c0 + c] += alpha[c] * p[k];
-                const double rv = r[k] - alpha[c] * q[k];
-                r[k] = rv;
-                acc[0][c] += rv * rv;
-                acc[1][c] += rv * d * rv;
-            }
-        }
-        block_sum(acc, 2);
-        double beta[CGC];
-#pragma unroll
-        for (int c = 0; c < CGC; ++c) {
-            beta[c] = (s_rz[c] != 0.0) ? acc[1][c] / s_rz[c] : 0.0;
-        }
-        __syncthreads();
-        if (tid == 0) {
-#pragma unroll
-            for (int c = 0; c < CGC; ++c) {
-                if (s_active[c] && acc[0][c] <= tol2 * s_bb[c]) s_active[c] = 0;
-                s_rz[c] = acc[1][c];
-            }
-        }
-        // p = z + beta p
-        for (int v = tid; v < n; v += 256) {
-            const double d = Dinv[v];
-#pragma unroll
-            for (int c = 0; c < CGC; ++c) {
-                const int64_t k = (int64_t)v * CGC + c;
-                p[k] = d * r[k] + beta[c] * p[k];
-            }
-        }
-        __syncthreads();
-    }
-    if (tid == 0) atomicMax(&ctr[1], it);
-}
-
 // ---------------------------------------------------------------- LDS block PCG
 // The block PCG of small meshes with its working set on chip: the mass
 // matrix and the search directions p live in LDS (copied there once), and
 // each thread keeps r, q and Y of its R rows (tid + k * 256) in registers.
 // The loop touches no global memory; Y is written once at the end.  Same
-// algorithm, stopping rule and counters as pcg_block_kernel.
+// algorithm and stopping rule as the multi-block PCG below; ctr[1] receives
+// the largest iteration count over the work-groups (zeroed before the launch)
+// and pcg_total_kernel adds it to ctr[2].
 // The matrix is held in one of two forms:
 //  * CSR (TRI = false): val (nnz doubles), ptr (n + 1 ints), idx (nnz ints);
 //  * triangles (TRI = true), for meshes whose CSR does not fit: per triangle
@@ -796,13 +654,9 @@ __global__ __launch_bounds__(256) void pcg_block_kernel(int n, const int32_t* __
 // pcg_lds_bytes gives the dynamic LDS size of either form.
 constexpr int kPcgLdsThreads = 256;
 constexpr size_t kPcgLdsMaxBytes = 150u * 1024u;
-// Columns per work-group of the LDS block PCG (a build-time setting).  One
-// column per work-group measured fastest on MI355X: FarField solves with 32
-// plane waves took 25-40 % less time than with four (more work-groups, fewer
-// rows x columns per thread, and a smaller LDS footprint).
-#ifndef BAEMM_PCG_LDS_CGC_MAX
-#define BAEMM_PCG_LDS_CGC_MAX 1
-#endif
+// One column per work-group: FarField solves with 32 plane waves measured
+// 25-40 % faster on MI355X than with four columns per work-group (more
+// work-groups, fewer rows x columns per thread, a smaller LDS footprint).
 
 struct PcgMatrix {
     int n, nnz, nt, nvt;    // nvt: vertex-to-triangle incidences
@@ -1059,8 +913,6 @@ static void launch_pcg_lds_any(int CGC, int R, Handle& h, int nb, size_t lds, co
         return;                                                                                   \
     }
     BAEMM_PCGL(1, 1) BAEMM_PCGL(1, 2) BAEMM_PCGL(1, 4) BAEMM_PCGL(1, 8) BAEMM_PCGL(1, 16)
-    BAEMM_PCGL(2, 1) BAEMM_PCGL(2, 2) BAEMM_PCGL(2, 4) BAEMM_PCGL(2, 8) BAEMM_PCGL(2, 16)
-    BAEMM_PCGL(4, 1) BAEMM_PCGL(4, 2) BAEMM_PCGL(4, 4) BAEMM_PCGL(4, 8) BAEMM_PCGL(4, 16)
 #undef BAEMM_PCGL
     throw ArgError{"internal: no LDS PCG variant"};
 }
@@ -1084,60 +936,29 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
 {
     const int64_t n = h.nv;
     const int64_t N = n * ncols;
-    const int CGC = ncols >= 64 ? 4 : (ncols >= 16 ? 2 : 1);
-    const int nb = (ncols + CGC - 1) / CGC;
     // LDS block PCG: the CSR form when it fits, else the triangle form (16-bit
-    // vertex and triangle indices); fewer columns per work-group (more
-    // work-groups) when that makes it fit
+    // vertex and triangle indices)
     const int64_t nnz = (int64_t)h.h_M_idx.size(), nvt = (int64_t)h.h_vt_idx.size();
-    bool fit_csr = false, fit_tri = false;
-    int lcgc = BAEMM_PCG_LDS_CGC_MAX < CGC ? BAEMM_PCG_LDS_CGC_MAX : CGC;
-    size_t lds = 0;
-    for (; lcgc >= 1 && !fit_csr && !fit_tri; lcgc /= 2) {
-        const size_t b_csr = pcg_lds_bytes(n, nnz, h.nt, nvt, lcgc, false);
-        const size_t b_tri = pcg_lds_bytes(n, nnz, h.nt, nvt, lcgc, true);
-        fit_csr = n <= 8 * kPcgLdsThreads && b_csr <= kPcgLdsMaxBytes;
-        fit_tri = !fit_csr && n <= 16 * kPcgLdsThreads && h.nt <= 65535 && nvt <= 65535 &&
-                  b_tri <= kPcgLdsMaxBytes;
-        lds = fit_csr ? b_csr : b_tri;
-        if (fit_csr || fit_tri) break;
-    }
-    if (fit_csr || fit_tri || n <= kPcgBlockMaxRows) {
+    const size_t b_csr = pcg_lds_bytes(n, nnz, h.nt, nvt, 1, false);
+    const size_t b_tri = pcg_lds_bytes(n, nnz, h.nt, nvt, 1, true);
+    const bool fit_csr = n <= 8 * kPcgLdsThreads && b_csr <= kPcgLdsMaxBytes;
+    const bool fit_tri = !fit_csr && n <= 16 * kPcgLdsThreads && h.nt <= 65535 && nvt <= 65535 &&
+                         b_tri <= kPcgLdsMaxBytes;
+    if (fit_csr || fit_tri) {
+        // working set on chip: one launch with no global traffic in the loop
         int* ctr = dev_cg_counters(ws, s);
-        if (fit_csr || fit_tri) {
-            // working set on chip: one launch with no global traffic in the loop
-            ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
-            hipLaunchKernelGGL(pcg_reset_kernel, dim3(1), dim3(1), 0, s, ctr);
-            int R = 1;
-            while (R * kPcgLdsThreads < n) R *= 2;
-            const int lnb = (ncols + lcgc - 1) / lcgc;
-            if (fit_csr)
-                launch_pcg_lds_any<false>(lcgc, R, h, lnb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s);
-            else
-                launch_pcg_lds_any<true>(lcgc, R, h, lnb, lds, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s);
-            hipLaunchKernelGGL(pcg_total_kernel, dim3(1), dim3(1), 0, s, ctr);
-            BAEMM_HIP(hipGetLastError());
-            ws.cg_block = true;
-            return;
-        }
-        ws.cg.reserve(sizeof(double) * 3 * n * (int64_t)CGC * nb);
         ws.scal.reserve(sizeof(double) * 8 * ncols + sizeof(int) * ncols);
-        // the convergence flags read by dev_mass_cg_result: all columns converged
-        // or stopped at max_iter; the block version reports via the iteration count
         hipLaunchKernelGGL(pcg_reset_kernel, dim3(1), dim3(1), 0, s, ctr);
-#define BAEMM_PCGB(C_)                                                                                          \
-    hipLaunchKernelGGL(pcg_block_kernel<C_>, dim3(nb), dim3(256), 0, s, (int)n, h.d_M_ptr, h.d_M_idx, h.d_M_val,   \
-                       h.d_lumped_inv, X, ldX, Y, ldY, ncols, tol * tol, max_iter, ws.cg.as<double>(), ctr)
-        if (CGC == 4) BAEMM_PCGB(4);
-        else if (CGC == 2) BAEMM_PCGB(2);
-        else BAEMM_PCGB(1);
-#undef BAEMM_PCGB
+        int R = 1;
+        while (R * kPcgLdsThreads < n) R *= 2;
+        if (fit_csr)
+            launch_pcg_lds_any<false>(1, R, h, ncols, b_csr, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s);
+        else
+            launch_pcg_lds_any<true>(1, R, h, ncols, b_tri, X, ldX, Y, ldY, ncols, tol, max_iter, ctr, s);
         hipLaunchKernelGGL(pcg_total_kernel, dim3(1), dim3(1), 0, s, ctr);
         BAEMM_HIP(hipGetLastError());
-        ws.cg_block = true;
         return;
     }
-    ws.cg_block = false;
     ws.cg.reserve(sizeof(double) * (4 * N));
     // partial-sum blocks of the SpMV and update passes: about 16 rows per block
     // (4 per wave), so the row loops stay short; the column sums read them with
@@ -1203,27 +1024,13 @@ int* dev_cg_counters(Workspace& ws, hipStream_t s)
     return ws.cgctr.as<int>();
 }
 
-// Reads the iteration count and the convergence of the last dev_mass_cg
-// (synchronises the stream).
-int dev_mass_cg_result(Workspace& ws, int ncols, bool* converged, hipStream_t s)
+// Iterations of the last dev_mass_cg (the largest over its columns;
+// synchronises the stream).
+int dev_mass_cg_result(Workspace& ws, hipStream_t s)
 {
-    if (ws.cg_block) {
-        int ctr[4] = {0, 0, 0, 0};
-        BAEMM_HIP(hipMemcpyAsync(ctr, ws.cgctr.ptr, sizeof(ctr), hipMemcpyDeviceToHost, s));
-        BAEMM_HIP(hipStreamSynchronize(s));
-        if (converged) *converged = ctr[1] < 20;
-        return ctr[1];
-    }
-    const double* bb = ws.scal.as<double>() + 6 * ncols;
-    const int* active = reinterpret_cast<const int*>(bb + 2 * ncols);
-    std::vector<int> a(ncols);
     int ctr[4] = {0, 0, 0, 0};
-    BAEMM_HIP(hipMemcpyAsync(a.data(), active, sizeof(int) * ncols, hipMemcpyDeviceToHost, s));
     BAEMM_HIP(hipMemcpyAsync(ctr, ws.cgctr.ptr, sizeof(ctr), hipMemcpyDeviceToHost, s));
     BAEMM_HIP(hipStreamSynchronize(s));
-    bool conv = true;
-    for (int c = 0; c < ncols; ++c) conv = conv && !a[c];
-    if (converged) *converged = conv;
     return ctr[1];
 }
 

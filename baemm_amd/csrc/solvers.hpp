@@ -10,15 +10,6 @@
 namespace baemm {
 
 constexpr int kRedBlocks = 240;   // blocks of the first pass of column reductions
-// Meshes up to this many vertices run the one-launch block PCG, larger ones the
-// multi-block PCG (a build-time setting: -DBAEMM_PCG_BLOCK_MAX_ROWS=n).  The
-// block PCG runs one work-group per group of up to four columns, so it uses few
-// CUs; FarField solves measured on MI355X put the crossover between 640 and
-// 1,200 vertices.
-#ifndef BAEMM_PCG_BLOCK_MAX_ROWS
-#define BAEMM_PCG_BLOCK_MAX_ROWS 1024
-#endif
-constexpr int kPcgBlockMaxRows = BAEMM_PCG_BLOCK_MAX_ROWS;
 
 void dev_mass(Handle& h, double alpha, const double* X, int64_t ldX, double beta, double* Y, int64_t ldY, int ncols,
               hipStream_t s, const int* skip = nullptr);
@@ -42,7 +33,7 @@ void dev_inv_norm(int W, const double2* hn, double* sc, hipStream_t s);
 void dev_sub_c(int64_t N, const double2* b, const double2* t, double2* r, hipStream_t s);
 void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
                  int max_iter, hipStream_t s);
-int dev_mass_cg_result(Workspace& ws, int ncols, bool* converged, hipStream_t s);
+int dev_mass_cg_result(Workspace& ws, hipStream_t s);
 int* dev_cg_counters(Workspace& ws, hipStream_t s);
 int64_t dev_cg_total(Workspace& ws, hipStream_t s);
 

@@ -68,7 +68,7 @@ std::vector<char> graph_key(Handle& h, int j, double cg_tol)
     int rmin_bits = 0;
     std::memcpy(&rmin_bits, &h.rmin, sizeof(float));
     const int64_t v[] = {j, h.nv, h.prm.W, h.prm.wcs, h.prm.K, h.precision, h.use_diagonal ? 1 : 0, h.segments,
-                         h.gmres_max_iter, rmin_bits};
+                         h.gmres_max_iter, rmin_bits, h.mixed_kernel};
     for (size_t i = 0; i < sizeof(v) / sizeof(v[0]); ++i) k.val[i] = v[i];
     k.tol = cg_tol;
     std::vector<char> key(reinterpret_cast<const char*>(&k), reinterpret_cast<const char*>(&k) + sizeof(k));
@@ -379,7 +379,7 @@ void mass_inverse_dev(Handle& h, const double* X, int64_t ldX, double* Y, int64_
                       hipStream_t s)
 {
     dev_mass_cg(h, h.sws, X, ldX, Y, ldY, ncols, tol, 20, s);
-    h.stats.cg_iterations += dev_mass_cg_result(h.sws, ncols, nullptr, s);
+    h.stats.cg_iterations += dev_mass_cg_result(h.sws, s);
 }
 
 // DotWithNormals_PL (FarField.hpp:956-1022): Xn (device, V) = M^{-1} AvT ((n . Av X)/a).
@@ -520,7 +520,7 @@ int baemm_apply_mass_inverse(baemm_handle h, const double* X, int64_t ldX, doubl
         BAEMM_HIP(copy_rows_async(dx.p, sizeof(double) * nrhs, X, sizeof(double) * ldX, sizeof(double) * nrhs, hh.nv,
                                    hipMemcpyHostToDevice, s));
         dev_mass_cg(hh, hh.sws, dx.as<double>(), nrhs, dy.as<double>(), nrhs, (int)nrhs, cg_tol, 20, s);
-        hh.stats.cg_iterations += dev_mass_cg_result(hh.sws, (int)nrhs, nullptr, s);
+        hh.stats.cg_iterations += dev_mass_cg_result(hh.sws, s);
         BAEMM_HIP(copy_rows_async(Y, sizeof(double) * ldY, dy.p, sizeof(double) * nrhs, sizeof(double) * nrhs, hh.nv,
                                    hipMemcpyDeviceToHost, s));
         BAEMM_HIP(hipStreamSynchronize(s));

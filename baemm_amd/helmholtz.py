@@ -461,7 +461,9 @@ class _SolverMixin:
             return _lib.REAL_OP(cb)
 
         Mc, Pc = wrap(M), wrap(P)
-        Bv = np.require(B, dtype=np.float64, requirements=["C"])
+        Bv = B
+        if not (isinstance(B, np.ndarray) and B.dtype == np.float64 and B.ndim == 2 and B.strides[1] == 8):
+            Bv = np.require(B, dtype=np.float64, requirements=["C"])   # keeps the row stride (ldB) otherwise
         pp, keep = self._du_arg(du_dn)
         ok = ctypes.c_int(0)
         st = self._L.baemm_gauss_newton_solve(self._h, _ptr(k), int(wave_chunk_count), _ptr(d), int(wave_chunk_size),

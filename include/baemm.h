@@ -403,6 +403,14 @@ int64_t baemm_server_request_bytes(const baemm_gn_request* request);
 int baemm_set_timing(baemm_handle h, int on);
 int baemm_get_timing(baemm_handle h, double* total_ms, int64_t* launches);
 
+/* Diagnostic, no reference counterpart: the hardware sine and cosine
+ * (v_sin_f32 / v_cos_f32, argument in revolutions: sin(2 pi rev)) that every
+ * pair kernel uses for e^{i kappa r}, evaluated on `device` for rev[0..n).
+ * The pair kernels accept phases kappa r / (2 pi) up to 65,536 revolutions
+ * (baemm_load_parameters rejects kappa * mesh extent beyond that); this entry
+ * point lets a test confirm the hardware's accuracy over that range. */
+int baemm_hw_sincos(int device, const float* rev, int64_t n, float* sin_out, float* cos_out);
+
 /* Device memory helpers for the *_dev entry points. */
 int baemm_dev_alloc(void** ptr, size_t bytes);
 int baemm_dev_free(void* ptr);

@@ -44,11 +44,14 @@ PINS = {
     "mass / Av / AvT": "1^T M 1 = sum a, M 1 = AvT 1 (test_oracle_geometry)",
     "bdr_f64": "sphere spectra of S, K, K'; A_S^T = A_S; A_K^T = A_K'; Gauss law (test_oracle_operators)",
     "bdr_cl_f32": "agrees with bdr_f64 to FP32 accuracy (test_oracle_operators)",
+    "far_cl_f32": "agrees with far_f64 to FP32 accuracy in every Re/Im branch (test_oracle_operators)",
     "far_f64": "Funk-Hecke closed form, far field of the SL of a plane-wave density (test_oracle_operators)",
     "herglotz_f64": "Funk-Hecke / j0 closed form (test_oracle_operators)",
     "near_f64": "limit of the near field equals the far field; SL of the sphere exterior (test_oracle_operators)",
     "incident": "plane-wave closed form (test_oracle_operators)",
     "solves": "dense LU; Mie series for the sound-soft sphere; adjoint dot-product test (test_oracle_solvers)",
+    "gauss_newton": "its dense operator reproduces the pinned derivative_ff / adjoint_derivative_ff compositions "
+                    "column by column; the solution satisfies the normal equations (test_oracle_solvers)",
 }
 
 
@@ -456,6 +459,22 @@ class F32Chain:
                 C[:, sl] = alpha * np.complex128(c32[q, 0]) * MB[:, sl] + add_to * C[:, sl]
         return C
 
+    def far_field(self, alpha, B, beta, F, kappa, coeff, W, wcs):
+        """FP32 restatement of ApplyFarFieldOperators_PL (ApplyFarFieldOperators.hpp:
+        49-104): b = Av B in complex64, FarFieldOperatorKernel.cl arithmetic
+        (orc_far_cl_f32, coefficient columns 1 and 2 only), then alpha/beta."""
+        kap32, c32, bits, fl = self.params32(kappa, coeff, W, wcs)
+        F = np.array(F, dtype=np.complex128, copy=True)
+        bits &= 15
+        if not bits:
+            return F   # no single or double layer: F_out untouched
+        b32 = np.ascontiguousarray((self.Av @ _c128(B)).astype(np.complex64))
+        m = self.dirs4.shape[0]
+        out = np.zeros((m, int(W)), np.complex64)
+        lib().orc_far_cl_f32(self.mesh.nt, _p(self.mid4), _p(self.nrm4), m, _p(self.dirs4), int(W), int(wcs),
+                             _p(kap32), _p(c32), int(bits), _p(b32), _p(out))
+        return alpha * out.astype(np.complex128) + beta * F
+
 
 # --------------------------------------------------------------------------
 # Dense operator assembly and solver compositions (FarField.hpp), FP64
@@ -576,6 +595,77 @@ def adjoint_derivative_ff(mesh: Mesh, kappa, dirs, wcs, g, du_dn=None, wave_type
     dv_dn = dtn(mesh, kappa, herg, wcs)
     prod = np.sum(du_dn * dv_dn, axis=1)
     return multiply_with_normals(mesh, prod, -1.0 / W), du_dn
+
+
+def gauss_newton_operator(mesh: Mesh, kappa, dirs, wcs, metric, du_dn=None, wave_type="plane"):
+    """Dense matrix of the outer GMRES operator of GaussNewtonSolve
+    (FarField.hpp:551-572): A x = metric x + (Mass (x) I_3) DF* DF x on vec(X),
+    X V x 3 row-major (entry 3 v + d).  ``metric`` is a dense (3V x 3V) matrix
+    (the caller's M).  Returns (A, du_dn).
+
+    DF and DF* are derivative_ff and adjoint_derivative_ff with each of their
+    linear stages written as a dense matrix (the boundary operators factorised
+    once per chunk, exact mass solves), so A is built for all 3V unit vectors
+    at once:
+      DF:  xn = N vec(X) (DotWithNormals), Y[:, c] = -FF_q A_q^{-1} M (xn * du[:, c])
+      DF*: dv[:, c] = D_q^{-1} H_q conj(g[:, c]), p = sum_c du[:, c] dv[:, c],
+           vec(Y) = R Re(p) (MultiplyWithNormals with factor -1/W)
+    with A_q the potential operator (1/2, -i kappa, 1, 0), D_q the DtN operator
+    (1/2, -i kappa, 0, 1), FF_q the far field and H_q the Herglotz wave
+    (0, -i kappa, 1, 0) of chunk q.
+    """
+    K = len(kappa)
+    W = K * wcs
+    nv, nt = mesh.nv, mesh.nt
+    m = mesh.meas.shape[0]
+    if du_dn is None:
+        du_dn = derivative_ff_helper(mesh, kappa, dirs, wcs, wave_type)
+    Avd = mesh.Av.toarray()
+    # N: vec(X) -> M^{-1} AvT ((n_t . (Av X)_t) / a_t)
+    T1 = np.zeros((nt, nv, 3))
+    for d in range(3):
+        T1[:, :, d] = Avd * (mesh.nrm[:, d] / mesh.area)[:, None]
+    N = mesh.mass_solve(Avd.T @ T1.reshape(nt, 3 * nv))
+    # R: p (nv) -> vec(-1/W M^{-1} AvT ((Av p) / a_t n_t))
+    R = np.zeros((nv, 3, nv))
+    for d in range(3):
+        R[:, d, :] = mesh.mass_solve(Avd.T @ (Avd * (mesh.nrm[:, d] / mesh.area)[:, None])) * (-1.0 / W)
+    R = R.reshape(3 * nv, nv)
+    Mfull = mesh.M.toarray()
+    DFx = np.zeros((m, W, 3 * nv), np.complex128)
+    P = np.zeros((nv, 3 * nv), np.complex128)
+    for q in range(K):
+        kq = kappa[q]
+        pot = np.array([0.5, -1j * kq, 1.0, 0.0])
+        inc = np.array([0.0, -1j * kq, 1.0, 0.0])
+        kp, cp = load_parameters([kq], pot, 1, 1)
+        Aq = dense_operator(mesh, kp[0], cp[0])
+        kd, cd = load_parameters([kq], [0.5, -1j * kq, 0.0, 1.0], 1, 1)
+        Dq = dense_operator(mesh, kd[0], cd[0])
+        FFq = apply_far_field(mesh, 1.0, np.eye(nv), 0.0, np.zeros((m, nv)), [kq], pot, nv, nv)
+        Hq = create_herglotz_wave(mesh, 1.0, np.eye(m), 0.0, np.zeros((nv, m)), [kq], inc, m, m)
+        G = FFq @ np.linalg.solve(Aq, Mfull)            # m x nv
+        L = np.linalg.solve(Dq, Hq)                      # nv x m
+        for c in range(q * wcs, (q + 1) * wcs):
+            DFx[:, c, :] = -G @ (du_dn[:, c][:, None] * N)
+            P += du_dn[:, c][:, None] * (L @ np.conj(DFx[:, c, :]))
+    Y = R @ P.real                                        # DF* DF on unit vectors, 3V x 3V
+    MY = (Mfull @ Y.reshape(nv, 3, 3 * nv).reshape(nv, 9 * nv)).reshape(nv, 3, 3 * nv).reshape(3 * nv, 3 * nv)
+    return np.asarray(metric, dtype=np.float64) + MY, du_dn
+
+
+def gauss_newton(mesh: Mesh, kappa, dirs, wcs, metric, B, alpha=1.0, beta=0.0, X=None, du_dn=None,
+                 wave_type="plane"):
+    """GaussNewtonSolve (FarField.hpp:457-604) in FP64 with an exact solve:
+    X_out = alpha A^{-1} vec(B) + beta X_out (the combine_matrices step of
+    :581-599) with A from gauss_newton_operator.  B, X: V x 3 real.
+    Returns (X_out, du_dn)."""
+    A, du_dn = gauss_newton_operator(mesh, kappa, dirs, wcs, metric, du_dn, wave_type)
+    x = np.linalg.solve(A, np.asarray(B, dtype=np.float64).reshape(-1)).reshape(mesh.nv, 3)
+    out = alpha * x
+    if X is not None and beta != 0.0:
+        out = out + beta * np.asarray(X, dtype=np.float64)
+    return out, du_dn
 
 
 def gmres_dense(A, b, tol=1e-10):

@@ -51,3 +51,14 @@ def relerr(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@functools.lru_cache(maxsize=None)
+def reference_mesh(name):
+    """A mesh of the reference's Meshes/ directory, from its committed fixture
+    (tests/golden/meshes/<name>.npz, written by tests/golden/make_mesh_fixtures.py)."""
+    with np.load(GOLDEN / "meshes" / f"{name}.npz") as z:
+        return z["V"].astype(np.float64), z["T"].astype(np.int64)

@@ -3,6 +3,8 @@ against the FP64 oracle and the FP32 restatement of the reference chain.
 
 Tolerance: 1e-5 relative (Frobenius) in both precisions, the bar the reference's
 FP32 apply sets; sampled vertex rows at the 81,920-triangle size."""
+import functools
+
 import numpy as np
 import pytest
 
@@ -133,26 +135,39 @@ def test_nonspherical_meshes(mode):
         assert relerr(C, ref) < TOL
 
 
-@pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
-def test_c2_size_sampled_rows(mode):
-    """Config 2 size: 81,920 triangles, kappa in {8, 16}, 2 x 32 RHS, SL+DL with
-    coefficients (0,1,1,0) and (1/2, -i kappa, 1, 0); 48 sampled vertex rows
-    against the FP64 oracle, plus a full-size bilinear symmetry check."""
+@functools.lru_cache(maxsize=None)
+def _c2_case():
     V, T = icosphere(6)
     m = oracle.Mesh(V, T)
     rng = np.random.default_rng(1)
     W, wcs = 64, 32
     B = _rand(rng, m.nv, W)
     coeff = np.array([0, 1, 1, 0, 0.5, -16j, 1, 0], np.complex128)
-    rows = np.sort(rng.choice(m.nv, 48, replace=False))
-    ref = oracle.apply_boundary_rows(m, 1.0, B, 0.0, np.zeros((48, W)), [8.0, 16.0], coeff, W, wcs, rows)
-    with Helmholtz_HIP(V, T) as H:
+    rows = np.sort(rng.choice(m.nv, 1024, replace=False))
+    ref = oracle.apply_boundary_rows(m, 1.0, B, 0.0, np.zeros((1024, W)), [8.0, 16.0], coeff, W, wcs, rows)
+    return m, B, coeff, rows, ref
+
+
+# achieved C2 errors vs FP64 (1,024 rows): FP32 about 4e-7, mixed 7e-7, FP32 MFMA 2e-7
+C2_BOUND = {"fp32": 2e-6, "mixed": 3e-6, "fp32_mfma": 2e-6}
+
+
+@pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
+def test_c2_size_sampled_rows(mode):
+    """Config 2 size: 81,920 triangles, kappa in {8, 16}, 2 x 32 RHS, SL+DL with
+    coefficients (0,1,1,0) and (1/2, -i kappa, 1, 0); 1,024 sampled vertex rows
+    against the FP64 oracle (contract 1e-5, and a bound near the achieved
+    error per precision), plus a full-size bilinear symmetry check."""
+    m, B, coeff, rows, ref = _c2_case()
+    W, wcs = 64, 32
+    rng = np.random.default_rng(2)
+    with Helmholtz_HIP(m.V, m.T) as H:
         H.SetPrecision(mode)
         C = np.zeros_like(B)
         H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [8.0, 16.0], coeff, W, wcs)
         e = relerr(C[rows], ref)
         print(f"C2 {mode}: sampled-row rel. error vs FP64 = {e:.3e}")
-        assert e < TOL
+        assert e < TOL and e < C2_BOUND[mode]
         # symmetry of the single layer at full size: v^T S u = u^T S v
         u = _rand(rng, m.nv, 1)
         v = _rand(rng, m.nv, 1)

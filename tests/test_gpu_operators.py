@@ -7,6 +7,7 @@ import pytest
 
 import oracle
 from baemm_amd import Helmholtz_HIP, WaveType
+from baemm_amd.meshes import icosphere
 from conftest import oracle_mesh, relerr
 
 pytestmark = pytest.mark.gpu
@@ -167,3 +168,50 @@ def test_mass_matrix_and_geometry(mesh_h):
     Y = np.ones_like(X)
     H.MassDot(2.0, X, 1.0, Y)
     assert relerr(Y, 2.0 * (m.M @ X) + 1.0) < 1e-6
+
+
+def test_hardware_sine_accuracy_over_the_accepted_phase_range():
+    """The pair kernels take e^{i kappa r} from the hardware sine and cosine
+    (argument in revolutions) without a range reduction of their own.  Over the
+    phases the library accepts (|kappa r / 2 pi| <= 65,536 revolutions; beyond
+    that baemm_load_parameters refuses kappa) they stay within 5e-7 of
+    sin/cos(2 pi t) evaluated in FP64 at the same FP32 argument t."""
+    import ctypes
+
+    from baemm_amd import _lib
+
+    rng = np.random.default_rng(12)
+    t = np.concatenate([
+        rng.uniform(-1.0, 1.0, 20000),
+        rng.uniform(-300.0, 300.0, 20000),
+        rng.uniform(-65536.0, 65536.0, 40000),
+        np.arange(-65536, 65537, 97, dtype=np.float64) + 0.125,
+        [65535.75, -65535.75, 0.25, 0.5, 0.75, 1e-8],
+    ]).astype(np.float32)
+    s = np.zeros_like(t)
+    c = np.zeros_like(t)
+    L = _lib.load()
+    st = L.baemm_hw_sincos(0, t.ctypes.data_as(ctypes.c_void_p), t.size, s.ctypes.data_as(ctypes.c_void_p),
+                           c.ctypes.data_as(ctypes.c_void_p))
+    assert st == 0
+    t64 = t.astype(np.float64)
+    es = np.abs(s - np.sin(2 * np.pi * t64))
+    ec = np.abs(c - np.cos(2 * np.pi * t64))
+    for lo, hi in ((0, 1), (1, 256), (256, 65537)):
+        sel = (np.abs(t64) >= lo) & (np.abs(t64) < hi)
+        print(f"|t| in [{lo}, {hi}): max error sin {es[sel].max():.2e} cos {ec[sel].max():.2e}")
+    assert es.max() < 5e-7 and ec.max() < 5e-7
+
+
+def test_wave_numbers_beyond_the_phase_range_are_refused():
+    """kappa * mesh extent / (2 pi) above 65,536 revolutions is an argument
+    error (the FP32 phase would resolve less than 2^-8 revolution there)."""
+    from baemm_amd import BaemmError
+
+    V, T = icosphere(1)
+    with Helmholtz_HIP(V, T) as H:
+        H.LoadBoundaryOperators_PL([1.0e5], [0, 1, 0, 0], 1, 1)   # extent 2 sqrt(3): about 55,000 revolutions
+        with pytest.raises(BaemmError):
+            H.LoadBoundaryOperators_PL([1.0e6], [0, 1, 0, 0], 1, 1)
+        with pytest.raises(BaemmError):
+            H.LoadBoundaryOperators_PL([float("nan")], [0, 1, 0, 0], 1, 1)

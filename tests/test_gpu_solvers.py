@@ -99,6 +99,48 @@ def test_gauss_newton_solves_its_equation(small):
     assert np.linalg.norm(res) < 1e-4 * np.linalg.norm(B)
 
 
+def test_gauss_newton_matches_dense_fp64_oracle():
+    """GaussNewtonSolve (FarField.hpp:457-604) against the oracle's dense FP64
+    solve of (M_user + Mass DF* DF) X = B, with DF and DF* the oracle's own
+    compositions: two wave numbers, a metric c I + d Mass (x) I_3, B and X in
+    scattered storage (ldB = 5, ldX = 4, FarField.hpp:516-536,581-599), and
+    alpha/beta in the combine step; columns 3.. of X are left untouched."""
+    m = oracle_mesh("sphere", 2, 64)
+    kappa, wcs = [2.0, 3.5], 2
+    dirs = DIRS[:2]
+    c, d = 0.05, 0.5
+    metric = c * np.eye(3 * m.nv) + d * np.kron(m.M.toarray(), np.eye(3))
+    rng = np.random.default_rng(8)
+    B = rng.standard_normal((m.nv, 3))
+    X0 = rng.standard_normal((m.nv, 3))
+    alpha, beta = 0.5, 2.0
+    ref, du_ref = oracle.gauss_newton(m, kappa, dirs, wcs, metric, B, alpha, beta, X0)
+    lumped = np.asarray(m.M.sum(axis=1)).ravel()
+    with Helmholtz_HIP(m.V, m.T, m.meas) as H:
+        H.SetGmresLimits(60, 4)
+
+        def Mop(x, y):
+            y[:] = c * x + d * (m.M @ x)
+
+        def Pop(x, y):
+            y[:] = x / (c + d * lumped)[:, None]
+
+        Bw = np.zeros((m.nv, 5))
+        Bw[:, :3] = B
+        Xw = np.full((m.nv, 4), 7.0)
+        Xw[:, :3] = X0
+        for mode in ("fp32", "mixed"):
+            H.SetPrecision(mode)
+            Xw[:, :3] = X0
+            ok, du = H.GaussNewtonSolve(kappa, 2, dirs, wcs, Mop, Pop, alpha, Bw[:, :3], beta, Xw[:, :3],
+                                        du_dn=None, cg_tol=1e-12, gmres_tol_inner=1e-10, gmres_tol_outer=1e-9)
+            assert ok
+            e_du, e_x = relerr(du, du_ref), relerr(Xw[:, :3], ref)
+            print(f"Gauss-Newton {mode}: du_dn rel. err {e_du:.2e}, X rel. err {e_x:.2e}", H.SolverStats())
+            assert e_du < 2e-5 and e_x < 1e-4, mode
+            assert np.all(Xw[:, 3] == 7.0)
+
+
 def test_far_field_mie_refinement():
     """Sound-soft unit sphere: far field error vs the Mie series shrinks under refinement."""
     errs = []
@@ -160,13 +202,19 @@ def test_graph_captured_arnoldi_steps_match_eager(small):
     """GMRES with each Arnoldi step replayed from a captured hipGraph gives the
     eager result exactly, on every replay (eager work and synchronisation in
     between), in every precision.  The small sphere (162 vertices) runs the
-    one-launch block PCG, the icosphere with 2,562 vertices the multi-block PCG
-    (device-side convergence flag, launches that return at once)."""
+    one-launch LDS-resident PCG; the torus with 3,000 vertices (its mass matrix
+    needs 162 KB in the triangle form, more than the 150 KB the LDS PCG takes)
+    runs the multi-block PCG: device-side convergence flag, ticket and
+    active-column counters reset every iteration, double-buffered directions,
+    launches that return at once after convergence."""
+    from baemm_amd.meshes import torus
+
     m, H = small
     _graph_vs_eager(H, m.nv)
-    V, T = icosphere(4)
-    with Helmholtz_HIP(V, T) as H4:
-        _graph_vs_eager(H4, V.shape[0])
+    V, T = torus(1.0, 0.4, 100, 30)
+    assert V.shape[0] == 3000
+    with Helmholtz_HIP(V, T) as Ht:
+        _graph_vs_eager(Ht, V.shape[0])
 
 
 def _graph_vs_eager(H, nv):

@@ -187,3 +187,28 @@ def test_f32_restatement_matches_f64():
     ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kap, coeff, W, wcs)
     got = ch.apply_boundary(1.0, B, 0.0, np.zeros_like(B), kap, coeff, W, wcs)
     assert relerr(got, ref) < 2e-6
+
+
+@pytest.mark.parametrize("c1,c2", [(1.0, 0.0), (1j, 0.0), (1 - 0.5j, 0.0), (0.0, 1.0), (0.0, 1j),
+                                   (0.0, -0.75 + 1j), (1 - 0.5j, -0.75 + 1j)])
+def test_far_field_f32_restatement_matches_f64(c1, c2):
+    """The FP32 restatement of FarFieldOperatorKernel.cl (orc_far_cl_f32)
+    agrees with the FP64 far field, itself pinned by Funk-Hecke above, to FP32
+    accuracy in each of its Re/Im single- and double-layer branches (a dropped
+    term or a wrong sign in any branch moves the result by O(1)); alpha/beta
+    and the untouched output of an all-zero coefficient row included."""
+    m = oracle_mesh("sphere", 3, 96)
+    ch = oracle.F32Chain(m)
+    rng = np.random.default_rng(5)
+    W, wcs = 6, 3
+    B = rng.standard_normal((m.nv, W)) + 1j * rng.standard_normal((m.nv, W))
+    F0 = rng.standard_normal((96, W)) + 1j * rng.standard_normal((96, W))
+    coeff = np.array([0.3, c1, c2, 0.7j, 0.0, 0.5 * c1, 2.0 * c2, 0.0], np.complex128)
+    kap = [3.0, 5.5]
+    alpha, beta = 0.7 - 0.2j, 0.5 + 1j
+    ref = oracle.apply_far_field(m, alpha, B, beta, F0, kap, coeff, W, wcs)
+    got = ch.far_field(alpha, B, beta, F0, kap, coeff, W, wcs)
+    assert relerr(got, ref) < 2e-6
+    # only the rows that differ from beta F0 carry the operator
+    assert relerr(got - beta * F0, ref - beta * F0) < 5e-6
+    np.testing.assert_array_equal(ch.far_field(1.0, B, 0.5, F0, kap, [1, 0, 0, 1j] * 2, W, wcs), F0)

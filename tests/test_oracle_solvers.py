@@ -72,3 +72,57 @@ def test_dense_solve_is_a_solution():
     X = oracle.solve_boundary(m, [3.0], [np.array([0.5, -3j, 1.0, 0.0])], rhs, 3)
     back = oracle.apply_boundary(m, 1.0, X, 0.0, np.zeros_like(X), [3.0], [0.5, -3j, 1.0, 0.0], 3, 3)
     assert np.linalg.norm(back - rhs) < 1e-10 * np.linalg.norm(rhs)
+
+
+def _gn_metric(m, c=0.05, d=0.5):
+    """A stand-in SPD metric on vec(X) (V x 3 row-major): c I + d (Mass (x) I_3)."""
+    Mf = m.M.toarray()
+    return c * np.eye(3 * m.nv) + d * np.kron(Mf, np.eye(3))
+
+
+def test_gauss_newton_operator_reproduces_the_pinned_compositions():
+    """Every column family of the dense Gauss-Newton operator equals
+    metric x + Mass DF*(DF x) evaluated with the pinned derivative_ff and
+    adjoint_derivative_ff (FarField.hpp:551-572) on random X; the solution of
+    gauss_newton then satisfies the normal equations through those same
+    functions, with alpha/beta applied as in FarField.hpp:581-599."""
+    m = oracle_mesh("sphere", 2, 48)
+    kappa, wcs = [2.0, 3.5], 2
+    dirs = np.array([[0.0, 0.0, 1.0], [0.6, 0.0, 0.8]])
+    metric = _gn_metric(m)
+    A, du = oracle.gauss_newton_operator(m, kappa, dirs, wcs, metric)
+    rng = np.random.default_rng(3)
+    for trial in range(3):
+        X = rng.standard_normal((m.nv, 3))
+        DFX, _ = oracle.derivative_ff(m, kappa, dirs, wcs, X, du)
+        Z, _ = oracle.adjoint_derivative_ff(m, kappa, dirs, wcs, DFX, du)
+        ref = (metric @ X.reshape(-1)).reshape(m.nv, 3) + m.M @ Z
+        assert relerr((A @ X.reshape(-1)).reshape(m.nv, 3), ref) < 1e-10
+    B = rng.standard_normal((m.nv, 3))
+    X0 = rng.standard_normal((m.nv, 3))
+    X, _ = oracle.gauss_newton(m, kappa, dirs, wcs, metric, B, alpha=0.5, beta=2.0, X=X0, du_dn=du)
+    x = (X - 2.0 * X0) / 0.5
+    DFx, _ = oracle.derivative_ff(m, kappa, dirs, wcs, x, du)
+    Zx, _ = oracle.adjoint_derivative_ff(m, kappa, dirs, wcs, DFx, du)
+    res = (metric @ x.reshape(-1)).reshape(m.nv, 3) + m.M @ Zx - B
+    assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(B)
+
+
+def test_gauss_newton_data_term_is_symmetric_positive():
+    """Mass DF* DF is (up to discretisation) the Gauss-Newton Hessian of the
+    misfit: x^T Mass DF* DF x = (4 pi / (m W)) |DF x|^2 >= 0 in the limit, so
+    the data term of the operator is nearly symmetric and its quadratic form
+    positive; a wrong sign or a dropped conjugation in DF* breaks both."""
+    m = oracle_mesh("sphere", 3, 128)
+    kappa, wcs = [2.0], 2
+    dirs = np.array([[0.0, 0.0, 1.0], [0.6, 0.0, 0.8]])
+    A, du = oracle.gauss_newton_operator(m, kappa, dirs, wcs, np.zeros((3 * m.nv, 3 * m.nv)))
+    asym = np.linalg.norm(A - A.T) / np.linalg.norm(A)
+    assert asym < 0.05
+    rng = np.random.default_rng(4)
+    for _ in range(4):
+        x = rng.standard_normal(3 * m.nv)
+        DFx, _ = oracle.derivative_ff(m, kappa, dirs, wcs, x.reshape(m.nv, 3), du)
+        quad = x @ A @ x
+        data = 4 * np.pi / (128 * wcs) * np.sum(np.abs(DFx) ** 2)
+        assert quad > 0 and abs(quad / data - 1) < 0.05
