@@ -47,6 +47,14 @@ SIGNATURES = {
     "baemm_set_graphs": (_INT, [_P, _INT]),
     "baemm_get_graph_stats": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "baemm_get_triangle_data": (_INT, [_P, _P, _P, _P, _P]),
+    "baemm_get_coordinates": (_INT, [_P, _P, _P]),
+    "baemm_device_info": (_INT, [_P, ctypes.c_char_p, _I64]),
+    "baemm_get_flags": (_INT, [_P, _P]),
+    "baemm_read_b": (_INT, [_P, _P, _I64, _I64]),
+    "baemm_write_b": (_INT, [_P, _P, _I64]),
+    "baemm_read_c": (_INT, [_P, _P, _I64, _I64]),
+    "baemm_write_c": (_INT, [_P, _P, _I64]),
+    "baemm_p0_buffer": (_INT, [_P, _INT, ctypes.POINTER(ctypes.POINTER(ctypes.c_float)), ctypes.POINTER(_I64)]),
     "baemm_mass_nnz": (_I64, [_P]),
     "baemm_get_mass_matrix": (_INT, [_P, _P, _P, _P]),
     "baemm_mass_dot": (_INT, [_P, _D, _P, _I64, _D, _P, _I64, _I64]),

@@ -238,6 +238,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
     for (int k = 0; k < n; ++k) {
         hipStream_t sk = es[k].h->stream;
         BAEMM_HIP(hipStreamWaitEvent(sk, bw.sync[0], 0));
+        p0_invalidate(*es[k].h);   // the batched launch leaves no P0 buffers
         bdr_begin(*es[k].h, es[k].a, prep[k].pl, sk);
         BAEMM_HIP(hipEventRecord(bw.sync[2 + k], sk));
     }

@@ -1,5 +1,6 @@
 // capi.cpp -- extern "C" entry points of libbaemm_hip (declared in include/baemm.h).
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -507,6 +508,123 @@ int baemm_apply_far_field_dev(baemm_handle h, const double* alpha, const void* B
         check_ld(ldF, hh.prm.W, "F");
         ApplyArgs a{cplx(alpha, 1.0), cplx(beta, 0.0), B, ldB, B_type, F, ldF, F_type};
         apply_far_dev(hh, a, stream_of(hh, stream));
+        return BAEMM_OK;
+    });
+}
+
+// ---------------------------------------------------------------- P0 buffers
+
+static int p0_read(baemm_handle h, int which, const float* input, int64_t ld_input, int64_t wave_count)
+{
+    return guard(h, [&](Handle& hh) {
+        if (!input || wave_count < 1 || ld_input < wave_count) throw ArgError{"invalid input, ld or wave_count"};
+        P0State& st = hh.p0;
+        std::vector<cfloat>& buf = st.host[which];
+        buf.resize((size_t)(hh.nt * wave_count));
+        const cfloat* in = reinterpret_cast<const cfloat*>(input);
+        for (int64_t i = 0; i < hh.nt; ++i)
+            std::copy(in + i * ld_input, in + i * ld_input + wave_count, buf.begin() + i * wave_count);
+        st.W[which] = wave_count;
+        st.host_current[which] = true;
+        return BAEMM_OK;
+    });
+}
+
+static int p0_write(baemm_handle h, int which, float* output, int64_t ld_output)
+{
+    return guard(h, [&](Handle& hh) {
+        if (!p0_sync(hh, which, hh.stream))
+            throw StateError{which == 0 ? "no B buffer: no operator apply yet"
+                                        : "no C buffer: no boundary-operator apply since the last other operator"};
+        const P0State& st = hh.p0;
+        const int64_t W = st.W[which];
+        if (!output || ld_output < W) throw ArgError{"invalid output or ld smaller than the wave count"};
+        cfloat* out = reinterpret_cast<cfloat*>(output);
+        for (int64_t i = 0; i < hh.nt; ++i)
+            std::copy(st.host[which].begin() + i * W, st.host[which].begin() + (i + 1) * W, out + i * ld_output);
+        return BAEMM_OK;
+    });
+}
+
+int baemm_read_b(baemm_handle h, const float* input, int64_t ld_input, int64_t wave_count)
+{
+    return p0_read(h, 0, input, ld_input, wave_count);
+}
+
+int baemm_write_b(baemm_handle h, float* output, int64_t ld_output) { return p0_write(h, 0, output, ld_output); }
+
+int baemm_read_c(baemm_handle h, const float* input, int64_t ld_input, int64_t wave_count)
+{
+    return p0_read(h, 1, input, ld_input, wave_count);
+}
+
+int baemm_write_c(baemm_handle h, float* output, int64_t ld_output) { return p0_write(h, 1, output, ld_output); }
+
+int baemm_p0_buffer(baemm_handle h, int which, float** data, int64_t* ld)
+{
+    return guard(h, [&](Handle& hh) {
+        if ((which != 0 && which != 1) || !data) throw ArgError{"which must be 0 (B) or 1 (C), data not NULL"};
+        if (!p0_sync(hh, which, hh.stream))
+            throw StateError{which == 0 ? "no B buffer: no operator apply yet"
+                                        : "no C buffer: no boundary-operator apply since the last other operator"};
+        *data = reinterpret_cast<float*>(hh.p0.host[which].data());
+        if (ld) *ld = hh.p0.W[which];
+        return BAEMM_OK;
+    });
+}
+
+int baemm_get_flags(baemm_handle h, int* flags)
+{
+    return guard(h, [&](Handle& hh) {
+        if (!flags) throw ArgError{"flags is NULL"};
+        for (int j = 0; j < 4; ++j) {
+            flags[2 * j] = hh.prm.re[j] ? 1 : 0;
+            flags[2 * j + 1] = hh.prm.im[j] ? 1 : 0;
+        }
+        return BAEMM_OK;
+    });
+}
+
+int baemm_device_info(baemm_handle h, char* buf, int64_t size)
+{
+    return guard(h, [&](Handle& hh) {
+        if (!buf || size < 1) throw ArgError{"buffer is NULL or empty"};
+        hipDeviceProp_t prop;
+        BAEMM_HIP(hipGetDeviceProperties(&prop, hh.device));
+        int drv = 0, rt = 0;
+        BAEMM_HIP(hipDriverGetVersion(&drv));
+        BAEMM_HIP(hipRuntimeGetVersion(&rt));
+        char text[1024];
+        std::snprintf(text, sizeof(text),
+                      "Helmholtz_HIP::DeviceInfo\n"
+                      "DEVICE_VENDOR                  = AMD\n"
+                      "DEVICE_NAME                    = %s\n"
+                      "DEVICE_VERSION                 = %s (%d compute units)\n"
+                      "DRIVER_VERSION                 = %d (HIP runtime %d)\n"
+                      "DEVICE_MAX_MEM_ALLOC_SIZE      = %zu\n"
+                      "DEVICE_GLOBAL_MEM_SIZE         = %zu\n"
+                      "DEVICE_LOCAL_MEM_SIZE          = %zu\n"
+                      "DEVICE_MAX_CONSTANT_BUFFER_SIZE = %zu\n",
+                      prop.name[0] ? prop.name : "AMD Instinct GPU", prop.gcnArchName, prop.multiProcessorCount, drv, rt,
+                      (size_t)prop.totalGlobalMem,
+                      (size_t)prop.totalGlobalMem, (size_t)prop.sharedMemPerBlock, (size_t)prop.totalConstMem);
+        std::snprintf(buf, (size_t)size, "%s", text);
+        return BAEMM_OK;
+    });
+}
+
+int baemm_get_coordinates(baemm_handle h, float* vertex_coords, float* triangle_coords)
+{
+    return guard(h, [&](Handle& hh) {
+        if (vertex_coords)
+            for (int64_t k = 0; k < 3 * hh.nv; ++k) vertex_coords[k] = (float)hh.coords[k];
+        if (triangle_coords)
+            for (int64_t t = 0; t < hh.nt; ++t)
+                for (int c = 0; c < 3; ++c) {
+                    const int64_t v = hh.tris[3 * t + c];
+                    for (int d = 0; d < 3; ++d) triangle_coords[12 * t + 4 * c + d] = (float)hh.coords[3 * v + d];
+                    triangle_coords[12 * t + 4 * c + 3] = 0.0f;
+                }
         return BAEMM_OK;
     });
 }

@@ -106,6 +106,30 @@ struct SolverStats {
     bool converged = true;
 };
 
+// The reference class's P0 buffers B and C (InputOutput.hpp:5-89): B holds
+// the triangle values b = Av B_in of the last operator apply, C the triangle
+// values (K + D) b of the last boundary-operator apply before AvT.  Here they
+// are filled on demand: the last apply records where its device data lives
+// (src), and the host copies are brought up to date when they are read.
+struct P0Source {
+    bool b = false, c = false;          // device data of B / C available
+    const float2* bptr = nullptr;       // staged sources, ldb floats2 per row
+    int ldb = 0, WCSP = 0, wcs = 0, K = 0, W = 0;
+    const float* part = nullptr;        // segment partials of the pair kernels
+    int S = 0, ldp = 0;
+    int64_t seg_stride = 0;
+    const float* unscale = nullptr;     // per column (mixed precision) or null
+    bool diag = false;                  // single-layer diagonal term on
+    float rotr[64], roti[64], c1r[64], c1i[64], kap[64];
+};
+
+struct P0State {
+    std::vector<cfloat> host[2];        // B, C: simplex_count x W[k]
+    int64_t W[2] = {0, 0};
+    bool host_current[2] = {false, false};   // host[k] holds the buffer's value
+    P0Source src;
+};
+
 // A captured Arnoldi step (one per Krylov index j) and the key it was captured for.
 struct GraphSlot {
     std::vector<char> key;
@@ -155,6 +179,8 @@ struct Handle {
     // workspaces
     DevBuf ws_B, ws_out, ws_b32, ws_bop, ws_part, ws_scale, ws_misc, ws_pts, ws_g, ws_nrm;
     DevBuf io[6];                        // device copies of the host arrays of the solver entry points
+    DevBuf ws_p0;                        // P0 buffer read back for ReadB/WriteB/ReadC/WriteC
+    P0State p0;
     Workspace sws;
     SolverStats stats;
     // hipGraph capture of the GMRES Arnoldi steps (on by default: replaying a
@@ -202,12 +228,24 @@ struct ApplyArgs {
     void* C; int64_t ldC; int Ctype;
 };
 
-void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s);
+// scratch (optional): a fixed device array of scratch_bytes that holds every
+// workspace of the apply instead of the handle's grow-only buffers; the call
+// then allocates nothing and fails with ArgError before any launch when the
+// apply needs more.
+void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s, char* scratch = nullptr,
+                        size_t scratch_bytes = 0);
 void apply_far_dev(Handle& h, const ApplyArgs& a, hipStream_t s);
 void apply_near_dev(Handle& h, const ApplyArgs& a, const float4* d_pts, int64_t P, hipStream_t s);
 void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s);
 void incident_dev(Handle& h, const ApplyArgs& a, const double* dirs_host, int64_t inc_count, int wave_type, hipStream_t s);
 float compute_rmin(Handle& h, hipStream_t s);
+// P0 buffers (InputOutput.hpp): which = 0 (B) or 1 (C).  p0_invalidate: an
+// apply rewrote the workspaces the buffers are read from; p0_sync: brings
+// host copy `which` up to date (synchronises s); false if the buffer has no
+// value (no apply yet, or C after an operator other than the boundary apply).
+void p0_invalidate(Handle& h);
+bool p0_sync(Handle& h, int which, hipStream_t s);
+
 // Hardware sine and cosine (v_sin_f32 / v_cos_f32, argument in revolutions)
 // of rev[0..n) as the pair kernels evaluate them; device pointers.
 void hw_sincos_dev(const float* rev, int64_t n, float* sin_out, float* cos_out, hipStream_t s);

@@ -93,6 +93,12 @@ __global__ void stage_rows_kernel(int nrows, int64_t total, const void* __restri
     b[idx] = out;
 }
 
+__global__ void fill_u32_kernel(unsigned* __restrict__ p, int n, unsigned v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void zero_u32_kernel(unsigned* __restrict__ p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -485,6 +491,8 @@ __global__ void incident_kernel(IncArgs a)
 
 // --------------------------------------------------------------- dispatch
 
+static void p0_record(Handle& h, const float2* b, int ldb, int WCSP, const EpiArgs* ea);
+
 CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk, const cfloat* rot)
 {
     CoefPack cp{};
@@ -638,28 +646,33 @@ static void run_plain(Handle& h, int op, bool adj, int WCSP, const float4* tx, c
     tm.done();
 }
 
-// Minimum distance between the targets and the centroids (synchronises s).
-static float min_distance(Handle& h, const float4* tgt, int ntgt, bool self, hipStream_t s)
+// Minimum distance between the targets and the centroids, reduced in the
+// 4-byte device word buf (synchronises s).
+static float min_distance(Handle& h, const float4* tgt, int ntgt, bool self, unsigned* buf, hipStream_t s)
 {
-    h.ws_misc.reserve(256);
-    unsigned init = 0x7f7fffffu;
-    BAEMM_HIP(hipMemcpyAsync(h.ws_misc.ptr, &init, sizeof(unsigned), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, buf, 1, 0x7f7fffffu);
     hipLaunchKernelGGL(rmin_kernel, dim3((ntgt + 255) / 256), dim3(256), 0, s, tgt, ntgt, h.d_mid, (int)h.nt,
-                       self ? 1 : 0, h.ws_misc.as<unsigned>());
+                       self ? 1 : 0, buf);
     BAEMM_HIP(hipGetLastError());
     unsigned out = 0;
-    BAEMM_HIP(hipMemcpyAsync(&out, h.ws_misc.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    BAEMM_HIP(hipMemcpyAsync(&out, buf, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     BAEMM_HIP(hipStreamSynchronize(s));
     float r2;
     memcpy(&r2, &out, sizeof(float));
     return std::sqrt(std::max(r2, 1e-30f));
 }
 
-float compute_rmin(Handle& h, hipStream_t s)
+static float compute_rmin(Handle& h, WsSource& ws, hipStream_t s)
 {
     if (h.rmin >= 0.0f) return h.rmin;
-    h.rmin = min_distance(h, h.d_mid, (int)h.nt, true, s);
+    h.rmin = min_distance(h, h.d_mid, (int)h.nt, true, static_cast<unsigned*>(ws.take(h.ws_misc, 256)), s);
     return h.rmin;
+}
+
+float compute_rmin(Handle& h, hipStream_t s)
+{
+    WsSource ws;
+    return compute_rmin(h, ws, s);
 }
 
 template <int BT, int CT>
@@ -745,14 +758,13 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
         pl.rot.assign(p.K, cfloat(1.0f, 0.0f));
         if (!pl.adj)
             for (int q = 0; q < p.K; ++q) pl.rot[q] = cfloat(p.c[4 * q + 2]);
-        pl.nst = (int)(h.nt_pad / 8);
         const int NCB = 2 * pl.WCSP / 32;
         const int64_t nbop = (int64_t)p.K * pl.nst * NCB * 64;
-        h.ws_bop.reserve(sizeof(uint4) * nbop * 2);
+        uint4* bop = static_cast<uint4*>(pl.ws.take(h.ws_bop, sizeof(uint4) * nbop * 2));
         hipLaunchKernelGGL(stage_f32_kernel, dim3((unsigned)((nbop + 255) / 256)), dim3(256), 0, s, pl.b, pl.ldb,
-                           pl.nst, pl.WCSP, (int)p.K, h.ws_bop.as<uint4>());
+                           pl.nst, pl.WCSP, (int)p.K, bop);
         BAEMM_HIP(hipGetLastError());
-        pl.bop = h.ws_bop.as<uint4>();
+        pl.bop = bop;
         pl.unscale = nullptr;
         return;
     }
@@ -778,10 +790,10 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
         std::frexp(std::max(amax, 1e-30), &e);
         const float sig = (float)std::ldexp(1.0, 14 - e);
         for (int q = 0; q < p.K; ++q) pl.sA[q] = csc.sA[q] = sig;
-        h.ws_nrm.reserve(sizeof(float4) * h.nt_pad);
+        float4* nrm = static_cast<float4*>(pl.ws.take(h.ws_nrm, sizeof(float4) * h.nt_pad));
         hipLaunchKernelGGL(scale_nrm_kernel, dim3((unsigned)((h.nt_pad + 255) / 256)), dim3(256), 0, s, h.d_nrm,
-                           (int)h.nt_pad, sig, h.ws_nrm.as<float4>());
-        pl.nrm = h.ws_nrm.as<float4>();
+                           (int)h.nt_pad, sig, nrm);
+        pl.nrm = nrm;
     } else {
         for (int q = 0; q < p.K; ++q) {
             const double c1 = std::abs(std::complex<double>(p.c[4 * q + 1]));
@@ -796,32 +808,30 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
         }
     }
     const int ncols = pl.ldb;
-    h.ws_scale.reserve(sizeof(float) * 3 * ncols);
-    unsigned* colmax = h.ws_scale.as<unsigned>();
+    unsigned* colmax = static_cast<unsigned*>(pl.ws.take(h.ws_scale, sizeof(float) * 3 * ncols));
     float* scale = reinterpret_cast<float*>(colmax + ncols);
     float* unscale = scale + ncols;
     hipLaunchKernelGGL(zero_u32_kernel, dim3((ncols + 255) / 256), dim3(256), 0, s, colmax, ncols);
     hipLaunchKernelGGL(colmax_kernel, dim3(256), dim3(256), 0, s, pl.b, (int)h.nt, pl.ldb, colmax);
     hipLaunchKernelGGL(colscale_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, colmax, ncols, pl.WCSP, csc, scale,
                        unscale);
-    pl.nst = (int)(h.nt_pad / 8);
     const int NCB = 2 * pl.WCSP / 32;
     const int64_t nbop = (int64_t)p.K * pl.nst * NCB * 64;
-    h.ws_bop.reserve(sizeof(uint4) * nbop * 2);
+    uint4* bop = static_cast<uint4*>(pl.ws.take(h.ws_bop, sizeof(uint4) * nbop * 2));
     hipLaunchKernelGGL(split_kernel, dim3((unsigned)((nbop + 255) / 256)), dim3(256), 0, s, pl.b, pl.ldb, pl.nst,
-                       pl.WCSP, (int)p.K, scale, h.ws_bop.as<uint4>());
+                       pl.WCSP, (int)p.K, scale, bop);
     BAEMM_HIP(hipGetLastError());
-    pl.bop = h.ws_bop.as<uint4>();
+    pl.bop = bop;
     pl.unscale = unscale;
 }
 
-void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
+void bdr_setup(Handle& h, BdrPlan& pl)
 {
     const Params& p = h.prm;
     if (!p.loaded) throw ArgError{"operators not loaded"};
+    pl.planned = true;
     pl.kernel_on = p.kernel_on();
     pl.mass_on = p.mass_on();
-    if (!pl.kernel_on && !pl.mass_on) return;   // the reference leaves C_out untouched
     pl.mixed = (h.precision != BAEMM_PRECISION_FP32) && pl.kernel_on;
     pl.f32 = pl.mixed && h.precision == BAEMM_PRECISION_FP32_MFMA;
     pl.WCSP = pl.mixed ? wcsp_mixed((int)p.wcs) : wcsp_plain((int)p.wcs);
@@ -833,6 +843,59 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     if (pl.mixed)
         for (int q = 0; q < p.K; ++q)
             if (p.c[4 * q + 2] == cfloat(0.0f, 0.0f)) pl.adj = true;
+    if (!pl.kernel_on) return;
+    if (!pl.mixed) {
+        const int tiles = (int)(h.nt_pad / kTile);
+        const int max_seg = (int)(h.nt_pad / kSrcBlock);
+        int S0 = choose_segments(h, tiles, max_seg, h.nt);
+        if (h.segments == 0)
+            S0 = balance_segments(tiles, max_seg, S0, max_seg,
+                                  resident_blocks((const void*)plain_fn(OP_BDR, pl.adj, pl.WCSP,
+                                                                        pick_kg(pl.WCSP, (int)p.K, false)),
+                                                  kTile));
+        pl.seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
+        pl.S = (int)((h.nt_pad + pl.seg_len - 1) / pl.seg_len);
+        return;
+    }
+    pl.nst = (int)(h.nt_pad / 8);
+    const int tiles = (int)(h.nt_pad / kMixTile);
+    int S0 = choose_segments(h, tiles, pl.nst / 8, h.nt);
+    if (h.segments == 0) {
+        // balanced for the wave-specialised kernel (the default); the
+        // single-wave kernel takes the same segments, so the two stay
+        // bit-identical
+        const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
+        S0 = balance_segments(tiles, pl.nst, S0, pl.nst / 8,
+                              resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512));
+    }
+    pl.steps = (pl.nst + S0 - 1) / S0;
+    pl.S = (pl.nst + pl.steps - 1) / pl.steps;
+}
+
+size_t bdr_workspace_bytes(const Handle& h, const BdrPlan& pl, bool need_rmin)
+{
+    if (!pl.kernel_on) return 0;
+    const auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    const Params& p = h.prm;
+    size_t n = al(sizeof(float2) * h.nt_pad * pl.ldb);                       // staged sources b
+    if (pl.mixed) {
+        const int NCB = 2 * pl.WCSP / 32;
+        n += al(sizeof(uint4) * 2 * (size_t)p.K * pl.nst * NCB * 64);       // MFMA B operand
+        if (!pl.f32) {
+            if (need_rmin) n += al(256);                                    // minimum distance
+            if (!pl.adj) n += al(sizeof(float4) * h.nt_pad);                // scaled source normals
+            n += al(sizeof(float) * 3 * pl.ldb);                             // column maxima and scales
+        }
+    }
+    n += al(sizeof(float) * (size_t)pl.S * h.nt_pad * pl.ldp);              // segment partials
+    return n;
+}
+
+void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
+{
+    const Params& p = h.prm;
+    if (!pl.planned) bdr_setup(h, pl);
+    if (!pl.kernel_on && !pl.mass_on) return;   // the reference leaves C_out untouched
     pl.nrm = h.d_nrm;
     EpiArgs& ea = pl.ea;
     ea = base_epi(h, a, pl.WCSP);
@@ -840,15 +903,14 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     ea.kernel_on = pl.kernel_on;
     ea.diag_on = pl.kernel_on && h.use_diagonal && p.any(1);
     if (!pl.kernel_on) return;
-    h.ws_b32.reserve(sizeof(float2) * h.nt_pad * pl.ldb);
-    pl.b = h.ws_b32.as<float2>();
+    pl.b = static_cast<float2*>(pl.ws.take(h.ws_b32, sizeof(float2) * h.nt_pad * pl.ldb));
     launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
     ea.b = pl.b;
     ea.ldp = pl.ldp;
     ea.ldb = pl.ldb;
     ea.unscale = nullptr;
     if (!pl.mixed) return;
-    stage_mfma(h, pl, pl.f32 ? 0.0f : compute_rmin(h, s), s);
+    stage_mfma(h, pl, pl.f32 ? 0.0f : compute_rmin(h, pl.ws, s), s);
     ea.unscale = pl.unscale;
     for (int q = 0; q < p.K; ++q) {
         ea.rotr[q] = pl.rot[q].real();
@@ -859,61 +921,39 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
 float* bdr_reserve_part(Handle& h, BdrPlan& pl, int S)
 {
     const int64_t seg_stride = h.nt_pad * (int64_t)pl.ldp;
-    h.ws_part.reserve(sizeof(float) * seg_stride * S);
+    float* part = static_cast<float*>(pl.ws.take(h.ws_part, sizeof(float) * seg_stride * S));
     pl.ea.S = S;
     pl.ea.seg_stride = seg_stride;
-    pl.ea.part = h.ws_part.as<float>();
-    return h.ws_part.as<float>();
+    pl.ea.part = part;
+    return part;
 }
 
 void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
 {
     if (!pl.kernel_on) return;
     const Params& p = h.prm;
+    float* part = bdr_reserve_part(h, pl, pl.S);
     if (!pl.mixed) {
-        const int tiles = (int)(h.nt_pad / kTile);
-        const int max_seg = (int)(h.nt_pad / kSrcBlock);
-        int S0 = choose_segments(h, tiles, max_seg, h.nt);
-        if (h.segments == 0)
-            S0 = balance_segments(tiles, max_seg, S0, max_seg,
-                                  resident_blocks((const void*)plain_fn(OP_BDR, pl.adj, pl.WCSP,
-                                                                        pick_kg(pl.WCSP, (int)p.K, false)),
-                                                  kTile));
-        const int seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
-        const int S = (int)((h.nt_pad + seg_len - 1) / seg_len);
-        float* part = bdr_reserve_part(h, pl, S);
         run_plain(h, OP_BDR, pl.adj, pl.WCSP, h.d_mid, h.d_nrm, (int)h.nt, (int)h.nt_pad, h.d_mid, h.d_nrm,
-                  (int)h.nt, (int)h.nt_pad, pl.b, pl.ldb, part, pl.ea.seg_stride, pl.ldp, S, seg_len, s);
+                  (int)h.nt, (int)h.nt_pad, pl.b, pl.ldb, part, pl.ea.seg_stride, pl.ldp, pl.S, pl.seg_len, s);
         return;
     }
     const int nst = pl.nst;
     const int tiles = (int)(h.nt_pad / kMixTile);
     // the FP32 MFMA contraction exists in the wave-specialised kernel only
     const bool ws = h.mixed_kernel == 1 || pl.f32;
-    int S0 = choose_segments(h, tiles, nst / 8, h.nt);
-    if (h.segments == 0) {
-        // balanced for the wave-specialised kernel (the default); the
-        // single-wave kernel takes the same segments, so the two stay
-        // bit-identical
-        const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
-        S0 = balance_segments(tiles, nst, S0, nst / 8,
-                              resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512));
-    }
-    const int steps = (nst + S0 - 1) / S0;
-    const int S = (nst + steps - 1) / steps;
-    float* part = bdr_reserve_part(h, pl, S);
     Timed tm(h, s);
     for (int q0 = 0; q0 < p.K;) {
         int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
         if (ws) kg = std::min(kg, 2);
         const CoefPack cp = make_pack(p, q0, kg, pl.sA.data(), pl.rot.data());
         if (ws)
-            hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32), dim3(tiles, S), dim3(512), 0, s, h.d_mid,
+            hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32), dim3(tiles, pl.S), dim3(512), 0, s, h.d_mid,
                                (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
-                               pl.ldp, q0, steps);
+                               pl.ldp, q0, pl.steps);
         else
-            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg), dim3(tiles, S), dim3(256), 0, s, h.d_mid, pl.nrm,
-                               (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride, pl.ldp, q0, steps);
+            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg), dim3(tiles, pl.S), dim3(256), 0, s, h.d_mid, pl.nrm,
+                               (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride, pl.ldp, q0, pl.steps);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;
     }
@@ -926,12 +966,24 @@ void bdr_end(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     bdr_epilogue(h, pl.ea, a.Btype, a.Ctype, s);
 }
 
-void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
+void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s, char* scratch, size_t scratch_bytes)
 {
     BdrPlan pl;
+    bdr_setup(h, pl);
+    if (scratch) {
+        const size_t need = bdr_workspace_bytes(h, pl, h.rmin < 0.0f);
+        if (need > scratch_bytes)
+            throw ArgError{"the apply needs " + std::to_string(need) + " bytes of workspace; the fixed scratch "
+                           "array holds " + std::to_string(scratch_bytes)};
+        pl.ws = WsSource{scratch, scratch_bytes, 0};
+    }
     bdr_begin(h, a, pl, s);
     bdr_pairs(h, pl, s);
     bdr_end(h, a, pl, s);
+    // the fixed scratch array is shared by every handle of the device: its
+    // contents are no P0 buffer of this handle
+    if (scratch) p0_invalidate(h);
+    else if (pl.kernel_on) p0_record(h, pl.b, pl.ldb, pl.WCSP, &pl.ea);
 }
 
 static void rows_epilogue(int nrows, const Params& p, int WCSP, int S, int64_t seg_stride, int ldp, const float* part,
@@ -984,6 +1036,7 @@ static void apply_rows_op(Handle& h, int op, const ApplyArgs& a, const float4* t
     run_plain(h, op, false, WCSP, tx, nullptr, ntgt, ntgt_pad, h.d_mid, h.d_nrm, (int)h.nt, (int)h.nt_pad, b, ldb,
               h.ws_part.as<float>(), seg_stride, ldp, S, seg_len, s);
     rows_epilogue(ntgt, p, WCSP, S, seg_stride, ldp, h.ws_part.as<float>(), a, s);
+    p0_record(h, b, ldb, WCSP, nullptr);
 }
 
 void apply_far_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
@@ -1010,10 +1063,13 @@ static void apply_near_mfma(Handle& h, const ApplyArgs& a, const float4* d_pts, 
     for (int q = 0; q < p.K; ++q)
         if (p.c[4 * q + 2] == cfloat(0.0f, 0.0f)) pl.adj = true;
     pl.nrm = h.d_nrm;
-    h.ws_b32.reserve(sizeof(float2) * h.nt_pad * pl.ldb);
-    pl.b = h.ws_b32.as<float2>();
+    pl.nst = (int)(h.nt_pad / 8);
+    pl.b = static_cast<float2*>(pl.ws.take(h.ws_b32, sizeof(float2) * h.nt_pad * pl.ldb));
     launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
-    stage_mfma(h, pl, pl.f32 ? 0.0f : min_distance(h, d_pts, P, false, s), s);
+    stage_mfma(h, pl,
+               pl.f32 ? 0.0f
+                      : min_distance(h, d_pts, P, false, static_cast<unsigned*>(pl.ws.take(h.ws_misc, 256)), s),
+               s);
     const int P_pad = (P + kMixTile - 1) / kMixTile * kMixTile;
     const int nst = pl.nst;
     const int tiles = P_pad / kMixTile;
@@ -1040,6 +1096,7 @@ static void apply_near_mfma(Handle& h, const ApplyArgs& a, const float4* d_pts, 
     }
     tm.done();
     rows_epilogue(P, p, pl.WCSP, S, seg_stride, pl.ldp, part, a, s, pl.unscale, &pl.rot);
+    p0_record(h, pl.b, pl.ldb, pl.WCSP, nullptr);
 }
 
 void apply_near_dev(Handle& h, const ApplyArgs& a, const float4* d_pts, int64_t P, hipStream_t s)
@@ -1055,6 +1112,7 @@ void apply_near_dev(Handle& h, const ApplyArgs& a, const float4* d_pts, int64_t 
 
 void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
 {
+    p0_invalidate(h);   // rewrites the workspaces; no P0 buffer of the reference kind
     const Params& p = h.prm;
     if (!p.loaded) throw ArgError{"parameters not loaded"};
     if (!(p.any(1) || p.any(2))) return;   // CreateHerglotzWave.hpp:200
@@ -1105,6 +1163,7 @@ void herglotz_dev(Handle& h, const ApplyArgs& a, hipStream_t s)
 void incident_dev(Handle& h, const ApplyArgs& a, const double* dirs, int64_t inc_count, int wave_type,
                   hipStream_t s)
 {
+    p0_invalidate(h);   // rewrites the workspaces; no P0 buffer of the reference kind
     const Params& p = h.prm;
     if (!p.loaded) throw ArgError{"parameters not loaded"};
     if (!(p.any(1) || p.any(2))) return;   // CreateIncidentWave.hpp:95
@@ -1151,6 +1210,104 @@ void incident_dev(Handle& h, const ApplyArgs& a, const double* dirs, int64_t inc
     bdr_epilogue(h, ea, kC64, a.Ctype, s);
 }
 
+
+// --------------------------------------------------------------- P0 buffers
+
+// The reference's P0 buffers (InputOutput.hpp) from the device data of the
+// last apply, as complex float, simplex_count x W: which = 0: B, the staged
+// sources b[t][k]; which = 1: C, the triangle values before AvT,
+// rot_q * unscale_k * sum_s part_s[t][k] + D_t b[t][k] (the inner term of
+// bdr_epilogue_kernel, summed in the same order in FP64).
+__global__ void p0_kernel(P0Source src, int which, int nt, const float* __restrict__ sdiag,
+                          float2* __restrict__ out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nt * src.W) return;
+    const int t = (int)(idx / src.W), k = (int)(idx % src.W);
+    const int q = k / src.wcs, kk = k % src.wcs;
+    const int pc = q * src.WCSP + kk;
+    const float2 bv = src.bptr[(int64_t)t * src.ldb + pc];
+    if (which == 0) {
+        out[idx] = bv;
+        return;
+    }
+    double pr = 0.0, pi = 0.0;
+    const float* pp = src.part + (int64_t)t * src.ldp + q * 2 * src.WCSP + kk;
+    for (int sg = 0; sg < src.S; ++sg) {
+        pr += (double)pp[(int64_t)sg * src.seg_stride];
+        pi += (double)pp[(int64_t)sg * src.seg_stride + src.WCSP];
+    }
+    const double us = src.unscale ? (double)src.unscale[pc] : 1.0;
+    double qr = pr * us, qi = pi * us;
+    const double rr = src.rotr[q], ri = src.roti[q];
+    const double tr = qr * rr - qi * ri;
+    qi = qr * ri + qi * rr;
+    qr = tr;
+    if (src.diag) {
+        const double fr = (double)src.c1r[q] * sdiag[t] - (double)src.c1i[q] * src.kap[q];
+        const double fi = (double)src.c1i[q] * sdiag[t] + (double)src.c1r[q] * src.kap[q];
+        qr += fr * bv.x - fi * bv.y;
+        qi += fr * bv.y + fi * bv.x;
+    }
+    out[idx] = make_float2((float)qr, (float)qi);
+}
+
+// Records the device data of an apply as the source of the P0 buffers: b
+// (B) and, for a boundary-operator apply (ea != null), the partials behind C.
+static void p0_record(Handle& h, const float2* b, int ldb, int WCSP, const EpiArgs* ea)
+{
+    const Params& p = h.prm;
+    P0Source& src = h.p0.src;
+    src.b = b != nullptr;
+    src.bptr = b;
+    src.ldb = ldb;
+    src.WCSP = WCSP;
+    src.wcs = (int)p.wcs;
+    src.K = (int)p.K;
+    src.W = (int)p.W;
+    h.p0.host_current[0] = false;
+    h.p0.host_current[1] = false;
+    src.c = ea != nullptr && ea->kernel_on;
+    if (!src.c) return;
+    src.part = ea->part;
+    src.S = ea->S;
+    src.ldp = ea->ldp;
+    src.seg_stride = ea->seg_stride;
+    src.unscale = ea->unscale;
+    src.diag = ea->diag_on != 0;
+    for (int q = 0; q < p.K; ++q) {
+        src.rotr[q] = ea->rotr[q];
+        src.roti[q] = ea->roti[q];
+        src.c1r[q] = ea->c1r[q];
+        src.c1i[q] = ea->c1i[q];
+        src.kap[q] = ea->kap[q];
+    }
+}
+
+void p0_invalidate(Handle& h)
+{
+    h.p0.src.b = h.p0.src.c = false;
+    h.p0.host_current[0] = h.p0.host_current[1] = false;
+}
+
+bool p0_sync(Handle& h, int which, hipStream_t s)
+{
+    P0State& st = h.p0;
+    if (st.host_current[which]) return true;
+    const P0Source& src = st.src;
+    if (!(which == 0 ? src.b : src.c)) return false;
+    const int64_t n = h.nt * (int64_t)src.W;
+    h.ws_p0.reserve(sizeof(float2) * std::max<int64_t>(n, 1));
+    hipLaunchKernelGGL(p0_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, which, (int)h.nt, h.d_sdiag,
+                       h.ws_p0.as<float2>());
+    BAEMM_HIP(hipGetLastError());
+    st.host[which].resize((size_t)n);
+    BAEMM_HIP(hipMemcpyAsync(st.host[which].data(), h.ws_p0.ptr, sizeof(float2) * n, hipMemcpyDeviceToHost, s));
+    BAEMM_HIP(hipStreamSynchronize(s));
+    st.W[which] = src.W;
+    st.host_current[which] = true;
+    return true;
+}
 
 // --------------------------------------------------------------- diagnostics
 

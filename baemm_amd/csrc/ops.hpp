@@ -57,6 +57,25 @@ struct Timed {
     }
 };
 
+// Where an apply takes its device workspaces from: the handle's grow-only
+// buffers (base == nullptr), or consecutive 256-byte-aligned pieces of a fixed
+// device array of cap bytes (the no-allocation entry point).
+struct WsSource {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    void* take(DevBuf& buf, size_t n)
+    {
+        if (!base) {
+            buf.reserve(n);
+            return buf.ptr;
+        }
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off + n > cap) throw ArgError{"internal: workspace larger than its planned size"};
+        used = off + n;
+        return base + off;
+    }
+};
+
 // State of one apply between its phases.
 struct BdrPlan {
     bool kernel_on = false, mass_on = false, mixed = false, adj = false;
@@ -69,9 +88,22 @@ struct BdrPlan {
     float2* b = nullptr;         // staged sources b = Av B
     const uint4* bop = nullptr;  // split MFMA operand (mixed)
     float* unscale = nullptr;    // per-column 1 / (A scale x b scale) of the mixed path
+    // launch shape of the single-instance pair kernels (bdr_setup)
+    bool planned = false;
+    int S = 0;                   // source segments
+    int seg_len = 0;             // plain kernels: sources per segment (multiple of 64)
+    int steps = 0;               // MFMA kernels: 8-source steps per segment
+    WsSource ws;
     EpiArgs ea{};
 };
 
+// Phase 0 (no launches): arithmetic, operand widths and the launch shape of
+// the single-instance pair kernels for the loaded parameters.
+void bdr_setup(Handle& h, BdrPlan& pl);
+// Device workspace bytes of a single-instance apply planned by bdr_setup
+// (an upper bound of what its phases take from pl.ws); need_rmin: the minimum
+// centroid distance is not yet known and is computed in the workspace.
+size_t bdr_workspace_bytes(const Handle& h, const BdrPlan& pl, bool need_rmin);
 // Phase 1: parameters, Av gather and (mixed) the scaled hi/lo operand split.
 void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s);
 // Reserves the partial buffer for S segment slices and records it in pl.ea.

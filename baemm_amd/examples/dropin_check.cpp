@@ -127,6 +127,68 @@ int main()
         std::vector<C64> Cfd(Cf.begin(), Cf.end());
         check("complex<float> vs complex<double> apply", rel(Cfd, C1), 2e-6);
 
+        // ---- P0 buffers of the last boundary apply (InputOutput.hpp)
+        {
+            std::vector<C32> Bp((size_t)nt * W), Cp((size_t)nt * W);
+            H.WriteB(Bp.data(), W);
+            H.WriteC(Cp.data(), W);
+            std::vector<C32> b_ref((size_t)nt * W);
+            const float* a = H.TriangleAreas();
+            for (int64_t t = 0; t < nt; ++t)
+                for (int64_t k = 0; k < W; ++k)
+                    b_ref[t * W + k] = C32(a[t] / 3.0f) * (Bf[T[3 * t] * W + k] + Bf[T[3 * t + 1] * W + k] +
+                                                           Bf[T[3 * t + 2] * W + k]);
+            check("WriteB = Av B of the last apply", rel(Bp, b_ref), 1e-6);
+            // AvT C + c0 M B reproduces the apply's result
+            std::vector<C64> Cv((size_t)nv * W, C64(0, 0));
+            for (int64_t t = 0; t < nt; ++t)
+                for (int c = 0; c < 3; ++c)
+                    for (int64_t k = 0; k < W; ++k) Cv[T[3 * t + c] * W + k] += C64(a[t] / 3.0) * C64(Cp[t * W + k]);
+            std::vector<double> Br((size_t)nv * 2 * W), Mr((size_t)nv * 2 * W, 0.0);
+            for (size_t k = 0; k < B.size(); ++k) {
+                Br[2 * k] = Bf[k].real();
+                Br[2 * k + 1] = Bf[k].imag();
+            }
+            H.MassMatrix().Dot<16>(1.0, Br.data(), int64_t(16), 0.0, Mr.data(), int64_t(16));
+            for (int64_t v = 0; v < nv; ++v)
+                for (int64_t k = 0; k < W; ++k)
+                    Cv[v * W + k] += C64(coeff_f[4 * (k / wcs)]) * C64(Mr[v * 16 + 2 * k], Mr[v * 16 + 2 * k + 1]);
+            check("AvT WriteC + c0 M B = the apply's result", rel(Cv, Cfd), 1e-6);
+            check("B(i,k) = WriteB", std::abs(H.B(5, 3) - Bp[5 * W + 3]) + std::abs(H.B(nt - 1, W - 1) - Bp.back()), 0);
+            check("C(i,k) = WriteC", std::abs(H.C(7, 2) - Cp[7 * W + 2]) + std::abs(H.C(0, 0) - Cp[0]), 0);
+            // ReadB / ReadC replace the buffers (ld > wave_count on input)
+            std::vector<C32> in((size_t)nt * 5);
+            for (size_t k = 0; k < in.size(); ++k) in[k] = C32(0.5f * k, -1.0f * k);
+            H.ReadB(in.data(), 5, 3);
+            H.ReadC(in.data(), 5, 2);
+            std::vector<C32> ob((size_t)nt * 4), oc((size_t)nt * 2);
+            H.WriteB(ob.data(), 4);
+            H.WriteC(oc.data(), 2);
+            double d = 0;
+            for (int64_t t = 0; t < nt; ++t) {
+                for (int k = 0; k < 3; ++k) d += std::abs(ob[t * 4 + k] - in[t * 5 + k]);
+                for (int k = 0; k < 2; ++k) d += std::abs(oc[t * 2 + k] - in[t * 5 + k]);
+            }
+            check("ReadB/ReadC then WriteB/WriteC round trip", d, 0);
+            check("B(i,k) after ReadB", std::abs(H.B(9, 2) - in[9 * 5 + 2]), 0);
+            H.B(9, 2) = C32(4.0f, 2.0f);
+            check("B(i,k) is a writable reference", std::abs(H.B(9, 2) - C32(4.0f, 2.0f)), 0);
+            // vertex and triangle coordinates, device information, flags
+            double dv = 0;
+            for (int64_t k = 0; k < 3 * nv; ++k) dv += std::abs(H.VertexCoordinates()[k] - (float)V[k]);
+            for (int64_t t = 0; t < nt; ++t)
+                for (int c = 0; c < 3; ++c) {
+                    for (int e = 0; e < 3; ++e)
+                        dv += std::abs(H.TriangleCoordinates()[12 * t + 4 * c + e] - (float)V[3 * T[3 * t + c] + e]);
+                    dv += std::abs(H.TriangleCoordinates()[12 * t + 4 * c + 3]);
+                }
+            check("VertexCoordinates / TriangleCoordinates", dv, 0);
+            const std::string info = H.DeviceInfo();
+            std::printf("%s", info.c_str());
+            check("DeviceInfo names the device", info.find("DEVICE_NAME") != std::string::npos ? 0.0 : 1.0, 0);
+            H.PrintBooleans();
+        }
+
         // ---- far field, potential, waves
         std::vector<C64> F((size_t)m * W), U(7 * W);
         H.ApplyFarFieldOperators_PL<0>(C64(1, 0), B.data(), W, C64(0, 0), F.data(), W, kappa, coeff, W, wcs);
@@ -149,6 +211,11 @@ int main()
         double hn = 0;
         for (auto& z : Hw) hn += std::norm(z);
         check("Herglotz wave finite and nonzero", hn > 0 ? 0.0 : 1.0, 0);
+        {
+            std::vector<float> tmp((size_t)2 * nt * W);
+            check("WriteC after a Herglotz wave: no value",
+                  baemm_write_c(H.Handle(), tmp.data(), W) == BAEMM_ERR_STATE ? 0.0 : 1.0, 0);
+        }
 
         // ---- mass matrix and its inverse
         std::vector<double> X((size_t)nv * 3), Y((size_t)nv * 3), Z((size_t)nv * 3, 0.0);

@@ -3,13 +3,19 @@
 // Example_OpenCL/main.cpp:25-249 (FarField, Derivative_FF with h = vertex
 // coordinates, AdjointDerivative_FF of DF h), on the HIP library.
 //
-// Usage: example_far_field [mesh.txt [meas.txt]] [--level L] [--waves W]
+// Usage: example_far_field [mesh.txt [meas.txt]] [--level L] [--waves W] [--out DIR]
 //   mesh.txt / meas.txt are in the reference's text mesh format
 //   (ReadMeshFromFile.hpp:30-97).  Without files an icosphere of the given
 //   subdivision level (default 5, 20480 triangles) is the obstacle and the
 //   vertices of a level-4 icosphere (2562 points) are the measurement
 //   directions.  Prints the L2 norm of the far field, the norms of DF h and
-//   DF* DF h, and the wall time of each call.
+//   DF* DF h, and the wall time of each call.  With --out, writes the three
+//   results as the reference example does (Example_OpenCL/main.cpp:191,220,233):
+//   DIR/F_New.txt (far field, meas_count x W), DIR/DF_New.txt (DF h) and
+//   DIR/DFT_New.txt (DF* DF h, vertex_count x 3).  One row per line; a complex
+//   entry is written as its real and imaginary parts, so a row of W complex
+//   values holds 2 W numbers (the reference prints with its Tools library's
+//   ToString, which is not part of its repository).
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -18,6 +24,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
@@ -88,6 +95,29 @@ double fro(const std::vector<Complex>& a)
     return std::sqrt(s);
 }
 
+void write_complex(const std::string& path, const Complex* a, int64_t rows, int64_t cols)
+{
+    std::FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    for (int64_t i = 0; i < rows; ++i) {
+        for (int64_t k = 0; k < cols; ++k)
+            std::fprintf(f, "%s%.17g %.17g", k ? " " : "", a[i * cols + k].real(), a[i * cols + k].imag());
+        std::fputc('\n', f);
+    }
+    std::fclose(f);
+}
+
+void write_real(const std::string& path, const double* a, int64_t rows, int64_t cols)
+{
+    std::FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    for (int64_t i = 0; i < rows; ++i) {
+        for (int64_t k = 0; k < cols; ++k) std::fprintf(f, "%s%.17g", k ? " " : "", a[i * cols + k]);
+        std::fputc('\n', f);
+    }
+    std::fclose(f);
+}
+
 double seconds_since(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -98,6 +128,7 @@ double seconds_since(std::chrono::steady_clock::time_point t0)
 int run(int argc, char** argv)
 {
     std::vector<std::string> files;
+    std::string out_dir;
     int level = 5;
     int wave_count = 16;
     for (int i = 1; i < argc; ++i) {
@@ -105,6 +136,8 @@ int run(int argc, char** argv)
             level = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "--waves") && i + 1 < argc)
             wave_count = std::atoi(argv[++i]);
+        else if (!std::strcmp(argv[i], "--out") && i + 1 < argc)
+            out_dir = argv[++i];
         else
             files.push_back(argv[i]);
     }
@@ -170,6 +203,7 @@ int run(int argc, char** argv)
     std::printf("FarField: %.3f s\n", seconds_since(t0));
     const double factor = 2.0 * M_PI / double(m * wave_count);
     std::printf("L^2-norm of C = %.10e\n", std::sqrt(factor) * fro(F));
+    if (!out_dir.empty()) write_complex(out_dir + "/F_New.txt", F.data(), m, wave_count);
 
     std::vector<double> h(V), w((size_t)3 * nv);
     std::vector<Complex> DFh((size_t)m * wave_count);
@@ -179,6 +213,7 @@ int run(int argc, char** argv)
                        BAEMM::WaveType::Plane, cg_tol, gmres_tol);
     std::printf("Derivative_FF: %.3f s\n", seconds_since(t0));
     std::printf("|DF h| = %.10e\n", fro(DFh));
+    if (!out_dir.empty()) write_complex(out_dir + "/DF_New.txt", DFh.data(), m, wave_count);
 
     t0 = std::chrono::steady_clock::now();
     H.AdjointDerivative_FF<0>(kappa.data(), wave_chunk_count, dirs.data(), wave_chunk_size, DFh.data(), w.data(),
@@ -187,6 +222,7 @@ int run(int argc, char** argv)
     double s = 0;
     for (double x : w) s += x * x;
     std::printf("|DF* DF h| = %.10e\n", std::sqrt(s));
+    if (!out_dir.empty()) write_real(out_dir + "/DFT_New.txt", w.data(), nv, 3);
 
     if (du_dn != nullptr) std::free(du_dn);
     return 0;

@@ -196,6 +196,31 @@ def random_directions(n: int, seed: int):
     return np.ascontiguousarray(d / np.linalg.norm(d, axis=1, keepdims=True))
 
 
+def volume_grid(shape, lo, hi):
+    """Points of a regular box grid: shape = (nx, ny, nz) samples from lo to hi
+    (inclusive) per axis, x slowest; (nx ny nz) x 3.  Evaluation points for the
+    potential (ApplyNearFieldOperators_PL) on a volume, for visualisation."""
+    axes = [np.linspace(l, h, n) for n, l, h in zip(shape, lo, hi)]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+
+def shell_points(V: np.ndarray, T: np.ndarray, offsets, per_offset: int, seed: int):
+    """Near-surface points: for each offset d, per_offset triangles drawn at
+    random (seeded) and the point c_t + d n_t at signed distance d from the
+    centroid c_t along the unit outward normal n_t (d < 0: inside)."""
+    rng = np.random.default_rng(seed)
+    A = V[T[:, 0]]
+    c = (A + V[T[:, 1]] + V[T[:, 2]]) / 3.0
+    n = np.cross(V[T[:, 1]] - A, V[T[:, 2]] - A)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    out = []
+    for d in offsets:
+        t = rng.choice(T.shape[0], per_offset, replace=False)
+        out.append(c[t] + d * n[t])
+    return np.concatenate(out, axis=0)
+
+
 def signed_volume(V: np.ndarray, T: np.ndarray) -> float:
     """Signed enclosed volume (positive for outward orientation)."""
     a, b, c = V[T[:, 0]], V[T[:, 1]], V[T[:, 2]]

@@ -252,6 +252,70 @@ public:
 
     std::string ClassName() const { return "Helmholtz_HIP"; }
 
+    // ------------------------------------------------ Helmholtz_OpenCL.hpp:333-391
+    // FP32 copies of the input: V x 3, and T x 12 (three vertices, each x, y, z, 0).
+    const float* VertexCoordinates() const
+    {
+        if (vcoords_.empty()) {
+            vcoords_.resize(3 * (size_t)VertexCount());
+            baemm_get_coordinates(h_, vcoords_.data(), nullptr);
+        }
+        return vcoords_.data();
+    }
+
+    const float* TriangleCoordinates() const
+    {
+        if (tcoords_.empty()) {
+            tcoords_.resize(12 * (size_t)SimplexCount());
+            baemm_get_coordinates(h_, nullptr, tcoords_.data());
+        }
+        return tcoords_.data();
+    }
+
+    std::string DeviceInfo()
+    {
+        char buf[1024] = {0};
+        detail::report(h_, baemm_device_info(h_, buf, sizeof(buf)), "DeviceInfo");
+        return std::string(buf);
+    }
+
+    // ------------------------------------------------ LoadParameters.hpp:85-95
+    void PrintBooleans() const
+    {
+        static const char* names[8] = {"Re_mass_matrix",  "Im_mass_matrix",  "Re_single_layer", "Im_single_layer",
+                                       "Re_double_layer", "Im_double_layer", "Re_adjdbl_layer", "Im_adjdbl_layer"};
+        int f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        detail::report(h_, baemm_get_flags(h_, f), "PrintBooleans");
+        for (int j = 0; j < 8; ++j) std::printf("%s = %s\n", names[j], f[j] ? "true" : "false");
+    }
+
+    // ------------------------------------------------ InputOutput.hpp:5-89
+    // The P0 buffers: B = Av B_in of the last operator apply, C = (K + D) b of
+    // the last boundary-operator apply, simplex_count x wave_count (Complex).
+    Complex& B(const Int i, const Int k) { return p0_ref(0, i, k); }
+    const Complex& B(const Int i, const Int k) const { return p0_ref(0, i, k); }
+    Complex& C(const Int i, const Int k) { return p0_ref(1, i, k); }
+    const Complex& C(const Int i, const Int k) const { return p0_ref(1, i, k); }
+
+    void ReadB(const Complex* input, const Int ld_input, const Int wave_count_)
+    {
+        detail::report(h_, baemm_read_b(h_, reinterpret_cast<const float*>(input), ld_input, wave_count_), "ReadB");
+    }
+    void ReadB(const Complex* input, const Int wave_count_) { ReadB(input, wave_count_, wave_count_); }
+    void WriteB(Complex* output, const Int ld_output) const
+    {
+        detail::report(h_, baemm_write_b(h_, reinterpret_cast<float*>(output), ld_output), "WriteB");
+    }
+    void ReadC(const Complex* input, const Int ld_input, const Int wave_count_)
+    {
+        detail::report(h_, baemm_read_c(h_, reinterpret_cast<const float*>(input), ld_input, wave_count_), "ReadC");
+    }
+    void ReadC(const Complex* input, const Int wave_count_) { ReadC(input, wave_count_, wave_count_); }
+    void WriteC(Complex* output, const Int ld_output) const
+    {
+        detail::report(h_, baemm_write_c(h_, reinterpret_cast<float*>(output), ld_output), "WriteC");
+    }
+
     // ----------------------------------------------------- LoadParameters.hpp
     template <typename R_ext, typename C_ext, typename I_ext>
     void LoadParameters(const R_ext* kappa_list, const C_ext* coeff_list, I_ext wave_count_, I_ext wave_chunk_size_,
@@ -544,6 +608,8 @@ private:
     std::vector<double> vertex_coords_;
     std::vector<int64_t> triangles_;
     mutable std::vector<Real> tri_[4];
+    mutable std::vector<float> vcoords_, tcoords_;
+    mutable Complex p0_dummy_;
 
     const Real* tri_data(int which) const
     {
@@ -555,6 +621,21 @@ private:
             baemm_get_triangle_data(h_, p[0], p[1], p[2], p[3]);
         }
         return tri_[which].data();
+    }
+
+    // Element (i, k) of the handle's host copy of P0 buffer `which` (kept by
+    // the library; brought up to date with the device on each access).
+    Complex& p0_ref(int which, Int i, Int k) const
+    {
+        float* data = nullptr;
+        int64_t ld = 0;
+        const int st = baemm_p0_buffer(h_, which, &data, &ld);
+        if (st != BAEMM_OK || !data) {
+            detail::report(h_, st == BAEMM_OK ? BAEMM_ERR_STATE : st, which == 0 ? "B" : "C");
+            p0_dummy_ = Complex(0.0f, 0.0f);
+            return p0_dummy_;
+        }
+        return reinterpret_cast<Complex*>(data)[ld * (int64_t)i + k];
     }
 
     bool check_wc(Int WC, Int W, const char* where) const

@@ -118,7 +118,6 @@ int baemm_get_precision(baemm_handle h);
  * knob, no reference counterpart); 0 selects automatically. */
 int baemm_set_segments(baemm_handle h, int segments);
 
-/* gmres_max_iter / gmres_max_restarts public members, MemberVariables.hpp:64-65. */
 /* No reference counterpart: the MFMA kernel variant of the mixed precision
  * for the boundary operators.
  * 0 = one wave generates its kernel values and issues their MFMAs;
@@ -127,11 +126,9 @@ int baemm_set_segments(baemm_handle h, int segments);
  * potential always use the wave-specialised kernel. */
 int baemm_set_mixed_kernel(baemm_handle h, int variant);
 
+/* gmres_max_iter / gmres_max_restarts public members, MemberVariables.hpp:64-65. */
 int baemm_set_gmres_limits(baemm_handle h, int max_iter, int max_restarts);
 
-/* TriangleMidpoints / TriangleNormals (float4 padded, 4*T floats),
- * TriangleAreas, SingleLayerDiagonal (T floats), GetSetters.hpp:95-113.
- * Any pointer may be NULL. */
 /* No reference counterpart: capture each GMRES Arnoldi step (operator apply,
  * PCG preconditioner, Gram-Schmidt, normalisation) in a hipGraph and replay
  * it (on != 0), instead of launching its ~100 kernels one by one.  A graph is
@@ -140,8 +137,48 @@ int baemm_set_gmres_limits(baemm_handle h, int max_iter, int max_restarts);
 int baemm_set_graphs(baemm_handle h, int on);
 int baemm_get_graph_stats(baemm_handle h, int64_t* captures, int64_t* launches);
 
+/* TriangleMidpoints / TriangleNormals (float4 padded, 4*T floats),
+ * TriangleAreas, SingleLayerDiagonal (T floats), GetSetters.hpp:95-113.
+ * Any pointer may be NULL. */
 int baemm_get_triangle_data(baemm_handle h, float* midpoints4, float* normals4,
                             float* areas, float* single_diag);
+
+/* VertexCoordinates / TriangleCoordinates, Helmholtz_OpenCL.hpp:333-341 (FP32):
+ * vertex_coords V x 3; triangle_coords T x 12 (the three vertices of each
+ * triangle as x, y, z, 0, Initialize.hpp:82-93).  Any pointer may be NULL. */
+int baemm_get_coordinates(baemm_handle h, float* vertex_coords, float* triangle_coords);
+
+/* DeviceInfo, Helmholtz_OpenCL.hpp:343-391: vendor, name, architecture,
+ * driver and memory sizes of the handle's GPU as text (NUL-terminated,
+ * truncated to size bytes). */
+int baemm_device_info(baemm_handle h, char* buf, int64_t size);
+
+/* PrintBooleans, LoadParameters.hpp:85-95: the eight flags of the loaded
+ * parameters, flags[2j] = Re and flags[2j+1] = Im of coefficient column j
+ * (mass matrix, single layer, double layer, adjoint double layer). */
+int baemm_get_flags(baemm_handle h, int* flags);
+
+/* ReadB / WriteB / ReadC / WriteC, InputOutput.hpp:30-89: the reference's
+ * P0 buffers, simplex_count x wave_count complex float (interleaved, the
+ * reference's internal Complex), rows = triangles.  B holds the triangle
+ * values b = Av B_in of the last operator apply (boundary, far field or
+ * potential), C the triangle values (K + D) b of the last boundary-operator
+ * apply before AvT (ApplyBoundaryOperators.hpp:120-140), also when the apply
+ * ran inside a solver.  Read* overwrite the handle's copy (ld_input >=
+ * wave_count) until the next apply replaces it, as in the reference.
+ * Write* return BAEMM_ERR_STATE while the buffer has no value: B before the
+ * first apply; C before the first boundary apply or after a Herglotz wave,
+ * incident wave, batched or no-allocation apply rewrote the workspaces. */
+int baemm_read_b(baemm_handle h, const float* input, int64_t ld_input, int64_t wave_count);
+int baemm_write_b(baemm_handle h, float* output, int64_t ld_output);
+int baemm_read_c(baemm_handle h, const float* input, int64_t ld_input, int64_t wave_count);
+int baemm_write_c(baemm_handle h, float* output, int64_t ld_output);
+
+/* B(i,k) / C(i,k), InputOutput.hpp:5-25: the handle's host copy of buffer
+ * `which` (0 = B, 1 = C), brought up to date with the device, as *data
+ * (complex float interleaved, element (i, k) at 2 (ld i + k)); valid until
+ * the next call on h.  Same states as baemm_write_b/_c. */
+int baemm_p0_buffer(baemm_handle h, int which, float** data, int64_t* ld);
 
 /* MassMatrix(), GetSetters.hpp:18-21: CSR of the P1 mass matrix
  * (V x V, nnz = baemm_mass_nnz).  Any pointer may be NULL. */

@@ -170,8 +170,8 @@ def test_cpp_dropin_example_matches_python(tmp_path):
     Vm, Tm = icosphere(4)
     write_mesh_file(tmp_path / "obstacle.txt", V, T)
     write_mesh_file(tmp_path / "meas.txt", Vm, Tm)
-    out = subprocess.run([str(exe), str(tmp_path / "obstacle.txt"), str(tmp_path / "meas.txt")],
-                         capture_output=True, text=True, timeout=240)
+    out = subprocess.run([str(exe), str(tmp_path / "obstacle.txt"), str(tmp_path / "meas.txt"), "--out",
+                          str(tmp_path)], capture_output=True, text=True, timeout=240)
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     norm_cpp = float(re.search(r"L\^2-norm of C = (\S+)", out.stdout).group(1))
@@ -194,8 +194,18 @@ def test_cpp_dropin_example_matches_python(tmp_path):
     with Helmholtz_HIP(V, T, Vm) as H:
         H.UseDiagonal(True)
         F = H.FarField([5 * np.pi], 1, inc, 16, cg_tol=1e-5, gmres_tol=1e-4)
+        DFh, du = H.Derivative_FF([5 * np.pi], 1, inc, 16, V.copy(), cg_tol=1e-5, gmres_tol=1e-4)
+        w, _ = H.AdjointDerivative_FF([5 * np.pi], 1, inc, 16, DFh, du_dn=du, cg_tol=1e-5, gmres_tol=1e-4)
     norm_py = np.sqrt(2 * np.pi / (Vm.shape[0] * 16)) * np.linalg.norm(F)
     assert abs(norm_cpp - norm_py) < 1e-6 * norm_py
+    # the result files of Example_OpenCL/main.cpp:191,220,233 hold the same arrays
+    def cplx(name):
+        a = np.loadtxt(tmp_path / name)
+        return a[:, 0::2] + 1j * a[:, 1::2]
+
+    assert relerr(cplx("F_New.txt"), F) < 1e-12
+    assert relerr(cplx("DF_New.txt"), DFh) < 1e-12
+    assert relerr(np.loadtxt(tmp_path / "DFT_New.txt"), w) < 1e-12
 
 
 def test_graph_captured_arnoldi_steps_match_eager(small):
