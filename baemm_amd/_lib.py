@@ -70,6 +70,8 @@ SIGNATURES = {
     "baemm_autotune_segments": (_INT, [_P, _INT, ctypes.POINTER(_INT), ctypes.POINTER(_D)]),
     "baemm_apply_boundary_dev": (_INT, [_P, _P, _P, _I64, _INT, _P, _P, _I64, _INT, _P]),
     "baemm_apply_far_field_dev": (_INT, [_P, _P, _P, _I64, _INT, _P, _P, _I64, _INT, _P]),
+    "baemm_apply_boundary_dev_noalloc": (_INT, [_P, _P, _P, _I64, _INT, _P, _P, _I64, _INT, _P]),
+    "baemm_noalloc_scratch_bytes": (_I64, []),
     "baemm_apply_boundary_batch": (_INT, [_INT, _P, _P, _P, _P, _INT, _P, _P, _P, _INT, _P]),
     "baemm_batch_set_tuning": (_INT, [_INT, _INT, _INT, _INT]),
     "baemm_batch_get_tuning": (_INT, [ctypes.POINTER(_INT)] * 4),

@@ -256,6 +256,23 @@ int baemm_apply_boundary_dev(baemm_handle h, const double* alpha, const void* B,
 int baemm_apply_far_field_dev(baemm_handle h, const double* alpha, const void* B, int64_t ldB, int B_type,
                               const double* beta, void* F, int64_t ldF, int F_type, void* stream);
 
+/* No reference counterpart: baemm_apply_boundary_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; it calls no
+ * hipMalloc/hipFree: every workspace of the apply lives in one fixed device
+ * array of baemm_noalloc_scratch_bytes() bytes (256 MiB) that each GPU holds
+ * from the first call on it until the process ends.  An apply that needs more
+ * (mixed precision at 64 columns: about 1.5 KB per triangle plus 0.5 KB
+ * per triangle and source segment, so meshes up to roughly 50,000 triangles)
+ * fails with BAEMM_ERR_ARG before launching anything.  All calls on one GPU share that
+ * array, so calls that overlap on one GPU (other streams or threads) race;
+ * the array keeps the last call's contents.  The first mixed-precision call
+ * of a handle synchronises its stream once (minimum centroid distance). */
+int baemm_apply_boundary_dev_noalloc(baemm_handle h, const double* alpha, const void* B, int64_t ldB, int B_type,
+                                     const double* beta, void* C, int64_t ldC, int C_type, void* stream);
+/* Size of the fixed scratch array of baemm_apply_boundary_dev_noalloc. */
+int64_t baemm_noalloc_scratch_bytes(void);
+
 /* ------------------------------------------------------------ batching
  * No reference counterpart.  Applies the loaded boundary operators of n
  * handles (different meshes, wave numbers and W) in one persistent launch per
