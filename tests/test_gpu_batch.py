@@ -112,3 +112,60 @@ def test_batch_rejects_fp32_mfma_instances(batch):
             ApplyBoundaryOperatorsBatch([H], [batch[0]["B"]])
     finally:
         H.SetPrecision(batch[0]["prec"])
+
+
+def _c4_mix(seed=3):
+    """The batched-instances workload: 24 perturbed icospheres (radius
+    1 + 0.1 sum of real spherical harmonics of degree <= 4 with seeded
+    N(0,1)/(l+1)^2 weights) of 1,280, 5,120 or 20,480 triangles, kappa uniform
+    in [1, 16], W = 2^p columns with p uniform in 0..6, SL+DL with complex
+    coefficients; the same draws as the batch leg of bench.py."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(24):
+        level = int(rng.integers(3, 6))
+        V, T = perturbed_sphere(level, seed=1000 + k)
+        W = 2 ** int(rng.integers(0, 7))
+        kappa = [float(rng.uniform(1.0, 16.0))]
+        coeff = np.array([0.0, 1.0 + 0.2j, 1.0 - 0.1j, 0.0])
+        B = rng.standard_normal((V.shape[0], W)) + 1j * rng.standard_normal((V.shape[0], W))
+        out.append(dict(V=V, T=T, W=W, kappa=kappa, coeff=coeff, B=B))
+    return out
+
+
+def test_c4_mix_in_one_launch_matches_oracle():
+    """The full C4 mix (24 instances, including the 20,480-triangle meshes) in
+    one persistent mixed-precision launch with autotuned tile shapes: every
+    instance within 1e-5 of the FP64 oracle on 48 sampled vertex rows, and
+    bit-identical to the same instance batched alone."""
+    mix = _c4_mix()
+    assert {it["T"].shape[0] for it in mix} == {1280, 5120, 20480}
+    Hs = []
+    try:
+        for it in mix:
+            H = Helmholtz_HIP(it["V"], it["T"])
+            H.SetPrecision("mixed")
+            H.LoadBoundaryOperators_PL(it["kappa"], it["coeff"], it["W"], it["W"])
+            Hs.append(H)
+        Bs = [it["B"] for it in mix]
+        saved = BatchTuning()
+        BatchAutotune(Hs, Bs, repeats=1)
+        out = ApplyBoundaryOperatorsBatch(Hs, Bs)
+        rng = np.random.default_rng(4)
+        worst = 0.0
+        for k, (it, C) in enumerate(zip(mix, out)):
+            m = oracle.Mesh(it["V"], it["T"])
+            rows = np.sort(rng.choice(m.nv, 48, replace=False))
+            ref = oracle.apply_boundary_rows(m, 1.0, it["B"], 0.0, np.zeros((48, it["W"])), it["kappa"],
+                                             it["coeff"], it["W"], it["W"], rows)
+            e = relerr(C[rows], ref)
+            worst = max(worst, e)
+            assert e < 1e-5, (k, it["T"].shape[0], it["W"], e)
+        print("C4 mix: worst sampled-row error vs FP64", worst)
+        for k in (0, 11, 23):
+            alone = ApplyBoundaryOperatorsBatch([Hs[k]], [Bs[k]])
+            assert np.array_equal(alone[0], out[k]), k
+        BatchTuning(**saved)
+    finally:
+        for H in Hs:
+            H.close()

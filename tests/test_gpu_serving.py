@@ -130,3 +130,53 @@ def test_admission_by_memory(reqs):
         rid = srv.submit(reqs[0]["V"], reqs[0]["T"], MEAS, reqs[0]["kappa"], reqs[0]["dirs"])
         with pytest.raises(BaemmError):
             srv.wait(rid)
+
+
+def test_c5_mix_with_large_meshes_alone_and_in_the_mix():
+    """Serving at the sizes of the request stream: perturbed icospheres of 320
+    to 327,680 triangles, kappa up to 32 (at least 10 points per wavelength),
+    1 to 64 columns, GMRES to 1e-6, mixed and FP32 requests.  Every request
+    completes; each result is bit-identical run alone and in the mix of four
+    concurrent slots; the two small requests match the FP64 oracle's dense
+    solves (FarField, Derivative_FF_Helper, AdjointDerivative_FF)."""
+    import oracle
+
+    rng = np.random.default_rng(55)
+    specs = [  # level, kappa, wcs, precision
+        (2, [2.0], 64, "mixed"),
+        (3, [2.0, 4.0], 2, "fp32"),
+        (4, [8.0], 1, "mixed"),
+        (6, [16.0], 8, "mixed"),
+        (7, [32.0], 2, "mixed"),
+    ]
+    reqs = []
+    for k, (level, kappa, wcs, prec) in enumerate(specs):
+        V, T = perturbed_sphere(level, seed=300 + k)
+        dirs = random_directions(wcs, seed=400 + k)
+        W = len(kappa) * wcs
+        fobs = 0.1 * (rng.standard_normal((MEAS.shape[0], W)) + 1j * rng.standard_normal((MEAS.shape[0], W)))
+        reqs.append(dict(V=V, T=T, kappa=kappa, dirs=dirs, fobs=fobs, mesh_id=5000 + k, prec=prec))
+    assert [r["T"].shape[0] for r in reqs] == [320, 1280, 5120, 81920, 327680]
+    alone = []
+    with GNServer(slots=1) as srv:
+        for r in reqs:
+            alone.extend(_run(srv, [r]))
+    with GNServer(slots=4) as srv:
+        mixed = _run(srv, reqs[::-1])[::-1]   # the largest first, so the small ones run beside it
+        st = srv.stats()
+    print("C5 mix stats", st)
+    assert st["completed"] == len(reqs) and st["failed"] == 0 and st["max_in_flight"] >= 2
+    for k, (a, b) in enumerate(zip(alone, mixed)):
+        assert a[0] == 0 and b[0] == 0, (k, a[0], b[0])
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), k
+    for k in (0, 1):
+        r = reqs[k]
+        m = oracle.Mesh(r["V"], r["T"], MEAS)
+        wcs = r["dirs"].shape[0]
+        F, _ = oracle.far_field(m, r["kappa"], r["dirs"], wcs)
+        du = oracle.derivative_ff_helper(m, r["kappa"], r["dirs"], wcs)
+        G, _ = oracle.adjoint_derivative_ff(m, r["kappa"], r["dirs"], wcs, F - r["fobs"], du)
+        eF, eG = relerr(alone[k][1], F), relerr(alone[k][2], G)
+        print(f"C5 request {k} ({r['T'].shape[0]} triangles, {r['prec']}): far field vs FP64 {eF:.2e}, "
+              f"gradient vs FP64 {eG:.2e}")
+        assert eF < 1e-5 and eG < 2e-5, k   # measured 1.4e-7 to 1.2e-6
