@@ -152,6 +152,9 @@ int baemm_destroy(baemm_handle h)
         free_device(*hh);
         if (hh->ev0) (void)hipEventDestroy(hh->ev0);
         if (hh->ev1) (void)hipEventDestroy(hh->ev1);
+        for (hipEvent_t e : hh->ev_step)
+            if (e) (void)hipEventDestroy(e);
+        if (hh->pin_ctl) (void)hipHostFree(hh->pin_ctl);
         if (hh->stream) (void)hipStreamDestroy(hh->stream);
     }
     delete hh;

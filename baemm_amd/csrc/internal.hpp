@@ -95,6 +95,7 @@ struct Workspace {
     DevBuf a, b, c, d, e;                 // composition temporaries
     DevBuf cgctr;                         // PCG counters: done, iterations, total
     DevBuf sc;                            // per-column scales of the Krylov vectors
+    DevBuf gm;                            // GMRES Hessenberg / Givens state on the device
 };
 
 struct SolverStats {
@@ -187,6 +188,11 @@ struct Handle {
     // step saves its launch overheads, 4-15 % of a solve on small meshes)
     bool use_graphs = true;
     bool capturing = false;              // a capture is being recorded on the handle's stream
+    // Device stop flag of a GMRES cycle (null outside one): the pair kernels and
+    // the PCG of an Arnoldi step return at once when it is set.
+    const int* step_skip = nullptr;
+    int* pin_ctl = nullptr;              // pinned host copies of the GMRES flags (2 slots x 4 ints, 2 doubles)
+    hipEvent_t ev_step[2] = {nullptr, nullptr};
     std::vector<GraphSlot> graphs;
     int64_t graph_captures = 0, graph_launches = 0;
     // profiling of the dominant pair kernel

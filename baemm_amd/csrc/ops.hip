@@ -515,7 +515,7 @@ CoefPack make_pack(const Params& p, int q0, int kg, const float* scale_per_chunk
     if (WCSP == 64 && KG == 1) return &pair_plain_kernel<OP, 64, 1, ADJ>;
 
 using PlainFn = void (*)(const float4*, const float4*, int, const float4*, const float4*, int, const float2*, int,
-                         CoefPack, float*, int64_t, int, int, int, int);
+                         CoefPack, float*, int64_t, int, int, int, int, const int*);
 
 static PlainFn plain_fn(int op, bool adj, int WCSP, int KG)
 {
@@ -528,10 +528,10 @@ static PlainFn plain_fn(int op, bool adj, int WCSP, int KG)
 }
 
 using MixedFn = void (*)(const float4*, const float4*, int, const uint4*, int, CoefPack, float*, int64_t, int, int,
-                         int);
+                         int, const int*);
 
 using WsFn = void (*)(const float4*, int, int, const float4*, const float4*, int, const uint4*, int, CoefPack, float*,
-                      int64_t, int, int, int);
+                      int64_t, int, int, int, const int*);
 
 static WsFn mixed_ws_fn(bool adj, int WCSP, int KG, bool f32)
 {
@@ -639,7 +639,7 @@ static void run_plain(Handle& h, int op, bool adj, int WCSP, const float4* tx, c
         PlainFn fn = plain_fn(op, adj, WCSP, kg);
         dim3 grid(ntgt_pad / kTile, S);
         hipLaunchKernelGGL(fn, grid, dim3(kTile), 0, s, tx, tn, ntgt, sy, sn, nsrc, b, ldb, cp, part, seg_stride, ldp,
-                           q0, seg_len, nsrc_pad);
+                           q0, seg_len, nsrc_pad, op == OP_BDR ? h.step_skip : nullptr);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;
     }
@@ -950,10 +950,10 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
         if (ws)
             hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32), dim3(tiles, pl.S), dim3(512), 0, s, h.d_mid,
                                (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
-                               pl.ldp, q0, pl.steps);
+                               pl.ldp, q0, pl.steps, h.step_skip);
         else
             hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg), dim3(tiles, pl.S), dim3(256), 0, s, h.d_mid, pl.nrm,
-                               (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride, pl.ldp, q0, pl.steps);
+                               (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride, pl.ldp, q0, pl.steps, h.step_skip);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;
     }
@@ -1090,7 +1090,8 @@ static void apply_near_mfma(Handle& h, const ApplyArgs& a, const float4* d_pts, 
         const int kg = std::min(pick_kg(pl.WCSP, (int)p.K - q0, true), 2);
         const CoefPack cp = make_pack(p, q0, kg, pl.sA.data(), pl.rot.data());
         hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32), dim3(tiles, S), dim3(512), 0, s, d_pts, P, 0,
-                           h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, seg_stride, pl.ldp, q0, steps);
+                           h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, seg_stride, pl.ldp, q0, steps,
+                           nullptr);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;
     }

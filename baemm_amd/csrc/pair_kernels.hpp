@@ -195,8 +195,10 @@ __global__ __launch_bounds__(kTile) void pair_plain_kernel(
     const float4* __restrict__ tx, const float4* __restrict__ tn, int ntgt,
     const float4* __restrict__ sy, const float4* __restrict__ sn_, int nsrc,
     const float2* __restrict__ b, int ldb, CoefPack cp,
-    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int seg_len, int nsrc_pad)
+    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int seg_len, int nsrc_pad,
+    const int* __restrict__ skip)
 {
+    if (skip && *skip) return;   // a converged solve: the step's result is not used
     __shared__ float4 s_y[kSrcBlock];
     __shared__ float4 s_n[kSrcBlock];
     __shared__ float2 s_b[kSrcBlock * WCSP * KG];
@@ -518,8 +520,9 @@ template <int WCSP, int KG, bool ADJ>
 __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
     const uint4* __restrict__ bop, int nst, CoefPack cp,
-    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
+    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;   // a converged solve: the step's result is not used
     using L = MixedLds<WCSP, KG>;
     __shared__ uint4 s_op[L::OP_ELEMS];
     __shared__ float4 s_geo[L::GEO_ELEMS];
@@ -745,8 +748,9 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
     const float4* __restrict__ tgt, int ntgt, int self,
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
     const uint4* __restrict__ bop, int nst, CoefPack cp,
-    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg)
+    float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;   // a converged solve: the step's result is not used
     using L = WsLds<WCSP, KG>;
     __shared__ uint4 s_op[L::B_ELEMS];
     __shared__ uint4 s_a[L::A_ELEMS];

@@ -248,11 +248,14 @@ __global__ __launch_bounds__(64) void gs_dots_sum_kernel(int nblocks, int nbW, c
 }
 
 // w[r][k] -= sum_{i<nb} h[i][k] V_i[r][k]   (or x += ... with sign = +1)
+// nbp (may be null): the number of basis vectors is min(nb, *nbp).
 __global__ void gs_update_kernel(int n, int W, int nb, const double2* __restrict__ V, int64_t vstride,
-                                 const double2* __restrict__ h, double2* __restrict__ w, double sign)
+                                 const double2* __restrict__ h, double2* __restrict__ w, double sign,
+                                 const int* __restrict__ nbp)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * W) return;
+    if (nbp) nb = min(nb, *nbp);
     const int k = (int)(t % W);
     double sr = 0.0, si = 0.0;
     for (int i0 = 0; i0 < nb; i0 += GS_BATCH) {
@@ -292,8 +295,9 @@ __global__ void scale_cols_kernel(int n, int W, const double2* __restrict__ src,
 // solve (all columns converged or the iteration limit reached).
 
 // active = bb > 0; done = no column active; iterations = 0
+// skip (may be null): a converged GMRES cycle; the PCG is then done at once.
 __global__ void cg_init_kernel(int ncols, const double* __restrict__ bb, int* __restrict__ active,
-                               int* __restrict__ done, int* __restrict__ iters)
+                               int* __restrict__ done, int* __restrict__ iters, const int* __restrict__ skip)
 {
     __shared__ int any;
     if (threadIdx.x == 0) any = 0;
@@ -305,7 +309,7 @@ __global__ void cg_init_kernel(int ncols, const double* __restrict__ bb, int* __
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        *done = !any;
+        *done = !any || (skip && *skip);
         *iters = 0;
         done[3] = 0;   // ticket and active count of cg_check_beta_kernel
         done[4] = 0;
@@ -314,8 +318,10 @@ __global__ void cg_init_kernel(int ncols, const double* __restrict__ bb, int* __
 
 // dst (n x ncols, ld ncols) = src (ld lds); zero (n x ncols, ld ldz) = 0
 __global__ void cg_start_kernel(int64_t n, int ncols, const double* __restrict__ src, int64_t lds,
-                                double* __restrict__ dst, double* __restrict__ zero, int64_t ldz)
+                                double* __restrict__ dst, double* __restrict__ zero, int64_t ldz,
+                                const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * ncols) return;
     const int64_t v = t / ncols;
@@ -413,9 +419,10 @@ void dev_gs_dots(Workspace& ws, int64_t n, int W, int nb, const double2* V, int6
 }
 
 void dev_gs_update(int64_t n, int W, int nb, const double2* V, int64_t vstride, const double2* hcoef, double2* w,
-                   double sign, hipStream_t s)
+                   double sign, hipStream_t s, const int* nbp)
 {
-    hipLaunchKernelGGL(gs_update_kernel, grid_for(n * W), dim3(256), 0, s, (int)n, W, nb, V, vstride, hcoef, w, sign);
+    hipLaunchKernelGGL(gs_update_kernel, grid_for(n * W), dim3(256), 0, s, (int)n, W, nb, V, vstride, hcoef, w, sign,
+                       nbp);
     BAEMM_HIP(hipGetLastError());
 }
 
@@ -429,6 +436,187 @@ void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, doub
 {
     hipLaunchKernelGGL(scale_cols_kernel, grid_for(n * W), dim3(256), 0, s, (int)n, W, src, sc, dst);
     BAEMM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- GMRES on the device
+// The Hessenberg matrix, Givens rotations and least squares of the GMRES
+// cycles, in FP64 on the device (one work-group, threads over the columns),
+// so that an Arnoldi step needs no host round trip.  State per column k:
+// H[k][i][j] ((m+1) x m, rotated in place), rotations cs[k][j], sn[k][j],
+// rotated right-hand side g[k][0..m], initial norm bnorm[k], residual res[k].
+// ctl[0]: stop flag (set once every column meets tol; the pair kernels and
+// the PCG of later steps then return at once, and later Givens kernels do
+// nothing), ctl[1]: completed steps of the cycle, ctl[2]: converged.
+// worst[0]: the largest relative residual over the columns.
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// Largest value over the work-group (256 threads); every thread gets it.
+__device__ double block_max(double v)
+{
+    __shared__ double sm[4];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double r = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+    __syncthreads();
+    return r;
+}
+
+// Start of a cycle from the preconditioned residual r (its column norms^2 in
+// hv[k].x): beta_k = |r_k|, bnorm_k = beta_k on the first cycle, g_k = beta_k e_1,
+// sc_k = 1 / beta_k (the caller scales r to V_0 with it).
+__global__ __launch_bounds__(256) void gm_start_kernel(int W, int m, const double2* __restrict__ hv, int first,
+                                                       double tol, double* __restrict__ bnorm,
+                                                       double* __restrict__ res, double2* __restrict__ g,
+                                                       double* __restrict__ sc, int* __restrict__ ctl,
+                                                       double* __restrict__ worst)
+{
+    double wmax = 0.0;
+    for (int k = threadIdx.x; k < W; k += blockDim.x) {
+        const double beta = sqrt(fmax(0.0, hv[k].x));
+        if (first) bnorm[k] = beta;
+        const double r = bnorm[k] > 0.0 ? beta / bnorm[k] : 0.0;
+        res[k] = r;
+        wmax = fmax(wmax, r);
+        sc[k] = beta > 0.0 ? 1.0 / beta : 0.0;
+        double2* gk = g + (int64_t)k * (m + 1);
+        gk[0] = make_double2(beta, 0.0);
+        for (int i = 1; i <= m; ++i) gk[i] = make_double2(0.0, 0.0);
+    }
+    wmax = block_max(wmax);
+    if (threadIdx.x == 0) {
+        worst[0] = wmax;
+        ctl[1] = 0;
+        ctl[2] = wmax <= tol ? 1 : 0;
+        ctl[0] = ctl[2];
+    }
+}
+
+// Step j of a cycle: Hessenberg column j from the Gram-Schmidt projections
+// (h1 + h2, rows 0..j) and the new vector's norm^2 (h2[j+1].x), the previous
+// rotations applied, a new rotation that zeroes H[j+1][j], g rotated.
+__global__ __launch_bounds__(256) void gm_givens_kernel(int W, int m, int j, const double2* __restrict__ h1,
+                                                        const double2* __restrict__ h2, double tol,
+                                                        const double* __restrict__ bnorm, double2* __restrict__ H,
+                                                        double2* __restrict__ cs, double2* __restrict__ sn,
+                                                        double2* __restrict__ g, double* __restrict__ res,
+                                                        int* __restrict__ ctl, double* __restrict__ worst)
+{
+    if (ctl[0]) return;   // converged at an earlier step of this cycle
+    double wmax = 0.0;
+    for (int k = threadIdx.x; k < W; k += blockDim.x) {
+        double2* Hk = H + (int64_t)k * (m + 1) * m;
+        double2* ck = cs + (int64_t)k * m;
+        double2* sk = sn + (int64_t)k * m;
+        double2* gk = g + (int64_t)k * (m + 1);
+        for (int i = 0; i <= j; ++i) Hk[i * m + j] = zadd(h1[i * W + k], h2[i * W + k]);
+        const double hn = sqrt(fmax(0.0, h2[(j + 1) * W + k].x));
+        for (int i = 0; i < j; ++i) {
+            const double2 c = ck[i], s = sk[i];
+            const double2 a = Hk[i * m + j], b = Hk[(i + 1) * m + j];
+            Hk[i * m + j] = zadd(zmul(c, a), zmul(s, b));
+            Hk[(i + 1) * m + j] = zsub(zmul(c, b), zmul(zconj(s), a));
+        }
+        const double2 a = Hk[j * m + j];
+        const double na = sqrt(a.x * a.x + a.y * a.y), nu = sqrt(na * na + hn * hn);
+        double2 c, s;
+        if (nu == 0.0) {
+            c = make_double2(1.0, 0.0);
+            s = make_double2(0.0, 0.0);
+        } else if (na == 0.0) {
+            c = make_double2(0.0, 0.0);
+            s = make_double2(1.0, 0.0);
+        } else {
+            c = make_double2(na / nu, 0.0);
+            s = make_double2(a.x / na * hn / nu, a.y / na * hn / nu);
+        }
+        ck[j] = c;
+        sk[j] = s;
+        Hk[j * m + j] = zadd(zmul(c, a), make_double2(s.x * hn, s.y * hn));
+        Hk[(j + 1) * m + j] = make_double2(0.0, 0.0);
+        const double2 gj = gk[j];
+        gk[j] = zmul(c, gj);
+        const double2 gn = zmul(zconj(s), gj);
+        gk[j + 1] = make_double2(-gn.x, -gn.y);
+        const double r = bnorm[k] > 0.0 ? sqrt(gk[j + 1].x * gk[j + 1].x + gk[j + 1].y * gk[j + 1].y) / bnorm[k]
+                                        : 0.0;
+        res[k] = r;
+        wmax = fmax(wmax, r);
+    }
+    wmax = block_max(wmax);
+    if (threadIdx.x == 0) {
+        worst[0] = wmax;
+        ctl[1] = j + 1;
+        if (wmax <= tol) {
+            ctl[2] = 1;
+            ctl[0] = 1;
+        }
+    }
+}
+
+// End of a cycle: y_k = R_k^{-1} g_k over the ctl[1] completed steps (upper
+// triangular back substitution), written as y[i * W + k] for the basis update.
+__global__ __launch_bounds__(256) void gm_backsub_kernel(int W, int m, const double2* __restrict__ H,
+                                                         const double2* __restrict__ g, const int* __restrict__ ctl,
+                                                         double2* __restrict__ y)
+{
+    const int jd = ctl[1];
+    for (int k = threadIdx.x; k < W; k += blockDim.x) {
+        const double2* Hk = H + (int64_t)k * (m + 1) * m;
+        const double2* gk = g + (int64_t)k * (m + 1);
+        for (int i = jd - 1; i >= 0; --i) {
+            double2 acc = gk[i];
+            for (int l = i + 1; l < jd; ++l) acc = zsub(acc, zmul(Hk[i * m + l], y[l * W + k]));
+            const double2 d = Hk[i * m + i];
+            const double dd = d.x * d.x + d.y * d.y;
+            y[i * W + k] = dd > 0.0 ? make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd)
+                                    : make_double2(0.0, 0.0);
+        }
+    }
+}
+
+void dev_gm_start(int W, int m, const double2* hv, bool first, double tol, GmState& st, double* sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(gm_start_kernel, dim3(1), dim3(256), 0, s, W, m, hv, first ? 1 : 0, tol, st.bnorm, st.res,
+                       st.g, sc, st.ctl, st.worst);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_gm_givens(int W, int m, int j, const double2* h1, const double2* h2, double tol, GmState& st,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(gm_givens_kernel, dim3(1), dim3(256), 0, s, W, m, j, h1, h2, tol, st.bnorm, st.H, st.cs,
+                       st.sn, st.g, st.res, st.ctl, st.worst);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_gm_backsub(int W, int m, GmState& st, double2* y, hipStream_t s)
+{
+    hipLaunchKernelGGL(gm_backsub_kernel, dim3(1), dim3(256), 0, s, W, m, st.H, st.g, st.ctl, y);
+    BAEMM_HIP(hipGetLastError());
+}
+
+GmState dev_gm_state(Workspace& ws, int W, int m)
+{
+    const size_t nH = (size_t)W * (m + 1) * m, nr = (size_t)W * m, ng = (size_t)W * (m + 1);
+    ws.gm.reserve(sizeof(double2) * (nH + 2 * nr + ng) + sizeof(double) * (2 * (size_t)W + 1) + 16 * sizeof(int));
+    GmState st;
+    st.H = ws.gm.as<double2>();
+    st.cs = st.H + nH;
+    st.sn = st.cs + nr;
+    st.g = st.sn + nr;
+    st.bnorm = reinterpret_cast<double*>(st.g + ng);
+    st.res = st.bnorm + W;
+    st.worst = st.res + W;
+    st.ctl = reinterpret_cast<int*>(st.worst + 1);
+    return st;
 }
 
 // ---------------------------------------------------------------- fused PCG steps
@@ -679,8 +867,10 @@ template <int CGC, int R, bool TRI>
 __global__ __launch_bounds__(kPcgLdsThreads) void pcg_lds_kernel(PcgMatrix A, const double* __restrict__ Dinv,
                                                                  const double* __restrict__ X, int64_t ldX,
                                                                  double* __restrict__ Y, int64_t ldY, int ncols,
-                                                                 double tol2, int max_iter, int* __restrict__ ctr)
+                                                                 double tol2, int max_iter, int* __restrict__ ctr,
+                                                                 const int* __restrict__ skip)
 {
+    if (skip && *skip) return;   // a converged GMRES cycle: the step's result is not used
     constexpr int NT = kPcgLdsThreads, NW = NT / 64;
     const int n = A.n, nnz = A.nnz, nt = A.nt;
     extern __shared__ double pcg_lds[];
@@ -899,7 +1089,7 @@ static void launch_pcg_lds(Handle& h, int nb, size_t lds, const double* X, int64
     const PcgMatrix A{(int)h.nv, (int)h.h_M_idx.size(), (int)h.nt, (int)h.h_vt_idx.size(), h.d_M_ptr, h.d_M_idx,
                       h.d_M_val, h.d_tris, h.d_area, h.d_vt_ptr, h.d_vt_idx};
     hipLaunchKernelGGL((pcg_lds_kernel<CGC, R, TRI>), dim3(nb), dim3(kPcgLdsThreads), lds, s, A, h.d_lumped_inv, X,
-                       ldX, Y, ldY, ncols, tol * tol, max_iter, ctr);
+                       ldX, Y, ldY, ncols, tol * tol, max_iter, ctr, h.step_skip);
 }
 
 // The LDS block PCG for CGC columns per work-group and R rows per thread.
@@ -983,9 +1173,9 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
     int* done = ctr;
     int* iters = ctr + 1;
     // r = X (contiguous), Y = 0
-    hipLaunchKernelGGL(cg_start_kernel, grid_for(N), dim3(256), 0, s, n, ncols, X, ldX, r, Y, ldY);
-    dev_coldot(ws, n, ncols, r, ncols, r, ncols, bb, s);
-    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(256), 0, s, ncols, bb, active, done, iters);
+    hipLaunchKernelGGL(cg_start_kernel, grid_for(N), dim3(256), 0, s, n, ncols, X, ldX, r, Y, ldY, h.step_skip);
+    dev_coldot(ws, n, ncols, r, ncols, r, ncols, bb, s, h.step_skip);
+    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(256), 0, s, ncols, bb, active, done, iters, h.step_skip);
     dev_lumped(h, r, ncols, pa, ncols, ncols, s, done);      // p = z = D r
     dev_coldot(ws, n, ncols, r, ncols, pa, ncols, rz, s, done);
     // four launches per iteration: p = z + beta p (after the first) and q = M p

@@ -27,13 +27,30 @@ void dev_coldot(Workspace& ws, int64_t n, int ncols, const double* X, int64_t ld
 void dev_gs_dots(Workspace& ws, int64_t n, int W, int nb, const double2* V, int64_t vstride, const double2* w,
                  double2* out, hipStream_t s);
 void dev_gs_update(int64_t n, int W, int nb, const double2* V, int64_t vstride, const double2* hcoef, double2* w,
-                   double sign, hipStream_t s);
+                   double sign, hipStream_t s, const int* nbp = nullptr);
 void dev_scale_cols(int64_t n, int W, const double2* src, const double* sc, double2* dst, hipStream_t s);
 void dev_inv_norm(int W, const double2* hn, double* sc, hipStream_t s);
 void dev_sub_c(int64_t N, const double2* b, const double2* t, double2* r, hipStream_t s);
 void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double* Y, int64_t ldY, int ncols, double tol,
                  int max_iter, hipStream_t s);
 int dev_mass_cg_result(Workspace& ws, hipStream_t s);
+
+// GMRES cycle state on the device (Hessenberg, rotations, least squares).
+struct GmState {
+    double2* H = nullptr;   // W x (m+1) x m
+    double2* cs = nullptr;  // W x m
+    double2* sn = nullptr;  // W x m
+    double2* g = nullptr;   // W x (m+1)
+    double* bnorm = nullptr;
+    double* res = nullptr;
+    double* worst = nullptr;
+    int* ctl = nullptr;     // stop flag, completed steps, converged
+};
+GmState dev_gm_state(Workspace& ws, int W, int m);
+void dev_gm_start(int W, int m, const double2* hv, bool first, double tol, GmState& st, double* sc, hipStream_t s);
+void dev_gm_givens(int W, int m, int j, const double2* h1, const double2* h2, double tol, GmState& st,
+                   hipStream_t s);
+void dev_gm_backsub(int W, int m, GmState& st, double2* y, hipStream_t s);
 int* dev_cg_counters(Workspace& ws, hipStream_t s);
 int64_t dev_cg_total(Workspace& ws, hipStream_t s);
 

@@ -6,8 +6,8 @@
 // inputs and outputs cross PCIe.  GMRES: left-preconditioned, restarted
 // (restart length gmres_max_iter, at most gmres_max_restarts cycles), one
 // Krylov space per column advanced together, classical Gram-Schmidt with
-// re-orthogonalisation on the device, Hessenberg / Givens / least squares in
-// FP64 on the host (W x 31 x 30 numbers).  The preconditioner is the mass
+// re-orthogonalisation on the device, and the Hessenberg matrix, Givens
+// rotations and least squares in FP64 on the device as well.  The preconditioner is the mass
 // inverse by PCG, as in the reference (lumped_mass_as_prec_for_intopsQ false,
 // MemberVariables.hpp:95).
 #include <cmath>
@@ -52,17 +52,17 @@ void apply_prec(Handle& h, const double2* in, double2* out, double cg_tol, hipSt
 struct GraphKeyHead {
     const void* ptr[16];
     int64_t val[12];
-    double tol;
+    double tol, gm_tol;
 };
 
-std::vector<char> graph_key(Handle& h, int j, double cg_tol)
+std::vector<char> graph_key(Handle& h, int j, double cg_tol, double gm_tol)
 {
     GraphKeyHead k;
     std::memset(&k, 0, sizeof(k));
     const Workspace& ws = h.sws;
     const void* p[] = {ws.V.ptr, ws.w.ptr, ws.t.ptr, ws.hc.ptr, ws.sc.ptr, ws.red.ptr, ws.cg.ptr, ws.scal.ptr,
                        ws.cgctr.ptr, h.ws_b32.ptr, h.ws_bop.ptr, h.ws_part.ptr, h.ws_scale.ptr, h.ws_misc.ptr,
-                       h.ws_nrm.ptr};
+                       h.ws_nrm.ptr, ws.gm.ptr};
     static_assert(sizeof(p) / sizeof(p[0]) <= 16, "key size");
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); ++i) k.ptr[i] = p[i];
     int rmin_bits = 0;
@@ -71,6 +71,7 @@ std::vector<char> graph_key(Handle& h, int j, double cg_tol)
                          h.gmres_max_iter, rmin_bits, h.mixed_kernel};
     for (size_t i = 0; i < sizeof(v) / sizeof(v[0]); ++i) k.val[i] = v[i];
     k.tol = cg_tol;
+    k.gm_tol = gm_tol;
     std::vector<char> key(reinterpret_cast<const char*>(&k), reinterpret_cast<const char*>(&k) + sizeof(k));
     const auto add = [&](const void* d, size_t n) {
         key.insert(key.end(), static_cast<const char*>(d), static_cast<const char*>(d) + n);
@@ -85,11 +86,11 @@ std::vector<char> graph_key(Handle& h, int j, double cg_tol)
 }
 
 template <class F>
-void run_graph(Handle& h, int j, double cg_tol, hipStream_t s, F&& enqueue)
+void run_graph(Handle& h, int j, double cg_tol, double gm_tol, hipStream_t s, F&& enqueue)
 {
     if ((int)h.graphs.size() <= j) h.graphs.resize(j + 1);
     GraphSlot& g = h.graphs[j];
-    std::vector<char> key = graph_key(h, j, cg_tol);
+    std::vector<char> key = graph_key(h, j, cg_tol, gm_tol);
     if (!g.exec || g.key != key) {
         if (g.exec) {
             // the previous launch of this graph may still run on s
@@ -120,7 +121,38 @@ void run_graph(Handle& h, int j, double cg_tol, hipStream_t s, F&& enqueue)
     h.graph_launches++;
 }
 
+// Pinned host words the GMRES flags are copied to (two slots, one per step in
+// flight: ctl[0..3] as ints, then the largest residual as a double), and the
+// events that mark each step's copy.
+void gm_host_state(Handle& h)
+{
+    if (!h.pin_ctl) BAEMM_HIP(hipHostMalloc((void**)&h.pin_ctl, 64, hipHostMallocDefault));
+    for (hipEvent_t& e : h.ev_step)
+        if (!e) BAEMM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+// Copies the GMRES flags and the largest residual into slot `slot` of the
+// pinned words, after the work enqueued so far, and records the slot's event.
+void gm_snapshot(Handle& h, const GmState& st, int slot, hipStream_t s)
+{
+    int* dst = h.pin_ctl + 8 * slot;
+    BAEMM_HIP(hipMemcpyAsync(dst, st.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    BAEMM_HIP(hipMemcpyAsync(dst + 4, st.worst, sizeof(double), hipMemcpyDeviceToHost, s));
+    BAEMM_HIP(hipEventRecord(h.ev_step[slot], s));
+}
+
 // Solves (P A) X = P B for the loaded boundary operator A on W columns.
+//
+// Left-preconditioned restarted GMRES, one Krylov space per column advanced
+// together.  An Arnoldi step (operator apply, PCG preconditioner, classical
+// Gram-Schmidt twice, normalisation, then the Hessenberg column, the Givens
+// rotations and the residual estimate of every column) runs on the device
+// with no host round trip; the host enqueues step j + 1 before it waits for
+// the flags of step j, so the device never idles between steps.  Once every
+// column meets tol, a device flag makes the pair kernels and the PCG of the
+// step already in flight return at once, and its Givens kernel does nothing;
+// the cycle's least squares (back substitution) uses the steps completed
+// before the flag.  The host synchronises once per cycle start.
 GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, double cg_tol, hipStream_t s)
 {
     Workspace& ws = h.sws;
@@ -140,13 +172,11 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
     double2* hc1 = ws.hc.as<double2>();
     double2* hc2 = hc1 + (m + 2) * W;
     double* sc = ws.sc.as<double>();
+    GmState st = dev_gm_state(ws, W, m);
+    gm_host_state(h);
     const int64_t cg0 = dev_cg_total(ws, s);
 
     BAEMM_HIP(hipMemsetAsync(X, 0, sizeof(double2) * NW, s));
-    std::vector<double> bnorm(W, 0.0), beta(W, 0.0), res(W, 0.0), scale(W);
-    std::vector<cdouble> H((size_t)W * (m + 1) * m), g((size_t)W * (m + 1)), cs_((size_t)W * m), sn_((size_t)W * m);
-    std::vector<cdouble> hh1((size_t)(m + 2) * W), hh2((size_t)(m + 2) * W), y((size_t)(m + 1) * W);
-    auto Hk = [&](int k, int i, int j) -> cdouble& { return H[((size_t)k * (m + 1) + i) * m + j]; };
 
     // Iterative refinement around GMRES(m): every restart cycle starts from the
     // true preconditioned residual P (B - A X), accumulated in FP64 from the
@@ -173,32 +203,21 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
             apply_prec(h, w, V, cg_tol, s);
         }
         dev_gs_dots(ws, n, W, 1, V, NW, V, hc1, s);
-        BAEMM_HIP(hipMemcpyAsync(hh1.data(), hc1, sizeof(double2) * W, hipMemcpyDeviceToHost, s));
-        BAEMM_HIP(hipStreamSynchronize(s));
-        double worst = 0.0;
-        for (int k = 0; k < W; ++k) {
-            beta[k] = std::sqrt(std::max(0.0, hh1[k].real()));
-            if (cyc == 0) bnorm[k] = beta[k];
-            res[k] = bnorm[k] > 0.0 ? beta[k] / bnorm[k] : 0.0;
-            worst = std::max(worst, res[k]);
-        }
-        out.rel_residual = worst;
-        if (worst <= tol) {
+        dev_gm_start(W, m, hc1, cyc == 0, tol, st, sc, s);
+        gm_snapshot(h, st, 0, s);
+        BAEMM_HIP(hipEventSynchronize(h.ev_step[0]));
+        out.rel_residual = *reinterpret_cast<const double*>(h.pin_ctl + 4);
+        if (h.pin_ctl[2]) {
             out.converged = true;
             break;
         }
         out.cycles = cyc + 1;
-        for (int k = 0; k < W; ++k) scale[k] = beta[k] > 0.0 ? 1.0 / beta[k] : 0.0;
-        BAEMM_HIP(hipMemcpyAsync(sc, scale.data(), sizeof(double) * W, hipMemcpyHostToDevice, s));
         dev_scale_cols(n, W, V, sc, V, s);
-        std::fill(g.begin(), g.end(), cdouble(0.0, 0.0));
-        for (int k = 0; k < W; ++k) g[(size_t)k * (m + 1)] = beta[k];
-        int jdone = 0;
-        for (int j = 0; j < m; ++j) {
-            // One Arnoldi step on the device: V_{j+1} = normalised (P A V_j
-            // minus its projections), projections in hc1 + hc2, norm^2 in
-            // hc2[j+1].  The first step of a solve runs eagerly (it sizes every
-            // workspace); later steps replay a captured graph when enabled.
+        // One Arnoldi step: V_{j+1} = normalised (P A V_j minus its
+        // projections), projections in hc1 + hc2, norm^2 in hc2[j+1], then the
+        // Givens update.  The first step of a solve runs eagerly (it sizes
+        // every workspace); later steps replay a captured graph when enabled.
+        auto enqueue_step = [&](int j) {
             auto step = [&]() {
                 double2* Vj = V + (int64_t)j * NW;
                 double2* Vn = V + (int64_t)(j + 1) * NW;
@@ -212,71 +231,42 @@ GmresResult gmres_boundary(Handle& h, const double2* B, double2* X, double tol, 
                 dev_gs_dots(ws, n, W, 1, w, NW, w, hc2 + (j + 1) * W, s);
                 dev_inv_norm(W, hc2 + (j + 1) * W, sc, s);
                 dev_scale_cols(n, W, w, sc, Vn, s);
+                dev_gm_givens(W, m, j, hc1, hc2, tol, st, s);
             };
-            if (h.use_graphs && !(cyc == 0 && j == 0)) {
-                run_graph(h, j, cg_tol, s, step);
-                h.stats.operator_applies++;
-            } else {
-                step();
-            }
-            BAEMM_HIP(hipMemcpyAsync(hh1.data(), hc1, sizeof(double2) * (j + 1) * W, hipMemcpyDeviceToHost, s));
-            BAEMM_HIP(hipMemcpyAsync(hh2.data(), hc2, sizeof(double2) * (j + 2) * W, hipMemcpyDeviceToHost, s));
-            BAEMM_HIP(hipStreamSynchronize(s));
-            worst = 0.0;
-            for (int k = 0; k < W; ++k) {
-                for (int i = 0; i <= j; ++i) Hk(k, i, j) = hh1[(size_t)i * W + k] + hh2[(size_t)i * W + k];
-                const double hn = std::sqrt(std::max(0.0, hh2[(size_t)(j + 1) * W + k].real()));
-                // previous rotations
-                for (int i = 0; i < j; ++i) {
-                    const cdouble c = cs_[(size_t)k * m + i], sn = sn_[(size_t)k * m + i];
-                    const cdouble a = Hk(k, i, j), b = Hk(k, i + 1, j);
-                    Hk(k, i, j) = c * a + sn * b;
-                    Hk(k, i + 1, j) = -std::conj(sn) * a + c * b;
-                }
-                const cdouble a = Hk(k, j, j);
-                const double b = hn;
-                const double na = std::abs(a), nu = std::sqrt(na * na + b * b);
-                cdouble c, sn;
-                if (nu == 0.0) {
-                    c = 1.0;
-                    sn = 0.0;
-                } else if (na == 0.0) {
-                    c = 0.0;
-                    sn = 1.0;
+            h.step_skip = st.ctl;
+            try {
+                if (h.use_graphs && !(cyc == 0 && j == 0)) {
+                    run_graph(h, j, cg_tol, tol, s, step);
+                    h.stats.operator_applies++;
                 } else {
-                    c = na / nu;
-                    sn = (a / na) * b / nu;
+                    step();
                 }
-                cs_[(size_t)k * m + j] = c;
-                sn_[(size_t)k * m + j] = sn;
-                Hk(k, j, j) = c * a + sn * b;
-                Hk(k, j + 1, j) = 0.0;
-                const cdouble gj = g[(size_t)k * (m + 1) + j];
-                g[(size_t)k * (m + 1) + j] = c * gj;
-                g[(size_t)k * (m + 1) + j + 1] = -std::conj(sn) * gj;
-                res[k] = bnorm[k] > 0.0 ? std::abs(g[(size_t)k * (m + 1) + j + 1]) / bnorm[k] : 0.0;
-                worst = std::max(worst, res[k]);
+            } catch (...) {
+                h.step_skip = nullptr;
+                throw;
             }
-            out.iterations++;
-            h.stats.gmres_iterations++;
-            jdone = j + 1;
-            out.rel_residual = worst;
-            if (worst <= tol) break;
-        }
-        // y = R^{-1} g per column (R upper triangular jdone x jdone); X += V y
-        for (int k = 0; k < W; ++k) {
-            for (int i = jdone - 1; i >= 0; --i) {
-                cdouble acc = g[(size_t)k * (m + 1) + i];
-                for (int l = i + 1; l < jdone; ++l) acc -= Hk(k, i, l) * y[(size_t)l * W + k];
-                const cdouble d = Hk(k, i, i);
-                y[(size_t)i * W + k] = (std::abs(d) > 0.0) ? acc / d : cdouble(0.0, 0.0);
+            h.step_skip = nullptr;
+            gm_snapshot(h, st, j & 1, s);
+        };
+        enqueue_step(0);
+        for (int j = 0; j < m; ++j) {
+            if (j + 1 < m) enqueue_step(j + 1);
+            // flags after step j (step j + 1 runs meanwhile)
+            BAEMM_HIP(hipEventSynchronize(h.ev_step[j & 1]));
+            const int* c = h.pin_ctl + 8 * (j & 1);
+            out.rel_residual = *reinterpret_cast<const double*>(c + 4);
+            if (c[1] == j + 1) {
+                out.iterations++;
+                h.stats.gmres_iterations++;
             }
+            if (c[0]) break;
         }
-        BAEMM_HIP(hipMemcpyAsync(hc1, y.data(), sizeof(double2) * jdone * W, hipMemcpyHostToDevice, s));
-        dev_gs_update(n, W, jdone, V, NW, hc1, X, 1.0, s);
-        BAEMM_HIP(hipStreamSynchronize(s));
+        // y = R^{-1} g per column over the completed steps; X += V y
+        dev_gm_backsub(W, m, st, hc1, s);
+        dev_gs_update(n, W, m, V, NW, hc1, X, 1.0, s, st.ctl + 1);
         if (out.rel_residual <= tol) {
             out.converged = true;
+            BAEMM_HIP(hipStreamSynchronize(s));
             break;
         }
     }
