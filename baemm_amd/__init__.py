@@ -8,9 +8,9 @@ utilities.  Importing the package does not load the GPU library; creating a
 """
 from ._lib import BaemmError, LIB_PATH
 from .helmholtz import (ApplyBoundaryOperatorsBatch, BatchAutotune, BatchStats, BatchTuning, Helmholtz_HIP,
-                        WaveType)
+                        SolveLoadedBatch, WaveType)
 from . import meshes
 from .serving import GNServer
 
 __all__ = ["Helmholtz_HIP", "WaveType", "BaemmError", "LIB_PATH", "meshes", "ApplyBoundaryOperatorsBatch",
-           "BatchAutotune", "BatchTuning", "BatchStats", "GNServer"]
+           "BatchAutotune", "BatchTuning", "BatchStats", "GNServer", "SolveLoadedBatch"]

@@ -79,6 +79,7 @@ SIGNATURES = {
     "baemm_batch_autotune": (_INT, [_INT, _P, _P, _P, _INT, _P, _P, _INT, _INT, ctypes.POINTER(_D)]),
     "baemm_server_create": (_INT, [_INT, _INT, _I64, ctypes.POINTER(_P)]),
     "baemm_server_set_policy": (_INT, [_P, _INT, _INT]),
+    "baemm_server_set_fusion": (_INT, [_P, _INT]),
     "baemm_server_destroy": (_INT, [_P]),
     "baemm_server_submit_gn": (_INT, [_P, _P, _P, _P, ctypes.POINTER(_I64)]),
     "baemm_server_wait": (_INT, [_P, _I64, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
@@ -96,6 +97,8 @@ SIGNATURES = {
     "baemm_adjoint_derivative_ff": (_INT, [_P, _P, _I64, _P, _I64, _P, _P, ctypes.POINTER(_P), _INT, _D, _D]),
     "baemm_gauss_newton_solve": (_INT, [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _D, _P, _I64, _D, _P, _I64,
                                         ctypes.POINTER(_P), _INT, _D, _D, _D, ctypes.POINTER(_INT)]),
+    "baemm_gmres_dev": (_INT, [_P, _P, _P, _D, _D, ctypes.POINTER(_I64), _P]),
+    "baemm_gmres_batch": (_INT, [_INT, _P, _P, _P, _D, _D, _P, _P, _P]),
     "baemm_get_solver_stats": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_D),
                                       ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "baemm_reset_solver_stats": (_INT, [_P]),

@@ -3,7 +3,9 @@
 // launch per arithmetic mode.
 //
 // Each instance is prepared as in the single-instance path (Av gather and,
-// for the MFMA path, the scaled hi/lo operand split), then a work list of
+// for the MFMA path, the scaled hi/lo operand split).  An instance large
+// enough to fill the device with its own pair launch runs that launch on its
+// stream.  For the others a work list of
 // items (instance, target tile, source range, chunk group) is built on the
 // host, ordered by decreasing cost, and walked by a persistent grid: every
 // block takes the next item from a device counter (one vector atomic per
@@ -31,6 +33,7 @@ struct BatchInst {
     int ldp, ldb;
     int nt, nt_pad, nst;
     int coef_off;         // first chunk coefficient of the instance in the coefficient table
+    const int* skip;      // GMRES stop flag of the instance (null: none): its items return at once
 };
 
 struct BatchItem {
@@ -83,6 +86,7 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
         if (it >= n_items) break;
         const BatchItem w = items[it];
         const BatchInst in = insts[w.inst];
+        if (in.skip && *in.skip) continue;   // uniform over the block: s_item was read by every thread
         const ChunkCoef* cf = coefs + in.coef_off + w.q0;
         float* out = in.part + (int64_t)w.seg * in.seg_stride;
 #define BAEMM_MIXB(ID, WCSP_, KG_, ADJ_)                                                                          \
@@ -163,6 +167,7 @@ __global__ __launch_bounds__(kTile) void bdr_plain_batch_kernel(const BatchInst*
         if (it >= n_items) break;
         const BatchItem w = items[it];
         const BatchInst in = insts[w.inst];
+        if (in.skip && *in.skip) continue;
         CoefPack cp;
         cp.c[0] = coefs[in.coef_off + w.q0];
         float* out = in.part + (int64_t)w.seg * in.seg_stride;
@@ -235,14 +240,27 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
         bw.sync.push_back(e);
     }
     BAEMM_HIP(hipEventRecord(bw.sync[0], s));
+    // An instance whose own pair launch fills the device at least once runs
+    // that launch (the single-instance kernels, on its stream, beside the
+    // others); the rest share the persistent launch, whose work list they
+    // fill.  The choice depends on the instance alone, so its result is the
+    // same in any mix.
+    std::vector<char> own(n, 0);
     for (int k = 0; k < n; ++k) {
         hipStream_t sk = es[k].h->stream;
         BAEMM_HIP(hipStreamWaitEvent(sk, bw.sync[0], 0));
         p0_invalidate(*es[k].h);   // the batched launch leaves no P0 buffers
-        bdr_begin(*es[k].h, es[k].a, prep[k].pl, sk);
+        BdrPlan& pl = prep[k].pl;
+        bdr_setup(*es[k].h, pl);
+        bdr_begin(*es[k].h, es[k].a, pl, sk);
+        if (pl.kernel_on && pl.rounds >= 1.0) {
+            own[k] = 1;
+            bdr_pairs(*es[k].h, pl, sk);
+        }
         BAEMM_HIP(hipEventRecord(bw.sync[2 + k], sk));
     }
-    for (int k = 0; k < n; ++k) BAEMM_HIP(hipStreamWaitEvent(s, bw.sync[2 + k], 0));
+    for (int k = 0; k < n; ++k)
+        if (!own[k]) BAEMM_HIP(hipStreamWaitEvent(s, bw.sync[2 + k], 0));
 
     for (int mode = 0; mode < 2; ++mode) {     // 0: MFMA instances, 1: FP32 instances
         const bool mixed = (mode == 0);
@@ -253,7 +271,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
         std::vector<int> owner;
         for (int k = 0; k < n; ++k) {
             BdrPlan& pl = prep[k].pl;
-            if (!pl.kernel_on || pl.mixed != mixed) continue;
+            if (!pl.kernel_on || pl.mixed != mixed || own[k]) continue;
             Handle& h = *es[k].h;
             const Params& p = h.prm;
             BatchInst in{};
@@ -267,6 +285,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
             in.nt_pad = (int)h.nt_pad;
             in.nst = (int)(h.nt_pad / 8);
             in.coef_off = (int)coefs.size();
+            in.skip = h.step_skip;
             for (int q = 0; q < p.K; ++q)
                 coefs.push_back(make_pack(p, q, 1, mixed ? pl.sA.data() : nullptr, mixed ? pl.rot.data() : nullptr).c[0]);
             // source ranges of equal length across the batch
@@ -375,7 +394,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
     BAEMM_HIP(hipEventRecord(bw.sync[1], s));
     for (int k = 0; k < n; ++k) {
         hipStream_t sk = es[k].h->stream;
-        BAEMM_HIP(hipStreamWaitEvent(sk, bw.sync[1], 0));
+        if (!own[k]) BAEMM_HIP(hipStreamWaitEvent(sk, bw.sync[1], 0));
         bdr_end(*es[k].h, es[k].a, prep[k].pl, sk);
         BAEMM_HIP(hipEventRecord(bw.sync[2 + n + k], sk));
     }

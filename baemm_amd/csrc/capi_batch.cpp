@@ -214,4 +214,41 @@ int baemm_batch_autotune(int n, const baemm_handle* hs, const void* const* B, co
     });
 }
 
+int baemm_gmres_batch(int n, const baemm_handle* hs, const void* const* B, void* const* X, double cg_tol,
+                      double gmres_tol, int64_t* iterations, int* converged, void* stream)
+{
+    Handle* first = (n > 0 && hs && hs[0]) ? H(hs[0]) : nullptr;
+    return batch_guard(first, [&]() {
+        std::vector<Handle*> v = handles_of(n, hs);
+        if (!B || !X) throw ArgError{"NULL array argument"};
+        std::lock_guard<std::mutex> g(g_batch_mtx);
+        BatchLock lk(v);
+        BAEMM_HIP(hipSetDevice(v[0]->device));
+        if (v[0]->device >= kMaxDevices) throw ArgError{"device index too large"};
+        std::vector<GmresBatchItem> items;
+        for (int k = 0; k < n; ++k) {
+            Handle& h = *v[k];
+            if (h.device != v[0]->device) throw ArgError{"all handles of a batch must be on one device"};
+            if (!h.prm.loaded) throw StateError{"operators are not loaded on handle " + std::to_string(k)};
+            if (h.precision == BAEMM_PRECISION_FP32_MFMA)
+                throw ArgError{"batched GMRES: instances must use BAEMM_PRECISION_FP32 or BAEMM_PRECISION_MIXED"};
+            if (!B[k] || !X[k]) throw ArgError{"B and X must not be NULL"};
+            GmresBatchItem it;
+            it.h = &h;
+            it.B = static_cast<const double2*>(B[k]);
+            it.X = static_cast<double2*>(X[k]);
+            items.push_back(it);
+        }
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : v[0]->stream;
+        gmres_batch(items, g_bw[v[0]->device], gmres_tol, cg_tol, s);
+        bool all = true;
+        for (int k = 0; k < n; ++k) {
+            if (iterations) iterations[k] = items[k].iterations;
+            if (converged) converged[k] = items[k].converged ? 1 : 0;
+            all = all && items[k].converged;
+        }
+        return all ? BAEMM_OK : BAEMM_ERR_NOCONV;
+    });
+}
+
 }  // extern "C"

@@ -848,28 +848,25 @@ void bdr_setup(Handle& h, BdrPlan& pl)
         const int tiles = (int)(h.nt_pad / kTile);
         const int max_seg = (int)(h.nt_pad / kSrcBlock);
         int S0 = choose_segments(h, tiles, max_seg, h.nt);
-        if (h.segments == 0)
-            S0 = balance_segments(tiles, max_seg, S0, max_seg,
-                                  resident_blocks((const void*)plain_fn(OP_BDR, pl.adj, pl.WCSP,
-                                                                        pick_kg(pl.WCSP, (int)p.K, false)),
-                                                  kTile));
+        const int res = resident_blocks((const void*)plain_fn(OP_BDR, pl.adj, pl.WCSP, pick_kg(pl.WCSP, (int)p.K, false)),
+                                        kTile);
+        if (h.segments == 0) S0 = balance_segments(tiles, max_seg, S0, max_seg, res);
         pl.seg_len = (int)(((h.nt_pad + S0 - 1) / S0 + kSrcBlock - 1) / kSrcBlock) * kSrcBlock;
         pl.S = (int)((h.nt_pad + pl.seg_len - 1) / pl.seg_len);
+        pl.rounds = (double)tiles * pl.S / res;
         return;
     }
     pl.nst = (int)(h.nt_pad / 8);
     const int tiles = (int)(h.nt_pad / kMixTile);
     int S0 = choose_segments(h, tiles, pl.nst / 8, h.nt);
-    if (h.segments == 0) {
-        // balanced for the wave-specialised kernel (the default); the
-        // single-wave kernel takes the same segments, so the two stay
-        // bit-identical
-        const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
-        S0 = balance_segments(tiles, pl.nst, S0, pl.nst / 8,
-                              resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512));
-    }
+    const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
+    const int res = resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512);
+    // balanced for the wave-specialised kernel (the default); the single-wave
+    // kernel takes the same segments, so the two stay bit-identical
+    if (h.segments == 0) S0 = balance_segments(tiles, pl.nst, S0, pl.nst / 8, res);
     pl.steps = (pl.nst + S0 - 1) / S0;
     pl.S = (pl.nst + pl.steps - 1) / pl.steps;
+    pl.rounds = (double)tiles * pl.S / res;
 }
 
 size_t bdr_workspace_bytes(const Handle& h, const BdrPlan& pl, bool need_rmin)

@@ -93,6 +93,7 @@ struct BdrPlan {
     int S = 0;                   // source segments
     int seg_len = 0;             // plain kernels: sources per segment (multiple of 64)
     int steps = 0;               // MFMA kernels: 8-source steps per segment
+    double rounds = 0.0;         // work-groups of one pair launch / resident work-groups of the device
     WsSource ws;
     EpiArgs ea{};
 };

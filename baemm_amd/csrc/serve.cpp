@@ -31,6 +31,7 @@
 #include <thread>
 #include <vector>
 
+#include "batch.hpp"
 #include "capi_util.hpp"
 
 namespace baemm {
@@ -122,9 +123,16 @@ struct Server {
     Clock::time_point first_submit, last_done;
     bool any_submit = false;
 
-    void worker();
+    // fused mode: worker 0 gathers up to `slots` admitted requests and runs
+    // them as one batch whose solves are batched GMRES; the other workers idle
+    bool fused = false;
+    BatchWorkspace bw;
+
+    void worker(int index);
+    void fused_worker();
     bool admit(std::shared_ptr<Request>& rq, Cached*& slot);
     void run(Request& rq, Cached& c);
+    void finish(std::shared_ptr<Request>& rq, Cached* c);   // with mtx held
 };
 
 // Picks the next queued request that fits; returns false when none does.
@@ -259,17 +267,56 @@ void Server::run(Request& rq, Cached& c)
     rq.status = st;
 }
 
-void Server::worker()
+// Bookkeeping of a finished request (mtx held): memory, cache, statistics.
+void Server::finish(std::shared_ptr<Request>& rq, Cached* c)
+{
+    --running;
+    reserved -= rq->need;
+    const size_t bytes = c->h ? handle_device_bytes(*H(c->h)) : 0;
+    resident += bytes;
+    resident -= c->bytes;
+    c->bytes = bytes;
+    peak = std::max(peak, resident + reserved);
+    c->busy = false;
+    c->last_used = Clock::now();
+    if (!c->h || rq->mesh_id < 0) {   // not cacheable: release now
+        if (c->h) baemm_destroy(c->h);
+        resident -= c->bytes;
+        for (auto ci = cache.begin(); ci != cache.end(); ++ci)
+            if (&*ci == c) {
+                cache.erase(ci);
+                break;
+            }
+    }
+    rq->t_done = Clock::now();
+    rq->done = true;
+    if (rq->status == BAEMM_OK || rq->status == BAEMM_ERR_NOCONV) {
+        ++completed;
+        latencies_ms.push_back(std::chrono::duration<double, std::milli>(rq->t_done - rq->t_submit).count());
+        total_gmres += rq->gmres_iterations;
+        last_done = rq->t_done;
+    } else {
+        ++failed;
+    }
+}
+
+void Server::worker(int index)
 {
     (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mtx);
     for (;;) {
+        if (fused && index == 0) {
+            lk.unlock();
+            fused_worker();
+            lk.lock();
+            if (stop && queue.empty()) return;
+            continue;
+        }
         std::shared_ptr<Request> rq;
         Cached* c = nullptr;
-        cv.wait(lk, [&] { return stop || (!queue.empty() && admit(rq, c)); });
+        cv.wait(lk, [&] { return stop || (fused && index == 0) || (!fused && !queue.empty() && admit(rq, c)); });
         if (!rq) {
             if (stop && queue.empty()) return;
-            if (stop) continue;
             continue;
         }
         ++running;
@@ -287,34 +334,120 @@ void Server::worker()
             rq->error = "unknown error";
         }
         lk.lock();
-        --running;
-        reserved -= rq->need;
-        const size_t bytes = c->h ? handle_device_bytes(*H(c->h)) : 0;
-        resident += bytes;
-        resident -= c->bytes;
-        c->bytes = bytes;
+        finish(rq, c);
+        cv.notify_all();
+    }
+}
+
+// Fused mode: gathers the queued requests that fit (up to `slots`) and runs
+// them as one batch through gn_step_batch, phase by phase, with each of the
+// three solves one batched GMRES; requests that arrive meanwhile join the next
+// batch.  Returns when fused mode is switched off or the server stops.
+void Server::fused_worker()
+{
+    std::unique_lock<std::mutex> lk(mtx);
+    while (fused) {
+        cv.wait(lk, [&] { return stop || !fused || !queue.empty(); });
+        if (!fused || (stop && queue.empty())) return;
+        std::vector<std::pair<std::shared_ptr<Request>, Cached*>> batch;
+        for (;;) {
+            if ((int)batch.size() >= slots || queue.empty()) break;
+            std::shared_ptr<Request> rq;
+            Cached* c = nullptr;
+            if (!admit(rq, c)) break;
+            if (!rq) break;
+            batch.emplace_back(rq, c);
+        }
+        if (batch.empty()) {
+            // nothing fits now; admit() fails requests that can never fit
+            if (running == 0 && !queue.empty()) cv.wait_for(lk, std::chrono::milliseconds(2));
+            continue;
+        }
+        running += (int)batch.size();
+        max_running = std::max(max_running, running);
         peak = std::max(peak, resident + reserved);
-        c->busy = false;
-        c->last_used = Clock::now();
-        if (!c->h || rq->mesh_id < 0) {   // not cacheable: release now
-            if (c->h) baemm_destroy(c->h);
-            resident -= c->bytes;
-            for (auto ci = cache.begin(); ci != cache.end(); ++ci)
-                if (&*ci == c) {
-                    cache.erase(ci);
+        const auto t0 = Clock::now();
+        for (auto& e : batch) e.first->t_start = t0;
+        lk.unlock();
+        std::vector<GnItem> items;
+        std::vector<size_t> which;
+        for (size_t k = 0; k < batch.size(); ++k) {
+            Request& rq = *batch[k].first;
+            Cached& c = *batch[k].second;
+            int st = BAEMM_OK;
+            if (!c.h) {
+                st = baemm_create(rq.coords.data(), rq.nv, rq.tris.data(), rq.nt, rq.meas.data(), rq.m, device, &c.h);
+                if (st != BAEMM_OK) {
+                    rq.error = baemm_last_error(nullptr);
+                    rq.status = st;
+                    c.h = nullptr;
+                    continue;
+                }
+            }
+            baemm_set_precision(c.h, rq.precision);
+            baemm_reset_solver_stats(c.h);
+            GnItem it;
+            it.h = H(c.h);
+            it.kappa = rq.kappa.data();
+            it.K = rq.K;
+            it.wcs = rq.wcs;
+            it.dirs = rq.dirs.data();
+            it.wave_type = rq.wave_type;
+            it.fobs = rq.fobs.empty() ? nullptr : rq.fobs.data();
+            it.F_out = rq.F_out;
+            it.grad_out = rq.grad_out;
+            it.cg_tol = rq.cg_tol;
+            it.gmres_tol = rq.gmres_tol;
+            items.push_back(it);
+            which.push_back(k);
+        }
+        // requests with other tolerances than the first run in batches of their own
+        std::vector<std::vector<size_t>> groups;
+        for (size_t k = 0; k < items.size(); ++k) {
+            bool placed = false;
+            for (auto& g : groups)
+                if (items[g[0]].gmres_tol == items[k].gmres_tol && items[g[0]].cg_tol == items[k].cg_tol) {
+                    g.push_back(k);
+                    placed = true;
                     break;
                 }
+            if (!placed) groups.push_back({k});
         }
-        rq->t_done = Clock::now();
-        rq->done = true;
-        if (rq->status == BAEMM_OK || rq->status == BAEMM_ERR_NOCONV) {
-            ++completed;
-            latencies_ms.push_back(std::chrono::duration<double, std::milli>(rq->t_done - rq->t_submit).count());
-            total_gmres += rq->gmres_iterations;
-            last_done = rq->t_done;
-        } else {
-            ++failed;
+        for (auto& g : groups) {
+            std::vector<GnItem> part;
+            for (size_t k : g) part.push_back(items[k]);
+            int st = BAEMM_OK;
+            std::string err;
+            try {
+                hipStream_t s0 = part[0].h->stream;
+                gn_step_batch(part, bw, s0);
+            } catch (const ArgError& e) {
+                st = BAEMM_ERR_ARG;
+                err = e.msg;
+            } catch (const HipError& e) {
+                st = BAEMM_ERR_HIP;
+                err = std::string("HIP error '") + hipGetErrorString(e.err) + "' in " + e.what;
+            } catch (const std::exception& e) {
+                st = BAEMM_ERR_INTERNAL;
+                err = e.what();
+            }
+            for (size_t i = 0; i < g.size(); ++i) {
+                Request& rq = *batch[which[g[i]]].first;
+                if (st != BAEMM_OK) {
+                    rq.status = st;
+                    rq.error = err;
+                    continue;
+                }
+                rq.status = part[i].status;
+                if (rq.status == BAEMM_ERR_NOCONV) rq.error = "GMRES did not reach its tolerance";
+                int64_t its = 0, cyc = 0, cgit = 0, apps = 0;
+                double res = 0.0;
+                baemm_get_solver_stats(reinterpret_cast<baemm_handle>(part[i].h), &its, &cyc, &res, &cgit, &apps);
+                rq.gmres_iterations = its;
+            }
         }
+        lk.lock();
+        for (auto& e : batch) finish(e.first, e.second);
         cv.notify_all();
     }
 }
@@ -345,7 +478,7 @@ int baemm_server_create(int device, int slots, int64_t hbm_budget_bytes, baemm_s
     s.slots = slots;
     s.budget = (size_t)hbm_budget_bytes;
     try {
-        for (int k = 0; k < slots; ++k) s.workers.emplace_back([&s] { s.worker(); });
+        for (int k = 0; k < slots; ++k) s.workers.emplace_back([&s, k] { s.worker(k); });
     } catch (...) {
         {
             std::lock_guard<std::mutex> lk(s.mtx);
@@ -374,7 +507,21 @@ int baemm_server_destroy(baemm_server srv)
     for (auto& t : s.workers) t.join();
     for (Cached& c : s.cache)
         if (c.h) baemm_destroy(c.h);
+    (void)hipSetDevice(s.device);
+    s.bw.release();
     delete srv;
+    return BAEMM_OK;
+}
+
+int baemm_server_set_fusion(baemm_server srv, int on)
+{
+    if (!srv) return BAEMM_ERR_ARG;
+    Server& s = srv->s;
+    {
+        std::lock_guard<std::mutex> lk(s.mtx);
+        s.fused = on != 0;
+    }
+    s.cv.notify_all();
     return BAEMM_OK;
 }
 

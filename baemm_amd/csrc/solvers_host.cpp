@@ -16,6 +16,7 @@
 #include <cstring>
 #include <vector>
 
+#include "batch.hpp"
 #include "capi_util.hpp"
 #include "solvers.hpp"
 
@@ -494,6 +495,334 @@ double2* ensure_du_dn(Handle& h, double** du_dn, const double* kappa, int64_t K,
 
 }  // namespace
 
+// Batched GMRES: the cycles of gmres_boundary for several systems in lockstep.
+// Per Arnoldi step, one batched launch applies the operators of every system
+// still iterating (apply_boundary_batch; a system that converged at the step
+// before has its items return at once through its stop flag), then each
+// system's PCG, Gram-Schmidt and Givens run on its handle's stream, so those
+// of different systems overlap.  As in gmres_boundary, step j + 1 is enqueued
+// before the flags of step j are read, restart residuals of the mixed
+// precision use the FP32 operator, and each system's result is the one its
+// own GMRES reaches (the batched apply's segment split is the batch's own, so
+// results differ from gmres_boundary's in the last bits).
+void gmres_batch(std::vector<GmresBatchItem>& items, BatchWorkspace& bw, double tol, double cg_tol, hipStream_t s)
+{
+    struct Sys {
+        Handle* h;
+        GmresBatchItem* it;
+        int W;
+        int64_t n, NW;
+        double2 *V, *w, *t, *hc1, *hc2;
+        double* sc;
+        GmState st;
+        int64_t cg0;
+        bool done = false;      // converged (or out of cycles)
+        bool iterating = false; // in the current cycle, not yet stopped
+    };
+    const int nsys = (int)items.size();
+    if (nsys == 0) return;
+    int m = items[0].h->gmres_max_iter, cycles = items[0].h->gmres_max_restarts;
+    std::vector<Sys> sy(nsys);
+    for (int k = 0; k < nsys; ++k) {
+        Sys& q = sy[k];
+        q.h = items[k].h;
+        q.it = &items[k];
+        Handle& h = *q.h;
+        if (h.gmres_max_iter != m || h.gmres_max_restarts != cycles)
+            throw ArgError{"batched GMRES: all handles need the same gmres limits"};
+        Workspace& ws = h.sws;
+        q.W = (int)h.prm.W;
+        q.n = h.nv;
+        q.NW = q.n * q.W;
+        ws.V.reserve(sizeof(double2) * q.NW * (m + 1));
+        ws.w.reserve(sizeof(double2) * q.NW);
+        ws.t.reserve(sizeof(double2) * q.NW);
+        ws.hc.reserve(sizeof(double2) * (2 * (m + 2) * q.W));
+        ws.sc.reserve(sizeof(double) * q.W);
+        ws.red.reserve(sizeof(double2) * kRedBlocks * (m + 2) * q.W);
+        q.V = ws.V.as<double2>();
+        q.w = ws.w.as<double2>();
+        q.t = ws.t.as<double2>();
+        q.hc1 = ws.hc.as<double2>();
+        q.hc2 = q.hc1 + (m + 2) * q.W;
+        q.sc = ws.sc.as<double>();
+        q.st = dev_gm_state(ws, q.W, m);
+        gm_host_state(h);
+        q.cg0 = dev_cg_total(ws, h.stream);
+        BAEMM_HIP(hipMemsetAsync(q.it->X, 0, sizeof(double2) * q.NW, h.stream));
+    }
+    // the operator applies of the listed systems, in: in_k, out: t_k
+    auto batch_apply = [&](const std::vector<int>& ks, auto&& in_of) {
+        std::vector<BatchEntry> es;
+        for (int k : ks) {
+            Sys& q = sy[k];
+            es.push_back(BatchEntry{q.h, ApplyArgs{cdouble(1.0, 0.0), cdouble(0.0, 0.0), in_of(k), q.W, kC128, q.t,
+                                                   q.W, kC128}});
+            q.h->stats.operator_applies++;
+        }
+        apply_boundary_batch(es, bw, s, nullptr);
+    };
+    auto enqueue_step = [&](int j, const std::vector<int>& ks) {
+        for (int k : ks) sy[k].h->step_skip = sy[k].st.ctl;
+        try {
+            batch_apply(ks, [&](int k) { return (const void*)(sy[k].V + (int64_t)j * sy[k].NW); });
+            for (int k : ks) {
+                Sys& q = sy[k];
+                Handle& h = *q.h;
+                Workspace& ws = h.sws;
+                hipStream_t sk = h.stream;
+                double2* Vn = q.V + (int64_t)(j + 1) * q.NW;
+                apply_prec(h, q.t, q.w, cg_tol, sk);
+                dev_gs_dots(ws, q.n, q.W, j + 1, q.V, q.NW, q.w, q.hc1, sk);
+                dev_gs_update(q.n, q.W, j + 1, q.V, q.NW, q.hc1, q.w, -1.0, sk);
+                dev_gs_dots(ws, q.n, q.W, j + 1, q.V, q.NW, q.w, q.hc2, sk);
+                dev_gs_update(q.n, q.W, j + 1, q.V, q.NW, q.hc2, q.w, -1.0, sk);
+                dev_gs_dots(ws, q.n, q.W, 1, q.w, q.NW, q.w, q.hc2 + (j + 1) * q.W, sk);
+                dev_inv_norm(q.W, q.hc2 + (j + 1) * q.W, q.sc, sk);
+                dev_scale_cols(q.n, q.W, q.w, q.sc, Vn, sk);
+                dev_gm_givens(q.W, m, j, q.hc1, q.hc2, tol, q.st, sk);
+                gm_snapshot(h, q.st, j & 1, sk);
+            }
+        } catch (...) {
+            for (Sys& q : sy) q.h->step_skip = nullptr;
+            throw;
+        }
+        for (int k : ks) sy[k].h->step_skip = nullptr;
+    };
+    for (int cyc = 0; cyc < cycles; ++cyc) {
+        std::vector<int> act;
+        for (int k = 0; k < nsys; ++k)
+            if (!sy[k].done) act.push_back(k);
+        if (act.empty()) break;
+        if (cyc == 0) {
+            for (int k : act) apply_prec(*sy[k].h, sy[k].it->B, sy[k].V, cg_tol, sy[k].h->stream);
+        } else {
+            // restart residuals P (B - A X), the mixed precision's with the FP32 operator
+            std::vector<int> prec(nsys);
+            for (int k : act) {
+                prec[k] = sy[k].h->precision;
+                if (prec[k] == BAEMM_PRECISION_MIXED) sy[k].h->precision = BAEMM_PRECISION_FP32;
+            }
+            try {
+                batch_apply(act, [&](int k) { return (const void*)sy[k].it->X; });
+            } catch (...) {
+                for (int k : act) sy[k].h->precision = prec[k];
+                throw;
+            }
+            for (int k : act) {
+                Sys& q = sy[k];
+                q.h->precision = prec[k];
+                dev_sub_c(q.NW, q.it->B, q.t, q.w, q.h->stream);
+                apply_prec(*q.h, q.w, q.V, cg_tol, q.h->stream);
+            }
+        }
+        for (int k : act) {
+            Sys& q = sy[k];
+            dev_gs_dots(q.h->sws, q.n, q.W, 1, q.V, q.NW, q.V, q.hc1, q.h->stream);
+            dev_gm_start(q.W, m, q.hc1, cyc == 0, tol, q.st, q.sc, q.h->stream);
+            gm_snapshot(*q.h, q.st, 0, q.h->stream);
+        }
+        std::vector<int> run;
+        for (int k : act) {
+            Sys& q = sy[k];
+            BAEMM_HIP(hipEventSynchronize(q.h->ev_step[0]));
+            q.it->rel_residual = *reinterpret_cast<const double*>(q.h->pin_ctl + 4);
+            if (q.h->pin_ctl[2]) {
+                q.done = true;
+                q.it->converged = true;
+                continue;
+            }
+            q.it->cycles++;
+            q.iterating = true;
+            dev_scale_cols(q.n, q.W, q.V, q.sc, q.V, q.h->stream);
+            run.push_back(k);
+        }
+        if (run.empty()) break;
+        const std::vector<int> cycle_set = run;
+        enqueue_step(0, run);
+        std::vector<int> inflight = run;   // systems whose step j is enqueued
+        for (int j = 0; j < m && !inflight.empty(); ++j) {
+            std::vector<int> next;
+            if (j + 1 < m && !run.empty()) {
+                next = run;
+                enqueue_step(j + 1, next);
+            }
+            for (int k : inflight) {
+                Sys& q = sy[k];
+                BAEMM_HIP(hipEventSynchronize(q.h->ev_step[j & 1]));
+                const int* c = q.h->pin_ctl + 8 * (j & 1);
+                q.it->rel_residual = *reinterpret_cast<const double*>(c + 4);
+                if (c[1] == j + 1) {
+                    q.it->iterations++;
+                    q.h->stats.gmres_iterations++;
+                }
+                if (c[0]) q.iterating = false;
+            }
+            run.clear();
+            for (int k : cycle_set)
+                if (sy[k].iterating) run.push_back(k);
+            // systems that stopped at step j are in step j + 1 already (their
+            // work there returns at once); step j + 2 leaves them out
+            inflight = next;
+        }
+        for (int k : cycle_set) {
+            Sys& q = sy[k];
+            dev_gm_backsub(q.W, m, q.st, q.hc1, q.h->stream);
+            dev_gs_update(q.n, q.W, m, q.V, q.NW, q.hc1, q.it->X, 1.0, q.h->stream, q.st.ctl + 1);
+            if (q.it->rel_residual <= tol) {
+                q.done = true;
+                q.it->converged = true;
+            }
+        }
+    }
+    for (Sys& q : sy) {
+        BAEMM_HIP(hipStreamSynchronize(q.h->stream));
+        Handle& h = *q.h;
+        h.stats.cg_iterations += dev_cg_total(h.sws, h.stream) - q.cg0;
+        h.stats.gmres_cycles += q.it->cycles;
+        h.stats.gmres_rel_residual = q.it->rel_residual;
+        h.stats.converged = h.stats.converged && q.it->converged;
+    }
+    BAEMM_HIP(hipStreamSynchronize(s));
+}
+
+// Fused Gauss-Newton steps: the per-request pipeline of the serving loop
+// (FarField, Derivative_FF_Helper, AdjointDerivative_FF) with each of its three
+// solves batched over the requests.  Device arrays per request live in the
+// handle's I/O slots: [0] right-hand side, [1] potential, [2] far field,
+// [3] Neumann data du, [4] Herglotz solve dv, [5] misfit, product and gradient.
+void gn_step_batch(std::vector<GnItem>& items, BatchWorkspace& bw, hipStream_t s)
+{
+    const int n = (int)items.size();
+    if (n == 0) return;
+    struct Buf {
+        double2 *rhs, *phi, *F, *du, *dv, *g, *prod;
+        double* Y;
+        std::vector<double> coeff;
+        int64_t W, NW;
+    };
+    std::vector<Buf> b(n);
+    for (int k = 0; k < n; ++k) {
+        GnItem& it = items[k];
+        Handle& h = *it.h;
+        check_common(h, it.kappa, it.K, it.wcs);
+        if (!it.dirs || !it.F_out || !it.grad_out) throw ArgError{"request arrays must not be NULL"};
+        if (h.m < 1) throw ArgError{"no measurement directions"};
+        Buf& q = b[k];
+        q.W = it.K * it.wcs;
+        q.NW = h.nv * q.W;
+        q.rhs = DevArray(h, 0, sizeof(double2) * q.NW).as<double2>();
+        q.phi = DevArray(h, 1, sizeof(double2) * q.NW).as<double2>();
+        q.F = DevArray(h, 2, sizeof(double2) * h.m * q.W).as<double2>();
+        q.du = DevArray(h, 3, sizeof(double2) * q.NW).as<double2>();
+        q.dv = DevArray(h, 4, sizeof(double2) * q.NW).as<double2>();
+        char* p5 = DevArray(h, 5, sizeof(double2) * (h.m * q.W + h.nv) + sizeof(double) * 3 * h.nv).as<char>();
+        q.g = reinterpret_cast<double2*>(p5);
+        q.prod = q.g + h.m * q.W;
+        q.Y = reinterpret_cast<double*>(q.prod + h.nv);
+        h.stats.converged = true;
+    }
+    auto solve = [&](auto&& rhs_of, auto&& x_of, double tol_of_first) {
+        (void)tol_of_first;
+        std::vector<GmresBatchItem> g(n);
+        for (int k = 0; k < n; ++k) {
+            g[k].h = items[k].h;
+            g[k].B = rhs_of(k);
+            g[k].X = x_of(k);
+        }
+        // one tolerance per batch: the requests of a fused batch share theirs
+        gmres_batch(g, bw, items[0].gmres_tol, items[0].cg_tol, s);
+        for (int k = 0; k < n; ++k)
+            if (!g[k].converged) items[k].status = BAEMM_ERR_NOCONV;
+    };
+    for (int k = 1; k < n; ++k)
+        if (items[k].gmres_tol != items[0].gmres_tol || items[k].cg_tol != items[0].cg_tol)
+            throw ArgError{"a fused batch needs one GMRES and one CG tolerance"};
+    // 1. incident wave (0, -1, 0, 0) and the potential operator (1/2, -i kappa, 1, 0)
+    for (int k = 0; k < n; ++k) {
+        GnItem& it = items[k];
+        Handle& h = *it.h;
+        std::vector<cdouble> c12(2 * it.K, cdouble(0.0, 0.0));
+        for (int64_t q = 0; q < it.K; ++q) c12[2 * q] = -1.0;
+        incident_rows_dev(h, it.kappa, it.K, it.wcs, it.dirs, it.wave_type, c12, b[k].rhs, h.stream);
+        std::vector<cdouble> rows(4 * it.K);
+        for (int64_t q = 0; q < it.K; ++q) {
+            rows[4 * q] = {0.5, 0.0};
+            rows[4 * q + 1] = {0.0, -it.kappa[q]};
+            rows[4 * q + 2] = {1.0, 0.0};
+            rows[4 * q + 3] = {0.0, 0.0};
+        }
+        b[k].coeff = rows_coeff(it.K, rows);
+        load_params(h, it.kappa, b[k].coeff.data(), b[k].W, it.wcs, 1.0);
+    }
+    solve([&](int k) { return (const double2*)b[k].rhs; }, [&](int k) { return b[k].phi; }, 0.0);
+    // 2. far field; incident wave (0, -i kappa, 1, 0) and the DtN operator (1/2, -i kappa, 0, 1)
+    for (int k = 0; k < n; ++k) {
+        GnItem& it = items[k];
+        Handle& h = *it.h;
+        far_field_dev(h, it.kappa, it.K, it.wcs, b[k].coeff, b[k].phi, b[k].F, h.stream);
+        BAEMM_HIP(hipMemcpyAsync(it.F_out, b[k].F, sizeof(double2) * h.m * b[k].W, hipMemcpyDeviceToHost, h.stream));
+        std::vector<cdouble> c12(2 * it.K);
+        for (int64_t q = 0; q < it.K; ++q) {
+            c12[2 * q] = {0.0, -it.kappa[q]};
+            c12[2 * q + 1] = {1.0, 0.0};
+        }
+        incident_rows_dev(h, it.kappa, it.K, it.wcs, it.dirs, it.wave_type, c12, b[k].rhs, h.stream);
+        std::vector<cdouble> rows(4 * it.K);
+        for (int64_t q = 0; q < it.K; ++q) {
+            rows[4 * q] = {0.5, 0.0};
+            rows[4 * q + 1] = {0.0, -it.kappa[q]};
+            rows[4 * q + 2] = {0.0, 0.0};
+            rows[4 * q + 3] = {1.0, 0.0};
+        }
+        const std::vector<double> c = rows_coeff(it.K, rows);
+        load_params(h, it.kappa, c.data(), b[k].W, it.wcs, 1.0);
+    }
+    solve([&](int k) { return (const double2*)b[k].rhs; }, [&](int k) { return b[k].du; }, 0.0);
+    // 3. misfit g = F - F_obs, its Herglotz wave (0, -i kappa, 1, 0) with factor 4 pi / m, DtN again
+    for (int k = 0; k < n; ++k) {
+        GnItem& it = items[k];
+        Handle& h = *it.h;
+        BAEMM_HIP(hipStreamSynchronize(h.stream));     // F_out is on the host
+        const int64_t mW = h.m * b[k].W;
+        std::vector<double> g(it.F_out, it.F_out + 2 * mW);
+        if (it.fobs)
+            for (int64_t e = 0; e < 2 * mW; ++e) g[e] -= it.fobs[e];
+        BAEMM_HIP(hipMemcpyAsync(b[k].g, g.data(), sizeof(double2) * mW, hipMemcpyHostToDevice, h.stream));
+        std::vector<cdouble> rows(4 * it.K);
+        for (int64_t q = 0; q < it.K; ++q) {
+            rows[4 * q] = 0.0;
+            rows[4 * q + 1] = {0.0, -it.kappa[q]};
+            rows[4 * q + 2] = 1.0;
+            rows[4 * q + 3] = 0.0;
+        }
+        const std::vector<double> c = rows_coeff(it.K, rows);
+        load_params(h, it.kappa, c.data(), b[k].W, it.wcs, 4.0 * M_PI / (double)h.m);
+        ApplyArgs a{cdouble(1.0, 0.0), cdouble(0.0, 0.0), b[k].g, b[k].W, kC128, b[k].rhs, b[k].W, kC128};
+        herglotz_dev(h, a, h.stream);
+        BAEMM_HIP(hipStreamSynchronize(h.stream));     // the host copy of g is released here
+        std::vector<cdouble> dr(4 * it.K);
+        for (int64_t q = 0; q < it.K; ++q) {
+            dr[4 * q] = {0.5, 0.0};
+            dr[4 * q + 1] = {0.0, -it.kappa[q]};
+            dr[4 * q + 2] = {0.0, 0.0};
+            dr[4 * q + 3] = {1.0, 0.0};
+        }
+        const std::vector<double> cd = rows_coeff(it.K, dr);
+        load_params(h, it.kappa, cd.data(), b[k].W, it.wcs, 1.0);
+    }
+    solve([&](int k) { return (const double2*)b[k].rhs; }, [&](int k) { return b[k].dv; }, 0.0);
+    // 4. gradient: M^{-1} AvT ((-1/W) Re(sum_k du dv) averaged to P0 / a n)
+    for (int k = 0; k < n; ++k) {
+        GnItem& it = items[k];
+        Handle& h = *it.h;
+        dev_row_product_sum(h, (int)b[k].W, b[k].du, b[k].dv, b[k].prod, h.stream);
+        multiply_with_normals_dev(h, b[k].prod, b[k].Y, -1.0 / (double)b[k].W, it.cg_tol, h.stream);
+        BAEMM_HIP(hipMemcpyAsync(it.grad_out, b[k].Y, sizeof(double) * 3 * h.nv, hipMemcpyDeviceToHost, h.stream));
+    }
+    for (int k = 0; k < n; ++k) BAEMM_HIP(hipStreamSynchronize(items[k].h->stream));
+}
+
 }  // namespace baemm
 
 using namespace baemm;
@@ -549,6 +878,21 @@ int baemm_boundary_potential(baemm_handle h, const double* kappa, int64_t K, con
         const GmresResult r = boundary_potential_dev(hh, kappa, eta ? eta : kappa, K, wcs, dw.as<double2>(),
                                                      dp.as<double2>(), cg_tol, gmres_tol, s, coeff);
         download(phi, dp.p, sizeof(double2) * NW, s);
+        return r.converged ? BAEMM_OK : BAEMM_ERR_NOCONV;
+    });
+}
+
+int baemm_gmres_dev(baemm_handle h, const void* B, void* X, double cg_tol, double gmres_tol, int64_t* iterations,
+                    void* stream)
+{
+    return guard(h, [&](Handle& hh) {
+        if (!hh.prm.loaded) throw StateError{"operators are not loaded"};
+        if (!B || !X) throw ArgError{"B and X must not be NULL"};
+        hipStream_t s = stream_of(hh, stream);
+        const GmresResult r = gmres_boundary(hh, static_cast<const double2*>(B), static_cast<double2*>(X), gmres_tol,
+                                             cg_tol, s);
+        BAEMM_HIP(hipStreamSynchronize(s));
+        if (iterations) *iterations = r.iterations;
         return r.converged ? BAEMM_OK : BAEMM_ERR_NOCONV;
     });
 }

@@ -327,6 +327,28 @@ class _SolverMixin:
             self._check(st)
         return st == _lib.OK
 
+    def SolveLoaded(self, B, cg_tol=1e-10, gmres_tol=1e-6):
+        """X with (P A) X = P B for the loaded boundary operator A (the GMRES of the
+        solvers, baemm_gmres_dev); returns (X, iterations, converged).  No
+        reference counterpart."""
+        L = self._L
+        W = self.GetWaveCount()
+        Bh = np.ascontiguousarray(_c128_in(B, "B", self.VertexCount()))
+        if Bh.shape[1] != W:
+            raise ValueError(f"B has {Bh.shape[1]} columns, the loaded wave count is {W}")
+        dB, dX = _DevArray(L, Bh), _DevArray(L, np.zeros_like(Bh))
+        try:
+            its = ctypes.c_int64(0)
+            st = L.baemm_gmres_dev(self._h, dB.ptr, dX.ptr, float(cg_tol), float(gmres_tol), ctypes.byref(its), None)
+            if st not in (_lib.OK, _lib.ERR_NOCONV):
+                self._check(st)
+            X = np.empty_like(Bh)
+            dX.download(X)
+            return X, int(its.value), st == _lib.OK
+        finally:
+            dB.close()
+            dX.close()
+
     def ApplyMassInverse(self, X, Y, cg_tol):
         """ApplyMassInverse.hpp:6-156: Y = M^{-1} X (real or complex columns)."""
         X = np.asarray(X)
@@ -556,6 +578,37 @@ def ApplyBoundaryOperatorsBatch(instances, B_list, C_list=None, alpha=None, beta
         return out
     finally:
         for d in dB + dC:
+            d.close()
+
+
+def SolveLoadedBatch(instances, B_list, cg_tol=1e-10, gmres_tol=1e-6):
+    """X_k with (P A_k) X_k = P B_k for the loaded operator A_k of each instance,
+    all solved together (baemm_gmres_batch): each Arnoldi step applies every
+    operator still iterating in one batched launch.  Returns (X list,
+    iterations list, converged list).  No reference counterpart."""
+    L = _lib.load()
+    n = len(instances)
+    Bs, Xs = _batch_arrays(instances, B_list, None)
+    dB = [_DevArray(L, b) for b in Bs]
+    dX = [_DevArray(L, x) for x in Xs]
+    try:
+        hs = (ctypes.c_void_p * n)(*[H._h.value for H in instances])
+        pB = (ctypes.c_void_p * n)(*[d.ptr.value for d in dB])
+        pX = (ctypes.c_void_p * n)(*[d.ptr.value for d in dX])
+        its = (ctypes.c_int64 * n)()
+        conv = (ctypes.c_int * n)()
+        st = L.baemm_gmres_batch(n, hs, pB, pX, float(cg_tol), float(gmres_tol), its, conv, None)
+        if st not in (_lib.OK, _lib.ERR_NOCONV):
+            raw = L.baemm_last_error(None)
+            raise BaemmError(st, raw.decode() if raw else "")
+        out = []
+        for d, x in zip(dX, Xs):
+            r = np.empty_like(x)
+            d.download(r)
+            out.append(r)
+        return out, list(its), [bool(c) for c in conv]
+    finally:
+        for d in dB + dX:
             d.close()
 
 

@@ -55,7 +55,7 @@ class GNServer:
     """Request loop on one device: ``slots`` concurrent requests, admitted while
     their estimated device memory fits into ``hbm_budget_bytes``."""
 
-    def __init__(self, slots=4, hbm_budget_bytes=64 << 30, device=0, policy="fifo", max_skips=8):
+    def __init__(self, slots=4, hbm_budget_bytes=64 << 30, device=0, policy="fifo", max_skips=8, fusion=False):
         self._L = _L()
         self._s = _P()
         st = self._L.baemm_server_create(int(device), int(slots), int(hbm_budget_bytes), ctypes.byref(self._s))
@@ -64,6 +64,15 @@ class GNServer:
         self._pending = {}
         if policy != "fifo":
             self.set_policy(policy, max_skips)
+        if fusion:
+            self.set_fusion(True)
+
+    def set_fusion(self, on=True):
+        """Fused mode: batches of up to ``slots`` queued requests run phase by
+        phase, each of their solves one batched GMRES (baemm_server_set_fusion)."""
+        st = self._L.baemm_server_set_fusion(self._s, 1 if on else 0)
+        if st != _lib.OK:
+            raise _err(self._L, st)
 
     def set_policy(self, policy, max_skips=8):
         """Admission order: "fifo" (oldest first) or "sjf" (least estimated work

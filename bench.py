@@ -189,11 +189,11 @@ def serving_workload(n, max_level, seed=7):
     return reqs
 
 
-def run_serving(reqs, rate, slots, precision, seed=11, policy="fifo"):
+def run_serving(reqs, rate, slots, precision, seed=11, policy="fifo", fusion=False):
     """Submits the requests at Poisson arrival times (rate per second) from a
     driver thread and waits for all; returns the server statistics.  policy:
     the admission order ("fifo", or "sjf": least estimated work first, with
-    aging)."""
+    aging); fusion: batches of up to `slots` requests with batched GMRES."""
     import threading
 
     from baemm_amd.serving import GNServer
@@ -201,7 +201,7 @@ def run_serving(reqs, rate, slots, precision, seed=11, policy="fifo"):
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
     ids = [None] * len(reqs)
-    with GNServer(slots=slots, hbm_budget_bytes=200 << 30, policy=policy) as srv:
+    with GNServer(slots=slots, hbm_budget_bytes=200 << 30, policy=policy, fusion=fusion) as srv:
         t0 = time.perf_counter()
 
         def driver():
@@ -218,7 +218,8 @@ def run_serving(reqs, rate, slots, precision, seed=11, policy="fifo"):
         statuses = []
         for k, i in enumerate(ids):
             statuses.append(srv.wait(i)[0])
-            log(f"serving ({precision}, {slots} slots, {policy}): request {k + 1}/{len(ids)} done")
+            log(f"serving ({precision}, {slots} slots, {policy}{', fused' if fusion else ''}): "
+                f"request {k + 1}/{len(ids)} done")
         wall = time.perf_counter() - t0
         st = srv.stats()
     st["wall_s"] = wall
@@ -315,13 +316,14 @@ def run_batch_leg(repeats=5, seed=3):
     from baemm_amd.meshes import perturbed_sphere
 
     rng = np.random.default_rng(seed)
-    inst, Bs = [], []
+    inst, Bs, kappas = [], [], []
     pairs_rhs = 0.0
     for k in range(24):
         level = int(rng.integers(3, 6))
         V, T = perturbed_sphere(level, seed=1000 + k)
         W = 2 ** int(rng.integers(0, 7))
         kappa = [float(rng.uniform(1.0, 16.0))]
+        kappas.append(kappa)
         coeff = np.array([0.0, 1.0 + 0.2j, 1.0 - 0.1j, 0.0])
         H = Helmholtz_HIP(V, T)
         H.SetPrecision("mixed")
@@ -369,6 +371,31 @@ def run_batch_leg(repeats=5, seed=3):
     batched()
     _lib.check(L.baemm_dev_synchronize())
     stats = BatchStats()
+    # GMRES on the same instances: each one's indirect potential operator
+    # (1/2, -i kappa, 1, 0) solved to 1e-6, all together with batched Arnoldi
+    # steps against one after the other
+    log("batch leg: batched GMRES")
+    for H, kap, b in zip(inst, kappas, Bs):
+        H.LoadBoundaryOperators_PL(kap, [0.5, -1j * kap[0], 1.0, 0.0], b.shape[1], b.shape[1])
+    its = (ctypes.c_int64 * n)()
+    conv = (ctypes.c_int * n)()
+
+    def gm_batched():
+        st = L.baemm_gmres_batch(n, hs, pB, pC, 1e-10, 1e-6, its, conv, None)
+        assert st == _lib.OK, st
+
+    def gm_sequential():
+        one_its = ctypes.c_int64(0)
+        for k, H in enumerate(inst):
+            _lib.check(L.baemm_gmres_dev(H._h, dB[k], dC[k], 1e-10, 1e-6, ctypes.byref(one_its), None), H._h)
+
+    gm_times = {}
+    for name, fn in (("batched", gm_batched), ("sequential", gm_sequential)):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(2):
+            fn()
+        gm_times[name] = (time.perf_counter() - t0) / 2
     for d in dB + dC:
         L.baemm_dev_free(d)
     for H in inst:
@@ -385,6 +412,15 @@ def run_batch_leg(repeats=5, seed=3):
         "tuning": tuning,
         "work_items": stats["items_mixed"],
         "persistent_grid": stats["grid_mixed"],
+        "gmres": {
+            "operator": "(1/2, -i kappa, 1, 0) per instance, GMRES(30) to 1e-6, PCG mass-inverse preconditioner",
+            "batched_s": gm_times["batched"],
+            "sequential_s": gm_times["sequential"],
+            "batched_solves_per_s": n / gm_times["batched"],
+            "sequential_solves_per_s": n / gm_times["sequential"],
+            "speedup": gm_times["sequential"] / gm_times["batched"],
+            "iterations": [int(x) for x in its],
+        },
     }
 
 
@@ -545,6 +581,7 @@ def main():
         reqs = serving_workload(args.serve_requests, args.serve_max_level)
         mixed_st = run_serving(reqs, args.serve_rate, args.serve_slots, "mixed")
         sjf_st = run_serving(reqs, args.serve_rate, args.serve_slots, "mixed", policy="sjf")
+        fused_st = run_serving(reqs, args.serve_rate, args.serve_slots, "mixed", fusion=True)
         seq_st = run_serving(reqs, args.serve_rate, 1, "fp32")
         keys = ("completed", "failed", "not_converged", "solves_per_s", "p50_ms", "p99_ms", "mean_ms", "max_in_flight",
                 "peak_bytes", "gmres_iterations", "wall_s")
@@ -556,8 +593,22 @@ def main():
             "offered_rate_per_s": args.serve_rate,
             "continuous_mixed": dict({k: mixed_st[k] for k in keys}, slots=args.serve_slots),
             "continuous_mixed_shortest_first": dict({k: sjf_st[k] for k in keys}, slots=args.serve_slots),
+            "fused_mixed": dict({k: fused_st[k] for k in keys}, slots=args.serve_slots,
+                                note="batches of up to `slots` queued requests, each solve one batched GMRES"),
             "sequential_fp32": dict({k: seq_st[k] for k in keys}, slots=1),
             "speedup_solves_per_s": mixed_st["solves_per_s"] / max(seq_st["solves_per_s"], 1e-12),
+        }
+        # a stream of small requests only (1,280 and 5,120 triangles), where the
+        # launches of each request, not the pair kernels, dominate
+        log("serving leg: small requests")
+        small = serving_workload(2 * args.serve_requests, 4, seed=8)
+        sm_cont = run_serving(small, 4 * args.serve_rate, args.serve_slots, "mixed")
+        sm_fused = run_serving(small, 4 * args.serve_rate, args.serve_slots, "mixed", fusion=True)
+        line["serving"]["small_requests"] = {
+            "workload": f"{2 * args.serve_requests} requests of 1280 or 5120 triangles, Poisson arrivals "
+                        f"{4 * args.serve_rate}/s, otherwise as above",
+            "continuous_mixed": dict({k: sm_cont[k] for k in keys}, slots=args.serve_slots),
+            "fused_mixed": dict({k: sm_fused[k] for k in keys}, slots=args.serve_slots),
         }
     if not args.no_cpu_baseline:
         log("cpu baseline (bounded sample)")

@@ -380,6 +380,24 @@ int baemm_gauss_newton_solve(baemm_handle h, const double* kappa, int64_t K, con
                              int64_t ldX, double** du_dn, int wave_type, double cg_tol,
                              double gmres_tol_inner, double gmres_tol_outer, int* succeeded);
 
+/* No reference counterpart: the GMRES of the solvers above on the loaded
+ * boundary operator A: solves (P A) X = P B (P = mass inverse by PCG, cg_tol)
+ * to gmres_tol.  B, X: device arrays, vertex_count x W complex double (ld W).
+ * *iterations (may be NULL) receives the Arnoldi steps.  Synchronous. */
+int baemm_gmres_dev(baemm_handle h, const void* B, void* X, double cg_tol, double gmres_tol, int64_t* iterations,
+                    void* stream);
+
+/* No reference counterpart: baemm_gmres_dev for n handles solved together
+ * (different meshes, wave numbers and W; precision FP32 or MIXED; equal
+ * gmres limits).  Each Arnoldi step applies the operators of every system
+ * still iterating in one batched persistent launch (as
+ * baemm_apply_boundary_batch); the preconditioner, Gram-Schmidt and Givens
+ * steps of the systems run on their handles' streams and overlap.
+ * iterations[k], converged[k] (either may be NULL) per system; returns
+ * BAEMM_ERR_NOCONV if any system stopped above gmres_tol.  Synchronous. */
+int baemm_gmres_batch(int n, const baemm_handle* handles, const void* const* B, void* const* X, double cg_tol,
+                      double gmres_tol, int64_t* iterations, int* converged, void* stream);
+
 /* Counters of the solvers since the last reset (no reference counterpart). */
 int baemm_get_solver_stats(baemm_handle h, int64_t* gmres_iterations, int64_t* gmres_cycles,
                            double* last_rel_residual, int64_t* cg_iterations,
@@ -434,6 +452,15 @@ int baemm_server_create(int device, int slots, int64_t hbm_budget_bytes, baemm_s
  * forever.  Applies to admissions after the call. */
 enum { BAEMM_SERVE_FIFO = 0, BAEMM_SERVE_SHORTEST_FIRST = 1 };
 int baemm_server_set_policy(baemm_server srv, int policy, int max_skips);
+/* Fused mode (on != 0; off by default): instead of running each request on
+ * its own worker, the loop gathers the queued requests that fit (up to
+ * `slots`) into a batch and runs them phase by phase, each of the batch's
+ * three boundary-integral solves one batched GMRES (baemm_gmres_batch: every
+ * Arnoldi step applies the operators of all requests still iterating in one
+ * launch).  Requests that arrive meanwhile join the next batch.  A request's
+ * result does not depend on the other requests of its batch.  Applies to
+ * admissions after the call. */
+int baemm_server_set_fusion(baemm_server srv, int on);
 /* Waits for every submitted request, then releases the cached handles. */
 int baemm_server_destroy(baemm_server srv);
 /* Inputs are copied; far_field_out (meas_count x W complex) and gradient_out

@@ -180,3 +180,26 @@ def test_c5_mix_with_large_meshes_alone_and_in_the_mix():
         print(f"C5 request {k} ({r['T'].shape[0]} triangles, {r['prec']}): far field vs FP64 {eF:.2e}, "
               f"gradient vs FP64 {eG:.2e}")
         assert eF < 1e-5 and eG < 2e-5, k   # measured 1.4e-7 to 1.2e-6
+
+
+def test_fused_batches_match_unfused_and_do_not_depend_on_the_batch(reqs):
+    """Fused mode (baemm_server_set_fusion): queued requests run as batches whose
+    solves are batched GMRES.  Each request's result is bit-identical whether
+    its batch holds it alone (one slot) or four requests, and agrees with the
+    unfused loop within the GMRES tolerance."""
+    fused_reqs = [r for r in reqs if r["prec"] in ("fp32", "mixed")]
+    with GNServer(slots=4) as srv:
+        plain = _run(srv, fused_reqs)
+    with GNServer(slots=1, fusion=True) as srv:
+        alone = _run(srv, fused_reqs)
+    with GNServer(slots=4, fusion=True) as srv:
+        together = _run(srv, fused_reqs)
+        st = srv.stats()
+    print("fused stats", st)
+    assert st["completed"] == len(fused_reqs) and st["failed"] == 0 and st["max_in_flight"] >= 2
+    for k in range(len(fused_reqs)):
+        assert alone[k][0] == 0 and together[k][0] == 0, k
+        assert np.array_equal(alone[k][1], together[k][1]) and np.array_equal(alone[k][2], together[k][2]), k
+        eF, eG = relerr(together[k][1], plain[k][1]), relerr(together[k][2], plain[k][2])
+        print(f"request {k}: fused vs unfused far field {eF:.2e}, gradient {eG:.2e}")
+        assert eF < 1e-5 and eG < 1e-4, k
