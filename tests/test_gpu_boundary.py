@@ -1,8 +1,9 @@
 """GPU parity of the boundary-operator apply (ApplyBoundaryOperators_PL)
 against the FP64 oracle and the FP32 restatement of the reference chain.
 
-Tolerance: 1e-5 relative (Frobenius) in both precisions, the bar the reference's
-FP32 apply sets; sampled vertex rows at the 81,920-triangle size."""
+Tolerance: 1e-5 relative (Frobenius) in every precision, the bar the
+reference's FP32 apply sets, and a bound near the error each case achieves
+(NEAR); sampled vertex rows at the 81,920- and 327,680-triangle sizes."""
 import functools
 
 import numpy as np
@@ -16,6 +17,9 @@ from conftest import oracle_mesh, relerr
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
+# the errors these cases achieve are 6e-8 to 1.4e-6: each apply is held to NEAR,
+# inside the 1e-5 contract, so that a tenfold loss of accuracy fails
+NEAR = 3e-6
 FULL_COEFF = [0.5 + 0.25j, 1 - 0.5j, -0.75 + 1j, 0.3 - 0.9j]
 
 
@@ -49,7 +53,7 @@ def test_boundary_apply_matches_oracle(sphere3, mode, W, wcs, kappa):
     C = np.zeros_like(B)
     H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)
     print(f"{mode} W={W} wcs={wcs} relerr={relerr(C, ref):.3e}")
-    assert relerr(C, ref) < TOL
+    assert relerr(C, ref) < NEAR
 
 
 @pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
@@ -69,7 +73,7 @@ def test_c1_all_operators_vs_fp32_restatement(mode):
         H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [4.0], FULL_COEFF, 16, 16)
     e64, e32 = relerr(C, ref64), relerr(C, ref32)
     print(f"C1 {mode}: vs FP64 {e64:.3e}, vs FP32 restatement {e32:.3e}")
-    assert e64 < TOL and e32 < TOL
+    assert e64 < NEAR and e32 < NEAR
 
 
 def test_alpha_beta_and_branch_semantics(sphere3):
@@ -83,7 +87,7 @@ def test_alpha_beta_and_branch_semantics(sphere3):
     ref = oracle.apply_boundary(m, alpha, B, beta, C0, [2.0], coeff, 4, 4)
     C = C0.copy()
     H.ApplyBoundaryOperators_PL(alpha, B, beta, C, [2.0], coeff, 4, 4)
-    assert relerr(C, ref) < TOL
+    assert relerr(C, ref) < NEAR
     # all-zero coefficients: C untouched (beta not applied), as in the reference
     C = C0.copy()
     H.ApplyBoundaryOperators_PL(1.0, B, 0.5, C, [2.0], [0, 0, 0, 0], 4, 4)
@@ -99,7 +103,7 @@ def test_alpha_beta_and_branch_semantics(sphere3):
     C = np.zeros_like(B)
     H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [2.0], coeff, 4, 4)
     H.UseDiagonal(True)
-    assert relerr(C, ref) < TOL
+    assert relerr(C, ref) < NEAR
     # strided (ld > W) input and output
     Bw = np.zeros((m.nv, 7), np.complex128)
     Bw[:, :4] = B
@@ -107,7 +111,7 @@ def test_alpha_beta_and_branch_semantics(sphere3):
     ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), [2.0], coeff, 4, 4)
     H.LoadBoundaryOperators_PL([2.0], coeff, 4, 4)
     H.ApplyBoundaryOperators_PL(1.0, Bw[:, :4], 0.0, Cw[:, :4])
-    assert relerr(Cw[:, :4], ref) < TOL
+    assert relerr(Cw[:, :4], ref) < NEAR
     assert np.all(Cw[:, 4:] == 0)
 
 
@@ -132,7 +136,7 @@ def test_nonspherical_meshes(mode):
             H.SetPrecision(mode)
             C = np.zeros_like(B)
             H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [3.0, 7.0], FULL_COEFF * 2, 32, 16)
-        assert relerr(C, ref) < TOL
+        assert relerr(C, ref) < NEAR
 
 
 @functools.lru_cache(maxsize=None)
@@ -208,7 +212,7 @@ def test_wave_specialised_mixed_kernel_is_identical(sphere3, W, wcs, kappa, c2, 
     H.SetPrecision("fp32")
     assert np.array_equal(outs[0], outs[1])
     ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
-    assert relerr(outs[1], ref) < TOL
+    assert relerr(outs[1], ref) < NEAR
 
 
 def test_kernel_variants_identical_at_default_segments():
@@ -249,7 +253,7 @@ def test_segment_autotuner_keeps_results(sphere3, mode):
     C = np.zeros((m.nv, W), np.complex128)
     H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C)
     ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
-    assert relerr(C, ref) < TOL
+    assert relerr(C, ref) < NEAR
     H.SetSegments(0)
     H.SetPrecision("fp32")
 
@@ -273,7 +277,7 @@ def test_largest_mesh_327680_triangles_sampled_rows():
             H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [32.0], coeff, W, wcs)
             e = relerr(C[rows], ref)
             print(f"327680 {mode}: sampled-row rel. error vs FP64 = {e:.3e}")
-            assert e < TOL
+            assert e < NEAR
 
 
 @pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
@@ -291,7 +295,7 @@ def test_smallest_mesh_icosahedron(mode):
         H.SetPrecision(mode)
         C = np.zeros_like(B)
         H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [1.0, 3.0], coeff, W, wcs)
-    assert relerr(C, ref) < TOL
+    assert relerr(C, ref) < NEAR
 
 
 @pytest.mark.gpu

@@ -14,6 +14,9 @@ from conftest import oracle_mesh, relerr
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
+# the errors these cases achieve are 6e-8 to 1.4e-6: each apply is held to NEAR,
+# inside the 1e-5 contract, so that a tenfold loss of accuracy fails
+NEAR = 3e-6
 CO = [0.5 + 0.25j, 1 - 0.5j, -0.75 + 1j, 0.3 - 0.9j]
 
 
@@ -39,7 +42,7 @@ def test_far_field_apply(mesh_h, W, wcs, kappa):
     ref = oracle.apply_far_field(m, 0.7 + 0.2j, B, 0.5j, F0, kappa, coeff, W, wcs)
     F = F0.copy()
     H.ApplyFarFieldOperators_PL(0.7 + 0.2j, B, 0.5j, F, kappa, coeff, W, wcs)
-    assert relerr(F, ref) < TOL
+    assert relerr(F, ref) < NEAR
 
 
 @pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
@@ -61,7 +64,7 @@ def test_near_field_apply(mesh_h, mode):
         U = np.zeros((300, W), np.complex128)
         H.ApplyNearFieldOperators_PL(1.0, B, 0.0, U, [2.0, 4.5], coeff, W, wcs, P)
         print(mode, "near field", relerr(U, ref))
-        assert relerr(U, ref) < TOL
+        assert relerr(U, ref) < NEAR
         # points near the surface, single- and double-layer rows with c2 = 0 in one chunk
         P2 = P / np.linalg.norm(P, axis=1, keepdims=True) * 1.02
         coeff2 = np.array([0, 1.0 - 0.5j, 0.0, 0, 0, 0.3j, 1.0 + 0.2j, 0], np.complex128)
@@ -70,7 +73,7 @@ def test_near_field_apply(mesh_h, mode):
         U2 = U0.copy()
         H.ApplyNearFieldOperators_PL(0.5 - 1j, B, 2.0, U2, [2.0, 4.5], coeff2, W, wcs, P2)
         print(mode, "near field at r = 1.02", relerr(U2, ref2))
-        assert relerr(U2, ref2) < TOL
+        assert relerr(U2, ref2) < NEAR
     finally:
         H.SetPrecision("fp32")
 
@@ -85,7 +88,7 @@ def test_herglotz_wave(mesh_h):
     ref = oracle.create_herglotz_wave(m, 1.0 - 1j, g, 2.0, C0, [2.0, 3.5], coeff, W, wcs)
     C = C0.copy()
     H.CreateHerglotzWave_PL(1.0 - 1j, g, 2.0, C, [2.0, 3.5], coeff, W, wcs)
-    assert relerr(C, ref) < TOL
+    assert relerr(C, ref) < NEAR
 
 
 @pytest.mark.parametrize("wtype", ["plane", "radial"])
@@ -103,7 +106,7 @@ def test_incident_wave(mesh_h, wtype):
     C = np.zeros((m.nv, W), np.complex128)
     H.CreateIncidentWave_PL(1.0, d, 0.0, C, [3.0, 5.0], coeff, W, wcs,
                             WaveType.Plane if wtype == "plane" else WaveType.Radial)
-    assert relerr(C, ref) < TOL
+    assert relerr(C, ref) < NEAR
 
 
 def test_mass_inverse(mesh_h):
