@@ -75,6 +75,8 @@ SIGNATURES = {
     "baemm_apply_boundary_batch": (_INT, [_INT, _P, _P, _P, _P, _INT, _P, _P, _P, _INT, _P]),
     "baemm_batch_set_tuning": (_INT, [_INT, _INT, _INT, _INT]),
     "baemm_batch_get_tuning": (_INT, [ctypes.POINTER(_INT)] * 4),
+    "baemm_batch_set_own_launch": (_INT, [ctypes.c_double]),
+    "baemm_batch_get_own_launch": (_INT, [ctypes.POINTER(ctypes.c_double)]),
     "baemm_batch_get_stats": (_INT, [ctypes.POINTER(_I64)] * 4),
     "baemm_batch_autotune": (_INT, [_INT, _P, _P, _P, _INT, _P, _P, _INT, _INT, ctypes.POINTER(_D)]),
     "baemm_server_create": (_INT, [_INT, _INT, _I64, ctypes.POINTER(_P)]),

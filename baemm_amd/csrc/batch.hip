@@ -240,11 +240,11 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
         bw.sync.push_back(e);
     }
     BAEMM_HIP(hipEventRecord(bw.sync[0], s));
-    // An instance whose own pair launch fills the device at least once runs
-    // that launch (the single-instance kernels, on its stream, beside the
-    // others); the rest share the persistent launch, whose work list they
-    // fill.  The choice depends on the instance alone, so its result is the
-    // same in any mix.
+    // An instance whose own pair launch fills the device at least
+    // tune.own_rounds times runs that launch (the single-instance kernels, on
+    // its stream, beside the others); the rest share the persistent launch,
+    // whose work list they fill.  The choice depends on the instance and the
+    // tuning alone, so its result is the same in any mix.
     std::vector<char> own(n, 0);
     for (int k = 0; k < n; ++k) {
         hipStream_t sk = es[k].h->stream;
@@ -253,7 +253,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
         BdrPlan& pl = prep[k].pl;
         bdr_setup(*es[k].h, pl);
         bdr_begin(*es[k].h, es[k].a, pl, sk);
-        if (pl.kernel_on && pl.rounds >= 1.0) {
+        if (pl.kernel_on && pl.rounds >= tune.own_rounds) {
             own[k] = 1;
             bdr_pairs(*es[k].h, pl, sk);
         }

@@ -19,6 +19,10 @@ struct BatchTuning {
     int plain_blocks = 16;      // 64-source blocks per item of the FP32 path (1024 sources)
     int mixed_grid_mult = 0;    // blocks per CU of the persistent grid (0: occupancy limit)
     int plain_grid_mult = 0;
+    // An instance whose own pair launch fills at least this many rounds of
+    // resident blocks runs that launch instead of joining the persistent one
+    // (0: every instance runs its own launch).
+    double own_rounds = 1.0;
 };
 BatchTuning& batch_tuning();
 

@@ -143,6 +143,22 @@ int baemm_batch_get_tuning(int* mixed_steps, int* plain_blocks, int* mixed_grid_
     return BAEMM_OK;
 }
 
+int baemm_batch_set_own_launch(double min_rounds)
+{
+    if (!(min_rounds >= 0.0) || min_rounds > 1e9) return BAEMM_ERR_ARG;
+    std::lock_guard<std::mutex> g(g_batch_mtx);
+    batch_tuning().own_rounds = min_rounds;
+    return BAEMM_OK;
+}
+
+int baemm_batch_get_own_launch(double* min_rounds)
+{
+    if (!min_rounds) return BAEMM_ERR_ARG;
+    std::lock_guard<std::mutex> g(g_batch_mtx);
+    *min_rounds = batch_tuning().own_rounds;
+    return BAEMM_OK;
+}
+
 int baemm_batch_get_stats(int64_t* items_mixed, int64_t* items_fp32, int64_t* grid_mixed, int64_t* grid_fp32)
 {
     std::lock_guard<std::mutex> g(g_batch_mtx);
@@ -178,6 +194,19 @@ int baemm_batch_autotune(int n, const baemm_handle* hs, const void* const* B, co
         BAEMM_HIP(hipEventCreate(&e1));
         BatchTuning best = saved;
         double best_t = 1e300;
+        auto time_candidate = [&]() {
+            apply_boundary_batch(es, g_bw[v[0]->device], s, nullptr);
+            BAEMM_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; ++r) apply_boundary_batch(es, g_bw[v[0]->device], s, nullptr);
+            BAEMM_HIP(hipEventRecord(e1, s));
+            BAEMM_HIP(hipEventSynchronize(e1));
+            float ms_t = 0.0f;
+            BAEMM_HIP(hipEventElapsedTime(&ms_t, e0, e1));
+            if (ms_t / reps < best_t) {
+                best_t = ms_t / reps;
+                best = t;
+            }
+        };
         try {
             apply_boundary_batch(es, g_bw[v[0]->device], s, nullptr);   // warm-up (allocations)
             for (int ms : msteps)
@@ -188,18 +217,14 @@ int baemm_batch_autotune(int n, const baemm_handle* hs, const void* const* B, co
                         t.plain_blocks = pb;
                         t.mixed_grid_mult = gm;
                         t.plain_grid_mult = gm;
-                        apply_boundary_batch(es, g_bw[v[0]->device], s, nullptr);
-                        BAEMM_HIP(hipEventRecord(e0, s));
-                        for (int r = 0; r < reps; ++r) apply_boundary_batch(es, g_bw[v[0]->device], s, nullptr);
-                        BAEMM_HIP(hipEventRecord(e1, s));
-                        BAEMM_HIP(hipEventSynchronize(e1));
-                        float ms_t = 0.0f;
-                        BAEMM_HIP(hipEventElapsedTime(&ms_t, e0, e1));
-                        if (ms_t / reps < best_t) {
-                            best_t = ms_t / reps;
-                            best = t;
-                        }
+                        t.own_rounds = 1.0;
+                        time_candidate();
                     }
+            // every instance on its own launch (no persistent launch): the
+            // schedule of separate applies, kept where it beats the work queue
+            t = saved;
+            t.own_rounds = 0.0;
+            time_candidate();
         } catch (...) {
             t = saved;
             (void)hipEventDestroy(e0);

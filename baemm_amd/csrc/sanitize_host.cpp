@@ -151,6 +151,8 @@ void null_arguments()
     NULL_CALL(baemm_get_timing);
     NULL_CALL(baemm_dev_alloc);
     NULL_CALL(baemm_batch_set_tuning);
+    NULL_CALL(baemm_batch_get_own_launch);
+    check(baemm_batch_set_own_launch(-1.0) == BAEMM_ERR_ARG, "baemm_batch_set_own_launch(-1)");
     check(baemm_batch_get_tuning(nullptr, nullptr, nullptr, nullptr) == BAEMM_OK ||
               baemm_batch_get_tuning(nullptr, nullptr, nullptr, nullptr) == BAEMM_ERR_ARG,
           "baemm_batch_get_tuning(NULL...)");

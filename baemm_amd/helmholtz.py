@@ -637,13 +637,17 @@ def BatchAutotune(instances, B_list, repeats=3):
             d.close()
 
 
-def BatchTuning(mixed_steps=None, plain_blocks=None, mixed_grid_mult=None, plain_grid_mult=None):
+def BatchTuning(mixed_steps=None, plain_blocks=None, mixed_grid_mult=None, plain_grid_mult=None, own_rounds=None):
     """Reads (and with arguments, sets) the process-wide tile shapes of the
-    persistent batched launch."""
+    persistent batched launch, and own_rounds: instances whose own pair launch
+    fills at least that many rounds of resident work-groups run it instead of
+    joining the persistent launch (0: every instance)."""
     L = _lib.load()
     v = [ctypes.c_int() for _ in range(4)]
     _lib.check(L.baemm_batch_get_tuning(*[ctypes.byref(x) for x in v]))
     cur = dict(zip(("mixed_steps", "plain_blocks", "mixed_grid_mult", "plain_grid_mult"), [x.value for x in v]))
+    own = ctypes.c_double()
+    _lib.check(L.baemm_batch_get_own_launch(ctypes.byref(own)))
     new = dict(cur)
     for k, val in (("mixed_steps", mixed_steps), ("plain_blocks", plain_blocks),
                    ("mixed_grid_mult", mixed_grid_mult), ("plain_grid_mult", plain_grid_mult)):
@@ -652,6 +656,10 @@ def BatchTuning(mixed_steps=None, plain_blocks=None, mixed_grid_mult=None, plain
     if new != cur:
         _lib.check(L.baemm_batch_set_tuning(new["mixed_steps"], new["plain_blocks"], new["mixed_grid_mult"],
                                             new["plain_grid_mult"]))
+    new["own_rounds"] = own.value
+    if own_rounds is not None and float(own_rounds) != own.value:
+        _lib.check(L.baemm_batch_set_own_launch(float(own_rounds)))
+        new["own_rounds"] = float(own_rounds)
     return new
 
 

@@ -292,6 +292,15 @@ int baemm_apply_boundary_batch(int n, const baemm_handle* handles, const double*
 int baemm_batch_set_tuning(int mixed_steps, int plain_blocks, int mixed_grid_mult, int plain_grid_mult);
 int baemm_batch_get_tuning(int* mixed_steps, int* plain_blocks, int* mixed_grid_mult, int* plain_grid_mult);
 
+/* Which instances skip the persistent launch (no reference counterpart): an
+ * instance whose own pair launch fills at least min_rounds rounds of resident
+ * work-groups runs that launch on its handle's stream, beside the others
+ * (default 1; 0: every instance, i.e. the schedule of separate applies).  The
+ * autotuner sets it too.  An instance's result depends on it only through the
+ * source segments its partial sums are split into (rounding). */
+int baemm_batch_set_own_launch(double min_rounds);
+int baemm_batch_get_own_launch(double* min_rounds);
+
 /* Work items and persistent grid size of the last batched call. */
 int baemm_batch_get_stats(int64_t* items_mixed, int64_t* items_fp32, int64_t* grid_mixed, int64_t* grid_fp32);
 
