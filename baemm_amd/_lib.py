@@ -99,6 +99,10 @@ SIGNATURES = {
     "baemm_adjoint_derivative_ff": (_INT, [_P, _P, _I64, _P, _I64, _P, _P, ctypes.POINTER(_P), _INT, _D, _D]),
     "baemm_gauss_newton_solve": (_INT, [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _D, _P, _I64, _D, _P, _I64,
                                         ctypes.POINTER(_P), _INT, _D, _D, _D, ctypes.POINTER(_INT)]),
+    "baemm_gauss_newton_solve_dev": (_INT, [_P, _P, _I64, _P, _I64, _P, _P, _P, _P, _D, _P, _I64, _D, _P, _I64,
+                                            _P, _INT, _INT, _D, _D, _D, ctypes.POINTER(_INT), _P]),
+    "baemm_mass_metric_apply": (None, [_P, _P, _P, _P]),
+    "baemm_mass_metric_jacobi": (None, [_P, _P, _P, _P]),
     "baemm_gmres_dev": (_INT, [_P, _P, _P, _D, _D, ctypes.POINTER(_I64), _P]),
     "baemm_gmres_batch": (_INT, [_INT, _P, _P, _P, _D, _D, _P, _P, _P]),
     "baemm_get_solver_stats": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_D),
@@ -122,6 +126,11 @@ class BaemmError(RuntimeError):
 
 
 REAL_OP = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+
+
+class MassMetric(ctypes.Structure):
+    """baemm_mass_metric: the device metric c I + d (Mass (x) I_3)."""
+    _fields_ = [("h", ctypes.c_void_p), ("c", ctypes.c_double), ("d", ctypes.c_double)]
 
 _lib = None
 

@@ -135,6 +135,7 @@ void null_arguments()
     NULL_CALL(baemm_derivative_ff);
     NULL_CALL(baemm_adjoint_derivative_ff);
     NULL_CALL(baemm_gauss_newton_solve);
+    NULL_CALL(baemm_gauss_newton_solve_dev);
     NULL_CALL(baemm_gmres_dev);
     NULL_CALL(baemm_gmres_batch);
     NULL_CALL(baemm_get_solver_stats);
@@ -160,6 +161,8 @@ void null_arguments()
     check(baemm_batch_get_stats(nullptr, nullptr, nullptr, nullptr) == BAEMM_OK &&
               baemm_batch_get_stats(st4, st4 + 1, st4 + 2, st4 + 3) == BAEMM_OK,
           "baemm_batch_get_stats");
+    baemm_mass_metric_apply(nullptr, nullptr, nullptr, nullptr);   // returns at once
+    baemm_mass_metric_jacobi(nullptr, nullptr, nullptr, nullptr);
     check(baemm_version() > 0, "baemm_version");
     // n < 0 is refused before the device is looked at
     check(baemm_hw_sincos(0, nullptr, -1, nullptr, nullptr) == BAEMM_ERR_ARG, "baemm_hw_sincos(n < 0)");

@@ -1254,3 +1254,106 @@ void dev_sub_c(int64_t N, const double2* b, const double2* t, double2* r, hipStr
 }
 
 }  // namespace baemm
+
+// ---------------------------------------------------------------------------
+// Real vectors of the device-side Gauss-Newton solve (baemm_gauss_newton_solve_dev):
+// the outer GMRES over vec(X) (3 V doubles) keeps its Krylov basis in HBM.
+// ---------------------------------------------------------------------------
+namespace baemm {
+
+constexpr int kRdotThreads = 256;
+
+// out[b] = sum_i V[b vs + i] w[i], b < gridDim.x: one block per vector, lane-
+// strided partial sums then a fixed tree, so results are reproducible.
+__global__ __launch_bounds__(kRdotThreads) void rdots_kernel(int64_t n, const double* __restrict__ V, int64_t vs,
+                                                             const double* __restrict__ w, double* __restrict__ out)
+{
+    const double* v = V + (int64_t)blockIdx.x * vs;
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kRdotThreads) acc = fma(v[i], w[i], acc);
+    __shared__ double red[kRdotThreads];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = kRdotThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// w[i] += sign sum_{b < nb} c[b] V[b vs + i]   (c on the device)
+__global__ void rcombine_kernel(int64_t n, int nb, const double* __restrict__ V, int64_t vs,
+                                const double* __restrict__ c, double sign, double* __restrict__ w)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = w[i];
+    for (int b = 0; b < nb; ++b) acc = fma(sign * c[b], V[b * vs + i], acc);
+    w[i] = acc;
+}
+
+// y = a x + b y  (b == 0: y is not read)
+__global__ void raxpby_kernel(int64_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = (b != 0.0) ? fma(a, x[i], b * y[i]) : a * x[i];
+}
+
+// Y (nv x 3, ld ldY) = a X (nv x 3, ld ldX) + b Y  (b == 0: Y is not read)
+__global__ void rows3_axpby_kernel(int64_t nv, double a, const double* __restrict__ X, int64_t ldX, double b,
+                                   double* __restrict__ Y, int64_t ldY)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 3 * nv) return;
+    const int64_t v = e / 3, d = e % 3;
+    const double x = X[v * ldX + d];
+    Y[v * ldY + d] = (b != 0.0) ? fma(a, x, b * Y[v * ldY + d]) : a * x;
+}
+
+// y = x / (c + d m_v) per vertex v (3 columns), m_v the mass-matrix row sum
+// 3 / lumped_inv[v] (the row sum of the P1 mass matrix is a third of the
+// areas around v; lumped_inv holds 9 over that area sum).
+__global__ void metric_jacobi_kernel(int64_t nv, double c, double d, const double* __restrict__ lumped_inv,
+                                     const double* __restrict__ x, double* __restrict__ y)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 3 * nv) return;
+    const double li = lumped_inv[e / 3];
+    const double mv = li > 0.0 ? 3.0 / li : 0.0;
+    y[e] = x[e] / (c + d * mv);
+}
+
+void dev_rdots(int64_t n, int nb, const double* V, int64_t vs, const double* w, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(rdots_kernel, dim3((unsigned)nb), dim3(kRdotThreads), 0, s, n, V, vs, w, out);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_rcombine(int64_t n, int nb, const double* V, int64_t vs, const double* c, double sign, double* w,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(rcombine_kernel, grid_for(n), dim3(256), 0, s, n, nb, V, vs, c, sign, w);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_raxpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s)
+{
+    hipLaunchKernelGGL(raxpby_kernel, grid_for(n), dim3(256), 0, s, n, a, x, b, y);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_rows3_axpby(int64_t nv, double a, const double* X, int64_t ldX, double b, double* Y, int64_t ldY,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(rows3_axpby_kernel, grid_for(3 * nv), dim3(256), 0, s, nv, a, X, ldX, b, Y, ldY);
+    BAEMM_HIP(hipGetLastError());
+}
+
+void dev_metric_jacobi(Handle& h, double c, double d, const double* x, double* y, hipStream_t s)
+{
+    hipLaunchKernelGGL(metric_jacobi_kernel, grid_for(3 * h.nv), dim3(256), 0, s, h.nv, c, d, h.d_lumped_inv, x, y);
+    BAEMM_HIP(hipGetLastError());
+}
+
+}  // namespace baemm

@@ -35,6 +35,20 @@ void dev_mass_cg(Handle& h, Workspace& ws, const double* X, int64_t ldX, double*
                  int max_iter, hipStream_t s);
 int dev_mass_cg_result(Workspace& ws, hipStream_t s);
 
+// Real vectors of the device-side Gauss-Newton GMRES.
+// out[b] = V_b . w for b < nb (V_b = V + b vs), one reproducible sum per vector.
+void dev_rdots(int64_t n, int nb, const double* V, int64_t vs, const double* w, double* out, hipStream_t s);
+// w += sign sum_b c[b] V_b (c on the device)
+void dev_rcombine(int64_t n, int nb, const double* V, int64_t vs, const double* c, double sign, double* w,
+                  hipStream_t s);
+// y = a x + b y (b == 0: y not read)
+void dev_raxpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t s);
+// Y = a X + b Y on nv x 3 arrays with leading dimensions (b == 0: Y not read)
+void dev_rows3_axpby(int64_t nv, double a, const double* X, int64_t ldX, double b, double* Y, int64_t ldY,
+                     hipStream_t s);
+// y = x / (c + d m_v) row by row (m_v: row sum of the mass matrix), nv x 3
+void dev_metric_jacobi(Handle& h, double c, double d, const double* x, double* y, hipStream_t s);
+
 // GMRES cycle state on the device (Hessenberg, rotations, least squares).
 struct GmState {
     double2* H = nullptr;   // W x (m+1) x m

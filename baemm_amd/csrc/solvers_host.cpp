@@ -1113,6 +1113,142 @@ int baemm_gauss_newton_solve(baemm_handle h, const double* kappa, int64_t K, con
     });
 }
 
+int baemm_gauss_newton_solve_dev(baemm_handle h, const double* kappa, int64_t K, const double* dirs, int64_t wcs,
+                                 baemm_dev_real_op M, void* M_user, baemm_dev_real_op P, void* P_user, double alpha,
+                                 const double* B, int64_t ldB, double beta, double* X, int64_t ldX, void* du_dn,
+                                 int du_dn_ready, int wave_type, double cg_tol, double gmres_tol_inner,
+                                 double gmres_tol_outer, int* succeeded, void* stream)
+{
+    return guard(h, [&](Handle& hh) {
+        check_common(hh, kappa, K, wcs);
+        if (!dirs || !M || !P || !B || !X || ldB < 3 || ldX < 3) throw ArgError{"invalid Gauss-Newton arguments"};
+        if (hh.m < 1) throw ArgError{"no measurement directions"};
+        const int64_t W = K * wcs, NW = hh.nv * W, n = hh.nv, N = 3 * n;
+        const int m = hh.gmres_max_iter;
+        hipStream_t s = stream_of(hh, stream);
+        void* sv = static_cast<void*>(s);
+        hh.stats.converged = true;
+        DevArray ddu(hh, 0, du_dn ? 16 : sizeof(double2) * NW);
+        double2* du = du_dn ? static_cast<double2*>(du_dn) : ddu.as<double2>();
+        if (!du_dn || !du_dn_ready) du_dn_dev(hh, kappa, K, wcs, dirs, wave_type, du, cg_tol, gmres_tol_inner, s);
+        // Krylov basis (m + 1) x N, then b, x, r, z, t and DF* DF x (N each), then
+        // two passes of Gram-Schmidt coefficients (m + 1 each)
+        DevArray dv(hh, 1, sizeof(double) * ((size_t)(m + 1) * N + 6 * (size_t)N + 2 * (size_t)(m + 1)));
+        double* Vb = dv.as<double>();
+        double* b = Vb + (size_t)(m + 1) * N;
+        double* xs = b + N;
+        double* r = xs + N;
+        double* z = r + N;
+        double* t = z + N;
+        double* dY = t + N;
+        double* hd = dY + N;
+        DevArray dF(hh, 2, sizeof(double2) * hh.m * W);
+        // A x = M x + Mass DF* DF x  (FarField.hpp:551-572), all on the device
+        auto A = [&](const double* x, double* y) {
+            M(M_user, x, y, sv);
+            derivative_dev(hh, kappa, K, wcs, x, du, dF.as<double2>(), cg_tol, gmres_tol_inner, s);
+            adjoint_dev(hh, kappa, K, wcs, dF.as<double2>(), du, dY, cg_tol, gmres_tol_inner, s);
+            dev_mass(hh, 1.0, dY, 3, 1.0, y, 3, 3, s);
+        };
+        auto norm = [&](const double* x) {
+            dev_rdots(N, 1, x, N, x, hd, s);
+            double v = 0.0;
+            download(&v, hd, sizeof(double), s);
+            return std::sqrt(v);
+        };
+        dev_rows3_axpby(n, 1.0, B, ldB, 0.0, b, 3, s);
+        BAEMM_HIP(hipMemsetAsync(xs, 0, sizeof(double) * N, s));
+        // Left-preconditioned restarted GMRES over vec(X): vectors in HBM, two
+        // passes of classical Gram-Schmidt per step, the (m + 1) x m Hessenberg
+        // matrix and its Givens rotations on the host (j + 1 numbers per step).
+        std::vector<double> Hm((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), yv(m), h1(m + 1), h2(m + 1);
+        bool ok = false;
+        double bnorm = 0.0;
+        for (int cyc = 0; cyc < hh.gmres_max_restarts && !ok; ++cyc) {
+            if (cyc == 0) {
+                P(P_user, b, z, sv);
+            } else {
+                A(xs, t);
+                dev_raxpby(N, 1.0, b, 0.0, r, s);
+                dev_raxpby(N, -1.0, t, 1.0, r, s);
+                P(P_user, r, z, sv);
+            }
+            const double bt = norm(z);
+            if (cyc == 0) bnorm = bt;
+            if (bnorm == 0.0 || bt <= gmres_tol_outer * bnorm) {
+                ok = true;
+                break;
+            }
+            dev_raxpby(N, 1.0 / bt, z, 0.0, Vb, s);
+            std::fill(g.begin(), g.end(), 0.0);
+            g[0] = bt;
+            int jd = 0;
+            for (int j = 0; j < m; ++j) {
+                A(Vb + (size_t)j * N, t);
+                P(P_user, t, z, sv);
+                dev_rdots(N, j + 1, Vb, N, z, hd, s);
+                dev_rcombine(N, j + 1, Vb, N, hd, -1.0, z, s);
+                dev_rdots(N, j + 1, Vb, N, z, hd + (m + 1), s);
+                dev_rcombine(N, j + 1, Vb, N, hd + (m + 1), -1.0, z, s);
+                download(h1.data(), hd, sizeof(double) * (j + 1), s);
+                download(h2.data(), hd + (m + 1), sizeof(double) * (j + 1), s);
+                for (int i = 0; i <= j; ++i) Hm[(size_t)i * m + j] = h1[i] + h2[i];
+                const double hn = norm(z);
+                dev_raxpby(N, hn > 0.0 ? 1.0 / hn : 0.0, z, 0.0, Vb + (size_t)(j + 1) * N, s);
+                for (int i = 0; i < j; ++i) {
+                    const double a0 = Hm[(size_t)i * m + j], a1 = Hm[(size_t)(i + 1) * m + j];
+                    Hm[(size_t)i * m + j] = cs[i] * a0 + sn[i] * a1;
+                    Hm[(size_t)(i + 1) * m + j] = -sn[i] * a0 + cs[i] * a1;
+                }
+                const double a0 = Hm[(size_t)j * m + j];
+                const double nu = std::sqrt(a0 * a0 + hn * hn);
+                cs[j] = nu > 0.0 ? a0 / nu : 1.0;
+                sn[j] = nu > 0.0 ? hn / nu : 0.0;
+                Hm[(size_t)j * m + j] = nu;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = cs[j] * g[j];
+                jd = j + 1;
+                hh.stats.gmres_iterations++;
+                if (std::fabs(g[j + 1]) <= gmres_tol_outer * bnorm || hn == 0.0) break;
+            }
+            for (int i = jd - 1; i >= 0; --i) {
+                double acc = g[i];
+                for (int l = i + 1; l < jd; ++l) acc -= Hm[(size_t)i * m + l] * yv[l];
+                yv[i] = Hm[(size_t)i * m + i] != 0.0 ? acc / Hm[(size_t)i * m + i] : 0.0;
+            }
+            // x += sum_i y_i V_i (the coefficients through the device)
+            upload(hd, yv.data(), sizeof(double) * jd, s);
+            dev_rcombine(N, jd, Vb, N, hd, 1.0, xs, s);
+            BAEMM_HIP(hipStreamSynchronize(s));   // yv is reused by the next cycle
+            hh.stats.gmres_rel_residual = std::fabs(g[jd]) / bnorm;
+            if (std::fabs(g[jd]) <= gmres_tol_outer * bnorm) ok = true;
+        }
+        // X = alpha (M + Mass DF* DF)^{-1} B + beta X
+        dev_rows3_axpby(n, alpha, xs, 3, beta, X, ldX, s);
+        BAEMM_HIP(hipStreamSynchronize(s));
+        if (succeeded) *succeeded = ok ? 1 : 0;
+        return BAEMM_OK;
+    });
+}
+
+void baemm_mass_metric_apply(void* user, const double* x, double* y, void* stream)
+{
+    const auto* mm = static_cast<const baemm_mass_metric*>(user);
+    if (!mm || !mm->h || !x || !y) return;
+    Handle& hh = *H(mm->h);
+    hipStream_t s = stream_of(hh, stream);
+    dev_mass(hh, mm->d, x, 3, 0.0, y, 3, 3, s);
+    dev_raxpby(3 * hh.nv, mm->c, x, 1.0, y, s);
+}
+
+void baemm_mass_metric_jacobi(void* user, const double* x, double* y, void* stream)
+{
+    const auto* mm = static_cast<const baemm_mass_metric*>(user);
+    if (!mm || !mm->h || !x || !y) return;
+    Handle& hh = *H(mm->h);
+    dev_metric_jacobi(hh, mm->c, mm->d, x, y, stream_of(hh, stream));
+}
+
 int baemm_get_solver_stats(baemm_handle h, int64_t* gmres_iterations, int64_t* gmres_cycles, double* rel_residual,
                            int64_t* cg_iterations, int64_t* operator_applies)
 {

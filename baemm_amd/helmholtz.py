@@ -498,6 +498,53 @@ class _SolverMixin:
         return bool(ok.value), self._du_result(pp, keep, W)
 
 
+    def GaussNewtonSolveDevice(self, kappa, wave_chunk_count, inc_directions, wave_chunk_size, c, d, alpha, B, beta,
+                               X, du_dn=None, type=WaveType.Plane, cg_tol=1e-5, gmres_tol_inner=5e-3,
+                               gmres_tol_outer=1e-2):
+        """GaussNewtonSolve with every vector on the device
+        (baemm_gauss_newton_solve_dev) and the device metric
+        M = c I + d (Mass (x) I_3) with its Jacobi preconditioner
+        (baemm_mass_metric_apply / _jacobi).  B, X: V x 3 float64 views
+        (ld >= 3 allowed); X is updated in place.  du_dn: V x W complex or None
+        (then computed).  Returns (succeeded, du_dn)."""
+        k, dr = self._kd(kappa, inc_directions, wave_chunk_size)
+        W = int(wave_chunk_count) * int(wave_chunk_size)
+        nv = self.VertexCount()
+        L = self._L
+        ldB, ldX = _ld(B, "B"), _ld(X, "X")
+        # device copies with the callers' leading dimensions (columns 3.. unused)
+        Bfull = np.zeros((nv, ldB))
+        Bfull[:, :3] = B[:, :3]
+        Xfull = np.zeros((nv, ldX))
+        Xfull[:, :3] = X[:, :3]
+        du = np.zeros((nv, W), np.complex128) if du_dn is None else np.require(du_dn, np.complex128, ["C"])
+        bufs = []
+        try:
+            for a in (Bfull, Xfull, du):
+                p = ctypes.c_void_p()
+                _lib.check(L.baemm_dev_alloc(ctypes.byref(p), a.nbytes))
+                bufs.append(p)
+                _lib.check(L.baemm_dev_upload(p, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+            metric = _lib.MassMetric(self._h.value, float(c), float(d))
+            Mf = ctypes.cast(L.baemm_mass_metric_apply, ctypes.c_void_p)
+            Pf = ctypes.cast(L.baemm_mass_metric_jacobi, ctypes.c_void_p)
+            mp = ctypes.cast(ctypes.pointer(metric), ctypes.c_void_p)
+            ok = ctypes.c_int(0)
+            st = L.baemm_gauss_newton_solve_dev(self._h, _ptr(k), int(wave_chunk_count), _ptr(dr),
+                                                int(wave_chunk_size), Mf, mp, Pf, mp, float(alpha), bufs[0], ldB,
+                                                float(beta), bufs[1], ldX, bufs[2], 0 if du_dn is None else 1,
+                                                int(type), float(cg_tol), float(gmres_tol_inner),
+                                                float(gmres_tol_outer), ctypes.byref(ok), None)
+            self._check(st)
+            _lib.check(L.baemm_dev_download(Xfull.ctypes.data_as(ctypes.c_void_p), bufs[1], Xfull.nbytes))
+            _lib.check(L.baemm_dev_download(du.ctypes.data_as(ctypes.c_void_p), bufs[2], du.nbytes))
+        finally:
+            for p in bufs:
+                L.baemm_dev_free(p)
+        X[:, :3] = Xfull[:, :3]
+        return bool(ok.value), du
+
+
 class Helmholtz_HIP(_HelmholtzBase, _SolverMixin):
     """Drop-in for ``BAEMM::Helmholtz_OpenCL`` (Helmholtz_OpenCL.hpp:34-399)."""
 

@@ -389,6 +389,39 @@ int baemm_gauss_newton_solve(baemm_handle h, const double* kappa, int64_t K, con
                              int64_t ldX, double** du_dn, int wave_type, double cg_tol,
                              double gmres_tol_inner, double gmres_tol_outer, int* succeeded);
 
+/* Device-side GaussNewtonSolve (no reference counterpart: it replaces the
+ * host callbacks of Example_GaussNewton/main.cpp:326-346).  The same solve as
+ * baemm_gauss_newton_solve, with every vector in HBM: the outer GMRES keeps its
+ * Krylov basis on the device, and M and P are called with device arrays
+ * (V x 3 doubles, ld 3) and the stream the solve runs on, so a metric written
+ * for the GPU runs without host copies.  B (ld ldB >= 3), X (ld ldX >= 3):
+ * device arrays.  du_dn: device array V x W complex double; when du_dn_ready
+ * is 0 it is filled (Derivative_FF_Helper), else used as is; NULL: computed
+ * into a buffer of the handle.  stream NULL: the handle's stream.
+ * Synchronous. */
+typedef void (*baemm_dev_real_op)(void* user, const double* x, double* y, void* stream);
+int baemm_gauss_newton_solve_dev(baemm_handle h, const double* kappa, int64_t K, const double* dirs,
+                                 int64_t wcs, baemm_dev_real_op M, void* M_user, baemm_dev_real_op P,
+                                 void* P_user, double alpha, const double* B, int64_t ldB, double beta,
+                                 double* X, int64_t ldX, void* du_dn, int du_dn_ready, int wave_type,
+                                 double cg_tol, double gmres_tol_inner, double gmres_tol_outer,
+                                 int* succeeded, void* stream);
+
+/* A metric for baemm_gauss_newton_solve_dev that runs on the device (no
+ * reference counterpart; Repulsor's TangentPointMetric0, the metric of
+ * Example_GaussNewton/main.cpp:301-346, is not part of BAEMM):
+ *   M x = c x + d (Mass (x) I_3) x,   P x = x / (c + d m_v) row by row,
+ * with m_v the row sums of the P1 mass matrix (a Jacobi preconditioner).
+ * user points to a baemm_mass_metric whose handle is the one being solved on;
+ * the functions are meant to be passed to baemm_gauss_newton_solve_dev, which
+ * calls them with that handle's lock held. */
+typedef struct {
+    baemm_handle h;
+    double c, d;
+} baemm_mass_metric;
+void baemm_mass_metric_apply(void* user, const double* x, double* y, void* stream);
+void baemm_mass_metric_jacobi(void* user, const double* x, double* y, void* stream);
+
 /* No reference counterpart: the GMRES of the solvers above on the loaded
  * boundary operator A: solves (P A) X = P B (P = mass inverse by PCG, cg_tol)
  * to gmres_tol.  B, X: device arrays, vertex_count x W complex double (ld W).

@@ -141,6 +141,55 @@ def test_gauss_newton_matches_dense_fp64_oracle():
             assert np.all(Xw[:, 3] == 7.0)
 
 
+def test_gauss_newton_on_the_device_matches_host_callbacks_and_oracle():
+    """baemm_gauss_newton_solve_dev: the outer GMRES with its Krylov basis in
+    HBM and the device metric c I + d (Mass (x) I_3) with its Jacobi
+    preconditioner (baemm_mass_metric_apply / _jacobi), no host callback.
+    Against the oracle's dense FP64 Gauss-Newton solve and against the same
+    solve with the metric as host callbacks (baemm_gauss_newton_solve), with
+    B and X in scattered storage (ldB = 5, ldX = 4) and alpha/beta; a second
+    call that passes the returned du_dn (not recomputed) gives the same X."""
+    m = oracle_mesh("sphere", 2, 64)
+    kappa, wcs = [2.0, 3.5], 2
+    dirs = DIRS[:2]
+    c, d = 0.05, 0.5
+    metric = c * np.eye(3 * m.nv) + d * np.kron(m.M.toarray(), np.eye(3))
+    rng = np.random.default_rng(18)
+    B = rng.standard_normal((m.nv, 3))
+    X0 = rng.standard_normal((m.nv, 3))
+    alpha, beta = 0.5, 2.0
+    ref, du_ref = oracle.gauss_newton(m, kappa, dirs, wcs, metric, B, alpha, beta, X0)
+    lumped = np.asarray(m.M.sum(axis=1)).ravel()
+    with Helmholtz_HIP(m.V, m.T, m.meas) as H:
+        H.SetGmresLimits(60, 4)
+        for mode in ("fp32", "mixed"):
+            H.SetPrecision(mode)
+            Bw = np.zeros((m.nv, 5))
+            Bw[:, :3] = B
+            Xd = np.full((m.nv, 4), 7.0)
+            Xd[:, :3] = X0
+            ok, du = H.GaussNewtonSolveDevice(kappa, 2, dirs, wcs, c, d, alpha, Bw[:, :3], beta, Xd[:, :3],
+                                              cg_tol=1e-12, gmres_tol_inner=1e-10, gmres_tol_outer=1e-9)
+            assert ok
+            assert np.all(Xd[:, 3] == 7.0)
+            Xh = X0.copy()
+            ok_h, _ = H.GaussNewtonSolve(kappa, 2, dirs, wcs, lambda x, y: y.__setitem__(slice(None), c * x + d * (m.M @ x)),
+                                         lambda x, y: y.__setitem__(slice(None), x / (c + d * lumped)[:, None]),
+                                         alpha, B, beta, Xh, du_dn=du, cg_tol=1e-12, gmres_tol_inner=1e-10,
+                                         gmres_tol_outer=1e-9)
+            assert ok_h
+            Xd2 = X0.copy()
+            ok2, _ = H.GaussNewtonSolveDevice(kappa, 2, dirs, wcs, c, d, alpha, B, beta, Xd2, du_dn=du,
+                                              cg_tol=1e-12, gmres_tol_inner=1e-10, gmres_tol_outer=1e-9)
+            e_x, e_du = relerr(Xd[:, :3], ref), relerr(du, du_ref)
+            e_host, e_again = relerr(Xd[:, :3], Xh), relerr(Xd2, Xd[:, :3])
+            print(f"device Gauss-Newton {mode}: X vs FP64 {e_x:.2e}, du_dn {e_du:.2e}, vs host callbacks {e_host:.2e}, "
+                  f"with du_dn passed {e_again:.2e}", H.SolverStats())
+            assert ok2
+            assert e_du < 2e-5 and e_x < 1e-4, mode
+            assert e_host < 1e-6 and e_again < 1e-6, mode
+
+
 def test_far_field_mie_refinement():
     """Sound-soft unit sphere: far field error vs the Mie series shrinks under refinement."""
     errs = []
