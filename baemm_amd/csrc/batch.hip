@@ -61,11 +61,9 @@ static int mixed_variant(int WCSP, int KG, bool adj)
 // Largest LDS need over the variants (all have 4 column blocks or fewer).
 constexpr int kWsOpMax = WsLds<32, 2>::B_ELEMS;
 constexpr int kWsAMax = WsLds<32, 2>::A_ELEMS;
-constexpr int kWsGeoMax = 2 * MixedLds<16, 1>::GEO_ELEMS;   // geometry of the two-tile single-wave items
 static_assert(WsLds<64, 1>::B_ELEMS <= kWsOpMax && WsLds<64, 1>::A_ELEMS <= kWsAMax, "LDS bound");
 static_assert(WsLds<16, 2>::B_ELEMS <= kWsOpMax && WsLds<16, 2>::A_ELEMS <= kWsAMax, "LDS bound");
 static_assert(2 * MixedLds<16, 1>::OP_ELEMS <= kWsOpMax + kWsAMax, "LDS bound");
-static_assert(2 * MixedLds<16, 1>::GEO_ELEMS <= kWsGeoMax, "LDS bound");
 
 // Wave-specialised blocks of 8 waves (generation waves feed MFMA waves), as
 // the single-instance MFMA kernel.
@@ -74,7 +72,6 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
     int n_items, unsigned* __restrict__ counter)
 {
     __shared__ uint4 s_opa[kWsOpMax + kWsAMax];
-    __shared__ float4 s_geo[kWsGeoMax];
     __shared__ int s_item;
     uint4* const s_op = s_opa;
     uint4* const s_a = s_opa + kWsOpMax;
@@ -111,8 +108,7 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
         const int t2 = 2 * w.tile + half;                                                                        \
         const bool real_tile = t2 < in.nt_pad / kMixTile;                                                        \
         mixed_block<16, KG_, ADJ_>(in.mid, in.nrm, in.nt, in.bop, in.nst, cp, real_tile ? out : nullptr, in.ldp,   \
-                                   w.q0, real_tile ? t2 : 2 * w.tile, w.s0, w.s1, s_op + half * LL::OP_ELEMS,       \
-                                   s_geo + half * LL::GEO_ELEMS);                                                 \
+                                   w.q0, real_tile ? t2 : 2 * w.tile, w.s0, w.s1, s_op + half * LL::OP_ELEMS);     \
         break;                                                                                                   \
     }
         switch (w.variant) {

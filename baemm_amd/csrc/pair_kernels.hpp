@@ -290,20 +290,6 @@ __device__ __forceinline__ void mixed_vals(const BdrGeo (&geo)[4], const CoefPac
     }
 }
 
-template <int KG, bool ADJ, bool MASK>
-__device__ __forceinline__ void mixed_gen(const float4 x, const float4 ni, const float4 (&y)[4], const float4 (&nj)[4],
-                                          int jbase, int i, int nt, const CoefPack& cp, uint32_t (&ah)[KG][4],
-                                          uint32_t (&al)[KG][4])
-{
-    BdrGeo geo[4];
-#pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-        const int j = jbase + jj;
-        geo[jj] = bdr_geo<ADJ, MASK>(x, ni, y[jj], nj[jj], (j != i) && (j < nt));
-    }
-    mixed_vals<KG, ADJ>(geo, cp, ah, al);
-}
-
 // Geometry of the 4 (target i, source jbase + jj) pairs of one lane when the
 // sources are spread over the lanes instead of read by every lane: lane
 // 16 r + jj (jj < 4) of each 16-lane row r holds source jj of its half-wave
@@ -378,37 +364,40 @@ __device__ __forceinline__ void mixed_geo_dpp(const float4 x, const float4 ni, c
 #define BAEMM_MIX_NS 2
 #endif
 #ifndef BAEMM_MIX_MINB
-#define BAEMM_MIX_MINB 3
+#define BAEMM_MIX_MINB 4
 #endif
-// LDS footprint of one mixed block: operand tiles (uint4) and source geometry
-// (float4) of NS steps, double buffered.
+// LDS footprint of one single-wave mixed block: the B operand tiles (uint4)
+// of NS steps, double buffered.
 template <int WCSP, int KG>
 struct MixedLds {
     static constexpr int NCB = 2 * WCSP / 32;
     static constexpr int OPS = KG * NCB * 2;     // 1-KB operand tiles per step (chunk x col-block x hi/lo)
     static constexpr int NS = BAEMM_MIX_NS;      // steps (8 sources each) per LDS stage
     static constexpr int TILES = NS * OPS;
-    static constexpr int GEO = 2 * NS * 8;       // float4 per stage: NS*8 centroids, then NS*8 normals
     static constexpr int OP_ELEMS = 2 * TILES * 64;
-    static constexpr int GEO_ELEMS = 2 * GEO;
 };
 
-// One block of the mixed kernel: targets [tile*128, tile*128+128) against the
-// source steps [st0, st1) of chunks q0..q0+KG-1, partial sums written to
-// part[row*ldp + ...] (one segment slice).  s_op / s_geo point to LDS of at
-// least MixedLds<WCSP,KG>::OP_ELEMS / GEO_ELEMS elements.  Called by whole
-// blocks of 256 threads; ends with a barrier so the caller may reuse the LDS.
+// One block of the single-wave mixed kernel: targets [tile*128, tile*128+128)
+// against the source steps [st0, st1) of chunks q0..q0+KG-1, partial sums
+// written to out[row*ldp + ...] (one segment slice; out NULL: a tile computed
+// only to keep the block's barriers in step).  Each wave generates the FP16
+// hi/lo A operands of its 32 targets in registers and contracts them itself.
+// Each lane loads the one source (centroid, normal) of each step it holds for
+// mixed_geo_dpp from global memory, one step ahead; only the B operands are
+// staged in LDS (s_op: MixedLds<WCSP,KG>::OP_ELEMS elements), NS steps per
+// stage and barrier.  Called by whole blocks of 256 threads (or 512 running
+// two tiles side by side); ends with a barrier so the caller may reuse the LDS.
 template <int WCSP, int KG, bool ADJ>
 __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
                                             const uint4* __restrict__ bop, int nst, const CoefPack& cp,
                                             float* __restrict__ out, int ldp, int q0, int tile, int st0, int st1,
-                                            uint4* s_op, float4* s_geo)
+                                            uint4* s_op)
 {
     using L = MixedLds<WCSP, KG>;
-    constexpr int NCB = L::NCB, OPS = L::OPS, NS = L::NS, TILES = L::TILES, GEO = L::GEO;
+    constexpr int NCB = L::NCB, OPS = L::OPS, NS = L::NS, TILES = L::TILES;
     const int lane = threadIdx.x & 63;
-    // 4 waves per tile; a 512-thread block may run two tiles side by side
-    const int wave = (threadIdx.x >> 6) & 3;
+    // wave-uniform (scalar register): 4 waves per tile
+    const int wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);
     const int g = lane >> 5;
     const int ibase = tile * kMixTile + wave * 32;
     const int i = ibase + (lane & 31);
@@ -425,6 +414,12 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
             for (int r = 0; r < 16; ++r) acc[q][cb][r] = 0.0f;
 
     const int nt_pad = nst * 8;
+    // steps to mask (wave-uniform): the 4 steps whose sources are this wave's
+    // targets, and steps reaching past nt
+    const int st_diag = __builtin_amdgcn_readfirstlane(ibase >> 3);
+    const int st_tail = nt >> 3;
+    // the source of each step this lane holds for mixed_geo_dpp
+    const int src_lane = 4 * g + (lane & 3);
 
     auto issue_stage = [&](int sbase, int buf) {
         for (int t = wave; t < TILES; t += 4) {
@@ -436,23 +431,15 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
             __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_op[(buf * TILES + t) * 64],
                                              16, 0, 0);
         }
-        if (wave == 0) {
-            // lane e of each 64-entry piece writes s_geo[buf][e0 + e]
-#pragma unroll
-            for (int e0 = 0; e0 < GEO; e0 += 64) {
-                const int e = e0 + lane;
-                if (e < GEO) {
-                    const int which = e / (NS * 8), k = e % (NS * 8);
-                    const int j = min(sbase * 8 + k, nt_pad - 1);
-                    const float4* src = which ? (nrm + j) : (mid + j);
-                    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&s_geo[buf * GEO + e0],
-                                                     16, 0, 0);
-                }
-            }
-        }
     };
 
-    if (st0 < st1) issue_stage(st0, 0);
+    float4 yv = make_float4(0.f, 0.f, 0.f, 0.f), nv = yv;
+    if (st0 < st1) {
+        const int j = min(st0 * 8 + src_lane, nt_pad - 1);
+        yv = mid[j];
+        nv = nrm[j];
+        issue_stage(st0, 0);
+    }
     int buf = 0;
     for (int sb = st0; sb < st1; sb += NS, buf ^= 1) {
         __builtin_amdgcn_s_waitcnt(0);
@@ -461,17 +448,18 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
         const int ns = min(NS, st1 - sb);
         for (int s = 0; s < ns; ++s) {
             const int st = sb + s;
-            float4 y[4], nj[4];
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                y[jj] = s_geo[buf * GEO + s * 8 + 4 * g + jj];
-                nj[jj] = s_geo[buf * GEO + NS * 8 + s * 8 + 4 * g + jj];
+            float4 yn = yv, nn = nv;
+            if (st + 1 < st1) {
+                const int j = min((st + 1) * 8 + src_lane, nt_pad - 1);
+                yn = mid[j];
+                nn = nrm[j];
             }
             uint32_t ah[KG][4], al[KG][4];
-            // Sources of this step overlap the wave's targets or the padding: mask.
-            const bool masked = (8 * st < ibase + 32 && 8 * st + 8 > ibase) || (8 * st + 8 > nt);
-            if (masked) mixed_gen<KG, ADJ, true>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
-            else mixed_gen<KG, ADJ, false>(x, ni, y, nj, 8 * st + 4 * g, i, nt, cp, ah, al);
+            BdrGeo geo[4];
+            const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
+            if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
+            else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
+            mixed_vals<KG, ADJ>(geo, cp, ah, al);
             half8_t a_hi[KG], a_lo[KG], b_hi[KG][NCB], b_lo[KG][NCB];
 #pragma unroll
             for (int q = 0; q < KG; ++q) {
@@ -500,9 +488,11 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
 #pragma unroll
                 for (int cb = 0; cb < NCB; ++cb)
                     acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo[q], b_hi[q][cb], acc[q][cb], 0, 0, 0);
+            yv = yn;
+            nv = nn;
         }
     }
-    if (out) {   // NULL: a tile computed only to keep the block's barriers in step
+    if (out) {
 #pragma unroll
         for (int q = 0; q < KG; ++q)
 #pragma unroll
@@ -516,8 +506,13 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
     __syncthreads();
 }
 
+// Waves per SIMD the single-wave kernel is built for: 4 (128 VGPRs) where the
+// registers fit without spilling in the step loop, else 3.
+template <int KG, bool ADJ>
+constexpr int mixed_min_waves() { return (!ADJ && KG <= 2) ? BAEMM_MIX_MINB : 3; }
+
 template <int WCSP, int KG, bool ADJ>
-__global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
+__global__ __launch_bounds__(256, (mixed_min_waves<KG, ADJ>())) void bdr_mixed_kernel(
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
     const uint4* __restrict__ bop, int nst, CoefPack cp,
     float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg, const int* __restrict__ skip)
@@ -525,11 +520,10 @@ __global__ __launch_bounds__(256, BAEMM_MIX_MINB) void bdr_mixed_kernel(
     if (skip && *skip) return;   // a converged solve: the step's result is not used
     using L = MixedLds<WCSP, KG>;
     __shared__ uint4 s_op[L::OP_ELEMS];
-    __shared__ float4 s_geo[L::GEO_ELEMS];
     const int st0 = blockIdx.y * steps_per_seg;
     const int st1 = min(st0 + steps_per_seg, nst);
     mixed_block<WCSP, KG, ADJ>(mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
-                               blockIdx.x, st0, st1, s_op, s_geo);
+                               blockIdx.x, st0, st1, s_op);
 }
 
 // ---------------------------------------------------------------------------
