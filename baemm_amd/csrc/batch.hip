@@ -2,10 +2,13 @@
 // uneven triangle counts, wave numbers and RHS widths) in ONE persistent
 // launch per arithmetic mode.
 //
-// Each instance is prepared as in the single-instance path (Av gather and,
-// for the MFMA path, the scaled hi/lo operand split).  An instance large
-// enough to fill the device with its own pair launch runs that launch on its
-// stream.  For the others a work list of
+// An instance large enough to fill the device with its own pair launch runs
+// the single-instance path on its stream (Av gather, for the MFMA path the
+// scaled hi/lo operand split, the pair launch, the vertex epilogue).  The
+// others are staged together: each stage (Av gather with the scaled normals,
+// column maxima, operand split, vertex epilogue) is one launch over all of
+// them on the call's stream, from per-instance descriptors, computing what
+// the single-instance kernels compute.  For their pair sums a work list of
 // items (instance, target tile, source range, chunk group) is built on the
 // host, ordered by decreasing cost, and walked by a persistent grid: every
 // block takes the next item from a device counter (one vector atomic per
@@ -20,6 +23,7 @@
 
 #include "batch.hpp"
 #include "ops.hpp"
+#include "stage.hpp"
 
 namespace baemm {
 
@@ -191,6 +195,114 @@ __global__ __launch_bounds__(kTile) void bdr_plain_batch_kernel(const BatchInst*
 
 __global__ void batch_zero_kernel(unsigned* p) { p[threadIdx.x] = 0u; }
 
+// ------------------------------------------------- batched staging, epilogue
+// Instance k owns the flattened range [off[k], off[k+1]) of a launch; off has
+// n + 1 entries, off[0] = 0.
+__device__ __forceinline__ int owner_of(const int64_t* __restrict__ off, int n, int64_t idx)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= idx) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Per instance: b = Av B (nt_pad x ldb), the scaled source normals (nt_pad,
+// non-adjoint MFMA form) and zeroed column maxima (ldb, MFMA path).
+template <int BT>
+__global__ void batch_stage_kernel(const StageDesc* __restrict__ sd, const int64_t* __restrict__ off, int n)
+{
+    const int64_t total = off[n];
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int k = owner_of(off, n, idx);
+        const StageDesc& d = sd[k];
+        int64_t e = idx - off[k];
+        const int64_t nav = (int64_t)d.nt_pad * d.ldb;
+        if (e < nav) {
+            d.b[e] = av_elem<BT>(d.tris, d.area, d.nt, d.B, d.ldB, d.wcs, d.WCSP, d.ldb, e);
+            continue;
+        }
+        e -= nav;
+        const int64_t nn = d.nrm_out ? d.nt_pad : 0;
+        if (e < nn) {
+            const float4 v = d.nrm_in[e];
+            d.nrm_out[e] = make_float4(d.nrm_s * v.x, d.nrm_s * v.y, d.nrm_s * v.z, v.w);
+            continue;
+        }
+        d.colmax[e - nn] = 0u;
+    }
+}
+
+constexpr int kColmaxRows = 256;   // rows per block of the batched column maxima
+
+// Column maxima of |re|, |im| of b (rows < nt): block = 4 row lanes x 64
+// columns over kColmaxRows rows of one instance; one atomicMax per column and
+// block (positive floats order like their bit patterns).
+__global__ __launch_bounds__(256) void batch_colmax_kernel(const StageDesc* __restrict__ sd,
+                                                           const int64_t* __restrict__ boff, int n)
+{
+    __shared__ float sm[4][64];
+    const int k = owner_of(boff, n, blockIdx.x);
+    const StageDesc& d = sd[k];
+    const int r0 = (int)(blockIdx.x - boff[k]) * kColmaxRows;
+    const int r1 = min(r0 + kColmaxRows, d.nt);
+    const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < d.ldb; c0 += 64) {
+        float m = 0.0f;
+        if (c0 + c < d.ldb)
+            for (int r = r0 + rl; r < r1; r += 4) {
+                const float2 v = d.b[(int64_t)r * d.ldb + c0 + c];
+                m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+            }
+        sm[rl][c] = m;
+        __syncthreads();
+        if (rl == 0 && c0 + c < d.ldb) {
+            m = fmaxf(fmaxf(sm[0][c], sm[1][c]), fmaxf(sm[2][c], sm[3][c]));
+            atomicMax(&d.colmax[c0 + c], __float_as_uint(m));
+        }
+        __syncthreads();
+    }
+}
+
+// Per MFMA instance: the hi/lo B operand (K x nst x NCB x 64 elements) with
+// the column scales taken from the maxima, then the scales and unscales
+// (ldb), as colscale_kernel and split_kernel compute them.
+__global__ void batch_split_kernel(const StageDesc* __restrict__ sd, const int64_t* __restrict__ off, int n)
+{
+    const int64_t total = off[n];
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int k = owner_of(off, n, idx);
+        const StageDesc& d = sd[k];
+        const int64_t e = idx - off[k];
+        const int64_t nb = (int64_t)d.K * d.nst * (2 * d.WCSP / 32) * 64;
+        if (e < nb) {
+            split_elem(d.b, d.ldb, d.nst, d.WCSP, e, [&](int pc) { return col_scale(d.colmax[pc]); }, d.bop);
+        } else {
+            const int pc = (int)(e - nb);
+            const float sc = col_scale(d.colmax[pc]);
+            d.scale[pc] = sc;
+            d.unscale[pc] = 1.0f / (sc * d.sA[pc / d.WCSP]);
+        }
+    }
+}
+
+// Vertex epilogues of the instances: instance k owns blocks [eoff[k], eoff[k+1]).
+template <int BT, int CT>
+__global__ __launch_bounds__(256) void batch_epilogue_kernel(const EpiArgs* __restrict__ ea,
+                                                             const int64_t* __restrict__ eoff, int n)
+{
+    const int k = owner_of(eoff, n, blockIdx.x);
+    const EpiArgs& a = ea[k];
+    const int lpv = epi_lanes(a.W);
+    const int v = (int)(blockIdx.x - eoff[k]) * (256 / lpv) + threadIdx.x / lpv;
+    if (v >= a.nv) return;
+    epi_vertex<BT, CT>(a, v, threadIdx.x & (lpv - 1), lpv);
+}
+
 // ----------------------------------------------------------------- host
 
 namespace {
@@ -198,6 +310,117 @@ namespace {
 struct Prepared {
     BdrPlan pl;
 };
+
+// Copies blob to the device buffer of slot through its pinned staging buffer
+// (after the previous copy from it has completed); returns the device copy.
+char* upload_blob(BatchWorkspace& bw, int slot, const std::vector<char>& blob, hipStream_t s)
+{
+    const size_t total = blob.size();
+    bw.dev[slot].reserve(total);
+    if (bw.host_bytes[slot] < total) {
+        if (bw.host[slot]) {
+            if (bw.ev[slot]) BAEMM_HIP(hipEventSynchronize(bw.ev[slot]));
+            BAEMM_HIP(hipHostFree(bw.host[slot]));
+            bw.host[slot] = nullptr;
+        }
+        BAEMM_HIP(hipHostMalloc(&bw.host[slot], total, hipHostMallocDefault));
+        bw.host_bytes[slot] = total;
+    }
+    if (!bw.ev[slot]) BAEMM_HIP(hipEventCreateWithFlags(&bw.ev[slot], hipEventDisableTiming));
+    BAEMM_HIP(hipEventSynchronize(bw.ev[slot]));
+    std::memcpy(bw.host[slot], blob.data(), total);
+    BAEMM_HIP(hipMemcpyAsync(bw.dev[slot].ptr, bw.host[slot], total, hipMemcpyHostToDevice, s));
+    BAEMM_HIP(hipEventRecord(bw.ev[slot], s));
+    return bw.dev[slot].as<char>();
+}
+
+template <class T>
+size_t append(std::vector<char>& blob, const T* data, size_t count)
+{
+    const size_t at = (blob.size() + 255) / 256 * 256;
+    blob.resize(at + sizeof(T) * count);
+    std::memcpy(blob.data() + at, data, sizeof(T) * count);
+    return at;
+}
+
+unsigned stride_grid(int64_t total)
+{
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 8192));
+}
+
+// The staging launches the deferred instances recorded (bdr_begin with
+// pl.defer), three launches for all of them on s.
+void stage_deferred(std::vector<Prepared>& prep, const std::vector<char>& own, int Btype, BatchWorkspace& bw,
+                    hipStream_t s)
+{
+    std::vector<StageDesc> sd;
+    std::vector<int64_t> off(1, 0), boff(1, 0), soff(1, 0);
+    for (size_t k = 0; k < prep.size(); ++k) {
+        const StageDesc& d = prep[k].pl.sd;
+        if (own[k] || !d.b) continue;
+        sd.push_back(d);
+        off.push_back(off.back() + (int64_t)d.nt_pad * d.ldb + (d.nrm_out ? d.nt_pad : 0) + (d.split ? d.ldb : 0));
+        boff.push_back(boff.back() + (d.split ? (d.nt + kColmaxRows - 1) / kColmaxRows : 0));
+        soff.push_back(soff.back() + (d.split ? (int64_t)d.K * d.nst * (2 * d.WCSP / 32) * 64 + d.ldb : 0));
+    }
+    if (sd.empty()) return;
+    const int m = (int)sd.size();
+    std::vector<char> blob;
+    const size_t a_sd = append(blob, sd.data(), sd.size());
+    const size_t a_off = append(blob, off.data(), off.size());
+    const size_t a_boff = append(blob, boff.data(), boff.size());
+    const size_t a_soff = append(blob, soff.data(), soff.size());
+    char* dp = upload_blob(bw, 2, blob, s);
+    const StageDesc* dsd = reinterpret_cast<const StageDesc*>(dp + a_sd);
+    const int64_t* doff = reinterpret_cast<const int64_t*>(dp + a_off);
+    if (Btype == kC128)
+        hipLaunchKernelGGL(batch_stage_kernel<kC128>, dim3(stride_grid(off.back())), dim3(256), 0, s, dsd, doff, m);
+    else
+        hipLaunchKernelGGL(batch_stage_kernel<kC64>, dim3(stride_grid(off.back())), dim3(256), 0, s, dsd, doff, m);
+    BAEMM_HIP(hipGetLastError());
+    if (boff.back() > 0) {
+        hipLaunchKernelGGL(batch_colmax_kernel, dim3((unsigned)boff.back()), dim3(256), 0, s, dsd,
+                           reinterpret_cast<const int64_t*>(dp + a_boff), m);
+        hipLaunchKernelGGL(batch_split_kernel, dim3(stride_grid(soff.back())), dim3(256), 0, s, dsd,
+                           reinterpret_cast<const int64_t*>(dp + a_soff), m);
+        BAEMM_HIP(hipGetLastError());
+    }
+}
+
+template <int BT, int CT>
+void launch_batch_epilogue(const EpiArgs* ea, const int64_t* eoff, int m, int64_t blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((batch_epilogue_kernel<BT, CT>), dim3((unsigned)blocks), dim3(256), 0, s, ea, eoff, m);
+}
+
+// The vertex epilogues of the deferred instances, one launch on s.
+void epilogue_deferred(std::vector<Prepared>& prep, const std::vector<char>& own, int Btype, int Ctype,
+                       BatchWorkspace& bw, hipStream_t s)
+{
+    std::vector<EpiArgs> ea;
+    std::vector<int64_t> eoff(1, 0);
+    for (size_t k = 0; k < prep.size(); ++k) {
+        const BdrPlan& pl = prep[k].pl;
+        if (own[k] || (!pl.kernel_on && !pl.mass_on)) continue;   // the reference leaves C_out untouched
+        ea.push_back(pl.ea);
+        const int vpb = 256 / epi_lanes(pl.ea.W);
+        eoff.push_back(eoff.back() + (pl.ea.nv + vpb - 1) / vpb);
+    }
+    if (ea.empty()) return;
+    const int m = (int)ea.size();
+    std::vector<char> blob;
+    const size_t a_ea = append(blob, ea.data(), ea.size());
+    const size_t a_off = append(blob, eoff.data(), eoff.size());
+    char* dp = upload_blob(bw, 3, blob, s);
+    const EpiArgs* dea = reinterpret_cast<const EpiArgs*>(dp + a_ea);
+    const int64_t* doff = reinterpret_cast<const int64_t*>(dp + a_off);
+    const int64_t blocks = eoff.back();
+    if (Btype == kC128 && Ctype == kC128) launch_batch_epilogue<kC128, kC128>(dea, doff, m, blocks, s);
+    else if (Btype == kC128 && Ctype == kC64) launch_batch_epilogue<kC128, kC64>(dea, doff, m, blocks, s);
+    else if (Btype == kC64 && Ctype == kC128) launch_batch_epilogue<kC64, kC128>(dea, doff, m, blocks, s);
+    else launch_batch_epilogue<kC64, kC64>(dea, doff, m, blocks, s);
+    BAEMM_HIP(hipGetLastError());
+}
 
 int persistent_grid(const void* fn, int block, int n_items, int grid_mult)
 {
@@ -226,37 +449,44 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
         if (es[k].h->precision == BAEMM_PRECISION_FP32_MFMA)
             throw ArgError{"batched apply: instances must use BAEMM_PRECISION_FP32 or BAEMM_PRECISION_MIXED"};
     std::vector<Prepared> prep(n);
-    // Per-instance preparation and epilogues are short launches; they run on
-    // each handle's own stream so that those of different instances overlap.
-    // Events: [0] fork, [1] after the persistent launches, then per instance
-    // [2 + k] prepared and [2 + n + k] finished.
-    while ((int)bw.sync.size() < 2 + 2 * n) {
+    // Events: [0] fork to the streams of the instances that run their own
+    // launches, [1] end of the work on s, [2 + k] instance k: done on its
+    // stream (own launch), or its stream's earlier work (staged on s).
+    while ((int)bw.sync.size() < 2 + n) {
         hipEvent_t e = nullptr;
         BAEMM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         bw.sync.push_back(e);
     }
     BAEMM_HIP(hipEventRecord(bw.sync[0], s));
     // An instance whose own pair launch fills the device at least
-    // tune.own_rounds times runs that launch (the single-instance kernels, on
-    // its stream, beside the others); the rest share the persistent launch,
-    // whose work list they fill.  The choice depends on the instance and the
-    // tuning alone, so its result is the same in any mix.
+    // tune.own_rounds times runs the single-instance path on its stream,
+    // beside the others; the rest are staged together on s and share the
+    // persistent launch, whose work list they fill.  The choice depends on the
+    // instance and the tuning alone, so its result is the same in any mix.
     std::vector<char> own(n, 0);
     for (int k = 0; k < n; ++k) {
-        hipStream_t sk = es[k].h->stream;
-        BAEMM_HIP(hipStreamWaitEvent(sk, bw.sync[0], 0));
-        p0_invalidate(*es[k].h);   // the batched launch leaves no P0 buffers
+        Handle& h = *es[k].h;
+        p0_invalidate(h);   // the batched apply leaves no P0 buffers
         BdrPlan& pl = prep[k].pl;
-        bdr_setup(*es[k].h, pl);
-        bdr_begin(*es[k].h, es[k].a, pl, sk);
+        bdr_setup(h, pl);
         if (pl.kernel_on && pl.rounds >= tune.own_rounds) {
             own[k] = 1;
-            bdr_pairs(*es[k].h, pl, sk);
+            hipStream_t sk = h.stream;
+            BAEMM_HIP(hipStreamWaitEvent(sk, bw.sync[0], 0));
+            bdr_begin(h, es[k].a, pl, sk);
+            bdr_pairs(h, pl, sk);
+            bdr_end(h, es[k].a, pl, sk);
+            BAEMM_HIP(hipEventRecord(bw.sync[2 + k], sk));
+        } else {
+            // staged on s: s waits for the instance stream's earlier work (its
+            // inputs may come from there, and its workspaces are reused)
+            BAEMM_HIP(hipEventRecord(bw.sync[2 + k], h.stream));
+            BAEMM_HIP(hipStreamWaitEvent(s, bw.sync[2 + k], 0));
+            pl.defer = true;
+            bdr_begin(h, es[k].a, pl, s);   // records the staging launches in pl.sd
         }
-        BAEMM_HIP(hipEventRecord(bw.sync[2 + k], sk));
     }
-    for (int k = 0; k < n; ++k)
-        if (!own[k]) BAEMM_HIP(hipStreamWaitEvent(s, bw.sync[2 + k], 0));
+    stage_deferred(prep, own, es[0].a.Btype, bw, s);
 
     for (int mode = 0; mode < 2; ++mode) {     // 0: MFMA instances, 1: FP32 instances
         const bool mixed = (mode == 0);
@@ -387,19 +617,19 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
             stats->instances[mode] = (int64_t)insts.size();
         }
     }
+    epilogue_deferred(prep, own, es[0].a.Btype, es[0].a.Ctype, bw, s);
+    // results and workspaces of the instances staged on s are done there;
+    // their streams wait for that before any later work on the handles
     BAEMM_HIP(hipEventRecord(bw.sync[1], s));
     for (int k = 0; k < n; ++k) {
-        hipStream_t sk = es[k].h->stream;
-        if (!own[k]) BAEMM_HIP(hipStreamWaitEvent(sk, bw.sync[1], 0));
-        bdr_end(*es[k].h, es[k].a, prep[k].pl, sk);
-        BAEMM_HIP(hipEventRecord(bw.sync[2 + n + k], sk));
+        if (own[k]) BAEMM_HIP(hipStreamWaitEvent(s, bw.sync[2 + k], 0));
+        else BAEMM_HIP(hipStreamWaitEvent(es[k].h->stream, bw.sync[1], 0));
     }
-    for (int k = 0; k < n; ++k) BAEMM_HIP(hipStreamWaitEvent(s, bw.sync[2 + n + k], 0));
 }
 
 void BatchWorkspace::release()
 {
-    for (int m = 0; m < 2; ++m) {
+    for (int m = 0; m < kSlots; ++m) {
         if (ev[m]) {
             (void)hipEventSynchronize(ev[m]);
             (void)hipEventDestroy(ev[m]);

@@ -26,12 +26,14 @@ struct BatchTuning {
 };
 BatchTuning& batch_tuning();
 
-// Device and pinned staging of the work lists (index 0: MFMA, 1: FP32).
+// Device and pinned staging of the work lists (index 0: MFMA, 1: FP32) and
+// of the batched staging and epilogue descriptors (2, 3).
 struct BatchWorkspace {
-    DevBuf dev[2];
-    void* host[2] = {nullptr, nullptr};
-    size_t host_bytes[2] = {0, 0};
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    static constexpr int kSlots = 4;
+    DevBuf dev[kSlots];
+    void* host[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+    size_t host_bytes[kSlots] = {0, 0, 0, 0};
+    hipEvent_t ev[kSlots] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> sync;   // fork/join events between the call's stream and the handles' streams
     void release();
 };

@@ -8,26 +8,11 @@
 #include <utility>
 
 #include "ops.hpp"
+#include "stage.hpp"
 
 namespace baemm {
 
 // --------------------------------------------------------------- helpers
-
-template <int T>
-struct Elem;
-template <>
-struct Elem<kC128> {
-    using type = double2;
-};
-template <>
-struct Elem<kC64> {
-    using type = float2;
-};
-
-__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
-__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.y); }
-__device__ __forceinline__ void from_d2(double2& dst, double2 v) { dst = v; }
-__device__ __forceinline__ void from_d2(float2& dst, double2 v) { dst = make_float2((float)v.x, (float)v.y); }
 
 static inline int next_pow2(int x)
 {
@@ -44,33 +29,15 @@ int wcsp_mixed(int wcs) { return std::max(16, next_pow2(wcs)); }
 
 // --------------------------------------------------------------- staging
 
-// b[t][q*WCSP+kk] = (a_t/3) (B[i0][k] + B[i1][k] + B[i2][k]),  k = q*wcs + kk
-// (AvOp.Dot, ApplyBoundaryOperators.hpp:120-124; Av values Initialize.hpp:133-143).
-// Rows t >= nt and columns kk >= wcs are zero (the reference leaves padded rows
-// of its host buffer unwritten, Q3; here they are defined).
+// b = Av B row by row (av_elem).
 template <int BT>
 __global__ void av_gather_kernel(const int4* __restrict__ tris, const float* __restrict__ area, int nt,
                                  int64_t total, const void* __restrict__ Bv, int64_t ldB, int wcs, int WCSP,
                                  int ldb, float2* __restrict__ b)
 {
-    using T = typename Elem<BT>::type;
-    const T* B = static_cast<const T*>(Bv);
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    const int t = (int)(idx / ldb);
-    const int pc = (int)(idx % ldb);
-    const int q = pc / WCSP, kk = pc % WCSP;
-    float2 out = make_float2(0.f, 0.f);
-    if (t < nt && kk < wcs) {
-        const int k = q * wcs + kk;
-        const int4 tv = tris[t];
-        const double2 v0 = to_d2(B[(int64_t)tv.x * ldB + k]);
-        const double2 v1 = to_d2(B[(int64_t)tv.y * ldB + k]);
-        const double2 v2 = to_d2(B[(int64_t)tv.z * ldB + k]);
-        const double w = (double)area[t] / 3.0;
-        out = make_float2((float)(w * (v0.x + v1.x + v2.x)), (float)(w * (v0.y + v1.y + v2.y)));
-    }
-    b[idx] = out;
+    b[idx] = av_elem<BT>(tris, area, nt, Bv, ldB, wcs, WCSP, ldb, idx);
 }
 
 // b[r][q*WCSP+kk] = (conj?) G[r][q*wcs+kk] for r < nrows, zero padding.
@@ -141,19 +108,12 @@ __global__ void colscale_kernel(const unsigned* __restrict__ colmax, int ncols, 
 {
     const int pc = blockIdx.x * blockDim.x + threadIdx.x;
     if (pc >= ncols) return;
-    const float m = __uint_as_float(colmax[pc]);
-    float s = 1.0f;
-    if (m > 0.0f) {
-        int e;
-        frexpf(m, &e);          // m in [2^(e-1), 2^e)
-        s = ldexpf(1.0f, 14 - e);
-    }
+    const float s = col_scale(colmax[pc]);
     scale[pc] = s;
     unscale[pc] = 1.0f / (s * cs.sA[pc / WCSP]);
 }
 
-// B-operand of the MFMA kernel, 16 bytes per (chunk, step, col-block, part, lane);
-// see bdr_mixed_kernel.  Values are scaled per column, then split hi/lo (RN).
+// B-operand of the MFMA kernel (split_elem), with the column scales of colscale_kernel.
 __global__ void split_kernel(const float2* __restrict__ b, int ldb, int nst, int WCSP, int K,
                              const float* __restrict__ scale, uint4* __restrict__ bop)
 {
@@ -161,36 +121,7 @@ __global__ void split_kernel(const float2* __restrict__ b, int ldb, int nst, int
     const int64_t total = (int64_t)K * nst * NCB * 64;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    const int lane = (int)(idx % 64);
-    int64_t rest = idx / 64;
-    const int cb = (int)(rest % NCB);
-    rest /= NCB;
-    const int st = (int)(rest % nst);
-    const int q = (int)(rest / nst);
-    const int g = lane >> 5;
-    const int c = cb * 32 + (lane & 31);
-    const bool imcol = c >= WCSP;
-    const int kk = imcol ? c - WCSP : c;
-    const int pc = q * WCSP + kk;
-    const float s = scale[pc];
-    _Float16 hi[8], lo[8];
-#pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-        const int j = 8 * st + 4 * g + jj;
-        const float2 v = b[(int64_t)j * ldb + pc];
-        const float br = v.x * s, bi = v.y * s;
-        // rows (j, re) and (j, im) of [[br, bi], [-bi, br]]
-        const float e0 = imcol ? bi : br;
-        const float e1 = imcol ? br : -bi;
-        const _Float16 h0 = (_Float16)e0, h1 = (_Float16)e1;
-        hi[2 * jj] = h0;
-        hi[2 * jj + 1] = h1;
-        lo[2 * jj] = (_Float16)(e0 - (float)h0);
-        lo[2 * jj + 1] = (_Float16)(e1 - (float)h1);
-    }
-    const int64_t base = ((((int64_t)q * nst + st) * NCB + cb) * 2) * 64 + lane;
-    bop[base] = __builtin_bit_cast(uint4, hi);
-    bop[base + 64] = __builtin_bit_cast(uint4, lo);
+    split_elem(b, ldb, nst, WCSP, idx, [&](int pc) { return scale[pc]; }, bop);
 }
 
 // out[j] = s * nrm[j]: source normals carrying the common scale of the
@@ -268,122 +199,14 @@ __global__ void rmin_kernel(const float4* __restrict__ tgt, int ntgt, const floa
 
 // --------------------------------------------------------------- epilogues
 
-// Batch sizes of the epilogue's loads: incident triangles, and segment partials
-// (also mass-matrix entries per row).
-constexpr int EPI_TRI = 2;
-constexpr int EPI_SEG = 8;
-
-// C[v][k] = alpha * ( sum_{t ni v} a_t/3 (sum_s part_s[t][k] * unscale + D_t b[t][k])
-//                     + c0_q (M B)[v][k] ) + beta * C[v][k]
-// (AvOpTransp.Dot ApplyBoundaryOperators.hpp:136-140, ApplySingleLayerDiagonal.hpp:36-50,
-//  MassOp.Dot :150-175).  lpv lanes (16, 32 or 64) per vertex, lanes over
-// columns, so narrow right-hand sides leave no lanes idle.
+// The boundary epilogue, one vertex per lpv lanes (epi_vertex).
 template <int BT, int CT>
 __global__ __launch_bounds__(256) void bdr_epilogue_kernel(EpiArgs a, int lpv)
 {
-    using TB = typename Elem<BT>::type;
-    using TC = typename Elem<CT>::type;
     const int lane = threadIdx.x & (lpv - 1);
     const int v = blockIdx.x * (256 / lpv) + threadIdx.x / lpv;
     if (v >= a.nv) return;
-    for (int k = lane; k < a.W; k += lpv) {
-        const int q = k / a.wcs, kk = k % a.wcs;
-        const int pc = q * a.WCSP + kk;
-        double sr = 0.0, si = 0.0;
-        if (a.kernel_on) {
-            const double us = a.unscale ? (double)a.unscale[pc] : 1.0;
-            const int e0 = a.vt_ptr[v], e1 = a.vt_ptr[v + 1];
-            // EPI_TRI triangles and EPI_SEG segments per batch: every load of a
-            // batch is issued before the first is added (indices clamped to the
-            // valid range, so there is no branch between them), and the sums run
-            // in the order of the plain loops (s ascending per triangle, then the
-            // triangles in list order)
-            for (int e = e0; e < e1; e += EPI_TRI) {
-                int t[EPI_TRI];
-                double pr[EPI_TRI], pi[EPI_TRI];
-#pragma unroll
-                for (int u = 0; u < EPI_TRI; ++u) {
-                    t[u] = a.vt_idx[min(e + u, e1 - 1)];
-                    pr[u] = pi[u] = 0.0;
-                }
-                for (int s0 = 0; s0 < a.S; s0 += EPI_SEG) {
-                    float xr[EPI_TRI][EPI_SEG], xi[EPI_TRI][EPI_SEG];
-#pragma unroll
-                    for (int u = 0; u < EPI_TRI; ++u) {
-                        const float* pp = a.part + (int64_t)t[u] * a.ldp + q * 2 * a.WCSP + kk;
-#pragma unroll
-                        for (int j = 0; j < EPI_SEG; ++j) {
-                            const int64_t o = (int64_t)min(s0 + j, a.S - 1) * a.seg_stride;
-                            xr[u][j] = pp[o];
-                            xi[u][j] = pp[o + a.WCSP];
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < EPI_TRI; ++u)
-#pragma unroll
-                        for (int j = 0; j < EPI_SEG; ++j)
-                            if (s0 + j < a.S) {
-                                pr[u] += (double)xr[u][j];
-                                pi[u] += (double)xi[u][j];
-                            }
-                }
-#pragma unroll
-                for (int u = 0; u < EPI_TRI; ++u) {
-                    if (e + u >= e1) break;
-                    double qr = pr[u] * us, qi = pi[u] * us;
-                    {
-                        const double rr = a.rotr[q], ri = a.roti[q];
-                        const double tr = qr * rr - qi * ri;
-                        qi = qr * ri + qi * rr;
-                        qr = tr;
-                    }
-                    if (a.diag_on) {
-                        const float2 bv = a.b[(int64_t)t[u] * a.ldb + pc];
-                        const double fr = (double)a.c1r[q] * a.sdiag[t[u]] - (double)a.c1i[q] * a.kap[q];
-                        const double fi = (double)a.c1i[q] * a.sdiag[t[u]] + (double)a.c1r[q] * a.kap[q];
-                        qr += fr * bv.x - fi * bv.y;
-                        qi += fr * bv.y + fi * bv.x;
-                    }
-                    const double w = (double)a.area[t[u]] / 3.0;
-                    sr += w * qr;
-                    si += w * qi;
-                }
-            }
-        }
-        if (a.mass_on) {
-            const TB* B = static_cast<const TB*>(a.B);
-            double mr = 0.0, mi = 0.0;
-            const int e0 = a.M_ptr[v], e1 = a.M_ptr[v + 1];
-            for (int e = e0; e < e1; e += EPI_SEG) {
-                double mv[EPI_SEG];
-                double2 bv[EPI_SEG];
-#pragma unroll
-                for (int j = 0; j < EPI_SEG; ++j) {
-                    const int ej = min(e + j, e1 - 1);
-                    mv[j] = a.M_val[ej];
-                    bv[j] = to_d2(B[(int64_t)a.M_idx[ej] * a.ldB + k]);
-                }
-#pragma unroll
-                for (int j = 0; j < EPI_SEG; ++j)
-                    if (e + j < e1) {
-                        mr += mv[j] * bv[j].x;
-                        mi += mv[j] * bv[j].y;
-                    }
-            }
-            sr += a.c0r[q] * mr - a.c0i[q] * mi;
-            si += a.c0r[q] * mi + a.c0i[q] * mr;
-        }
-        TC* C = static_cast<TC*>(a.C);
-        TC& dst = C[(int64_t)v * a.ldC + k];
-        double outr = a.ar * sr - a.ai * si;
-        double outi = a.ar * si + a.ai * sr;
-        if (a.br != 0.0 || a.bi != 0.0) {
-            const double2 c = to_d2(dst);
-            outr += a.br * c.x - a.bi * c.y;
-            outi += a.br * c.y + a.bi * c.x;
-        }
-        from_d2(dst, make_double2(outr, outi));
-    }
+    epi_vertex<BT, CT>(a, v, lane, lpv);
 }
 
 // Factors per wave-number chunk the MFMA near-field partials are multiplied by.
@@ -678,7 +501,7 @@ float compute_rmin(Handle& h, hipStream_t s)
 template <int BT, int CT>
 static void launch_bdr_epilogue(EpiArgs& ea, hipStream_t s)
 {
-    const int lpv = ea.W <= 16 ? 16 : (ea.W <= 32 ? 32 : 64);
+    const int lpv = epi_lanes(ea.W);
     const int vpb = 256 / lpv;
     hipLaunchKernelGGL((bdr_epilogue_kernel<BT, CT>), dim3((ea.nv + vpb - 1) / vpb), dim3(256), 0, s, ea, lpv);
 }
@@ -752,6 +575,7 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
 {
     const Params& p = h.prm;
     if (pl.f32) {
+        if (pl.defer) throw ArgError{"internal: deferred staging of the FP32 MFMA operand"};
         // FP32 MFMA: no FP16 range to fit, no scales; the non-adjoint form still
         // divides each chunk by c2 (the epilogue multiplies it back)
         pl.sA.assign(p.K, 1.0f);
@@ -791,8 +615,14 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
         const float sig = (float)std::ldexp(1.0, 14 - e);
         for (int q = 0; q < p.K; ++q) pl.sA[q] = csc.sA[q] = sig;
         float4* nrm = static_cast<float4*>(pl.ws.take(h.ws_nrm, sizeof(float4) * h.nt_pad));
-        hipLaunchKernelGGL(scale_nrm_kernel, dim3((unsigned)((h.nt_pad + 255) / 256)), dim3(256), 0, s, h.d_nrm,
-                           (int)h.nt_pad, sig, nrm);
+        if (pl.defer) {
+            pl.sd.nrm_in = h.d_nrm;
+            pl.sd.nrm_out = nrm;
+            pl.sd.nrm_s = sig;
+        } else {
+            hipLaunchKernelGGL(scale_nrm_kernel, dim3((unsigned)((h.nt_pad + 255) / 256)), dim3(256), 0, s, h.d_nrm,
+                               (int)h.nt_pad, sig, nrm);
+        }
         pl.nrm = nrm;
     } else {
         for (int q = 0; q < p.K; ++q) {
@@ -811,18 +641,30 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
     unsigned* colmax = static_cast<unsigned*>(pl.ws.take(h.ws_scale, sizeof(float) * 3 * ncols));
     float* scale = reinterpret_cast<float*>(colmax + ncols);
     float* unscale = scale + ncols;
+    const int NCB = 2 * pl.WCSP / 32;
+    const int64_t nbop = (int64_t)p.K * pl.nst * NCB * 64;
+    uint4* bop = static_cast<uint4*>(pl.ws.take(h.ws_bop, sizeof(uint4) * nbop * 2));
+    pl.bop = bop;
+    pl.unscale = unscale;
+    if (pl.defer) {
+        StageDesc& d = pl.sd;
+        d.split = 1;
+        d.K = (int)p.K;
+        d.nst = pl.nst;
+        d.colmax = colmax;
+        d.scale = scale;
+        d.unscale = unscale;
+        d.bop = bop;
+        for (int q = 0; q < p.K; ++q) d.sA[q] = csc.sA[q];
+        return;
+    }
     hipLaunchKernelGGL(zero_u32_kernel, dim3((ncols + 255) / 256), dim3(256), 0, s, colmax, ncols);
     hipLaunchKernelGGL(colmax_kernel, dim3(256), dim3(256), 0, s, pl.b, (int)h.nt, pl.ldb, colmax);
     hipLaunchKernelGGL(colscale_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, colmax, ncols, pl.WCSP, csc, scale,
                        unscale);
-    const int NCB = 2 * pl.WCSP / 32;
-    const int64_t nbop = (int64_t)p.K * pl.nst * NCB * 64;
-    uint4* bop = static_cast<uint4*>(pl.ws.take(h.ws_bop, sizeof(uint4) * nbop * 2));
     hipLaunchKernelGGL(split_kernel, dim3((unsigned)((nbop + 255) / 256)), dim3(256), 0, s, pl.b, pl.ldb, pl.nst,
                        pl.WCSP, (int)p.K, scale, bop);
     BAEMM_HIP(hipGetLastError());
-    pl.bop = bop;
-    pl.unscale = unscale;
 }
 
 void bdr_setup(Handle& h, BdrPlan& pl)
@@ -901,7 +743,21 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     ea.diag_on = pl.kernel_on && h.use_diagonal && p.any(1);
     if (!pl.kernel_on) return;
     pl.b = static_cast<float2*>(pl.ws.take(h.ws_b32, sizeof(float2) * h.nt_pad * pl.ldb));
-    launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
+    if (pl.defer) {
+        StageDesc& d = pl.sd;
+        d.tris = h.d_tris;
+        d.area = h.d_area;
+        d.nt = (int)h.nt;
+        d.nt_pad = (int)h.nt_pad;
+        d.B = a.B;
+        d.ldB = a.ldB;
+        d.wcs = (int)p.wcs;
+        d.WCSP = pl.WCSP;
+        d.ldb = pl.ldb;
+        d.b = pl.b;
+    } else {
+        launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
+    }
     ea.b = pl.b;
     ea.ldp = pl.ldp;
     ea.ldb = pl.ldb;

@@ -77,6 +77,32 @@ struct WsSource {
 };
 
 // State of one apply between its phases.
+// Staging launches of one boundary apply, recorded instead of launched when
+// its plan is deferred (BdrPlan::defer): the batched apply runs those of all
+// deferred instances of a call together (batch.hip).  b null: nothing staged.
+struct StageDesc {
+    // b = Av B (av_elem)
+    const int4* tris = nullptr;
+    const float* area = nullptr;
+    int nt = 0, nt_pad = 0;
+    const void* B = nullptr;
+    int64_t ldB = 0;
+    int wcs = 0, WCSP = 0, ldb = 0;
+    float2* b = nullptr;
+    // mixed precision: column maxima, scales and the hi/lo B operand
+    int split = 0;
+    int K = 0, nst = 0;
+    unsigned* colmax = nullptr;
+    float* scale = nullptr;
+    float* unscale = nullptr;
+    uint4* bop = nullptr;
+    float sA[64] = {};
+    // source normals scaled for the non-adjoint MFMA form (nrm_out null: none)
+    const float4* nrm_in = nullptr;
+    float4* nrm_out = nullptr;
+    float nrm_s = 1.0f;
+};
+
 struct BdrPlan {
     bool kernel_on = false, mass_on = false, mixed = false, adj = false;
     bool f32 = false;            // MFMA kernels in FP32 (BAEMM_PRECISION_FP32_MFMA); mixed is set too
@@ -96,6 +122,10 @@ struct BdrPlan {
     double rounds = 0.0;         // work-groups of one pair launch / resident work-groups of the device
     WsSource ws;
     EpiArgs ea{};
+    // bdr_begin records its staging launches in sd instead of launching them
+    // (mixed precision with the FP16 split, or FP32)
+    bool defer = false;
+    StageDesc sd{};
 };
 
 // Phase 0 (no launches): arithmetic, operand widths and the launch shape of

@@ -303,24 +303,26 @@ def run_potential_leg(repeats=3):
     return out
 
 
-def run_batch_leg(repeats=5, seed=3):
-    """Batched uneven instances: 24 perturbed icospheres (radius 1 + 0.1 sum of
-    real spherical harmonics of degree <= 4, seeded N(0,1)/(l+1)^2 weights) of
-    1,280, 5,120 or 20,480 triangles, kappa uniform in [1, 16], W = 2^p RHS with
-    p uniform in 0..6 (one wave-number chunk each), SL+DL with complex
+def run_batch_leg(repeats=5, seed=3, n_inst=24, levels=(3, 6), gmres=True):
+    """Batched uneven instances: n_inst perturbed icospheres (radius 1 + 0.1 sum
+    of real spherical harmonics of degree <= 4, seeded N(0,1)/(l+1)^2 weights)
+    of icosphere levels uniform in [levels[0], levels[1]) (the default: 1,280,
+    5,120 or 20,480 triangles), kappa uniform in [1, 16], W = 2^p RHS with p
+    uniform in 0..6 (one wave-number chunk each), SL+DL with complex
     coefficients, mixed precision.  Times one persistent launch per apply
     (autotuned tile shapes, or every instance on its own launch where that is
-    faster) against the 24 single-instance applies."""
+    faster) against the single-instance applies."""
     import ctypes
 
-    from baemm_amd import ApplyBoundaryOperatorsBatch, BatchAutotune, BatchStats, Helmholtz_HIP, _lib
+    from baemm_amd import ApplyBoundaryOperatorsBatch, BatchAutotune, BatchStats, BatchTuning, Helmholtz_HIP, _lib
     from baemm_amd.meshes import perturbed_sphere
 
     rng = np.random.default_rng(seed)
     inst, Bs, kappas = [], [], []
     pairs_rhs = 0.0
-    for k in range(24):
-        level = int(rng.integers(3, 6))
+    saved_tuning = BatchTuning()
+    for k in range(n_inst):
+        level = int(rng.integers(*levels))
         V, T = perturbed_sphere(level, seed=1000 + k)
         W = 2 ** int(rng.integers(0, 7))
         kappa = [float(rng.uniform(1.0, 16.0))]
@@ -380,6 +382,26 @@ def run_batch_leg(repeats=5, seed=3):
     batched()
     _lib.check(L.baemm_dev_synchronize())
     stats = BatchStats()
+    res = {
+        "pair_rhs_per_apply": pairs_rhs,
+        "batched_ms": times["batched"] * 1e3,
+        "single_ms": times["single"] * 1e3,
+        "single_pipelined_ms": times["single_pipelined"] * 1e3,
+        "batched_pair_rhs_per_s": pairs_rhs / times["batched"],
+        "single_pair_rhs_per_s": pairs_rhs / times["single"],
+        "speedup": times["single"] / times["batched"],
+        "speedup_vs_pipelined": times["single_pipelined"] / times["batched"],
+        "tuning": tuning,
+        "work_items": stats["items_mixed"],
+        "persistent_grid": stats["grid_mixed"],
+    }
+    if not gmres:
+        for d in dB + dC:
+            L.baemm_dev_free(d)
+        for H in inst:
+            H.close()
+        BatchTuning(**saved_tuning)
+        return res
     # GMRES on the same instances: each one's indirect potential operator
     # (1/2, -i kappa, 1, 0) solved to 1e-6, all together with batched Arnoldi
     # steps against one after the other
@@ -409,20 +431,11 @@ def run_batch_leg(repeats=5, seed=3):
         L.baemm_dev_free(d)
     for H in inst:
         H.close()
+    BatchTuning(**saved_tuning)
     return {
         "workload": "24 perturbed icospheres of 1280/5120/20480 triangles, kappa in [1,16], W = 2^p (p in 0..6), "
                     "SL+DL, mixed precision",
-        "pair_rhs_per_apply": pairs_rhs,
-        "batched_ms": times["batched"] * 1e3,
-        "single_ms": times["single"] * 1e3,
-        "single_pipelined_ms": times["single_pipelined"] * 1e3,
-        "batched_pair_rhs_per_s": pairs_rhs / times["batched"],
-        "single_pair_rhs_per_s": pairs_rhs / times["single"],
-        "speedup": times["single"] / times["batched"],
-        "speedup_vs_pipelined": times["single_pipelined"] / times["batched"],
-        "tuning": tuning,
-        "work_items": stats["items_mixed"],
-        "persistent_grid": stats["grid_mixed"],
+        **res,
         "gmres": {
             "operator": "(1/2, -i kappa, 1, 0) per instance, GMRES(30) to 1e-6, PCG mass-inverse preconditioner",
             "batched_s": gm_times["batched"],
@@ -582,6 +595,10 @@ def main():
     }
     if not args.no_batch:
         line["batch"] = run_batch_leg()
+        log("batch leg: many small instances")
+        small = run_batch_leg(seed=5, n_inst=96, levels=(2, 4), gmres=False)
+        line["batch"]["small_instances"] = {
+            "workload": "96 perturbed icospheres of 320 or 1280 triangles, otherwise as above", **small}
     if not args.no_potential:
         line["potential"] = run_potential_leg()
     if not args.no_solve:
