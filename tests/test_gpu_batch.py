@@ -114,6 +114,46 @@ def test_batch_rejects_fp32_mfma_instances(batch):
         H.SetPrecision(batch[0]["prec"])
 
 
+def test_batch_mass_only_and_empty_operators():
+    """Instances whose operator has no kernel part (coefficients (c0, 0, 0, 0):
+    the mass term only, no staging) or nothing at all (all zero: C is left as
+    it was, as the reference leaves C_out untouched) in one batch with ordinary
+    FP32 and mixed instances; each result equals its single-instance apply."""
+    rng = np.random.default_rng(31)
+    specs = [("mixed", [0.5, 1.0 - 0.5j, 1.0, 0.0]), ("fp32", [0.0, 1.0, 0.0, 0.0]),
+             ("mixed", [0.75 - 0.25j, 0.0, 0.0, 0.0]), ("fp32", [0.0, 0.0, 0.0, 0.0]),
+             ("mixed", [0.0, 0.0, 0.0, 0.0])]
+    Hs, Bs, Cs, singles = [], [], [], []
+    try:
+        for k, (prec, coeff) in enumerate(specs):
+            V, T = perturbed_sphere(2 + k % 2, seed=300 + k)
+            W = 4
+            H = Helmholtz_HIP(V, T)
+            H.SetPrecision(prec)
+            H.LoadBoundaryOperators_PL([3.0], np.array(coeff, np.complex128), W, W)
+            B = rng.standard_normal((V.shape[0], W)) + 1j * rng.standard_normal((V.shape[0], W))
+            C0 = np.full_like(B, 1.0 + 2.0j)
+            single = C0.copy()
+            H.ApplyBoundaryOperators_PL(1.0, B, 0.0, single)
+            Hs.append(H)
+            Bs.append(B)
+            Cs.append(C0)
+            singles.append(single)
+        # beta = 0 given, so that C_list is uploaded (the empty operators must leave it)
+        out = ApplyBoundaryOperatorsBatch(Hs, Bs, C_list=[c.copy() for c in Cs], beta=[0.0] * len(Hs))
+        for k, (C, single) in enumerate(zip(out, singles)):
+            print(k, specs[k], "batch vs single", relerr(C, single))
+            if not any(specs[k][1]):
+                assert np.array_equal(C, Cs[k]), k          # untouched
+            elif not any(specs[k][1][1:]):
+                assert relerr(C, single) < 1e-14, k         # mass term: same FP64 epilogue
+            else:
+                assert relerr(C, single) < 2e-6, k
+    finally:
+        for H in Hs:
+            H.close()
+
+
 def _c4_mix(seed=3):
     """The batched-instances workload: 24 perturbed icospheres (radius
     1 + 0.1 sum of real spherical harmonics of degree <= 4 with seeded
