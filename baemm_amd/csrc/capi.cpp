@@ -226,7 +226,7 @@ int baemm_set_segments(baemm_handle h, int segments)
 int baemm_set_mixed_kernel(baemm_handle h, int variant)
 {
     return guard(h, [&](Handle& hh) {
-        if (variant < 0 || variant > 1) throw ArgError{"mixed kernel variant must be 0 or 1"};
+        if (variant < 0 || variant > 2) throw ArgError{"mixed kernel variant must be 0, 1 or 2"};
         hh.mixed_kernel = variant;
         return BAEMM_OK;
     });

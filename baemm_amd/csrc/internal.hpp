@@ -172,7 +172,7 @@ struct Handle {
     int block_size = 64;
     int wave_chunk_size = 16;
     int precision = BAEMM_PRECISION_FP32;
-    int mixed_kernel = 1;                // 0: single-wave MFMA kernel, 1: wave-specialised (default)
+    int mixed_kernel = 2;                // 0: single-wave MFMA kernel, 1: wave-specialised, 2: automatic
     int segments = 0;                    // 0 = automatic split of the source range
     int gmres_max_iter = 30;
     int gmres_max_restarts = 3;

@@ -399,10 +399,10 @@ static int pick_kg(int WCSP, int remaining, bool mixed)
 // and at most kMaxSegSources sources per segment, which bounds the length of
 // each FP32 accumulation (the segment partials are summed in FP64).
 constexpr int kMaxSegSources = 32768;
-static int choose_segments(const Handle& h, int tiles, int max_seg, int64_t nsrc)
+static int choose_segments(const Handle& h, int tiles, int max_seg, int64_t nsrc, int target = 2048)
 {
     const int min_seg = (int)((nsrc + kMaxSegSources - 1) / kMaxSegSources);
-    const int want = h.segments > 0 ? h.segments : (2048 + tiles - 1) / tiles;
+    const int want = h.segments > 0 ? h.segments : (target + tiles - 1) / tiles;
     return std::max(1, std::min(std::max(want, min_seg), max_seg));
 }
 
@@ -447,6 +447,15 @@ static int balance_segments(int tiles, int nst, int S0, int max_seg, int residen
         }
     }
     return best;
+}
+
+// Which MFMA kernel a mixed plan runs: the wave-specialised one for the FP32
+// contraction, when selected, and (automatic selection, the default) for
+// operators with the adjoint double layer, whose single-wave kernel is built
+// for 3 waves per SIMD; else the single-wave kernel.
+static bool mixed_uses_ws(const Handle& h, const BdrPlan& pl)
+{
+    return pl.f32 || h.mixed_kernel == 1 || (h.mixed_kernel == 2 && pl.adj);
 }
 
 // Runs a plain pair kernel over all chunk groups: part has ntgt_pad rows.
@@ -701,11 +710,19 @@ void bdr_setup(Handle& h, BdrPlan& pl)
     pl.nst = (int)(h.nt_pad / 8);
     const int tiles = (int)(h.nt_pad / kMixTile);
     int S0 = choose_segments(h, tiles, pl.nst / 8, h.nt);
-    const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
-    const int res = resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512);
-    // balanced for the wave-specialised kernel (the default); the single-wave
-    // kernel takes the same segments, so the two stay bit-identical
-    if (h.segments == 0) S0 = balance_segments(tiles, pl.nst, S0, pl.nst / 8, res);
+    int res;
+    if (mixed_uses_ws(h, pl)) {
+        const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
+        res = resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512);
+    } else {
+        res = resident_blocks((const void*)mixed_fn(pl.adj, pl.WCSP, pick_kg(pl.WCSP, (int)p.K, true)), 256);
+        // The single-wave kernel runs 2-3 % faster with about four times the
+        // work-groups (finer last rounds): up to 16 segments where the rule
+        // above gives fewer, unless the plan must keep its partials small.
+        if (h.segments == 0 && !pl.lean && S0 < 16)
+            S0 = std::max(S0, std::min({16, (8192 + tiles - 1) / tiles, pl.nst / 8}));
+    }
+    if (h.segments == 0 && !pl.lean) S0 = balance_segments(tiles, pl.nst, S0, pl.nst / 8, res);
     pl.steps = (pl.nst + S0 - 1) / S0;
     pl.S = (pl.nst + pl.steps - 1) / pl.steps;
     pl.rounds = (double)tiles * pl.S / res;
@@ -794,7 +811,7 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
     const int nst = pl.nst;
     const int tiles = (int)(h.nt_pad / kMixTile);
     // the FP32 MFMA contraction exists in the wave-specialised kernel only
-    const bool ws = h.mixed_kernel == 1 || pl.f32;
+    const bool ws = mixed_uses_ws(h, pl);
     Timed tm(h, s);
     for (int q0 = 0; q0 < p.K;) {
         int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
@@ -823,6 +840,11 @@ void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s, char* scra
 {
     BdrPlan pl;
     bdr_setup(h, pl);
+    if (scratch && bdr_workspace_bytes(h, pl, h.rmin < 0.0f) > scratch_bytes) {
+        pl = BdrPlan{};   // fewer segments, smaller partials
+        pl.lean = true;
+        bdr_setup(h, pl);
+    }
     if (scratch) {
         const size_t need = bdr_workspace_bytes(h, pl, h.rmin < 0.0f);
         if (need > scratch_bytes)

@@ -116,6 +116,7 @@ struct BdrPlan {
     float* unscale = nullptr;    // per-column 1 / (A scale x b scale) of the mixed path
     // launch shape of the single-instance pair kernels (bdr_setup)
     bool planned = false;
+    bool lean = false;           // set before bdr_setup: the fewest automatic segments (small partials)
     int S = 0;                   // source segments
     int seg_len = 0;             // plain kernels: sources per segment (multiple of 64)
     int steps = 0;               // MFMA kernels: 8-source steps per segment

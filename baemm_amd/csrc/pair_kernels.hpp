@@ -247,47 +247,55 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t&
 // (with F32: the FP32 values, ah = Re A, al = Im A).  Written stage by stage
 // across the 4 sources so that the independent dependency chains interleave
 // (the kernel is bound by VALU latency otherwise).
+// The 4 pairs of one lane for one chunk (coefficients c): FP16 hi/lo A
+// operands (with F32: ah = Re A, al = Im A).  Written stage by stage across
+// the 4 sources so that the independent dependency chains interleave.
+template <bool ADJ, bool F32 = false>
+__device__ __forceinline__ void mixed_vals_q(const BdrGeo (&geo)[4], const ChunkCoef& c, uint32_t (&ah)[4],
+                                             uint32_t (&al)[4])
+{
+    float cs[4], sn[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const float t = geo[jj].r * c.kap_rev;
+        cs[jj] = hw_cos_rev(t);
+        sn[jj] = hw_sin_rev(t);
+    }
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+        const BdrGeo& g = geo[jj];
+        float Qr, Qi;
+        if constexpr (ADJ) {
+            Qr = fmaf(-c.c2r, g.hj, fmaf(-c.kc2i, g.gj, c.c1r * g.rinv));
+            Qi = fmaf(-c.c2i, g.hj, fmaf(c.kc2r, g.gj, c.c1i * g.rinv));
+            Qr = fmaf(-c.c3r, g.hi, fmaf(-c.kc3i, g.gi, Qr));
+            Qi = fmaf(-c.c3i, g.hi, fmaf(c.kc3r, g.gi, Qi));
+        } else {
+            // The host divides each chunk's coefficients by c2 / s (s the
+            // chunk scale; the epilogue multiplies c2 / s back) and the
+            // source normals carry the factor s, so gj = s (z.n_j)/r^2,
+            // hj = s (z.n_j)/r^3 and Q' = c1' / r - hj + i kappa gj.
+            Qr = fmaf(c.c1r, g.rinv, -g.hj);
+            Qi = fmaf(c.kap, g.gj, c.c1i * g.rinv);
+        }
+        const float Ar = fmaf(cs[jj], Qr, -sn[jj] * Qi);
+        const float Ai = fmaf(cs[jj], Qi, sn[jj] * Qr);
+        if constexpr (F32) {
+            ah[jj] = __float_as_uint(Ar);
+            al[jj] = __float_as_uint(Ai);
+        } else {
+            split2(Ar, Ai, ah[jj], al[jj]);
+        }
+    }
+}
+
+// FP16 hi/lo A operands of the 4 pairs of one lane for chunks q0..q0+KG-1.
 template <int KG, bool ADJ, bool F32 = false>
 __device__ __forceinline__ void mixed_vals(const BdrGeo (&geo)[4], const CoefPack& cp, uint32_t (&ah)[KG][4],
                                            uint32_t (&al)[KG][4])
 {
 #pragma unroll
-    for (int q = 0; q < KG; ++q) {
-        float cs[4], sn[4];
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const float t = geo[jj].r * cp.c[q].kap_rev;
-            cs[jj] = hw_cos_rev(t);
-            sn[jj] = hw_sin_rev(t);
-        }
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const ChunkCoef& c = cp.c[q];
-            const BdrGeo& g = geo[jj];
-            float Qr, Qi;
-            if constexpr (ADJ) {
-                Qr = fmaf(-c.c2r, g.hj, fmaf(-c.kc2i, g.gj, c.c1r * g.rinv));
-                Qi = fmaf(-c.c2i, g.hj, fmaf(c.kc2r, g.gj, c.c1i * g.rinv));
-                Qr = fmaf(-c.c3r, g.hi, fmaf(-c.kc3i, g.gi, Qr));
-                Qi = fmaf(-c.c3i, g.hi, fmaf(c.kc3r, g.gi, Qi));
-            } else {
-                // The host divides each chunk's coefficients by c2 / s (s the
-                // chunk scale; the epilogue multiplies c2 / s back) and the
-                // source normals carry the factor s, so gj = s (z.n_j)/r^2,
-                // hj = s (z.n_j)/r^3 and Q' = c1' / r - hj + i kappa gj.
-                Qr = fmaf(c.c1r, g.rinv, -g.hj);
-                Qi = fmaf(c.kap, g.gj, c.c1i * g.rinv);
-            }
-            const float Ar = fmaf(cs[jj], Qr, -sn[jj] * Qi);
-            const float Ai = fmaf(cs[jj], Qi, sn[jj] * Qr);
-            if constexpr (F32) {
-                ah[q][jj] = __float_as_uint(Ar);
-                al[q][jj] = __float_as_uint(Ai);
-            } else {
-                split2(Ar, Ai, ah[q][jj], al[q][jj]);
-            }
-        }
-    }
+    for (int q = 0; q < KG; ++q) mixed_vals_q<ADJ, F32>(geo, cp.c[q], ah[q], al[q]);
 }
 
 // Geometry of the 4 (target i, source jbase + jj) pairs of one lane when the
@@ -381,7 +389,9 @@ struct MixedLds {
 // against the source steps [st0, st1) of chunks q0..q0+KG-1, partial sums
 // written to out[row*ldp + ...] (one segment slice; out NULL: a tile computed
 // only to keep the block's barriers in step).  Each wave generates the FP16
-// hi/lo A operands of its 32 targets in registers and contracts them itself.
+// hi/lo A operands of its 32 targets in registers and contracts them itself,
+// one wave-number chunk at a time, so that the MFMAs of a chunk run while the
+// next chunk's values are generated.
 // Each lane loads the one source (centroid, normal) of each step it holds for
 // mixed_geo_dpp from global memory, one step ahead; only the B operands are
 // staged in LDS (s_op: MixedLds<WCSP,KG>::OP_ELEMS elements), NS steps per
@@ -454,40 +464,34 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
                 yn = mid[j];
                 nn = nrm[j];
             }
-            uint32_t ah[KG][4], al[KG][4];
             BdrGeo geo[4];
             const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
             if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
             else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
-            mixed_vals<KG, ADJ>(geo, cp, ah, al);
-            half8_t a_hi[KG], a_lo[KG], b_hi[KG][NCB], b_lo[KG][NCB];
+            // chunk by chunk: the MFMAs of chunk q run while chunk q + 1 is generated
 #pragma unroll
             for (int q = 0; q < KG; ++q) {
-                a_hi[q] = __builtin_bit_cast(half8_t, make_uint4(ah[q][0], ah[q][1], ah[q][2], ah[q][3]));
-                a_lo[q] = __builtin_bit_cast(half8_t, make_uint4(al[q][0], al[q][1], al[q][2], al[q][3]));
+                half8_t b_hi[NCB], b_lo[NCB];
 #pragma unroll
                 for (int cb = 0; cb < NCB; ++cb) {
                     const int o = (q * NCB + cb) * 2;
-                    b_hi[q][cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o) * 64 + lane]);
-                    b_lo[q][cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o + 1) * 64 + lane]);
+                    b_hi[cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o) * 64 + lane]);
+                    b_lo[cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o + 1) * 64 + lane]);
                 }
+                uint32_t ahq[4], alq[4];
+                mixed_vals_q<ADJ>(geo, cp.c[q], ahq, alq);
+                const half8_t a_hi = __builtin_bit_cast(half8_t, make_uint4(ahq[0], ahq[1], ahq[2], ahq[3]));
+                const half8_t a_lo = __builtin_bit_cast(half8_t, make_uint4(alq[0], alq[1], alq[2], alq[3]));
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb)
+                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_hi[cb], acc[q][cb], 0, 0, 0);
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb)
+                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_lo[cb], acc[q][cb], 0, 0, 0);
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb)
+                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo, b_hi[cb], acc[q][cb], 0, 0, 0);
             }
-            // Product-major order: consecutive MFMAs write different accumulators.
-#pragma unroll
-            for (int q = 0; q < KG; ++q)
-#pragma unroll
-                for (int cb = 0; cb < NCB; ++cb)
-                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi[q], b_hi[q][cb], acc[q][cb], 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < KG; ++q)
-#pragma unroll
-                for (int cb = 0; cb < NCB; ++cb)
-                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi[q], b_lo[q][cb], acc[q][cb], 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < KG; ++q)
-#pragma unroll
-                for (int cb = 0; cb < NCB; ++cb)
-                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo[q], b_hi[q][cb], acc[q][cb], 0, 0, 0);
             yv = yn;
             nv = nn;
         }

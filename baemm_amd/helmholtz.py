@@ -128,7 +128,8 @@ class _HelmholtzBase:
         self._check(self._L.baemm_set_precision(self._h, int(m)))
 
     def SetMixedKernel(self, variant):
-        """MFMA kernel variant of the mixed precision: 0 single-wave, 1 wave-specialised."""
+        """MFMA kernel variant of the mixed precision: 0 single-wave, 1 wave-specialised,
+        2 automatic (the default)."""
         self._check(self._L.baemm_set_mixed_kernel(self._h, int(variant)))
 
     def AutotuneSegments(self, repeats=3):

@@ -121,9 +121,13 @@ int baemm_set_segments(baemm_handle h, int segments);
 /* No reference counterpart: the MFMA kernel variant of the mixed precision
  * for the boundary operators.
  * 0 = one wave generates its kernel values and issues their MFMAs;
- * 1 = wave-specialised blocks (generation waves feed MFMA waves through LDS),
- * the default.  Both give identical results.  The FP32 MFMA precision and the
- * potential always use the wave-specialised kernel. */
+ * 1 = wave-specialised blocks (generation waves feed MFMA waves through LDS);
+ * 2 = automatic, the default: the single-wave kernel, the wave-specialised one
+ * for operators with the adjoint double layer.
+ * With the same source segments (baemm_set_segments) the variants give
+ * identical results; their automatic segment counts differ (results within
+ * rounding).  The FP32 MFMA precision and the potential always use the
+ * wave-specialised kernel. */
 int baemm_set_mixed_kernel(baemm_handle h, int variant);
 
 /* gmres_max_iter / gmres_max_restarts public members, MemberVariables.hpp:64-65. */

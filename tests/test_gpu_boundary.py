@@ -196,30 +196,39 @@ def test_c2_size_sampled_rows(mode):
 def test_wave_specialised_mixed_kernel_is_identical(sphere3, W, wcs, kappa, c2, c3):
     """The wave-specialised MFMA kernel (generation waves feeding MFMA waves
     through LDS, source geometry through DPP row broadcasts) computes exactly
-    the values of the single-wave kernel, and matches the FP64 oracle."""
+    the values of the single-wave kernel over the same source segments; with
+    each variant's own automatic segments (and the automatic choice) all match
+    the FP64 oracle."""
     m, H = sphere3
     rng = np.random.default_rng(W + wcs)
     B = _rand(rng, m.nv, W)
     coeff = np.array([0.25, 1.0 - 0.5j, c2, c3] * (W // wcs))
-    H.SetPrecision("mixed")
-    outs = []
-    for variant in (0, 1):
-        H.SetMixedKernel(variant)
-        C = np.zeros((m.nv, W), np.complex128)
-        H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)
-        outs.append(C)
-    H.SetMixedKernel(0)
-    H.SetPrecision("fp32")
-    assert np.array_equal(outs[0], outs[1])
     ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
-    assert relerr(outs[1], ref) < NEAR
+    H.SetPrecision("mixed")
+    try:
+        for segments in (3, 0):
+            H.SetSegments(segments)
+            outs = []
+            for variant in (0, 1, 2):
+                H.SetMixedKernel(variant)
+                C = np.zeros((m.nv, W), np.complex128)
+                H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)
+                outs.append(C)
+                assert relerr(C, ref) < NEAR, (segments, variant)
+            if segments:
+                assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+    finally:
+        H.SetSegments(0)
+        H.SetMixedKernel(2)
+        H.SetPrecision("fp32")
 
 
-def test_kernel_variants_identical_at_default_segments():
-    """At 20,480 triangles the automatic segment count is balanced against the
-    wave-specialised kernel's occupancy (it differs from the single-wave
-    kernel's); the single-wave kernel takes the same segments, so the two
-    variants stay bit-identical with the default settings too."""
+def test_kernel_variants_at_default_segments():
+    """At 20,480 triangles each MFMA kernel variant balances its automatic
+    segment count against its own occupancy (the single-wave kernel takes more
+    segments): the results agree within rounding, the automatic choice is the
+    single-wave kernel for this operator, and with the segment count fixed the
+    variants are bit-identical."""
     V, T = icosphere(5)
     rng = np.random.default_rng(21)
     W, wcs = 64, 32
@@ -227,13 +236,17 @@ def test_kernel_variants_identical_at_default_segments():
     coeff = np.array([0, 1, 1, 0, 0.5, -16j, 1, 0], np.complex128)
     with Helmholtz_HIP(V, T) as H:
         H.SetPrecision("mixed")
-        outs = []
-        for variant in (1, 0):
-            H.SetMixedKernel(variant)
-            C = np.zeros((V.shape[0], W), np.complex128)
-            H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [8.0, 16.0], coeff, W, wcs)
-            outs.append(C)
-    assert np.array_equal(outs[0], outs[1])
+        outs = {}
+        for segments in (0, 7):
+            H.SetSegments(segments)
+            for variant in (1, 0, 2):
+                H.SetMixedKernel(variant)
+                C = np.zeros((V.shape[0], W), np.complex128)
+                H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, [8.0, 16.0], coeff, W, wcs)
+                outs[segments, variant] = C
+    assert relerr(outs[0, 0], outs[0, 1]) < 1e-6
+    assert np.array_equal(outs[0, 2], outs[0, 0])
+    assert np.array_equal(outs[7, 0], outs[7, 1]) and np.array_equal(outs[7, 2], outs[7, 0])
 
 
 @pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
