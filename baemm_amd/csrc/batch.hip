@@ -111,7 +111,8 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
         const int half = (threadIdx.x >> 6) >= 4;                                                                \
         const int t2 = 2 * w.tile + half;                                                                        \
         const bool real_tile = t2 < in.nt_pad / kMixTile;                                                        \
-        mixed_block<16, KG_, ADJ_>(in.mid, in.nrm, in.nt, in.bop, in.nst, cp, real_tile ? out : nullptr, in.ldp,   \
+        mixed_block<16, KG_, ADJ_>(in.mid, in.nt_pad, true, in.mid, in.nrm, in.nt, in.bop, in.nst, cp,              \
+                                   real_tile ? out : nullptr, in.ldp,                                             \
                                    w.q0, real_tile ? t2 : 2 * w.tile, w.s0, w.s1, s_op + half * LL::OP_ELEMS);     \
         break;                                                                                                   \
     }

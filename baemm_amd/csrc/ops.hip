@@ -350,8 +350,8 @@ static PlainFn plain_fn(int op, bool adj, int WCSP, int KG)
     throw ArgError{"internal: no plain kernel instance"};
 }
 
-using MixedFn = void (*)(const float4*, const float4*, int, const uint4*, int, CoefPack, float*, int64_t, int, int,
-                         int, const int*);
+using MixedFn = void (*)(const float4*, int, int, const float4*, const float4*, int, const uint4*, int, CoefPack,
+                         float*, int64_t, int, int, int, const int*);
 
 using WsFn = void (*)(const float4*, int, int, const float4*, const float4*, int, const uint4*, int, CoefPack, float*,
                       int64_t, int, int, int, const int*);
@@ -822,8 +822,9 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
                                (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
                                pl.ldp, q0, pl.steps, h.step_skip);
         else
-            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg), dim3(tiles, pl.S), dim3(256), 0, s, h.d_mid, pl.nrm,
-                               (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride, pl.ldp, q0, pl.steps, h.step_skip);
+            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg), dim3(tiles, pl.S), dim3(256), 0, s, h.d_mid,
+                               (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
+                               pl.ldp, q0, pl.steps, h.step_skip);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;
     }
@@ -948,11 +949,19 @@ static void apply_near_mfma(Handle& h, const ApplyArgs& a, const float4* d_pts, 
     const int P_pad = (P + kMixTile - 1) / kMixTile * kMixTile;
     const int nst = pl.nst;
     const int tiles = P_pad / kMixTile;
+    // the kernel and its segments as for the boundary operators (bdr_setup)
+    const bool ws = mixed_uses_ws(h, pl);
     int S0 = choose_segments(h, tiles, nst / 8, h.nt);
     if (h.segments == 0) {
-        const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
-        S0 = balance_segments(tiles, nst, S0, nst / 8,
-                              resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512));
+        int res;
+        if (ws) {
+            const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
+            res = resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512);
+        } else {
+            res = resident_blocks((const void*)mixed_fn(pl.adj, pl.WCSP, pick_kg(pl.WCSP, (int)p.K, true)), 256);
+            if (S0 < 16) S0 = std::max(S0, std::min({16, (8192 + tiles - 1) / tiles, nst / 8}));
+        }
+        S0 = balance_segments(tiles, nst, S0, nst / 8, res);
     }
     const int steps = (nst + S0 - 1) / S0;
     const int S = (nst + steps - 1) / steps;
@@ -962,11 +971,16 @@ static void apply_near_mfma(Handle& h, const ApplyArgs& a, const float4* d_pts, 
     // every pair of the source range is written: no clearing needed
     Timed tm(h, s);
     for (int q0 = 0; q0 < p.K;) {
-        const int kg = std::min(pick_kg(pl.WCSP, (int)p.K - q0, true), 2);
+        int kg = pick_kg(pl.WCSP, (int)p.K - q0, true);
+        if (ws) kg = std::min(kg, 2);
         const CoefPack cp = make_pack(p, q0, kg, pl.sA.data(), pl.rot.data());
-        hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32), dim3(tiles, S), dim3(512), 0, s, d_pts, P, 0,
-                           h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, seg_stride, pl.ldp, q0, steps,
-                           nullptr);
+        if (ws)
+            hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32), dim3(tiles, S), dim3(512), 0, s, d_pts, P, 0,
+                               h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, seg_stride, pl.ldp, q0, steps,
+                               nullptr);
+        else
+            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg), dim3(tiles, S), dim3(256), 0, s, d_pts, P, 0, h.d_mid,
+                               pl.nrm, (int)h.nt, pl.bop, nst, cp, part, seg_stride, pl.ldp, q0, steps, nullptr);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;
     }

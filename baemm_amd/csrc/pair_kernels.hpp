@@ -388,7 +388,10 @@ struct MixedLds {
 // One block of the single-wave mixed kernel: targets [tile*128, tile*128+128)
 // against the source steps [st0, st1) of chunks q0..q0+KG-1, partial sums
 // written to out[row*ldp + ...] (one segment slice; out NULL: a tile computed
-// only to keep the block's barriers in step).  Each wave generates the FP16
+// only to keep the block's barriers in step).  The targets are tgt[0 .. ntgt)
+// (reads clamp there): the centroids themselves (self, boundary operators:
+// the pair i = j is left out, the target normals are nrm) or other points
+// (potential: no pair left out, no target normals).  Each wave generates the FP16
 // hi/lo A operands of its 32 targets in registers and contracts them itself,
 // one wave-number chunk at a time, so that the MFMAs of a chunk run while the
 // next chunk's values are generated.
@@ -398,7 +401,8 @@ struct MixedLds {
 // stage and barrier.  Called by whole blocks of 256 threads (or 512 running
 // two tiles side by side); ends with a barrier so the caller may reuse the LDS.
 template <int WCSP, int KG, bool ADJ>
-__device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
+__device__ __forceinline__ void mixed_block(const float4* __restrict__ tgt, int ntgt, bool self,
+                                            const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
                                             const uint4* __restrict__ bop, int nst, const CoefPack& cp,
                                             float* __restrict__ out, int ldp, int q0, int tile, int st0, int st1,
                                             uint4* s_op)
@@ -411,9 +415,10 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
     const int g = lane >> 5;
     const int ibase = tile * kMixTile + wave * 32;
     const int i = ibase + (lane & 31);
-    const float4 x = mid[i];
+    const float4 x = tgt[min(i, ntgt - 1)];
     float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
-    if constexpr (ADJ) ni = nrm[i];
+    if constexpr (ADJ)
+        if (self) ni = nrm[i];
 
     float16_t acc[KG][NCB];
 #pragma unroll
@@ -425,9 +430,10 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
 
     const int nt_pad = nst * 8;
     // steps to mask (wave-uniform): the 4 steps whose sources are this wave's
-    // targets, and steps reaching past nt
-    const int st_diag = __builtin_amdgcn_readfirstlane(ibase >> 3);
+    // targets (self only), and steps reaching past nt
+    const int st_diag = self ? __builtin_amdgcn_readfirstlane(ibase >> 3) : -(1 << 28);
     const int st_tail = nt >> 3;
+    const int i_mask = self ? i : -1;   // the source index left out
     // the source of each step this lane holds for mixed_geo_dpp
     const int src_lane = 4 * g + (lane & 3);
 
@@ -466,8 +472,8 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ mid, cons
             }
             BdrGeo geo[4];
             const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
-            if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
-            else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i, nt, geo);
+            if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
+            else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
             // chunk by chunk: the MFMAs of chunk q run while chunk q + 1 is generated
 #pragma unroll
             for (int q = 0; q < KG; ++q) {
@@ -517,6 +523,7 @@ constexpr int mixed_min_waves() { return (!ADJ && KG <= 2) ? BAEMM_MIX_MINB : 3;
 
 template <int WCSP, int KG, bool ADJ>
 __global__ __launch_bounds__(256, (mixed_min_waves<KG, ADJ>())) void bdr_mixed_kernel(
+    const float4* __restrict__ tgt, int ntgt, int self,
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
     const uint4* __restrict__ bop, int nst, CoefPack cp,
     float* __restrict__ part, int64_t seg_stride, int ldp, int q0, int steps_per_seg, const int* __restrict__ skip)
@@ -526,8 +533,8 @@ __global__ __launch_bounds__(256, (mixed_min_waves<KG, ADJ>())) void bdr_mixed_k
     __shared__ uint4 s_op[L::OP_ELEMS];
     const int st0 = blockIdx.y * steps_per_seg;
     const int st1 = min(st0 + steps_per_seg, nst);
-    mixed_block<WCSP, KG, ADJ>(mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
-                               blockIdx.x, st0, st1, s_op);
+    mixed_block<WCSP, KG, ADJ>(tgt, ntgt, self != 0, mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride,
+                               ldp, q0, blockIdx.x, st0, st1, s_op);
 }
 
 // ---------------------------------------------------------------------------

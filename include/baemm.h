@@ -126,8 +126,8 @@ int baemm_set_segments(baemm_handle h, int segments);
  * for operators with the adjoint double layer.
  * With the same source segments (baemm_set_segments) the variants give
  * identical results; their automatic segment counts differ (results within
- * rounding).  The FP32 MFMA precision and the potential always use the
- * wave-specialised kernel. */
+ * rounding).  The FP32 MFMA precision always uses the wave-specialised
+ * kernel; the potential (baemm_apply_near_field) follows the same choice. */
 int baemm_set_mixed_kernel(baemm_handle h, int variant);
 
 /* gmres_max_iter / gmres_max_restarts public members, MemberVariables.hpp:64-65. */
