@@ -103,6 +103,11 @@ SIGNATURES = {
                                             _P, _INT, _INT, _D, _D, _D, ctypes.POINTER(_INT), _P]),
     "baemm_mass_metric_apply": (None, [_P, _P, _P, _P]),
     "baemm_mass_metric_jacobi": (None, [_P, _P, _P, _P]),
+    "baemm_tp_energy": (_INT, [_P, _D, _D, ctypes.POINTER(_D), _P]),
+    "baemm_tp_metric_prepare": (_INT, [_P, _D, _D]),
+    "baemm_tp_metric_apply": (None, [_P, _P, _P, _P]),
+    "baemm_tp_metric_jacobi": (None, [_P, _P, _P, _P]),
+    "baemm_tp_metric_apply_host": (_INT, [_P, _D, _D, _D, _P, _P]),
     "baemm_gmres_dev": (_INT, [_P, _P, _P, _D, _D, ctypes.POINTER(_I64), _P]),
     "baemm_gmres_batch": (_INT, [_INT, _P, _P, _P, _D, _D, _P, _P, _P]),
     "baemm_get_solver_stats": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_D),
@@ -131,6 +136,11 @@ REAL_OP = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double
 class MassMetric(ctypes.Structure):
     """baemm_mass_metric: the device metric c I + d (Mass (x) I_3)."""
     _fields_ = [("h", ctypes.c_void_p), ("c", ctypes.c_double), ("d", ctypes.c_double)]
+
+
+class TPMetric(ctypes.Structure):
+    """baemm_tp_metric: scale times the tangent-point metric of exponents (q, p)."""
+    _fields_ = [("h", ctypes.c_void_p), ("q", ctypes.c_double), ("p", ctypes.c_double), ("scale", ctypes.c_double)]
 
 _lib = None
 

@@ -185,7 +185,8 @@ void free_device(Handle& h)
     DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_scale, &h.ws_misc,
                       &h.ws_pts, &h.ws_g, &h.ws_nrm, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
                       &h.sws.t, &h.sws.x, &h.sws.r, &h.sws.hc, &h.sws.a, &h.sws.b, &h.sws.c,
-                      &h.sws.d, &h.sws.e, &h.sws.cgctr, &h.sws.sc, &h.sws.gm, &h.ws_p0};
+                      &h.sws.d, &h.sws.e, &h.sws.cgctr, &h.sws.sc, &h.sws.gm, &h.ws_p0, &h.ws_tp_grad,
+                      &h.ws_tp_face, &h.ws_tp_part, &h.ws_tp_out, &h.ws_tp_diag};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : h.io) b.release();
     for (GraphSlot& g : h.graphs)
@@ -202,7 +203,8 @@ size_t handle_device_bytes(const Handle& h)
     const DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_scale, &h.ws_misc,
                             &h.ws_pts, &h.ws_g, &h.ws_nrm, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
                             &h.sws.t, &h.sws.x, &h.sws.r, &h.sws.hc, &h.sws.a, &h.sws.b, &h.sws.c,
-                            &h.sws.d, &h.sws.e, &h.sws.cgctr, &h.sws.sc, &h.sws.gm, &h.ws_p0};
+                            &h.sws.d, &h.sws.e, &h.sws.cgctr, &h.sws.sc, &h.sws.gm, &h.ws_p0, &h.ws_tp_grad,
+                            &h.ws_tp_face, &h.ws_tp_part, &h.ws_tp_out, &h.ws_tp_diag};
     for (const DevBuf* b : bufs) g += b->bytes;
     for (const DevBuf& b : h.io) g += b.bytes;
     return g;

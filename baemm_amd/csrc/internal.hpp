@@ -181,6 +181,13 @@ struct Handle {
     DevBuf ws_B, ws_out, ws_b32, ws_bop, ws_part, ws_scale, ws_misc, ws_pts, ws_g, ws_nrm;
     DevBuf io[6];                        // device copies of the host arrays of the solver entry points
     DevBuf ws_p0;                        // P0 buffer read back for ReadB/WriteB/ReadC/WriteC
+    // tangent-point energy and metric (tangent_point.hip)
+    DevBuf ws_tp_grad;                   // hat-function gradients per triangle (float, 9 each)
+    DevBuf ws_tp_face;                   // metric inputs per triangle: du (9), ubar (3) (float)
+    DevBuf ws_tp_part;                   // segment partials of the pair sums
+    DevBuf ws_tp_out;                    // per-triangle results (double)
+    DevBuf ws_tp_diag;                   // diagonal of the tangent-point metric per vertex (double)
+    double tp_diag_q = 0.0, tp_diag_p = 0.0;   // (q, p) of ws_tp_diag; p = 0: not computed
     P0State p0;
     Workspace sws;
     SolverStats stats;

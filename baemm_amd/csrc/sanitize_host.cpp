@@ -136,6 +136,9 @@ void null_arguments()
     NULL_CALL(baemm_adjoint_derivative_ff);
     NULL_CALL(baemm_gauss_newton_solve);
     NULL_CALL(baemm_gauss_newton_solve_dev);
+    NULL_CALL(baemm_tp_energy);
+    NULL_CALL(baemm_tp_metric_prepare);
+    NULL_CALL(baemm_tp_metric_apply_host);
     NULL_CALL(baemm_gmres_dev);
     NULL_CALL(baemm_gmres_batch);
     NULL_CALL(baemm_get_solver_stats);
@@ -163,6 +166,8 @@ void null_arguments()
           "baemm_batch_get_stats");
     baemm_mass_metric_apply(nullptr, nullptr, nullptr, nullptr);   // returns at once
     baemm_mass_metric_jacobi(nullptr, nullptr, nullptr, nullptr);
+    baemm_tp_metric_apply(nullptr, nullptr, nullptr, nullptr);
+    baemm_tp_metric_jacobi(nullptr, nullptr, nullptr, nullptr);
     check(baemm_version() > 0, "baemm_version");
     // n < 0 is refused before the device is looked at
     check(baemm_hw_sincos(0, nullptr, -1, nullptr, nullptr) == BAEMM_ERR_ARG, "baemm_hw_sincos(n < 0)");

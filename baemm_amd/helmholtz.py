@@ -501,13 +501,15 @@ class _SolverMixin:
 
     def GaussNewtonSolveDevice(self, kappa, wave_chunk_count, inc_directions, wave_chunk_size, c, d, alpha, B, beta,
                                X, du_dn=None, type=WaveType.Plane, cg_tol=1e-5, gmres_tol_inner=5e-3,
-                               gmres_tol_outer=1e-2):
+                               gmres_tol_outer=1e-2, tangent_point=None):
         """GaussNewtonSolve with every vector on the device
-        (baemm_gauss_newton_solve_dev) and the device metric
-        M = c I + d (Mass (x) I_3) with its Jacobi preconditioner
-        (baemm_mass_metric_apply / _jacobi).  B, X: V x 3 float64 views
-        (ld >= 3 allowed); X is updated in place.  du_dn: V x W complex or None
-        (then computed).  Returns (succeeded, du_dn)."""
+        (baemm_gauss_newton_solve_dev) and a device metric with its Jacobi
+        preconditioner: M = c I + d (Mass (x) I_3) (baemm_mass_metric_apply /
+        _jacobi), or, with tangent_point = (q, p, regpar), M = regpar times the
+        tangent-point metric as in Example_GaussNewton/main.cpp:301-346
+        (baemm_tp_metric_apply / _jacobi; c and d are then unused).
+        B, X: V x 3 float64 views (ld >= 3 allowed); X is updated in place.
+        du_dn: V x W complex or None (then computed).  Returns (succeeded, du_dn)."""
         k, dr = self._kd(kappa, inc_directions, wave_chunk_size)
         W = int(wave_chunk_count) * int(wave_chunk_size)
         nv = self.VertexCount()
@@ -526,9 +528,16 @@ class _SolverMixin:
                 _lib.check(L.baemm_dev_alloc(ctypes.byref(p), a.nbytes))
                 bufs.append(p)
                 _lib.check(L.baemm_dev_upload(p, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
-            metric = _lib.MassMetric(self._h.value, float(c), float(d))
-            Mf = ctypes.cast(L.baemm_mass_metric_apply, ctypes.c_void_p)
-            Pf = ctypes.cast(L.baemm_mass_metric_jacobi, ctypes.c_void_p)
+            if tangent_point is None:
+                metric = _lib.MassMetric(self._h.value, float(c), float(d))
+                Mf = ctypes.cast(L.baemm_mass_metric_apply, ctypes.c_void_p)
+                Pf = ctypes.cast(L.baemm_mass_metric_jacobi, ctypes.c_void_p)
+            else:
+                q, p, regpar = (float(v) for v in tangent_point)
+                self._check(L.baemm_tp_metric_prepare(self._h, q, p))
+                metric = _lib.TPMetric(self._h.value, q, p, regpar)
+                Mf = ctypes.cast(L.baemm_tp_metric_apply, ctypes.c_void_p)
+                Pf = ctypes.cast(L.baemm_tp_metric_jacobi, ctypes.c_void_p)
             mp = ctypes.cast(ctypes.pointer(metric), ctypes.c_void_p)
             ok = ctypes.c_int(0)
             st = L.baemm_gauss_newton_solve_dev(self._h, _ptr(k), int(wave_chunk_count), _ptr(dr),
@@ -544,6 +553,30 @@ class _SolverMixin:
                 L.baemm_dev_free(p)
         X[:, :3] = Xfull[:, :3]
         return bool(ok.value), du
+
+
+    def TangentPointEnergy(self, q=6.0, p=12.0, gradient=True):
+        """The tangent-point energy of the mesh (baemm_tp_energy): the
+        caller-side energy of Example_GaussNewton/main.cpp:308-309.
+        Returns (E, dE/dx as V x 3 float64, or None)."""
+        E = ctypes.c_double(0.0)
+        g = np.zeros((self.VertexCount(), 3)) if gradient else None
+        self._check(self._L.baemm_tp_energy(self._h, float(q), float(p), ctypes.byref(E),
+                                            None if g is None else g.ctypes.data_as(ctypes.c_void_p)))
+        return E.value, g
+
+    def TangentPointMetric(self, x, q=6.0, p=12.0, scale=1.0):
+        """scale times the tangent-point metric applied to x (V x 3 float64):
+        the operator of Example_GaussNewton/main.cpp:324-332
+        (baemm_tp_metric_apply_host)."""
+        xv = np.require(np.asarray(x, np.float64).reshape(-1, 3), np.float64, ["C"])
+        if xv.shape[0] != self.VertexCount():
+            raise ValueError("x must have vertex_count rows")
+        y = np.zeros_like(xv)
+        self._check(self._L.baemm_tp_metric_apply_host(self._h, float(q), float(p), float(scale),
+                                                       xv.ctypes.data_as(ctypes.c_void_p),
+                                                       y.ctypes.data_as(ctypes.c_void_p)))
+        return y
 
 
 class Helmholtz_HIP(_HelmholtzBase, _SolverMixin):

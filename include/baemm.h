@@ -426,6 +426,42 @@ typedef struct {
 void baemm_mass_metric_apply(void* user, const double* x, double* y, void* stream);
 void baemm_mass_metric_jacobi(void* user, const double* x, double* y, void* stream);
 
+/* The caller-side energy and metric of the Gauss-Newton example
+ * (Example_GaussNewton/main.cpp:301-346 builds them with Repulsor's
+ * TangentPointEnergy0 and TangentPointMetric0, which are not part of BAEMM),
+ * on the handle's mesh, restated from Yu, Brakensiek, Schumacher & Crane,
+ * "Repulsive Surfaces" (2021) with centroid quadrature over all pairs of
+ * distinct triangles S != T:
+ *   E = sum a_S a_T |n_S . (c_S - c_T)|^q / |c_S - c_T|^p          (q > 1, p > 0)
+ *   <<u, v>> = sum w_ST <du_S - du_T, dv_S - dv_T>_F
+ *            + sum w_ST k_ST <ubar_S - ubar_T, vbar_S - vbar_T>,
+ *   w_ST = a_S a_T |c_S - c_T|^(-2 sigma - 2), sigma = (p - 2)/q - 1,
+ *   k_ST = (n_S . (c_S - c_T))^2 / |c_S - c_T|^4,
+ * du_S the face gradient and ubar_S the face average of a P1 vector function.
+ * Pair terms in FP32, sums in FP64.
+ *
+ * baemm_tp_energy: E and (gradient non-NULL) dE/dx, vertex_count x 3 doubles,
+ * on the host.  Synchronous. */
+int baemm_tp_energy(baemm_handle h, double q, double p, double* energy, double* gradient);
+
+/* The metric as a device callback for baemm_gauss_newton_solve_dev:
+ * M x = scale * (matrix of <<., .>>) x on V x 3 device arrays, and the Jacobi
+ * preconditioner P x = x / (scale * d_v) row by row, d_v the diagonal of that
+ * matrix (equal for the three components of vertex v).
+ * baemm_tp_metric_prepare allocates the callbacks' buffers and computes the
+ * diagonal for (q, p); call it first (a callback that fails
+ * cannot return a status: it fills y with NaN and leaves a message in
+ * baemm_last_error).  Calls on one handle must not overlap, as for the
+ * mass metric above.  baemm_tp_metric_apply_host applies M to host arrays. */
+typedef struct {
+    baemm_handle h;
+    double q, p, scale;
+} baemm_tp_metric;
+int baemm_tp_metric_prepare(baemm_handle h, double q, double p);
+void baemm_tp_metric_apply(void* user, const double* x, double* y, void* stream);
+void baemm_tp_metric_jacobi(void* user, const double* x, double* y, void* stream);
+int baemm_tp_metric_apply_host(baemm_handle h, double q, double p, double scale, const double* x, double* y);
+
 /* No reference counterpart: the GMRES of the solvers above on the loaded
  * boundary operator A: solves (P A) X = P B (P = mass inverse by PCG, cg_tol)
  * to gmres_tol.  B, X: device arrays, vertex_count x W complex double (ld W).

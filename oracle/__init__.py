@@ -52,6 +52,11 @@ PINS = {
     "solves": "dense LU; Mie series for the sound-soft sphere; adjoint dot-product test (test_oracle_solvers)",
     "gauss_newton": "its dense operator reproduces the pinned derivative_ff / adjoint_derivative_ff compositions "
                     "column by column; the solution satisfies the normal equations (test_oracle_solvers)",
+    "tp_energy / tp_energy_gradient": "the gradient agrees with central finite differences of the energy; the "
+                                      "energy scales as lambda^(4-p+q) under dilation and is invariant under "
+                                      "rotations and translations (test_oracle_tangent_point)",
+    "tp_metric": "symmetric positive semi-definite; translations are in its kernel; it scales as lambda^(-2 sigma) "
+                 "under dilation; the face gradients reproduce the tangential projector (test_oracle_tangent_point)",
 }
 
 
@@ -666,6 +671,110 @@ def gauss_newton(mesh: Mesh, kappa, dirs, wcs, metric, B, alpha=1.0, beta=0.0, X
     if X is not None and beta != 0.0:
         out = out + beta * np.asarray(X, dtype=np.float64)
     return out, du_dn
+
+
+# --------------------------------------------------------------------------
+# Tangent-point energy and metric of the Gauss-Newton example's caller
+# (Example_GaussNewton/main.cpp:301-346 takes them from Repulsor's
+# TangentPointEnergy0 / TangentPointMetric0, which is not part of BAEMM and not
+# vendored).  Restated from the published discretisation (C. Yu, C.
+# Brakensiek, H. Schumacher, K. Crane, "Repulsive Surfaces", ACM Trans. Graph.
+# 40(6), 2021): centroid quadrature over pairs of distinct triangles, P1
+# vertex functions.  Parity with Repulsor is unpinned; these functions are
+# pinned by finite differences and by structure (PINS).
+# --------------------------------------------------------------------------
+
+def _tp_pairs(mesh):
+    c, n, a = mesh.mid, mesh.nrm, mesh.area
+    d = c[:, None, :] - c[None, :, :]          # d[S, T] = c_S - c_T
+    r2 = (d * d).sum(-1)
+    np.fill_diagonal(r2, 1.0)
+    return c, n, a, d, r2
+
+
+def tp_energy(mesh: Mesh, q: float, p: float) -> float:
+    """E = sum_{S != T} a_S a_T |n_S . (c_S - c_T)|^q / |c_S - c_T|^p  (FP64)."""
+    _, n, a, d, r2 = _tp_pairs(mesh)
+    s = np.einsum("sk,stk->st", n, d)
+    K = np.abs(s) ** q * r2 ** (-p / 2)
+    np.fill_diagonal(K, 0.0)
+    return float(a @ K @ a)
+
+
+def tp_energy_gradient(mesh: Mesh, q: float, p: float) -> np.ndarray:
+    """dE/dx_v (V x 3) of tp_energy.  Per ordered pair (S, T), with d = c_S - c_T,
+    s = n_S . d, R = |d|^-p and N = a n the area vector:
+      dTerm/dN_S = a_T R ((1 - q) |s|^q n_S + q |s|^(q-1) sgn(s) d)
+      dTerm/dN_T = a_S |s|^q R n_T
+      dTerm/dd   = a_S a_T (q |s|^(q-1) sgn(s) R n_S - p |s|^q R d / |d|^2),
+    c_S = (x0 + x1 + x2) / 3 and N = (x1 - x0) x (x2 - x0) / 2, so
+    d(g . N)/dx_k = (x_(k+1) - x_(k+2)) x g / 2."""
+    _, n, a, d, r2 = _tp_pairs(mesh)
+    nt = mesh.nt
+    s = np.einsum("sk,stk->st", n, d)
+    As = np.abs(s)
+    R = r2 ** (-p / 2)
+    np.fill_diagonal(R, 0.0)
+    P = As ** q
+    Pm = q * As ** (q - 1) * np.sign(s)
+    gN = ((a[None, :] * R * (1 - q) * P).sum(1))[:, None] * n + np.einsum("st,stk->sk", a[None, :] * R * Pm, d)
+    gN += ((a[:, None] * P * R).sum(0))[:, None] * n
+    aa = np.outer(a, a)
+    full = (aa * Pm * R)[:, :, None] * n[:, None, :] - (aa * p * P * R / r2)[:, :, None] * d
+    gc = full.sum(1) - full.sum(0)
+    X = mesh.V[mesh.T]                          # nt x 3 x 3
+    G = np.zeros_like(mesh.V)
+    for k in range(3):
+        e = X[:, (k + 1) % 3] - X[:, (k + 2) % 3]
+        np.add.at(G, mesh.T[:, k], gc / 3.0 + 0.5 * np.cross(e, gN))
+    assert G.shape == (mesh.nv, 3) and nt == gN.shape[0]
+    return G
+
+
+def face_gradients(mesh: Mesh) -> np.ndarray:
+    """Gradients of the three barycentric (hat) functions on each triangle,
+    nt x 3 (vertex) x 3 (space): grad lambda_k = n x (x_(k+2) - x_(k+1)) / (2 a)."""
+    X = mesh.V[mesh.T]
+    g = np.zeros((mesh.nt, 3, 3))
+    for k in range(3):
+        e = X[:, (k + 2) % 3] - X[:, (k + 1) % 3]
+        g[:, k] = np.cross(mesh.nrm, e) / (2.0 * mesh.area)[:, None]
+    return g
+
+
+def tp_metric(mesh: Mesh, q: float, p: float) -> np.ndarray:
+    """Dense (3V x 3V) matrix, on vec(X) with X V x 3 row-major (entry 3 v + c),
+    of <<u, v>> = B_sigma(u, v) + B0_sigma(u, v), sigma = (p - 2) / q - 1:
+      B_sigma = sum_{S != T} w_ST <du_S - du_T, dv_S - dv_T>_F,
+      B0      = sum_{S != T} w_ST k_ST <ubar_S - ubar_T, vbar_S - vbar_T>,
+    with w_ST = a_S a_T |c_S - c_T|^(-2 sigma - 2), k_ST = (n_S . (c_S -
+    c_T))^2 / |c_S - c_T|^4, du_S the face gradient (3 x 3) and ubar_S the
+    face average of a P1 vector function."""
+    sigma = (p - 2.0) / q - 1.0
+    _, n, a, d, r2 = _tp_pairs(mesh)
+    w = np.outer(a, a) * r2 ** (-(sigma + 1.0))
+    np.fill_diagonal(w, 0.0)
+    kst = np.einsum("sk,stk->st", n, d) ** 2 / r2 ** 2
+    wt = w * (kst + kst.T)
+    L = np.diag(w.sum(1)) - w
+    Lt = np.diag(wt.sum(1)) - wt
+    nt, nv = mesh.nt, mesh.nv
+    g = face_gradients(mesh)
+    # D: vec(u) -> (nt, 3 components, 3 space); A: vec(u) -> (nt, 3 components)
+    D = np.zeros((nt, 3, 3, nv, 3))
+    A = np.zeros((nt, 3, nv, 3))
+    for k in range(3):
+        for cc in range(3):
+            D[np.arange(nt), cc, :, mesh.T[:, k], cc] += g[:, k, :]
+            A[np.arange(nt), cc, mesh.T[:, k], cc] += 1.0 / 3.0
+    # Bs = 2 sum_st L_st D_s^T D_t, B0 = sum_st Lt_st A_s^T A_t, as two products each
+    D = D.reshape(nt, 9 * 3 * nv)
+    A = A.reshape(nt, 3 * 3 * nv)
+    LD = (L @ D).reshape(nt * 9, 3 * nv)
+    LA = (Lt @ A).reshape(nt * 3, 3 * nv)
+    Bs = 2.0 * D.reshape(nt * 9, 3 * nv).T @ LD
+    B0 = A.reshape(nt * 3, 3 * nv).T @ LA
+    return Bs + B0
 
 
 def gmres_dense(A, b, tol=1e-10):
