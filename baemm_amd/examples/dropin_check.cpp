@@ -112,6 +112,10 @@ int main()
         H.ApplyBoundaryOperators_PL<0>(C64(1, 0), B.data(), W, C64(0, 0), C1.data(), W, kappa, coeff, W, wcs);
         H.LoadBoundaryOperators_PL(kappa, coeff, W, wcs);
         check("GetWaveCount after load", std::abs((double)H.GetWaveCount() - W), 0);
+        H.SetWaveCount(5);
+        check("SetWaveCount then GetWaveCount", std::abs((double)H.GetWaveCount() - 5), 0);
+        H.LoadBoundaryOperators_PL(kappa, coeff, W, wcs);
+        check("LoadParameters overrides SetWaveCount", std::abs((double)H.GetWaveCount() - W), 0);
         check("GetWaveNumbers[1]", std::abs(H.GetWaveNumbers()[1] - 3.5), 1e-6);
         H.ApplyBoundaryOperators_PL<8>(C64(1, 0), B.data(), W, C64(0, 0), C2.data(), W);
         check("loaded vs one-shot apply", rel(C2, C1), 0);
@@ -236,6 +240,19 @@ int main()
             }
         H.MassMatrix().Dot<4>(1.0, Yr.data(), int64_t(4), 0.0, Zr.data(), int64_t(4));
         check("M (M^-1 X) = X, complex", rel(Zr, Xr), 1e-8);
+        // lumped inverse: 1 / (AvT (1/3)), i.e. three over the row sums of the mass matrix
+        std::vector<double> ones((size_t)nv, 1.0), rows((size_t)nv, 0.0);
+        H.MassMatrix().Dot<1>(1.0, ones.data(), int64_t(1), 0.0, rows.data(), int64_t(1));
+        std::vector<double> YL((size_t)nv * 3);
+        H.ApplyLumpedMassInverse<3>(X.data(), 3, YL.data(), 3);
+        std::vector<C64> YLc((size_t)nv * 2);
+        H.ApplyLumpedMassInverse<2>(Xc.data(), 2, YLc.data(), 2);
+        double dl = 0.0;
+        for (int64_t v = 0; v < nv; ++v) {
+            for (int c = 0; c < 3; ++c) dl = std::max(dl, std::abs(YL[v * 3 + c] * rows[v] / 3.0 - X[v * 3 + c]));
+            for (int c = 0; c < 2; ++c) dl = std::max(dl, std::abs(YLc[v * 2 + c] * (rows[v] / 3.0) - Xc[v * 2 + c]));
+        }
+        check("ApplyLumpedMassInverse = 3 / mass row sums, real and complex", dl, 1e-12);
 
         // ---- solvers
         const double cg_tol = 1e-12, gmres_tol = 1e-9;

@@ -209,10 +209,16 @@ public:
 
     Int GetWaveCount() const
     {
+        if (wave_count_set_ >= 0) return wave_count_set_;
         int64_t W = 0;
         baemm_get_parameters(h_, nullptr, nullptr, &W, nullptr, nullptr);
         return (Int)W;
     }
+    // GetSetters.hpp:40-45.  The reference's LoadParameters overwrites the value;
+    // until then it is what GetWaveCount reports and the width the preloaded
+    // applies expect (an apply whose width differs from the loaded parameters'
+    // fails with an argument error).
+    void SetWaveCount(Int wave_count_) { wave_count_set_ = wave_count_; }
     Int GetWaveChunkSize() const { return (Int)baemm_get_wave_chunk_size(h_); }
     void SetWaveChunkSize(Int wcs) { detail::report(h_, baemm_set_wave_chunk_size(h_, wcs), "SetWaveChunkSize"); }
     Int GetWaveChunkCount(Int wave_count) const
@@ -324,6 +330,7 @@ public:
         const int64_t K = (wave_count_ + wave_chunk_size_ - 1) / wave_chunk_size_;
         const std::vector<double> k = detail::to_double(kappa_list, K);
         const std::vector<double> c = detail::coeff_double(coeff_list, 4 * K);
+        wave_count_set_ = -1;
         detail::report(h_, baemm_load_parameters(h_, k.data(), c.data(), wave_count_, wave_chunk_size_, (double)factor),
                        "LoadParameters");
     }
@@ -471,6 +478,36 @@ public:
         }
     }
 
+    // ApplyMassInverse.hpp:161-197: Y = diag(areas_lumped_inv) X, row by row;
+    // real or complex columns.
+    template <Int NRHS = VarSize, typename X_T, typename Y_T>
+    void ApplyLumpedMassInverse(const X_T* X, const Int ldX, Y_T* Y, const Int ldY, const Int nrhs = NRHS)
+    {
+        constexpr int c = detail::is_complex<X_T>::value ? 2 : 1;
+        static_assert(detail::is_complex<Y_T>::value == (c == 2), "X and Y must both be real or both complex");
+        const int64_t n = VertexCount(), k = (int64_t)c * nrhs;
+        std::vector<double> x((size_t)n * k), y((size_t)n * k);
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < nrhs; ++j) {
+                if constexpr (c == 2) {
+                    x[i * k + 2 * j] = (double)X[i * ldX + j].real();
+                    x[i * k + 2 * j + 1] = (double)X[i * ldX + j].imag();
+                } else {
+                    x[i * k + j] = (double)X[i * ldX + j];
+                }
+            }
+        detail::report(h_, baemm_apply_lumped_mass_inverse(h_, x.data(), k, y.data(), k, k), "ApplyLumpedMassInverse");
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < nrhs; ++j) {
+                if constexpr (c == 2) {
+                    using R = typename Y_T::value_type;
+                    Y[i * ldY + j] = Y_T((R)y[i * k + 2 * j], (R)y[i * k + 2 * j + 1]);
+                } else {
+                    Y[i * ldY + j] = (Y_T)y[i * k + j];
+                }
+            }
+    }
+
     // ------------------------------------------------------------- FarField.hpp
     template <Int WC, typename I_ext, typename R_ext, typename C_ext>
     void FarField(const R_ext* kappa_, const I_ext wave_chunk_count_, R_ext* inc_directions,
@@ -605,6 +642,7 @@ public:
 private:
     baemm_handle h_ = nullptr;
     Int threads_ = 1;
+    Int wave_count_set_ = -1;   // SetWaveCount's value until the next LoadParameters (-1: none)
     std::vector<double> vertex_coords_;
     std::vector<int64_t> triangles_;
     mutable std::vector<Real> tri_[4];
