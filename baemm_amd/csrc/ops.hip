@@ -372,17 +372,61 @@ static WsFn mixed_ws_fn(bool adj, int WCSP, int KG, bool f32)
     throw ArgError{"internal: no wave-specialised mixed kernel instance"};
 }
 
-static MixedFn mixed_fn(bool adj, int WCSP, int KG)
+// Coefficient form of chunk c for the single-wave kernel (its mixed_vals_q):
+// 1 Re c1 = 0, 2 Im c1 = 0, else 0.
+static int coef_form(const ChunkCoef& c) { return c.c1r == 0.0f ? 1 : (c.c1i == 0.0f ? 2 : 0); }
+
+// The single-wave kernel for KG chunks with coefficients cp (NULL: the
+// general form).  Without the adjoint term, one or two chunks take their own
+// coefficient forms, four chunks a form they share; else the general form.
+static MixedFn mixed_fn(bool adj, int WCSP, int KG, const CoefPack* cp = nullptr)
 {
-#define MIX(W_, K_)                                                                 \
-    if (WCSP == W_ && KG == K_) return adj ? &bdr_mixed_kernel<W_, K_, true> : &bdr_mixed_kernel<W_, K_, false>;
-    MIX(16, 1)
-    MIX(16, 2)
-    MIX(16, 4)
-    MIX(32, 1)
-    MIX(32, 2)
-    MIX(64, 1)
-#undef MIX
+    int forms = 0;
+    if (cp && !adj) {
+        if (KG <= 2) {
+            for (int q = KG - 1; q >= 0; --q) forms = 3 * forms + coef_form(cp->c[q]);
+        } else {
+            const int f = coef_form(cp->c[0]);
+            bool same = true;
+            for (int q = 1; q < KG; ++q) same = same && coef_form(cp->c[q]) == f;
+            forms = same ? f * 40 : 0;   // 40 = 1 + 3 + 9 + 27: form f in every digit
+        }
+    }
+#define MIXF1(W_)                                                     \
+    if (WCSP == W_ && KG == 1) {                                      \
+        if (adj) return &bdr_mixed_kernel<W_, 1, true>;               \
+        if (forms == 1) return &bdr_mixed_kernel<W_, 1, false, 1>;    \
+        if (forms == 2) return &bdr_mixed_kernel<W_, 1, false, 2>;    \
+        return &bdr_mixed_kernel<W_, 1, false, 0>;                    \
+    }
+#define MIXF2(W_)                                                     \
+    if (WCSP == W_ && KG == 2) {                                      \
+        if (adj) return &bdr_mixed_kernel<W_, 2, true>;               \
+        switch (forms) {                                              \
+            case 1: return &bdr_mixed_kernel<W_, 2, false, 1>;        \
+            case 2: return &bdr_mixed_kernel<W_, 2, false, 2>;        \
+            case 3: return &bdr_mixed_kernel<W_, 2, false, 3>;        \
+            case 4: return &bdr_mixed_kernel<W_, 2, false, 4>;        \
+            case 5: return &bdr_mixed_kernel<W_, 2, false, 5>;        \
+            case 6: return &bdr_mixed_kernel<W_, 2, false, 6>;        \
+            case 7: return &bdr_mixed_kernel<W_, 2, false, 7>;        \
+            case 8: return &bdr_mixed_kernel<W_, 2, false, 8>;        \
+            default: return &bdr_mixed_kernel<W_, 2, false, 0>;       \
+        }                                                             \
+    }
+    MIXF1(16)
+    MIXF2(16)
+    MIXF1(32)
+    MIXF2(32)
+    MIXF1(64)
+#undef MIXF1
+#undef MIXF2
+    if (WCSP == 16 && KG == 4) {
+        if (adj) return &bdr_mixed_kernel<16, 4, true>;
+        if (forms == 40) return &bdr_mixed_kernel<16, 4, false, 40>;
+        if (forms == 80) return &bdr_mixed_kernel<16, 4, false, 80>;
+        return &bdr_mixed_kernel<16, 4, false, 0>;
+    }
     throw ArgError{"internal: no mixed kernel instance"};
 }
 
@@ -822,7 +866,7 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
                                (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
                                pl.ldp, q0, pl.steps, h.step_skip);
         else
-            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg), dim3(tiles, pl.S), dim3(256), 0, s, h.d_mid,
+            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg, &cp), dim3(tiles, pl.S), dim3(256), 0, s, h.d_mid,
                                (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
                                pl.ldp, q0, pl.steps, h.step_skip);
         BAEMM_HIP(hipGetLastError());
@@ -979,7 +1023,7 @@ static void apply_near_mfma(Handle& h, const ApplyArgs& a, const float4* d_pts, 
                                h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, seg_stride, pl.ldp, q0, steps,
                                nullptr);
         else
-            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg), dim3(tiles, S), dim3(256), 0, s, d_pts, P, 0, h.d_mid,
+            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg, &cp), dim3(tiles, S), dim3(256), 0, s, d_pts, P, 0, h.d_mid,
                                pl.nrm, (int)h.nt, pl.bop, nst, cp, part, seg_stride, pl.ldp, q0, steps, nullptr);
         BAEMM_HIP(hipGetLastError());
         q0 += kg;

@@ -13,6 +13,8 @@
 // onto this layout directly.
 #pragma once
 
+#include <utility>
+
 #include "internal.hpp"
 
 namespace baemm {
@@ -250,10 +252,17 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t&
 // The 4 pairs of one lane for one chunk (coefficients c): FP16 hi/lo A
 // operands (with F32: ah = Re A, al = Im A).  Written stage by stage across
 // the 4 sources so that the independent dependency chains interleave.
-template <bool ADJ, bool F32 = false>
+// FORM (no adjoint term only): which part of the chunk's single-layer
+// coefficient c1 is zero and left out of the arithmetic, as the reference
+// builds its kernels for the nonzero real and imaginary parts of its
+// coefficients (LoadParameters.hpp:47-80): 0 none, 1 Re c1 = 0, 2 Im c1 = 0.
+// The values are the same as with FORM 0 (a zero term adds nothing; at most
+// the sign of a zero differs).
+template <bool ADJ, bool F32 = false, int FORM = 0>
 __device__ __forceinline__ void mixed_vals_q(const BdrGeo (&geo)[4], const ChunkCoef& c, uint32_t (&ah)[4],
                                              uint32_t (&al)[4])
 {
+    static_assert(FORM == 0 || !ADJ, "coefficient forms are for operators without the adjoint term");
     float cs[4], sn[4];
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
@@ -275,8 +284,10 @@ __device__ __forceinline__ void mixed_vals_q(const BdrGeo (&geo)[4], const Chunk
             // chunk scale; the epilogue multiplies c2 / s back) and the
             // source normals carry the factor s, so gj = s (z.n_j)/r^2,
             // hj = s (z.n_j)/r^3 and Q' = c1' / r - hj + i kappa gj.
-            Qr = fmaf(c.c1r, g.rinv, -g.hj);
-            Qi = fmaf(c.kap, g.gj, c.c1i * g.rinv);
+            if constexpr (FORM == 1) Qr = -g.hj;
+            else Qr = fmaf(c.c1r, g.rinv, -g.hj);
+            if constexpr (FORM == 2) Qi = c.kap * g.gj;
+            else Qi = fmaf(c.kap, g.gj, c.c1i * g.rinv);
         }
         const float Ar = fmaf(cs[jj], Qr, -sn[jj] * Qi);
         const float Ai = fmaf(cs[jj], Qi, sn[jj] * Qr);
@@ -368,9 +379,37 @@ __device__ __forceinline__ void mixed_geo_dpp(const float4 x, const float4 ni, c
     }
 }
 
-#ifndef BAEMM_MIX_NS
-#define BAEMM_MIX_NS 2
-#endif
+// Buffer resource over the n float4 elements at p (raw, byte-addressed; word 3
+// as the gfx9 family takes it for untyped dword loads).  Loads past n * 16
+// bytes return zeros.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const float4* p, int n)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(p), (short)0, n * (int)sizeof(float4), 0x00020000);
+}
+
+// x, y, z of the float4 at byte offset voff + soff of r (w = 0).
+__device__ __forceinline__ float4 load_xyz(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), 0.0f);
+}
+
+// Coefficient forms of the chunks of a single-wave launch, FORM of chunk q in
+// base-3 digit q (see mixed_vals_q).
+constexpr int chunk_form(int forms, int q) { return q == 0 ? forms % 3 : chunk_form(forms / 3, q - 1); }
+
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<int, q>) for q = 0 .. N-1.
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 #ifndef BAEMM_MIX_MINB
 #define BAEMM_MIX_MINB 4
 #endif
@@ -380,7 +419,7 @@ template <int WCSP, int KG>
 struct MixedLds {
     static constexpr int NCB = 2 * WCSP / 32;
     static constexpr int OPS = KG * NCB * 2;     // 1-KB operand tiles per step (chunk x col-block x hi/lo)
-    static constexpr int NS = BAEMM_MIX_NS;      // steps (8 sources each) per LDS stage
+    static constexpr int NS = 2;                 // steps (8 sources each) per LDS stage
     static constexpr int TILES = NS * OPS;
     static constexpr int OP_ELEMS = 2 * TILES * 64;
 };
@@ -400,7 +439,7 @@ struct MixedLds {
 // staged in LDS (s_op: MixedLds<WCSP,KG>::OP_ELEMS elements), NS steps per
 // stage and barrier.  Called by whole blocks of 256 threads (or 512 running
 // two tiles side by side); ends with a barrier so the caller may reuse the LDS.
-template <int WCSP, int KG, bool ADJ>
+template <int WCSP, int KG, bool ADJ, int FORMS = 0>
 __device__ __forceinline__ void mixed_block(const float4* __restrict__ tgt, int ntgt, bool self,
                                             const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
                                             const uint4* __restrict__ bop, int nst, const CoefPack& cp,
@@ -449,58 +488,67 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ tgt, int 
         }
     };
 
-    float4 yv = make_float4(0.f, 0.f, 0.f, 0.f), nv = yv;
+    // The source of step st this lane holds is st * 8 + src_lane < nst * 8 =
+    // nt_pad: a wave-uniform byte offset st * 128 plus this lane's fixed one,
+    // read through buffer resources so that a step needs no address arithmetic.
+    const __amdgpu_buffer_rsrc_t mid_rsrc = buffer_rsrc(mid, nt_pad);
+    const __amdgpu_buffer_rsrc_t nrm_rsrc = buffer_rsrc(nrm, nt_pad);
+    const uint32_t src_off = (uint32_t)src_lane * (uint32_t)sizeof(float4);
+    float4 yv = make_float4(0.f, 0.f, 0.f, 0.f), nv = yv, yn = yv, nn = yv;
     if (st0 < st1) {
-        const int j = min(st0 * 8 + src_lane, nt_pad - 1);
-        yv = mid[j];
-        nv = nrm[j];
+        yv = load_xyz(mid_rsrc, src_off, st0 * 128);
+        nv = load_xyz(nrm_rsrc, src_off, st0 * 128);
         issue_stage(st0, 0);
     }
+
+    // Step st (slot s of the stage in buffer buf) on the source (yc, nc); the
+    // next step's source is loaded into (yx, nx).
+    auto step = [&](int st, int s, int buf, const float4 yc, const float4 nc, float4& yx, float4& nx) {
+        if (st + 1 < st1) {
+            yx = load_xyz(mid_rsrc, src_off, (st + 1) * 128);
+            nx = load_xyz(nrm_rsrc, src_off, (st + 1) * 128);
+        }
+        BdrGeo geo[4];
+        const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
+        if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yc, nc, 8 * st + 4 * g, i_mask, nt, geo);
+        else mixed_geo_dpp<ADJ, false>(x, ni, yc, nc, 8 * st + 4 * g, i_mask, nt, geo);
+        // chunk by chunk: the MFMAs of chunk q run while chunk q + 1 is generated
+        static_for<KG>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            half8_t b_hi[NCB], b_lo[NCB];
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const int o = (q * NCB + cb) * 2;
+                b_hi[cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o) * 64 + lane]);
+                b_lo[cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o + 1) * 64 + lane]);
+            }
+            uint32_t ahq[4], alq[4];
+            mixed_vals_q<ADJ, false, chunk_form(FORMS, q)>(geo, cp.c[q], ahq, alq);
+            const half8_t a_hi = __builtin_bit_cast(half8_t, make_uint4(ahq[0], ahq[1], ahq[2], ahq[3]));
+            const half8_t a_lo = __builtin_bit_cast(half8_t, make_uint4(alq[0], alq[1], alq[2], alq[3]));
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_hi[cb], acc[q][cb], 0, 0, 0);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_lo[cb], acc[q][cb], 0, 0, 0);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+                acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo, b_hi[cb], acc[q][cb], 0, 0, 0);
+        });
+    };
+
+    // Two steps per stage: the first on (yv, nv) loading (yn, nn), the second
+    // the other way round, so the source registers alternate without copies.
+    // Only the last stage can hold a single step.
+    static_assert(NS == 2, "the step loop alternates two source registers");
     int buf = 0;
     for (int sb = st0; sb < st1; sb += NS, buf ^= 1) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (sb + NS < st1) issue_stage(sb + NS, buf ^ 1);
-        const int ns = min(NS, st1 - sb);
-        for (int s = 0; s < ns; ++s) {
-            const int st = sb + s;
-            float4 yn = yv, nn = nv;
-            if (st + 1 < st1) {
-                const int j = min((st + 1) * 8 + src_lane, nt_pad - 1);
-                yn = mid[j];
-                nn = nrm[j];
-            }
-            BdrGeo geo[4];
-            const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
-            if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
-            else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
-            // chunk by chunk: the MFMAs of chunk q run while chunk q + 1 is generated
-#pragma unroll
-            for (int q = 0; q < KG; ++q) {
-                half8_t b_hi[NCB], b_lo[NCB];
-#pragma unroll
-                for (int cb = 0; cb < NCB; ++cb) {
-                    const int o = (q * NCB + cb) * 2;
-                    b_hi[cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o) * 64 + lane]);
-                    b_lo[cb] = __builtin_bit_cast(half8_t, s_op[(buf * TILES + s * OPS + o + 1) * 64 + lane]);
-                }
-                uint32_t ahq[4], alq[4];
-                mixed_vals_q<ADJ>(geo, cp.c[q], ahq, alq);
-                const half8_t a_hi = __builtin_bit_cast(half8_t, make_uint4(ahq[0], ahq[1], ahq[2], ahq[3]));
-                const half8_t a_lo = __builtin_bit_cast(half8_t, make_uint4(alq[0], alq[1], alq[2], alq[3]));
-#pragma unroll
-                for (int cb = 0; cb < NCB; ++cb)
-                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_hi[cb], acc[q][cb], 0, 0, 0);
-#pragma unroll
-                for (int cb = 0; cb < NCB; ++cb)
-                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_hi, b_lo[cb], acc[q][cb], 0, 0, 0);
-#pragma unroll
-                for (int cb = 0; cb < NCB; ++cb)
-                    acc[q][cb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_lo, b_hi[cb], acc[q][cb], 0, 0, 0);
-            }
-            yv = yn;
-            nv = nn;
-        }
+        step(sb, 0, buf, yv, nv, yn, nn);
+        if (sb + 1 < st1) step(sb + 1, 1, buf, yn, nn, yv, nv);
     }
     if (out) {
 #pragma unroll
@@ -521,7 +569,7 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ tgt, int 
 template <int KG, bool ADJ>
 constexpr int mixed_min_waves() { return (!ADJ && KG <= 2) ? BAEMM_MIX_MINB : 3; }
 
-template <int WCSP, int KG, bool ADJ>
+template <int WCSP, int KG, bool ADJ, int FORMS = 0>
 __global__ __launch_bounds__(256, (mixed_min_waves<KG, ADJ>())) void bdr_mixed_kernel(
     const float4* __restrict__ tgt, int ntgt, int self,
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
@@ -533,8 +581,8 @@ __global__ __launch_bounds__(256, (mixed_min_waves<KG, ADJ>())) void bdr_mixed_k
     __shared__ uint4 s_op[L::OP_ELEMS];
     const int st0 = blockIdx.y * steps_per_seg;
     const int st1 = min(st0 + steps_per_seg, nst);
-    mixed_block<WCSP, KG, ADJ>(tgt, ntgt, self != 0, mid, nrm, nt, bop, nst, cp, part + (int64_t)blockIdx.y * seg_stride,
-                               ldp, q0, blockIdx.x, st0, st1, s_op);
+    mixed_block<WCSP, KG, ADJ, FORMS>(tgt, ntgt, self != 0, mid, nrm, nt, bop, nst, cp,
+                                      part + (int64_t)blockIdx.y * seg_stride, ldp, q0, blockIdx.x, st0, st1, s_op);
 }
 
 // ---------------------------------------------------------------------------

@@ -223,6 +223,41 @@ def test_wave_specialised_mixed_kernel_is_identical(sphere3, W, wcs, kappa, c2, 
         H.SetPrecision("fp32")
 
 
+# single-layer coefficients whose real or imaginary part is zero: the
+# single-wave kernel leaves those parts out of its arithmetic, per chunk for
+# one or two chunks per launch, for four chunks when they share the form
+@pytest.mark.parametrize("W,wcs,kappa,c1", [
+    (32, 32, [5.0], [2j]), (32, 32, [5.0], [1.5]), (16, 16, [4.0], [0.0]), (64, 64, [6.0], [-4j]),
+    (64, 32, [8.0, 16.0], [1.0, -16j]), (64, 32, [8.0, 16.0], [0.7 - 0.2j, 3j]), (32, 16, [3.0, 5.0], [0.0, 2.0]),
+    (64, 16, [2.0, 3.0, 4.0, 5.0], [1j, 2j, 0.5j, -1j]), (64, 16, [2.0, 3.0, 4.0, 5.0], [1.0, 2.0, 0.5, 3.0]),
+    (64, 16, [2.0, 3.0, 4.0, 5.0], [1j, 2.0, 0.5j, 3.0])])
+def test_coefficient_forms_are_identical(sphere3, W, wcs, kappa, c1):
+    """The single-wave kernel's coefficient forms give the values of the
+    general form: bit-identical to the wave-specialised kernel (which always
+    takes the general form) over the same source segments, and near the FP64
+    oracle."""
+    m, H = sphere3
+    rng = np.random.default_rng(W + len(kappa))
+    B = _rand(rng, m.nv, W)
+    coeff = np.array([x for c in c1 for x in (0.25, c, 1.0, 0.0)])
+    ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
+    H.SetPrecision("mixed")
+    try:
+        H.SetSegments(5)
+        outs = []
+        for variant in (0, 1):
+            H.SetMixedKernel(variant)
+            C = np.zeros((m.nv, W), np.complex128)
+            H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)
+            outs.append(C)
+        assert np.array_equal(outs[0], outs[1])
+        assert relerr(outs[0], ref) < NEAR
+    finally:
+        H.SetSegments(0)
+        H.SetMixedKernel(2)
+        H.SetPrecision("fp32")
+
+
 def test_kernel_variants_at_default_segments():
     """At 20,480 triangles each MFMA kernel variant balances its automatic
     segment count against its own occupancy (the single-wave kernel takes more
