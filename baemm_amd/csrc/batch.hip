@@ -469,6 +469,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
         Handle& h = *es[k].h;
         p0_invalidate(h);   // the batched apply leaves no P0 buffers
         BdrPlan& pl = prep[k].pl;
+        pl.no_tn = true;   // the batched launch's bodies take the general adjoint form
         bdr_setup(h, pl);
         if (pl.kernel_on && pl.rounds >= tune.own_rounds) {
             own[k] = 1;

@@ -379,8 +379,25 @@ static int coef_form(const ChunkCoef& c) { return c.c1r == 0.0f ? 1 : (c.c1i == 
 // The single-wave kernel for KG chunks with coefficients cp (NULL: the
 // general form).  Without the adjoint term, one or two chunks take their own
 // coefficient forms, four chunks a form they share; else the general form.
-static MixedFn mixed_fn(bool adj, int WCSP, int KG, const CoefPack* cp = nullptr)
+static MixedFn mixed_fn(bool adj, int WCSP, int KG, const CoefPack* cp = nullptr, bool tn = false)
 {
+    if (tn) {
+        // the target-normal form: the general coefficient form, or every chunk
+        // without Re c1 (the Dirichlet-to-Neumann rows (1/2, -i kappa, 0, 1))
+        bool re0 = cp != nullptr;
+        for (int q = 0; cp && q < KG; ++q) re0 = re0 && coef_form(cp->c[q]) == 1;
+#define MIXTN(W_, K_, F_)                                                                      \
+    if (WCSP == W_ && KG == K_)                                                                \
+        return re0 ? &bdr_mixed_kernel<W_, K_, false, F_, true> : &bdr_mixed_kernel<W_, K_, false, 0, true>;
+        MIXTN(16, 1, 1)
+        MIXTN(16, 2, 4)
+        MIXTN(16, 4, 40)
+        MIXTN(32, 1, 1)
+        MIXTN(32, 2, 4)
+        MIXTN(64, 1, 1)
+#undef MIXTN
+        throw ArgError{"internal: no mixed kernel instance"};
+    }
     int forms = 0;
     if (cp && !adj) {
         if (KG <= 2) {
@@ -655,13 +672,16 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
         // with one scale s for all chunks, carried by the source normals
         // (Q' = c1/c2 s / r - s (z.n)/r^3 + i kappa s (z.n)/r^2); the epilogue
         // multiplies c2 / s back.
+        // (with pl.tn: c3 in place of c2, the target normals in place of the
+        // source normals)
+        const int cn = pl.tn ? 3 : 2;
         double amax = 0.0;
         for (int q = 0; q < p.K; ++q) {
-            const std::complex<double> c2(p.c[4 * q + 2]);
+            const std::complex<double> c2(p.c[4 * q + cn]);
             const double rho = std::abs(std::complex<double>(p.c[4 * q + 1]) / c2);
             const double kr = (double)std::fabs(p.kappa[q]) * rmin;
             amax = std::max(amax, rho / rmin + (1.0 + kr) / ((double)rmin * rmin));
-            pl.rot[q] = cfloat(p.c[4 * q + 2]);
+            pl.rot[q] = cfloat(p.c[4 * q + cn]);
         }
         int e = 0;
         std::frexp(std::max(amax, 1e-30), &e);
@@ -734,10 +754,23 @@ void bdr_setup(Handle& h, BdrPlan& pl)
     pl.ldp = 2 * pl.ldb;
     pl.adj = p.any(3);
     // The MFMA kernels' non-adjoint form divides each chunk by its c2; chunks
-    // without a double layer take the general (adjoint) form.
-    if (pl.mixed)
-        for (int q = 0; q < p.K; ++q)
+    // without a double layer take the general (adjoint) form, unless every
+    // chunk has the adjoint double layer and no double layer: the single-wave
+    // kernel then takes the non-adjoint form divided by c3, on the target
+    // normals (the adjoint double layer has the double layer's values with the
+    // target normal in place of the source normal).
+    pl.tn = false;
+    if (pl.mixed) {
+        bool tn = !pl.f32 && !pl.no_tn && h.mixed_kernel != 1;
+        for (int q = 0; q < p.K; ++q) {
             if (p.c[4 * q + 2] == cfloat(0.0f, 0.0f)) pl.adj = true;
+            tn = tn && p.c[4 * q + 2] == cfloat(0.0f, 0.0f) && p.c[4 * q + 3] != cfloat(0.0f, 0.0f);
+        }
+        if (tn) {
+            pl.tn = true;
+            pl.adj = false;
+        }
+    }
     if (!pl.kernel_on) return;
     if (!pl.mixed) {
         const int tiles = (int)(h.nt_pad / kTile);
@@ -759,7 +792,8 @@ void bdr_setup(Handle& h, BdrPlan& pl)
         const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
         res = resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512);
     } else {
-        res = resident_blocks((const void*)mixed_fn(pl.adj, pl.WCSP, pick_kg(pl.WCSP, (int)p.K, true)), 256);
+        res = resident_blocks((const void*)mixed_fn(pl.adj, pl.WCSP, pick_kg(pl.WCSP, (int)p.K, true), nullptr, pl.tn),
+                              256);
         // The single-wave kernel runs 2-3 % faster with about four times the
         // work-groups (finer last rounds): up to 16 segments where the rule
         // above gives fewer, unless the plan must keep its partials small.
@@ -866,7 +900,7 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
                                (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
                                pl.ldp, q0, pl.steps, h.step_skip);
         else
-            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg, &cp), dim3(tiles, pl.S), dim3(256), 0, s, h.d_mid,
+            hipLaunchKernelGGL(mixed_fn(pl.adj, pl.WCSP, kg, &cp, pl.tn), dim3(tiles, pl.S), dim3(256), 0, s, h.d_mid,
                                (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
                                pl.ldp, q0, pl.steps, h.step_skip);
         BAEMM_HIP(hipGetLastError());

@@ -105,12 +105,17 @@ struct StageDesc {
 
 struct BdrPlan {
     bool kernel_on = false, mass_on = false, mixed = false, adj = false;
+    // mixed, single-wave kernel: every chunk has c2 = 0 and c3 != 0 (single layer
+    // and adjoint double layer), taken as the non-adjoint form divided by c3 with
+    // the target normals in place of the source normals (adj is then false)
+    bool tn = false;
+    bool no_tn = false;          // set before bdr_setup: the general form for such operators
     bool f32 = false;            // MFMA kernels in FP32 (BAEMM_PRECISION_FP32_MFMA); mixed is set too
     int WCSP = 0, ldb = 0, ldp = 0;
     int nst = 0;                 // 8-source steps (mixed) of the padded source range
     std::vector<float> sA;       // per-chunk power-of-two scale of A (mixed)
-    std::vector<cfloat> rot;     // per-chunk factor the kernel values are divided by (mixed: c2, or 1)
-    const float4* nrm = nullptr; // source normals of the pair kernel (scaled in the non-adjoint mixed form)
+    std::vector<cfloat> rot;     // per-chunk factor the kernel values are divided by (mixed: c2, c3 with tn, or 1)
+    const float4* nrm = nullptr; // normals of the pair kernel (scaled in the non-adjoint mixed form; target normals with tn)
     float2* b = nullptr;         // staged sources b = Av B
     const uint4* bop = nullptr;  // split MFMA operand (mixed)
     float* unscale = nullptr;    // per-column 1 / (A scale x b scale) of the mixed path

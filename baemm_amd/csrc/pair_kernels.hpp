@@ -319,10 +319,15 @@ __device__ __forceinline__ void mixed_vals(const BdrGeo (&geo)[4], const CoefPac
 // The DPP source registers come from memory loads; s_nop 4 gives the wait
 // states a DPP read needs after a VALU or EXEC write, in case the compiler
 // moved a value just before.
-template <bool ADJ, bool MASK>
+// TN (no adjoint term): the normal products come from the target normal ni,
+// dn = -(z . n_i), so that gj and hj hold the adjoint double layer's
+// -(z . n_i)/r^2 and /r^3 (the gi, hi of the general form, same rounding); the
+// source normals nv are not read.
+template <bool ADJ, bool MASK, bool TN = false>
 __device__ __forceinline__ void mixed_geo_dpp(const float4 x, const float4 ni, const float4 yv, const float4 nv,
                                               int jbase, int i, int nt, BdrGeo (&geo)[4])
 {
+    static_assert(!(ADJ && TN), "the target-normal form has no separate adjoint term");
     float z0[4], z1[4], z2[4], dn[4];
     asm("s_nop 4\n\t"
         "v_sub_f32_dpp %0, %12, %15 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
@@ -340,6 +345,10 @@ __device__ __forceinline__ void mixed_geo_dpp(const float4 x, const float4 ni, c
         : "=&v"(z0[0]), "=&v"(z1[0]), "=&v"(z2[0]), "=&v"(z0[1]), "=&v"(z1[1]), "=&v"(z2[1]), "=&v"(z0[2]),
           "=&v"(z1[2]), "=&v"(z2[2]), "=&v"(z0[3]), "=&v"(z1[3]), "=&v"(z2[3])
         : "v"(yv.x), "v"(yv.y), "v"(yv.z), "v"(x.x), "v"(x.y), "v"(x.z));
+    if constexpr (TN) {
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) dn[jj] = -fmaf(z0[jj], ni.x, fmaf(z1[jj], ni.y, z2[jj] * ni.z));
+    } else {
     // z . n_j = (z2 n_z + z1 n_y) + z0 n_x, in the order of bdr_geo's fmaf chain
     asm("s_nop 4\n\t"
         "v_mul_f32_dpp %0, %6, %9 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
@@ -357,6 +366,7 @@ __device__ __forceinline__ void mixed_geo_dpp(const float4 x, const float4 ni, c
         : "=&v"(dn[0]), "=&v"(dn[1]), "=&v"(dn[2]), "=&v"(dn[3])
         : "v"(nv.x), "v"(nv.y), "v"(nv.z), "v"(z0[0]), "v"(z1[0]), "v"(z2[0]), "v"(z0[1]), "v"(z1[1]),
           "v"(z2[1]), "v"(z0[2]), "v"(z1[2]), "v"(z2[2]), "v"(z0[3]), "v"(z1[3]), "v"(z2[3]));
+    }
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
         const int j = jbase + jj;
@@ -439,7 +449,7 @@ struct MixedLds {
 // staged in LDS (s_op: MixedLds<WCSP,KG>::OP_ELEMS elements), NS steps per
 // stage and barrier.  Called by whole blocks of 256 threads (or 512 running
 // two tiles side by side); ends with a barrier so the caller may reuse the LDS.
-template <int WCSP, int KG, bool ADJ, int FORMS = 0>
+template <int WCSP, int KG, bool ADJ, int FORMS = 0, bool TN = false>
 __device__ __forceinline__ void mixed_block(const float4* __restrict__ tgt, int ntgt, bool self,
                                             const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
                                             const uint4* __restrict__ bop, int nst, const CoefPack& cp,
@@ -456,7 +466,7 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ tgt, int 
     const int i = ibase + (lane & 31);
     const float4 x = tgt[min(i, ntgt - 1)];
     float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
-    if constexpr (ADJ)
+    if constexpr (ADJ || TN)
         if (self) ni = nrm[i];
 
     float16_t acc[KG][NCB];
@@ -510,8 +520,8 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ tgt, int 
         }
         BdrGeo geo[4];
         const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
-        if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yc, nc, 8 * st + 4 * g, i_mask, nt, geo);
-        else mixed_geo_dpp<ADJ, false>(x, ni, yc, nc, 8 * st + 4 * g, i_mask, nt, geo);
+        if (masked) mixed_geo_dpp<ADJ, true, TN>(x, ni, yc, nc, 8 * st + 4 * g, i_mask, nt, geo);
+        else mixed_geo_dpp<ADJ, false, TN>(x, ni, yc, nc, 8 * st + 4 * g, i_mask, nt, geo);
         // chunk by chunk: the MFMAs of chunk q run while chunk q + 1 is generated
         static_for<KG>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
@@ -569,7 +579,7 @@ __device__ __forceinline__ void mixed_block(const float4* __restrict__ tgt, int 
 template <int KG, bool ADJ>
 constexpr int mixed_min_waves() { return (!ADJ && KG <= 2) ? BAEMM_MIX_MINB : 3; }
 
-template <int WCSP, int KG, bool ADJ, int FORMS = 0>
+template <int WCSP, int KG, bool ADJ, int FORMS = 0, bool TN = false>
 __global__ __launch_bounds__(256, (mixed_min_waves<KG, ADJ>())) void bdr_mixed_kernel(
     const float4* __restrict__ tgt, int ntgt, int self,
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
@@ -581,7 +591,7 @@ __global__ __launch_bounds__(256, (mixed_min_waves<KG, ADJ>())) void bdr_mixed_k
     __shared__ uint4 s_op[L::OP_ELEMS];
     const int st0 = blockIdx.y * steps_per_seg;
     const int st1 = min(st0 + steps_per_seg, nst);
-    mixed_block<WCSP, KG, ADJ, FORMS>(tgt, ntgt, self != 0, mid, nrm, nt, bop, nst, cp,
+    mixed_block<WCSP, KG, ADJ, FORMS, TN>(tgt, ntgt, self != 0, mid, nrm, nt, bop, nst, cp,
                                       part + (int64_t)blockIdx.y * seg_stride, ldp, q0, blockIdx.x, st0, st1, s_op);
 }
 

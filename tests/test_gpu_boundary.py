@@ -258,6 +258,51 @@ def test_coefficient_forms_are_identical(sphere3, W, wcs, kappa, c1):
         H.SetPrecision("fp32")
 
 
+# single layer + adjoint double layer without the double layer (the
+# Dirichlet-to-Neumann rows (1/2, -i kappa, 0, 1) and a general row): the
+# single-wave kernel takes the non-adjoint form on the target normals; the last
+# case has a double layer in one chunk and keeps the general form
+@pytest.mark.parametrize("W,wcs,kappa,rows", [
+    (16, 16, [4.0], [(0.5, -4j, 0, 1)]),
+    (64, 32, [8.0, 16.0], [(0.5, -8j, 0, 1), (0.5, -16j, 0, 1)]),
+    (32, 32, [5.0], [(0.25, 1 - 0.5j, 0, 0.7 + 0.2j)]),
+    (64, 16, [2.0, 3.0, 4.0, 5.0], [(0.5, -2j, 0, 1), (0.5, -3j, 0, 1), (0.5, -4j, 0, 1), (0.5, -5j, 0, 1)]),
+    (64, 64, [6.0], [(0.5, -6j, 0, 1)]),
+    (64, 32, [8.0, 16.0], [(0.5, -8j, 0, 1), (0.5, -16j, 1, 1)])])
+def test_adjoint_only_operators_on_target_normals(sphere3, W, wcs, kappa, rows):
+    """Operators whose chunks all have the adjoint double layer and no double
+    layer: the automatic choice (single-wave kernel, target-normal form), the
+    wave-specialised kernel (general adjoint form) and the batched launch
+    (general form) all stay near the FP64 oracle and within rounding of each
+    other."""
+    from baemm_amd import ApplyBoundaryOperatorsBatch
+
+    m, H = sphere3
+    rng = np.random.default_rng(W + 7 * len(kappa))
+    B = _rand(rng, m.nv, W)
+    coeff = np.array([c for r in rows for c in r], np.complex128)
+    ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
+    H.SetPrecision("mixed")
+    try:
+        outs = []
+        for variant in (2, 1):
+            H.SetMixedKernel(variant)
+            C = np.zeros((m.nv, W), np.complex128)
+            H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)
+            outs.append(C)
+        H.SetMixedKernel(2)
+        H.LoadBoundaryOperators_PL(kappa, coeff, W, wcs)
+        batched = ApplyBoundaryOperatorsBatch([H], [B])[0]
+        e = [relerr(x, ref) for x in outs + [batched]]
+        print(f"W={W} K={len(kappa)}: single-wave {e[0]:.2e}, wave-specialised {e[1]:.2e}, batched {e[2]:.2e}")
+        assert max(e) < NEAR
+        assert relerr(outs[0], outs[1]) < 1e-6 and relerr(batched, outs[1]) < 1e-6
+    finally:
+        H.UnloadBoundaryOperators_PL()
+        H.SetMixedKernel(2)
+        H.SetPrecision("fp32")
+
+
 def test_kernel_variants_at_default_segments():
     """At 20,480 triangles each MFMA kernel variant balances its automatic
     segment count against its own occupancy (the single-wave kernel takes more
