@@ -123,11 +123,15 @@ int baemm_set_segments(baemm_handle h, int segments);
  * 0 = one wave generates its kernel values and issues their MFMAs;
  * 1 = wave-specialised blocks (generation waves feed MFMA waves through LDS);
  * 2 = automatic, the default: the single-wave kernel, the wave-specialised one
- * for operators with the adjoint double layer.
+ * for operators with both the double and the adjoint double layer.
  * With the same source segments (baemm_set_segments) the variants give
  * identical results; their automatic segment counts differ (results within
- * rounding).  The FP32 MFMA precision always uses the wave-specialised
- * kernel; the potential (baemm_apply_near_field) follows the same choice. */
+ * rounding).  Operators whose chunks all have the adjoint double layer and no
+ * double layer (the Dirichlet-to-Neumann rows) take a form of their own on the
+ * single-wave kernel (0 or 2), the general adjoint form on the
+ * wave-specialised one and in the batched launch: those agree within rounding.
+ * The FP32 MFMA precision always uses the wave-specialised kernel; the
+ * potential (baemm_apply_near_field) follows the same choice. */
 int baemm_set_mixed_kernel(baemm_handle h, int variant);
 
 /* gmres_max_iter / gmres_max_restarts public members, MemberVariables.hpp:64-65. */
