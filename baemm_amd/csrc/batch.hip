@@ -51,15 +51,16 @@ struct BatchItem {
 };
 
 // ----------------------------------------------------------------- MFMA path
-// Variants: (WCSP, KG) in {(16,1),(16,2),(32,1),(32,2),(64,1)} x ADJ.
-static int mixed_variant(int WCSP, int KG, bool adj)
+// Variants: (WCSP, KG) in {(16,1),(16,2),(32,1),(32,2),(64,1)} x (non-adjoint
+// form, general adjoint form, non-adjoint form on the target normals).
+static int mixed_variant(int WCSP, int KG, bool adj, bool tn)
 {
     int v = -1;
     if (WCSP == 16) v = KG == 1 ? 0 : (KG == 2 ? 1 : -1);
     if (WCSP == 32) v = KG == 1 ? 2 : (KG == 2 ? 3 : -1);
     if (WCSP == 64 && KG == 1) v = 4;
     if (v < 0) throw ArgError{"internal: no batched mixed variant"};
-    return 2 * v + (adj ? 1 : 0);
+    return tn ? 10 + v : 2 * v + (adj ? 1 : 0);
 }
 
 // Largest LDS need over the variants (all have 4 column blocks or fewer).
@@ -90,11 +91,12 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
         if (in.skip && *in.skip) continue;   // uniform over the block: s_item was read by every thread
         const ChunkCoef* cf = coefs + in.coef_off + w.q0;
         float* out = in.part + (int64_t)w.seg * in.seg_stride;
-#define BAEMM_MIXB(ID, WCSP_, KG_, ADJ_)                                                                          \
+#define BAEMM_MIXB(ID, WCSP_, KG_, ADJ_, TN_)                                                                     \
     case ID: {                                                                                                   \
         CoefPack cp;                                                                                             \
         for (int g = 0; g < KG_; ++g) cp.c[g] = cf[g];                                                           \
-        mixed_ws_block<WCSP_, KG_, ADJ_>(in.mid, in.nt_pad, true, in.mid, in.nrm, in.nt, in.bop, in.nst, cp, out,  \
+        mixed_ws_block<WCSP_, KG_, ADJ_, false, TN_>(in.mid, in.nt_pad, true, in.mid, in.nrm, in.nt, in.bop,      \
+                                                     in.nst, cp, out,                                             \
                                          in.ldp, w.q0, w.tile,                                                     \
                                          w.s0, w.s1, s_op, s_a);                                                 \
         break;                                                                                                   \
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
         // single-wave block body on two target tiles at
         // once (waves 0-3 tile 2t, waves 4-7 tile 2t+1): every wave generates
         // kernel values, which these generation-bound items need.
-#define BAEMM_MIXB2(ID, KG_, ADJ_)                                                                               \
+#define BAEMM_MIXB2(ID, KG_, ADJ_, TN_)                                                                          \
     case ID: {                                                                                                   \
         CoefPack cp;                                                                                             \
         for (int g = 0; g < KG_; ++g) cp.c[g] = cf[g];                                                           \
@@ -111,22 +113,27 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_batch_kernel(
         const int half = (threadIdx.x >> 6) >= 4;                                                                \
         const int t2 = 2 * w.tile + half;                                                                        \
         const bool real_tile = t2 < in.nt_pad / kMixTile;                                                        \
-        mixed_block<16, KG_, ADJ_>(in.mid, in.nt_pad, true, in.mid, in.nrm, in.nt, in.bop, in.nst, cp,              \
+        mixed_block<16, KG_, ADJ_, 0, TN_>(in.mid, in.nt_pad, true, in.mid, in.nrm, in.nt, in.bop, in.nst, cp,     \
                                    real_tile ? out : nullptr, in.ldp,                                             \
                                    w.q0, real_tile ? t2 : 2 * w.tile, w.s0, w.s1, s_op + half * LL::OP_ELEMS);     \
         break;                                                                                                   \
     }
         switch (w.variant) {
-            BAEMM_MIXB2(0, 1, false)
-            BAEMM_MIXB(1, 16, 1, true)
-            BAEMM_MIXB(2, 16, 2, false)
-            BAEMM_MIXB(3, 16, 2, true)
-            BAEMM_MIXB(4, 32, 1, false)
-            BAEMM_MIXB(5, 32, 1, true)
-            BAEMM_MIXB(6, 32, 2, false)
-            BAEMM_MIXB(7, 32, 2, true)
-            BAEMM_MIXB(8, 64, 1, false)
-            BAEMM_MIXB(9, 64, 1, true)
+            BAEMM_MIXB2(0, 1, false, false)
+            BAEMM_MIXB(1, 16, 1, true, false)
+            BAEMM_MIXB(2, 16, 2, false, false)
+            BAEMM_MIXB(3, 16, 2, true, false)
+            BAEMM_MIXB(4, 32, 1, false, false)
+            BAEMM_MIXB(5, 32, 1, true, false)
+            BAEMM_MIXB(6, 32, 2, false, false)
+            BAEMM_MIXB(7, 32, 2, true, false)
+            BAEMM_MIXB(8, 64, 1, false, false)
+            BAEMM_MIXB(9, 64, 1, true, false)
+            BAEMM_MIXB2(10, 1, false, true)
+            BAEMM_MIXB(11, 16, 2, false, true)
+            BAEMM_MIXB(12, 32, 1, false, true)
+            BAEMM_MIXB(13, 32, 2, false, true)
+            BAEMM_MIXB(14, 64, 1, false, true)
             default: break;
         }
 #undef BAEMM_MIXB
@@ -469,7 +476,6 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
         Handle& h = *es[k].h;
         p0_invalidate(h);   // the batched apply leaves no P0 buffers
         BdrPlan& pl = prep[k].pl;
-        pl.no_tn = true;   // the batched launch's bodies take the general adjoint form
         bdr_setup(h, pl);
         if (pl.kernel_on && pl.rounds >= tune.own_rounds) {
             own[k] = 1;
@@ -540,7 +546,7 @@ void apply_boundary_batch(std::vector<BatchEntry>& es, BatchWorkspace& bw, hipSt
                     const int lim = std::min({maxkg, (int)p.K - q0, 2});
                     while (kg * 2 <= lim) kg *= 2;
                 }
-                const int var = mixed ? mixed_variant(pl.WCSP, kg, pl.adj) : plain_variant(pl.WCSP, pl.adj);
+                const int var = mixed ? mixed_variant(pl.WCSP, kg, pl.adj, pl.tn) : plain_variant(pl.WCSP, pl.adj);
                 // single 16-column chunks: items of two 128-target tiles
                 const bool two = mixed && pl.WCSP == 16 && kg == 1 && !pl.adj;
                 const int item_tiles = two ? (tiles + 1) / 2 : tiles;

@@ -356,10 +356,11 @@ using MixedFn = void (*)(const float4*, int, int, const float4*, const float4*, 
 using WsFn = void (*)(const float4*, int, int, const float4*, const float4*, int, const uint4*, int, CoefPack, float*,
                       int64_t, int, int, int, const int*);
 
-static WsFn mixed_ws_fn(bool adj, int WCSP, int KG, bool f32)
+static WsFn mixed_ws_fn(bool adj, int WCSP, int KG, bool f32, bool tn = false)
 {
 #define MIXWS(W_, K_)                                                                                    \
     if (WCSP == W_ && KG == K_) {                                                                        \
+        if (tn) return f32 ? &bdr_mixed_ws_kernel<W_, K_, false, true, true> : &bdr_mixed_ws_kernel<W_, K_, false, false, true>; \
         if (f32) return adj ? &bdr_mixed_ws_kernel<W_, K_, true, true> : &bdr_mixed_ws_kernel<W_, K_, false, true>; \
         return adj ? &bdr_mixed_ws_kernel<W_, K_, true> : &bdr_mixed_ws_kernel<W_, K_, false>;           \
     }
@@ -651,7 +652,7 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
         pl.sA.assign(p.K, 1.0f);
         pl.rot.assign(p.K, cfloat(1.0f, 0.0f));
         if (!pl.adj)
-            for (int q = 0; q < p.K; ++q) pl.rot[q] = cfloat(p.c[4 * q + 2]);
+            for (int q = 0; q < p.K; ++q) pl.rot[q] = cfloat(p.c[4 * q + (pl.tn ? 3 : 2)]);
         const int NCB = 2 * pl.WCSP / 32;
         const int64_t nbop = (int64_t)p.K * pl.nst * NCB * 64;
         uint4* bop = static_cast<uint4*>(pl.ws.take(h.ws_bop, sizeof(uint4) * nbop * 2));
@@ -755,13 +756,13 @@ void bdr_setup(Handle& h, BdrPlan& pl)
     pl.adj = p.any(3);
     // The MFMA kernels' non-adjoint form divides each chunk by its c2; chunks
     // without a double layer take the general (adjoint) form, unless every
-    // chunk has the adjoint double layer and no double layer: the single-wave
-    // kernel then takes the non-adjoint form divided by c3, on the target
+    // chunk has the adjoint double layer and no double layer: the MFMA
+    // kernels then take the non-adjoint form divided by c3, on the target
     // normals (the adjoint double layer has the double layer's values with the
     // target normal in place of the source normal).
     pl.tn = false;
     if (pl.mixed) {
-        bool tn = !pl.f32 && !pl.no_tn && h.mixed_kernel != 1;
+        bool tn = true;
         for (int q = 0; q < p.K; ++q) {
             if (p.c[4 * q + 2] == cfloat(0.0f, 0.0f)) pl.adj = true;
             tn = tn && p.c[4 * q + 2] == cfloat(0.0f, 0.0f) && p.c[4 * q + 3] != cfloat(0.0f, 0.0f);
@@ -790,7 +791,7 @@ void bdr_setup(Handle& h, BdrPlan& pl)
     int res;
     if (mixed_uses_ws(h, pl)) {
         const int kg0 = std::min(pick_kg(pl.WCSP, (int)p.K, true), 2);
-        res = resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32), 512);
+        res = resident_blocks((const void*)mixed_ws_fn(pl.adj, pl.WCSP, kg0, pl.f32, pl.tn), 512);
     } else {
         res = resident_blocks((const void*)mixed_fn(pl.adj, pl.WCSP, pick_kg(pl.WCSP, (int)p.K, true), nullptr, pl.tn),
                               256);
@@ -896,7 +897,7 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
         if (ws) kg = std::min(kg, 2);
         const CoefPack cp = make_pack(p, q0, kg, pl.sA.data(), pl.rot.data());
         if (ws)
-            hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32), dim3(tiles, pl.S), dim3(512), 0, s, h.d_mid,
+            hipLaunchKernelGGL(mixed_ws_fn(pl.adj, pl.WCSP, kg, pl.f32, pl.tn), dim3(tiles, pl.S), dim3(512), 0, s, h.d_mid,
                                (int)h.nt_pad, 1, h.d_mid, pl.nrm, (int)h.nt, pl.bop, nst, cp, part, pl.ea.seg_stride,
                                pl.ldp, q0, pl.steps, h.step_skip);
         else

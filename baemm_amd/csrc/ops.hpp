@@ -105,11 +105,10 @@ struct StageDesc {
 
 struct BdrPlan {
     bool kernel_on = false, mass_on = false, mixed = false, adj = false;
-    // mixed, single-wave kernel: every chunk has c2 = 0 and c3 != 0 (single layer
-    // and adjoint double layer), taken as the non-adjoint form divided by c3 with
-    // the target normals in place of the source normals (adj is then false)
+    // mixed: every chunk has c2 = 0 and c3 != 0 (single layer and adjoint double
+    // layer), taken as the non-adjoint form divided by c3 with the target normals
+    // in place of the source normals (adj is then false)
     bool tn = false;
-    bool no_tn = false;          // set before bdr_setup: the general form for such operators
     bool f32 = false;            // MFMA kernels in FP32 (BAEMM_PRECISION_FP32_MFMA); mixed is set too
     int WCSP = 0, ldb = 0, ldp = 0;
     int nst = 0;                 // 8-source steps (mixed) of the padded source range

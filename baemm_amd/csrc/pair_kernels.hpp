@@ -628,7 +628,7 @@ struct WsLds {
 // points (potential: no pair left out, no target normals).  s_op, s_a: LDS of
 // at least WsLds<WCSP,KG> B_ELEMS, A_ELEMS elements.  Ends with every wave past
 // a common barrier, so the caller may reuse the LDS.
-template <int WCSP, int KG, bool ADJ, bool F32 = false>
+template <int WCSP, int KG, bool ADJ, bool F32 = false, bool TN = false>
 __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ tgt, int ntgt, bool self,
                                                const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
                                                const uint4* __restrict__ bop, int nst, const CoefPack& cp,
@@ -668,7 +668,7 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ tgt, i
     if (producer) {
         const float4 x = tgt[min(i, ntgt - 1)];
         float4 ni = make_float4(0.f, 0.f, 0.f, 0.f);
-        if constexpr (ADJ)
+        if constexpr (ADJ || TN)
             if (self) ni = nrm[i];
         // the source of each step this lane holds for mixed_geo_dpp
         const int src_lane = 4 * g + (lane & 3);
@@ -692,8 +692,8 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ tgt, i
                 uint32_t ah[KG][4], al[KG][4];
                 const bool masked = (unsigned)(st - st_diag) < 4u || st >= st_tail;
                 BdrGeo geo[4];
-                if (masked) mixed_geo_dpp<ADJ, true>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
-                else mixed_geo_dpp<ADJ, false>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
+                if (masked) mixed_geo_dpp<ADJ, true, TN>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
+                else mixed_geo_dpp<ADJ, false, TN>(x, ni, yv, nv, 8 * st + 4 * g, i_mask, nt, geo);
                 mixed_vals<KG, ADJ, F32>(geo, cp, ah, al);
 #pragma unroll
                 for (int q = 0; q < KG; ++q) {
@@ -806,7 +806,7 @@ __device__ __forceinline__ void mixed_ws_block(const float4* __restrict__ tgt, i
     }
 }
 
-template <int WCSP, int KG, bool ADJ, bool F32 = false>
+template <int WCSP, int KG, bool ADJ, bool F32 = false, bool TN = false>
 __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
     const float4* __restrict__ tgt, int ntgt, int self,
     const float4* __restrict__ mid, const float4* __restrict__ nrm, int nt,
@@ -819,9 +819,9 @@ __global__ __launch_bounds__(512, 4) void bdr_mixed_ws_kernel(
     __shared__ uint4 s_a[L::A_ELEMS];
     const int st0 = blockIdx.y * steps_per_seg;
     const int st1 = min(st0 + steps_per_seg, nst);
-    mixed_ws_block<WCSP, KG, ADJ, F32>(tgt, ntgt, self != 0, mid, nrm, nt, bop, nst, cp,
-                                       part + (int64_t)blockIdx.y * seg_stride, ldp, q0,
-                                  blockIdx.x, st0, st1, s_op, s_a);
+    mixed_ws_block<WCSP, KG, ADJ, F32, TN>(tgt, ntgt, self != 0, mid, nrm, nt, bop, nst, cp,
+                                           part + (int64_t)blockIdx.y * seg_stride, ldp, q0, blockIdx.x, st0, st1,
+                                           s_op, s_a);
 }
 
 }  // namespace baemm

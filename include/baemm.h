@@ -127,9 +127,8 @@ int baemm_set_segments(baemm_handle h, int segments);
  * With the same source segments (baemm_set_segments) the variants give
  * identical results; their automatic segment counts differ (results within
  * rounding).  Operators whose chunks all have the adjoint double layer and no
- * double layer (the Dirichlet-to-Neumann rows) take a form of their own on the
- * single-wave kernel (0 or 2), the general adjoint form on the
- * wave-specialised one and in the batched launch: those agree within rounding.
+ * double layer (the Dirichlet-to-Neumann rows) take a faster form of their own
+ * on every MFMA kernel.
  * The FP32 MFMA precision always uses the wave-specialised kernel; the
  * potential (baemm_apply_near_field) follows the same choice. */
 int baemm_set_mixed_kernel(baemm_handle h, int variant);

@@ -259,9 +259,9 @@ def test_coefficient_forms_are_identical(sphere3, W, wcs, kappa, c1):
 
 
 # single layer + adjoint double layer without the double layer (the
-# Dirichlet-to-Neumann rows (1/2, -i kappa, 0, 1) and a general row): the
-# single-wave kernel takes the non-adjoint form on the target normals; the last
-# case has a double layer in one chunk and keeps the general form
+# Dirichlet-to-Neumann rows (1/2, -i kappa, 0, 1) and a general row): the MFMA
+# kernels take the non-adjoint form on the target normals; the last case has a
+# double layer in one chunk and keeps the general form
 @pytest.mark.parametrize("W,wcs,kappa,rows", [
     (16, 16, [4.0], [(0.5, -4j, 0, 1)]),
     (64, 32, [8.0, 16.0], [(0.5, -8j, 0, 1), (0.5, -16j, 0, 1)]),
@@ -271,10 +271,10 @@ def test_coefficient_forms_are_identical(sphere3, W, wcs, kappa, c1):
     (64, 32, [8.0, 16.0], [(0.5, -8j, 0, 1), (0.5, -16j, 1, 1)])])
 def test_adjoint_only_operators_on_target_normals(sphere3, W, wcs, kappa, rows):
     """Operators whose chunks all have the adjoint double layer and no double
-    layer: the automatic choice (single-wave kernel, target-normal form), the
-    wave-specialised kernel (general adjoint form) and the batched launch
-    (general form) all stay near the FP64 oracle and within rounding of each
-    other."""
+    layer: over the same source segments the single-wave and wave-specialised
+    kernels give identical values; with automatic segments every MFMA path (both
+    kernels, the batched launch, the FP32 MFMA contraction) stays near the FP64
+    oracle."""
     from baemm_amd import ApplyBoundaryOperatorsBatch
 
     m, H = sphere3
@@ -282,23 +282,36 @@ def test_adjoint_only_operators_on_target_normals(sphere3, W, wcs, kappa, rows):
     B = _rand(rng, m.nv, W)
     coeff = np.array([c for r in rows for c in r], np.complex128)
     ref = oracle.apply_boundary(m, 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
+
+    def apply():
+        C = np.zeros((m.nv, W), np.complex128)
+        H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)
+        return C
+
     H.SetPrecision("mixed")
     try:
-        outs = []
+        H.SetSegments(5)
+        fixed = []
+        for variant in (0, 1, 2):
+            H.SetMixedKernel(variant)
+            fixed.append(apply())
+        assert np.array_equal(fixed[0], fixed[1]) and np.array_equal(fixed[1], fixed[2])
+        H.SetSegments(0)
+        outs = {}
         for variant in (2, 1):
             H.SetMixedKernel(variant)
-            C = np.zeros((m.nv, W), np.complex128)
-            H.ApplyBoundaryOperators_PL(1.0, B, 0.0, C, kappa, coeff, W, wcs)
-            outs.append(C)
+            outs[variant] = apply()
         H.SetMixedKernel(2)
         H.LoadBoundaryOperators_PL(kappa, coeff, W, wcs)
-        batched = ApplyBoundaryOperatorsBatch([H], [B])[0]
-        e = [relerr(x, ref) for x in outs + [batched]]
-        print(f"W={W} K={len(kappa)}: single-wave {e[0]:.2e}, wave-specialised {e[1]:.2e}, batched {e[2]:.2e}")
-        assert max(e) < NEAR
-        assert relerr(outs[0], outs[1]) < 1e-6 and relerr(batched, outs[1]) < 1e-6
-    finally:
+        outs["batched"] = ApplyBoundaryOperatorsBatch([H], [B])[0]
         H.UnloadBoundaryOperators_PL()
+        H.SetPrecision("fp32_mfma")
+        outs["fp32_mfma"] = apply()
+        e = {k: relerr(v, ref) for k, v in outs.items()}
+        print(f"W={W} K={len(kappa)}:", {k: f"{v:.2e}" for k, v in e.items()})
+        assert max(e.values()) < NEAR
+    finally:
+        H.SetSegments(0)
         H.SetMixedKernel(2)
         H.SetPrecision("fp32")
 
