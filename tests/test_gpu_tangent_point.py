@@ -130,3 +130,37 @@ def test_gauss_newton_with_tangent_point_metric_matches_oracle():
                   H.SolverStats())
             assert ok, mode
             assert e_du < 2e-5 and e_x < 1e-5, mode
+
+
+def test_properties_at_scale():
+    """At 20,480 triangles (beyond the oracle's dense sizes), through
+    size-independent properties: the metric is symmetric (u . M v = v . M u)
+    and has the translations in its kernel, and the energy gradient matches a
+    central finite difference of the device energy along a random smooth
+    direction."""
+    V, T = perturbed_sphere(5, seed=31)
+    q, p = 6.0, 12.0
+    rng = np.random.default_rng(32)
+    u = rng.standard_normal(V.shape)
+    v = rng.standard_normal(V.shape)
+    with Helmholtz_HIP(V, T, MEAS) as H:
+        Mu = H.TangentPointMetric(u, q, p)
+        Mv = H.TangentPointMetric(v, q, p)
+        t = np.zeros_like(V)
+        t[:, 0] = 1.0
+        Mt = H.TangentPointMetric(t, q, p)
+        E, G = H.TangentPointEnergy(q, p)
+    sym = abs((u * Mv).sum() - (v * Mu).sum()) / np.sqrt((u * Mu).sum() * (v * Mv).sum())
+    trans = np.linalg.norm(Mt) / (np.linalg.norm(Mu) / np.linalg.norm(u) * np.linalg.norm(t))
+    # a smooth direction: a rotation-free stretch of the surface
+    d = np.stack([V[:, 0] * V[:, 1], V[:, 2] ** 2, np.sin(V[:, 0])], axis=1)
+    h = 5e-4
+    with Helmholtz_HIP(V + h * d, T, MEAS) as Hp, Helmholtz_HIP(V - h * d, T, MEAS) as Hm:
+        Ep, _ = Hp.TangentPointEnergy(q, p, gradient=False)
+        Em, _ = Hm.TangentPointEnergy(q, p, gradient=False)
+    fd = (Ep - Em) / (2 * h)
+    dg = (G * d).sum()
+    print(f"20,480 triangles: E {E:.6e}, symmetry {sym:.2e}, translation {trans:.2e}, "
+          f"FD {fd:.6e} vs gradient {dg:.6e} ({abs(fd - dg) / abs(dg):.2e})")
+    assert sym < 1e-8 and trans < 1e-6
+    assert abs(fd - dg) < 3e-4 * abs(dg)
