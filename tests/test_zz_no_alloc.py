@@ -55,19 +55,30 @@ def _apply_both(H, B, kappa, coeff, W, wcs, noalloc_first):
     return outs[::-1] if noalloc_first else outs
 
 
+def _rows(form, kappa):
+    """Coefficient rows per chunk: all four operators, single + double layer
+    (c1 with a zero real or imaginary part), or the Dirichlet-to-Neumann rows
+    (adjoint double layer without double layer)."""
+    if form == "full":
+        return np.array([0.5 + 0.25j, 1 - 0.5j, -0.75 + 1j, 0.3 - 0.9j] * len(kappa), np.complex128)
+    if form == "sl_dl":
+        return np.array([c for k in kappa for c in (0.5, -1j * k, 1.0, 0.0)], np.complex128)
+    return np.array([c for k in kappa for c in (0.5, -1j * k, 0.0, 1.0)], np.complex128)
+
+
+@pytest.mark.parametrize("form", ["full", "sl_dl", "dtn"])
 @pytest.mark.parametrize("mode", ["fp32", "mixed", "fp32_mfma"])
-def test_noalloc_apply_matches_allocating_apply(mode):
+def test_noalloc_apply_matches_allocating_apply(mode, form):
     rng = np.random.default_rng(31)
     cases = [(icosphere(4), [4.0, 9.0], 32, 16), (icosphere(3), [2.5], 8, 8)]   # second call: smaller input
     for first, ((V, T), kappa, W, wcs) in enumerate(cases):
-        K = W // wcs
-        coeff = np.array([0.5 + 0.25j, 1 - 0.5j, -0.75 + 1j, 0.3 - 0.9j] * K, np.complex128)
+        coeff = _rows(form, kappa)
         B = rng.standard_normal((V.shape[0], W)) + 1j * rng.standard_normal((V.shape[0], W))
         with Helmholtz_HIP(V, T) as H:
             H.SetPrecision(mode)
             ref_dev, got = _apply_both(H, B, kappa, coeff, W, wcs, noalloc_first=first == 0)
         ref = oracle.apply_boundary(oracle.Mesh(V, T), 1.0, B, 0.0, np.zeros_like(B), kappa, coeff, W, wcs)
-        print(f"{mode} {T.shape[0]} triangles W={W}: no-alloc vs existing {relerr(got, ref_dev):.2e}, "
+        print(f"{mode} {form} {T.shape[0]} triangles W={W}: no-alloc vs existing {relerr(got, ref_dev):.2e}, "
               f"vs FP64 oracle {relerr(got, ref):.2e}")
         # the same plan, kernels and segment split: identical results
         assert np.array_equal(got, ref_dev)
