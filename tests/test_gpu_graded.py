@@ -6,6 +6,8 @@ Bunny_00086632T, committed as fixtures): largest/smallest triangle area from
 vertex rows:
   * single layer only (c2 = 0: the general form of the MFMA kernels),
   * the potential operator (1/2, -i kappa, 1, 0) (the form divided by c2),
+  * the Dirichlet-to-Neumann operator (1/2, -i kappa, 0, 1) (the form divided
+    by c3 on the target normals),
   * all four operators with complex coefficients.
 The contract is 1e-5 relative (Frobenius); each precision is also held near
 the error it achieves, so that a tenfold loss of accuracy fails."""
@@ -27,6 +29,7 @@ NAMES = ["Spot_00005856T", "Bob_00010688T", "Blub_00014208T", "Bunny_00086632T"]
 COEFFS = {
     "SL": np.array([0, 1, 0, 0] * 3, np.complex128),
     "SL+DL": np.array(sum(([0.5, -1j * k, 1, 0] for k in KAPPA), []), np.complex128),
+    "DtN": np.array(sum(([0.5, -1j * k, 0, 1] for k in KAPPA), []), np.complex128),
     "all": np.array([0.5 + 0.25j, 1 - 0.5j, -0.75 + 1j, 0.3 - 0.9j] * 3, np.complex128),
 }
 # achieved-error guards per precision (about 5x the largest error measured on these meshes)
