@@ -165,6 +165,88 @@ class _HelmholtzBase:
     def SingleLayerDiagonal(self):
         return self._tri_data()[3]
 
+    def VertexCoordinates(self):
+        """Helmholtz_OpenCL.hpp:333-336: the vertex coordinates in FP32 (V x 3)."""
+        v = np.zeros((self.VertexCount(), 3), np.float32)
+        self._check(self._L.baemm_get_coordinates(self._h, _ptr(v), None))
+        return v
+
+    def TriangleCoordinates(self):
+        """Helmholtz_OpenCL.hpp:338-341: the three vertices of each triangle as
+        x, y, z, 0 (T x 12, FP32; Initialize.hpp:82-93)."""
+        t = np.zeros((self.SimplexCount(), 12), np.float32)
+        self._check(self._L.baemm_get_coordinates(self._h, None, _ptr(t)))
+        return t
+
+    def DeviceInfo(self):
+        """Helmholtz_OpenCL.hpp:343-391: the handle's GPU as text."""
+        buf = ctypes.create_string_buffer(8192)
+        self._check(self._L.baemm_device_info(self._h, buf, len(buf)))
+        return buf.value.decode()
+
+    def ClassName(self):
+        return "Helmholtz_HIP"
+
+    _FLAG_NAMES = ("Re_mass_matrix", "Im_mass_matrix", "Re_single_layer", "Im_single_layer",
+                   "Re_double_layer", "Im_double_layer", "Re_adjdbl_layer", "Im_adjdbl_layer")
+
+    def PrintBooleans(self):
+        """LoadParameters.hpp:85-95: prints the flags of the loaded coefficient
+        columns (nonzero real / imaginary parts over all chunks) and returns them."""
+        f = (ctypes.c_int * 8)()
+        self._check(self._L.baemm_get_flags(self._h, f))
+        flags = {n: bool(f[j]) for j, n in enumerate(self._FLAG_NAMES)}
+        for n, v in flags.items():
+            print(f"{n} = {'true' if v else 'false'}")
+        return flags
+
+    # ---- P0 buffers (InputOutput.hpp:5-89) ----
+    def _p0_width(self, which):
+        data = ctypes.POINTER(ctypes.c_float)()
+        ld = ctypes.c_int64()
+        self._check(self._L.baemm_p0_buffer(self._h, int(which), ctypes.byref(data), ctypes.byref(ld)))
+        return data, ld.value
+
+    def _p0_write(self, fn, which):
+        _, W = self._p0_width(which)
+        out = np.zeros((self.SimplexCount(), W), np.complex64)
+        self._check(fn(self._h, _ptr(out), W))
+        return out
+
+    def _p0_read(self, fn, data):
+        a = np.ascontiguousarray(data, dtype=np.complex64)
+        if a.ndim != 2 or a.shape[0] != self.SimplexCount():
+            raise ValueError("input must be simplex_count x wave_count")
+        self._check(fn(self._h, _ptr(a), a.shape[1], a.shape[1]))
+
+    def WriteB(self):
+        """B = Av B_in of the last operator apply, simplex_count x wave_count."""
+        return self._p0_write(self._L.baemm_write_b, 0)
+
+    def WriteC(self):
+        """C = (K + D) b of the last boundary-operator apply (before AvT)."""
+        return self._p0_write(self._L.baemm_write_c, 1)
+
+    def ReadB(self, data):
+        self._p0_read(self._L.baemm_read_b, data)
+
+    def ReadC(self, data):
+        self._p0_read(self._L.baemm_read_c, data)
+
+    def _p0_elem(self, which, i, k):
+        data, W = self._p0_width(which)
+        if not (0 <= int(i) < self.SimplexCount() and 0 <= int(k) < W):
+            raise IndexError("P0 buffer index out of range")
+        o = 2 * (W * int(i) + int(k))
+        return complex(data[o], data[o + 1])
+
+    def B(self, i, k):
+        """B(i, k) of the P0 buffer B (a value; the C++ header returns a reference)."""
+        return self._p0_elem(0, i, k)
+
+    def C(self, i, k):
+        return self._p0_elem(1, i, k)
+
     def MassMatrix(self):
         """CSR (indptr, indices, data) of the P1 mass matrix."""
         nnz = int(self._L.baemm_mass_nnz(self._h))
@@ -191,13 +273,26 @@ class _HelmholtzBase:
         K = -(-int(wave_count) // int(wave_chunk_size))
         if k.size < K or c.size < 4 * K:
             raise ValueError("kappa_list needs K entries and coeff_list 4K entries")
+        self._wave_count_set = None
         self._check(self._L.baemm_load_parameters(self._h, _ptr(k), _ptr(c), int(wave_count), int(wave_chunk_size),
                                                   float(factor)))
 
     def GetWaveCount(self):
+        if getattr(self, "_wave_count_set", None) is not None:
+            return self._wave_count_set
         W = ctypes.c_int64()
         self._check(self._L.baemm_get_parameters(self._h, None, None, ctypes.byref(W), None, None))
         return W.value
+
+    def SetWaveCount(self, wave_count):
+        """GetSetters.hpp:40-45: the wave count GetWaveCount reports until the
+        next LoadParameters (which sets it, as the reference's does)."""
+        self._wave_count_set = int(wave_count)
+
+    def GetWaveChunkCount(self, wave_count):
+        """GetSetters.hpp:59-62."""
+        wcs = self.GetWaveChunkSize()
+        return -(-int(wave_count) // wcs)
 
     def GetWaveNumbers(self):
         K = ctypes.c_int64()
