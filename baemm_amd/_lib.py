@@ -108,6 +108,7 @@ SIGNATURES = {
     "baemm_tp_metric_apply": (None, [_P, _P, _P, _P]),
     "baemm_tp_metric_jacobi": (None, [_P, _P, _P, _P]),
     "baemm_tp_metric_apply_host": (_INT, [_P, _D, _D, _D, _P, _P]),
+    "baemm_tp_metric_jacobi_host": (_INT, [_P, _D, _D, _D, _P, _P]),
     "baemm_gmres_dev": (_INT, [_P, _P, _P, _D, _D, ctypes.POINTER(_I64), _P]),
     "baemm_gmres_batch": (_INT, [_INT, _P, _P, _P, _D, _D, _P, _P, _P]),
     "baemm_get_solver_stats": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_D),

@@ -16,6 +16,8 @@
 #include <complex>
 #include <cstdint>
 #include <mutex>
+#include <thread>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -139,6 +141,7 @@ struct GraphSlot {
 
 struct Handle {
     std::mutex mtx;
+    std::atomic<std::thread::id> owner{};   // thread in a guarded call on this handle
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t nv = 0, nt = 0, m = 0;

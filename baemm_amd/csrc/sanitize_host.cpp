@@ -139,6 +139,7 @@ void null_arguments()
     NULL_CALL(baemm_tp_energy);
     NULL_CALL(baemm_tp_metric_prepare);
     NULL_CALL(baemm_tp_metric_apply_host);
+    NULL_CALL(baemm_tp_metric_jacobi_host);
     NULL_CALL(baemm_gmres_dev);
     NULL_CALL(baemm_gmres_batch);
     NULL_CALL(baemm_get_solver_stats);

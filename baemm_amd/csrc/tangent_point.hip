@@ -500,20 +500,35 @@ void baemm_tp_metric_jacobi(void* user, const double* x, double* y, void* stream
     }
 }
 
-int baemm_tp_metric_apply_host(baemm_handle h, double q, double p, double scale, const double* x, double* y)
+// The metric (jacobi false) or its Jacobi preconditioner (true) on host arrays.
+static int tp_host(baemm_handle h, double q, double p, double scale, const double* x, double* y, bool jacobi)
 {
     return guard(h, [&](Handle& hh) {
         if (!x || !y) throw ArgError{"x and y must not be NULL"};
+        if (jacobi && !(scale != 0.0 && std::isfinite(scale))) throw ArgError{"scale must be finite and nonzero"};
         hipStream_t s = hh.stream;
         const size_t bytes = sizeof(double) * 3 * hh.nv;
         hh.io[4].reserve(bytes);
         hh.io[5].reserve(bytes);
         BAEMM_HIP(hipMemcpyAsync(hh.io[4].ptr, x, bytes, hipMemcpyHostToDevice, s));
-        tp_metric_dev(hh, q, p, scale, hh.io[4].as<double>(), hh.io[5].as<double>(), s);
+        if (jacobi)
+            tp_jacobi_dev(hh, q, p, scale, hh.io[4].as<double>(), hh.io[5].as<double>(), s);
+        else
+            tp_metric_dev(hh, q, p, scale, hh.io[4].as<double>(), hh.io[5].as<double>(), s);
         BAEMM_HIP(hipMemcpyAsync(y, hh.io[5].ptr, bytes, hipMemcpyDeviceToHost, s));
         BAEMM_HIP(hipStreamSynchronize(s));
         return BAEMM_OK;
     });
+}
+
+int baemm_tp_metric_apply_host(baemm_handle h, double q, double p, double scale, const double* x, double* y)
+{
+    return tp_host(h, q, p, scale, x, y, false);
+}
+
+int baemm_tp_metric_jacobi_host(baemm_handle h, double q, double p, double scale, const double* x, double* y)
+{
+    return tp_host(h, q, p, scale, x, y, true);
 }
 
 }  // extern "C"

@@ -318,6 +318,84 @@ int main()
         std::vector<double> res((size_t)nv * 3);
         for (size_t k = 0; k < res.size(); ++k) res[k] = c * Xg[k] + MZ[k];
         check("Gauss-Newton residual", rel(res, Bg), 1e-5);
+
+        // The Gauss-Newton example's caller side (Example_GaussNewton/main.cpp:301-416)
+        // with the tangent-point energy and metric in place of Repulsor's
+        BAEMM::TangentPoint_HIP tp(H, 6.0, 12.0);
+        const double E = tp.Value();
+        check("tangent-point energy positive", (E > 0.0 && std::isfinite(E)) ? 0.0 : 1.0, 0);
+        std::vector<double> D1((size_t)nv * 3), D2((size_t)nv * 3, 1.0), D3((size_t)nv * 3);
+        tp.Differential(1.0, 0.0, D1.data(), int64_t(3));
+        tp.Differential(2.0, 0.5, D2.data(), int64_t(3));
+        for (size_t k = 0; k < D3.size(); ++k) D3[k] = 2.0 * D1[k] + 0.5;
+        check("Differential alpha dE + beta B", rel(D2, D3), 1e-14);
+        std::vector<double> u((size_t)nv * 3), w((size_t)nv * 3), Mu((size_t)nv * 3), Mw((size_t)nv * 3);
+        for (size_t k = 0; k < u.size(); ++k) {
+            u[k] = std::sin(0.91 * k + 0.3);
+            w[k] = std::cos(1.37 * k);
+        }
+        tp.MultiplyMetric(1.0, u.data(), int64_t(3), 0.0, Mu.data(), int64_t(3), int64_t(3));
+        tp.MultiplyMetric(1.0, w.data(), int64_t(3), 0.0, Mw.data(), int64_t(3), int64_t(3));
+        double uMw = 0, wMu = 0, uMu = 0, wMw = 0;
+        for (size_t k = 0; k < u.size(); ++k) {
+            uMw += u[k] * Mw[k];
+            wMu += w[k] * Mu[k];
+            uMu += u[k] * Mu[k];
+            wMw += w[k] * Mw[k];
+        }
+        check("tangent-point metric symmetric", std::abs(uMw - wMu) / std::sqrt(uMu * wMw), 1e-8);
+        std::vector<double> P1((size_t)nv * 3), P2((size_t)nv * 3, 3.0), P3((size_t)nv * 3);
+        tp.MultiplyPreconditioner(1.0, u.data(), int64_t(3), 0.0, P1.data(), int64_t(3), int64_t(3));
+        tp.MultiplyPreconditioner(2.0, u.data(), int64_t(3), -1.0, P2.data(), int64_t(3), int64_t(3));
+        for (size_t k = 0; k < P3.size(); ++k) P3[k] = 2.0 * P1[k] - 3.0;
+        check("MultiplyPreconditioner alpha P X + beta Y", rel(P2, P3), 1e-14);
+        // regpar so that both terms of the operator count on smooth fields: a third of
+        // the data term's gain on a smooth deformation s of the surface
+        std::vector<double> sm((size_t)nv * 3), Ms((size_t)nv * 3);
+        for (int64_t v = 0; v < nv; ++v) {
+            sm[3 * v] = V[3 * v] * V[3 * v + 1];
+            sm[3 * v + 1] = V[3 * v + 2] * V[3 * v + 2];
+            sm[3 * v + 2] = std::sin(V[3 * v]);
+        }
+        tp.MultiplyMetric(1.0, sm.data(), int64_t(3), 0.0, Ms.data(), int64_t(3), int64_t(3));
+        std::vector<C64> DFu((size_t)m * Ws);
+        std::vector<double> Zu((size_t)nv * 3), MZu((size_t)nv * 3, 0.0);
+        H.Derivative_FF<4>(kap_s, Ks, dirs_s, wcs_s, sm.data(), DFu.data(), du_gn, BAEMM::WaveType::Plane, cg_tol,
+                           gmres_tol);
+        H.AdjointDerivative_FF<4>(kap_s, Ks, dirs_s, wcs_s, DFu.data(), Zu.data(), du_gn, BAEMM::WaveType::Plane,
+                                  cg_tol, gmres_tol);
+        H.MassMatrix().Dot<3>(1.0, Zu.data(), int64_t(3), 0.0, MZu.data(), int64_t(3));
+        double nd = 0, nm = 0;
+        for (size_t k = 0; k < sm.size(); ++k) {
+            nd += MZu[k] * MZu[k];
+            nm += Ms[k] * Ms[k];
+        }
+        const double regpar = 0.3 * std::sqrt(nd / nm);
+        auto Atp = [&](const double* X, double* Y) {
+            tp.MultiplyMetric(regpar, X, int64_t(3), 0.0, Y, int64_t(3), int64_t(3));
+        };
+        auto Ptp = [&](const double* X, double* Y) {
+            tp.MultiplyPreconditioner(1.0 / regpar, X, int64_t(3), 0.0, Y, int64_t(3), int64_t(3));
+        };
+        H.gmres_max_iter = 100;
+        H.gmres_max_restarts = 8;
+        std::vector<double> Bt((size_t)nv * 3, 0.0), Xt((size_t)nv * 3, 0.0);
+        tp.Differential(regpar, 0.0, Bt.data(), int64_t(3));
+        for (size_t k = 0; k < Bt.size(); ++k) Bt[k] += 0.1 * u[k];
+        const bool ok_tp = H.GaussNewtonSolve<4>(kap_s, Ks, dirs_s, wcs_s, Atp, Ptp, 1.0, Bt.data(), int64_t(3), 0.0,
+                                                 Xt.data(), int64_t(3), du_gn, BAEMM::WaveType::Plane, cg_tol,
+                                                 gmres_tol, 1e-8);
+        check("GaussNewtonSolve with the tangent-point metric reports success", ok_tp ? 0.0 : 1.0, 0);
+        std::vector<double> Zt((size_t)nv * 3), MZt((size_t)nv * 3, 0.0), AX((size_t)nv * 3);
+        std::vector<C64> DFXt((size_t)m * Ws);
+        H.Derivative_FF<4>(kap_s, Ks, dirs_s, wcs_s, Xt.data(), DFXt.data(), du_gn, BAEMM::WaveType::Plane, cg_tol,
+                           gmres_tol);
+        H.AdjointDerivative_FF<4>(kap_s, Ks, dirs_s, wcs_s, DFXt.data(), Zt.data(), du_gn, BAEMM::WaveType::Plane,
+                                  cg_tol, gmres_tol);
+        H.MassMatrix().Dot<3>(1.0, Zt.data(), int64_t(3), 0.0, MZt.data(), int64_t(3));
+        Atp(Xt.data(), AX.data());
+        for (size_t k = 0; k < AX.size(); ++k) AX[k] += MZt[k];
+        check("Gauss-Newton residual with the tangent-point metric", rel(AX, Bt), 1e-5);
     } catch (const std::exception& e) {
         std::printf("exception: %s\n", e.what());
         return 2;

@@ -674,6 +674,19 @@ class _SolverMixin:
         return y
 
 
+    def TangentPointPreconditioner(self, x, q=6.0, p=12.0, scale=1.0):
+        """x / (scale d_v) row by row, d the diagonal of the tangent-point metric
+        (baemm_tp_metric_jacobi_host)."""
+        xv = np.require(np.asarray(x, np.float64).reshape(-1, 3), np.float64, ["C"])
+        if xv.shape[0] != self.VertexCount():
+            raise ValueError("x must have vertex_count rows")
+        y = np.zeros_like(xv)
+        self._check(self._L.baemm_tp_metric_jacobi_host(self._h, float(q), float(p), float(scale),
+                                                        xv.ctypes.data_as(ctypes.c_void_p),
+                                                        y.ctypes.data_as(ctypes.c_void_p)))
+        return y
+
+
 class Helmholtz_HIP(_HelmholtzBase, _SolverMixin):
     """Drop-in for ``BAEMM::Helmholtz_OpenCL`` (Helmholtz_OpenCL.hpp:34-399)."""
 

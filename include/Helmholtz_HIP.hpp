@@ -123,6 +123,8 @@ void scalar(const C& z, double (&out)[2])
 
 }  // namespace detail
 
+class TangentPoint_HIP;
+
 class Helmholtz_HIP {
 public:
     using Int = int32_t;
@@ -188,6 +190,7 @@ public:
         if (st != BAEMM_OK) throw std::runtime_error("Helmholtz_HIP: construction failed");
         vertex_coords_ = V;
         triangles_ = T;
+        device_ = (int)device_num;
     }
 
     ~Helmholtz_HIP()
@@ -645,6 +648,8 @@ private:
     Int wave_count_set_ = -1;   // SetWaveCount's value until the next LoadParameters (-1: none)
     std::vector<double> vertex_coords_;
     std::vector<int64_t> triangles_;
+    int device_ = 0;
+    friend class TangentPoint_HIP;
     mutable std::vector<Real> tri_[4];
     mutable std::vector<float> vcoords_, tcoords_;
     mutable Complex p0_dummy_;
@@ -701,5 +706,113 @@ private:
 
 // Drop-in name of the reference class.
 using Helmholtz_OpenCL = Helmholtz_HIP;
+
+// The tangent-point energy and metric that the Gauss-Newton example takes from
+// Repulsor (Example_GaussNewton/main.cpp:301-346: TangentPointEnergy0,
+// TangentPointMetric0, not part of BAEMM), on the mesh of a Helmholtz_HIP,
+// with the call shapes the example uses, the mesh argument dropped:
+//   tpe.Differential(*M, alpha, beta, B, ldB)  ->  tp.Differential(alpha, beta, B, ldB)
+//   tpm.MultiplyMetric(*M, alpha, X, ldX, beta, Y, ldY, DIM)
+//                                             ->  tp.MultiplyMetric(alpha, X, ldX, beta, Y, ldY, DIM)
+//   tpm.MultiplyPreconditioner(...)           ->  tp.MultiplyPreconditioner(...)
+// Every pair of distinct triangles is summed exactly (Repulsor approximates far
+// pairs), and the preconditioner is the metric's diagonal.  DIM must be 3.
+// It keeps a handle of its own on the same mesh and device, so that its
+// methods may be called from the M and P callbacks of GaussNewtonSolve, which
+// run while the Helmholtz_HIP handle is busy with the solve.
+class TangentPoint_HIP {
+public:
+    using Int = Helmholtz_HIP::Int;
+
+    TangentPoint_HIP(const Helmholtz_HIP& H, double q, double p) : q_(q), p_(p)
+    {
+        const int st = baemm_create(H.vertex_coords_.data(), (int64_t)H.vertex_coords_.size() / 3,
+                                    H.triangles_.data(), (int64_t)H.triangles_.size() / 3, nullptr, 0, H.device_, &h_);
+        if (st != BAEMM_OK) {
+            detail::report(nullptr, st, "TangentPoint_HIP");
+            throw std::runtime_error("TangentPoint_HIP: construction failed");
+        }
+        detail::report(h_, baemm_tp_metric_prepare(h_, q_, p_), "TangentPoint_HIP");
+    }
+
+    ~TangentPoint_HIP()
+    {
+        if (h_) baemm_destroy(h_);
+    }
+
+    TangentPoint_HIP(const TangentPoint_HIP&) = delete;
+    TangentPoint_HIP& operator=(const TangentPoint_HIP&) = delete;
+
+    double Value() const
+    {
+        double E = 0.0;
+        detail::report(h_, baemm_tp_energy(h_, q_, p_, &E, nullptr), "Value");
+        return E;
+    }
+
+    // B = alpha dE/dx + beta B (vertex_count x 3, leading dimension ldB)
+    template <typename R, typename I>
+    void Differential(const R alpha, const R beta, R* B, const I ldB) const
+    {
+        const int64_t n = baemm_vertex_count(h_);
+        std::vector<double> g((size_t)(3 * n));
+        const int st = baemm_tp_energy(h_, q_, p_, nullptr, g.data());
+        if (st != BAEMM_OK) {
+            detail::report(h_, st, "Differential");
+            return;
+        }
+        for (int64_t v = 0; v < n; ++v)
+            for (int c = 0; c < 3; ++c) {
+                R& b = B[v * (int64_t)ldB + c];
+                b = (R)((double)alpha * g[3 * v + c] + (beta == R(0) ? 0.0 : (double)beta * (double)b));
+            }
+    }
+
+    // Y = alpha M X + beta Y
+    template <typename R, typename I>
+    void MultiplyMetric(const R alpha, const R* X, const I ldX, const R beta, R* Y, const I ldY, const I nrhs) const
+    {
+        apply(false, (double)alpha, X, ldX, beta, Y, ldY, nrhs, "MultiplyMetric");
+    }
+
+    // Y = alpha P X + beta Y, P X = X / d row by row (d: the diagonal of M)
+    template <typename R, typename I>
+    void MultiplyPreconditioner(const R alpha, const R* X, const I ldX, const R beta, R* Y, const I ldY,
+                                const I nrhs) const
+    {
+        apply(true, (double)alpha, X, ldX, beta, Y, ldY, nrhs, "MultiplyPreconditioner");
+    }
+
+private:
+    template <typename R, typename I>
+    void apply(bool jacobi, double alpha, const R* X, I ldX, R beta, R* Y, I ldY, I nrhs, const char* where) const
+    {
+        if (nrhs != 3) {
+            std::fprintf(stderr, "TangentPoint_HIP::%s: nrhs must be 3.\n", where);
+            return;
+        }
+        const int64_t n = baemm_vertex_count(h_);
+        std::vector<double> x((size_t)(3 * n)), y((size_t)(3 * n), 0.0);
+        for (int64_t v = 0; v < n; ++v)
+            for (int c = 0; c < 3; ++c) x[3 * v + c] = (double)X[v * (int64_t)ldX + c];
+        if (alpha != 0.0) {
+            // the metric scaled by alpha; the preconditioner's x / (scale d) with scale = 1 / alpha
+            const int st = jacobi ? baemm_tp_metric_jacobi_host(h_, q_, p_, 1.0 / alpha, x.data(), y.data())
+                                  : baemm_tp_metric_apply_host(h_, q_, p_, alpha, x.data(), y.data());
+            if (st != BAEMM_OK) {
+                detail::report(h_, st, where);
+                return;
+            }
+        }
+        for (int64_t v = 0; v < n; ++v)
+            for (int c = 0; c < 3; ++c) {
+                R& o = Y[v * (int64_t)ldY + c];
+                o = (R)(y[3 * v + c] + (beta == R(0) ? 0.0 : (double)beta * (double)o));
+            }
+    }
+
+    baemm_handle h_ = nullptr;
+    double q_, p_;
+};
 
 }  // namespace BAEMM

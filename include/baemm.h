@@ -464,6 +464,9 @@ int baemm_tp_metric_prepare(baemm_handle h, double q, double p);
 void baemm_tp_metric_apply(void* user, const double* x, double* y, void* stream);
 void baemm_tp_metric_jacobi(void* user, const double* x, double* y, void* stream);
 int baemm_tp_metric_apply_host(baemm_handle h, double q, double p, double scale, const double* x, double* y);
+/* The Jacobi preconditioner on host arrays: y = x / (scale * d_v) row by row
+ * (scale finite and nonzero). */
+int baemm_tp_metric_jacobi_host(baemm_handle h, double q, double p, double scale, const double* x, double* y);
 
 /* No reference counterpart: the GMRES of the solvers above on the loaded
  * boundary operator A: solves (P A) X = P B (P = mass inverse by PCG, cg_tol)

@@ -164,3 +164,44 @@ def test_properties_at_scale():
           f"FD {fd:.6e} vs gradient {dg:.6e} ({abs(fd - dg) / abs(dg):.2e})")
     assert sym < 1e-8 and trans < 1e-6
     assert abs(fd - dg) < 3e-4 * abs(dg)
+
+
+def test_host_callbacks_with_the_tangent_point_metric():
+    """GaussNewtonSolve with host callbacks that apply the tangent-point metric,
+    as Example_GaussNewton/main.cpp:324-346 composes them: a call on the
+    solving handle from inside its own callback is refused with a state error
+    (it would wait for the solve that runs the callback), and a second handle
+    on the same mesh serves the callbacks; the result matches the device
+    callbacks (baemm_tp_metric_apply / _jacobi)."""
+    from baemm_amd import BaemmError
+
+    V, T = perturbed_sphere(2, seed=14)
+    meas = fibonacci_sphere(48)
+    q, p, regpar = 6.0, 12.0, 2e-4
+    kappa, wcs = [2.0], 2
+    dirs = np.array([[0.0, 0.0, 1.0], [0.6, 0.0, 0.8]])
+    B = np.random.default_rng(33).standard_normal((V.shape[0], 3))
+    refused = []
+    with Helmholtz_HIP(V, T, meas) as H, Helmholtz_HIP(V, T, meas) as Hm:
+        H.SetGmresLimits(100, 8)
+
+        def M(x, y):
+            try:
+                H.TangentPointMetric(x, q, p, regpar)
+            except BaemmError as e:
+                refused.append(e.status)
+            y[:] = Hm.TangentPointMetric(x, q, p, regpar)
+
+        def P(x, y):
+            y[:] = Hm.TangentPointPreconditioner(x, q, p, regpar)
+
+        X = np.zeros_like(B)
+        ok, du = H.GaussNewtonSolve(kappa, 1, dirs, wcs, M, P, 1.0, B, 0.0, X, cg_tol=1e-12,
+                                    gmres_tol_inner=1e-10, gmres_tol_outer=1e-9)
+        Xd = np.zeros_like(B)
+        ok_d, _ = H.GaussNewtonSolveDevice(kappa, 1, dirs, wcs, 0.0, 0.0, 1.0, B, 0.0, Xd, du_dn=du, cg_tol=1e-12,
+                                           gmres_tol_inner=1e-10, gmres_tol_outer=1e-9, tangent_point=(q, p, regpar))
+    print(f"host callbacks vs device callbacks: {relerr(X, Xd):.2e}; refused calls: {len(refused)}")
+    assert ok and ok_d
+    assert refused and set(refused) == {3}
+    assert relerr(X, Xd) < 1e-6
