@@ -21,7 +21,10 @@
  *  - Host-pointer functions are synchronous.  *_dev functions take device
  *    pointers and a hipStream_t (passed as void*, NULL = the handle's stream)
  *    and return without synchronising.
- *  - A handle is bound to one GPU (device index) and serialises its calls.
+ *  - A handle is bound to one GPU (device index) and serialises its calls.  A
+ *    call on a handle from inside a callback that a call on the same handle is
+ *    running (the M and P of baemm_gauss_newton_solve) returns BAEMM_ERR_STATE
+ *    (message in baemm_last_error(NULL)); such callbacks use another handle.
  */
 #ifndef BAEMM_H
 #define BAEMM_H
