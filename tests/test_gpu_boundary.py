@@ -309,7 +309,8 @@ def test_adjoint_only_operators_on_target_normals(sphere3, W, wcs, kappa, rows):
         outs["fp32_mfma"] = apply()
         e = {k: relerr(v, ref) for k, v in outs.items()}
         print(f"W={W} K={len(kappa)}:", {k: f"{v:.2e}" for k, v in e.items()})
-        assert max(e.values()) < NEAR
+        # measured 5e-8 to 3.4e-7 on these cases
+        assert max(e.values()) < 1e-6
     finally:
         H.SetSegments(0)
         H.SetMixedKernel(2)
