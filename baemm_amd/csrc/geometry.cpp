@@ -182,7 +182,7 @@ void free_device(Handle& h)
                     h.d_M_ptr, h.d_M_idx, h.d_M_val, h.d_lumped_inv, h.d_meas};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_scale, &h.ws_misc,
+    DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_tsum, &h.ws_scale, &h.ws_misc,
                       &h.ws_pts, &h.ws_g, &h.ws_nrm, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
                       &h.sws.t, &h.sws.x, &h.sws.r, &h.sws.hc, &h.sws.a, &h.sws.b, &h.sws.c,
                       &h.sws.d, &h.sws.e, &h.sws.cgctr, &h.sws.sc, &h.sws.gm, &h.ws_p0, &h.ws_tp_grad,
@@ -200,7 +200,7 @@ size_t handle_device_bytes(const Handle& h)
     size_t g = (size_t)h.nt_pad * (2 * sizeof(float4)) + (size_t)h.nt * (2 * sizeof(float) + sizeof(int4)) +
                h.h_vt_ptr.size() * 4 + h.h_vt_idx.size() * 4 + h.h_M_ptr.size() * 4 + h.h_M_idx.size() * 4 +
                h.h_M_val.size() * 8 + h.h_lumped_inv.size() * 8 + (size_t)h.m_pad * sizeof(float4);
-    const DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_scale, &h.ws_misc,
+    const DevBuf* bufs[] = {&h.ws_B, &h.ws_out, &h.ws_b32, &h.ws_bop, &h.ws_part, &h.ws_tsum, &h.ws_scale, &h.ws_misc,
                             &h.ws_pts, &h.ws_g, &h.ws_nrm, &h.sws.red, &h.sws.scal, &h.sws.cg, &h.sws.V, &h.sws.w,
                             &h.sws.t, &h.sws.x, &h.sws.r, &h.sws.hc, &h.sws.a, &h.sws.b, &h.sws.c,
                             &h.sws.d, &h.sws.e, &h.sws.cgctr, &h.sws.sc, &h.sws.gm, &h.ws_p0, &h.ws_tp_grad,

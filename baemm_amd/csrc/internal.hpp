@@ -182,6 +182,7 @@ struct Handle {
     Params prm;
     // workspaces
     DevBuf ws_B, ws_out, ws_b32, ws_bop, ws_part, ws_scale, ws_misc, ws_pts, ws_g, ws_nrm;
+    DevBuf ws_tsum;                      // per-triangle segment sums of the partials (double2)
     DevBuf io[6];                        // device copies of the host arrays of the solver entry points
     DevBuf ws_p0;                        // P0 buffer read back for ReadB/WriteB/ReadC/WriteC
     // tangent-point energy and metric (tangent_point.hip)

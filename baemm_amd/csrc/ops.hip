@@ -520,6 +520,12 @@ static bool mixed_uses_ws(const Handle& h, const BdrPlan& pl)
     return pl.f32 || h.mixed_kernel == 1 || (h.mixed_kernel == 2 && pl.adj);
 }
 
+// Whether the vertex epilogue reads per-triangle segment sums formed
+// beforehand (tri_segsum_kernel): with four or more segments, where re-reading
+// every segment's partials for each of a triangle's three vertices costs more
+// than one extra pass.
+static bool use_tsum(const BdrPlan& pl) { return pl.kernel_on && pl.S >= 4; }
+
 // Runs a plain pair kernel over all chunk groups: part has ntgt_pad rows.
 static void run_plain(Handle& h, int op, bool adj, int WCSP, const float4* tx, const float4* tn, int ntgt,
                       int ntgt_pad, const float4* sy, const float4* sn, int nsrc, int nsrc_pad, const float2* b,
@@ -807,6 +813,25 @@ void bdr_setup(Handle& h, BdrPlan& pl)
     pl.rounds = (double)tiles * pl.S / res;
 }
 
+// Per triangle t and column k of chunk q: the sum over the segments s (in
+// ascending order, FP64) of part_s[t][q][k], real and imaginary parts, at
+// tsum[t * ldb + q * WCSP + k % wcs]: the sums the vertex epilogue forms.
+__global__ void tri_segsum_kernel(int nt, int W, int wcs, int WCSP, int ldb, int S, int64_t seg_stride, int ldp,
+                                  const float* __restrict__ part, double2* __restrict__ tsum)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nt * W) return;
+    const int t = (int)(idx / W), k = (int)(idx % W);
+    const int q = k / wcs, kk = k % wcs;
+    const float* pp = part + (int64_t)t * ldp + q * 2 * WCSP + kk;
+    double pr = 0.0, pi = 0.0;
+    for (int s = 0; s < S; ++s) {
+        pr += (double)pp[(int64_t)s * seg_stride];
+        pi += (double)pp[(int64_t)s * seg_stride + WCSP];
+    }
+    tsum[(int64_t)t * ldb + q * WCSP + kk] = make_double2(pr, pi);
+}
+
 size_t bdr_workspace_bytes(const Handle& h, const BdrPlan& pl, bool need_rmin)
 {
     if (!pl.kernel_on) return 0;
@@ -823,6 +848,7 @@ size_t bdr_workspace_bytes(const Handle& h, const BdrPlan& pl, bool need_rmin)
         }
     }
     n += al(sizeof(float) * (size_t)pl.S * h.nt_pad * pl.ldp);              // segment partials
+    if (use_tsum(pl)) n += al(sizeof(double2) * h.nt_pad * pl.ldb);         // their sums per triangle
     return n;
 }
 
@@ -913,7 +939,20 @@ void bdr_pairs(Handle& h, BdrPlan& pl, hipStream_t s)
 void bdr_end(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
 {
     if (!pl.kernel_on && !pl.mass_on) return;
-    bdr_epilogue(h, pl.ea, a.Btype, a.Ctype, s);
+    EpiArgs& ea = pl.ea;
+    ea.tsum = nullptr;
+    if (use_tsum(pl)) {
+        // the segment sums once per triangle, then the vertex gather reads them
+        // (three vertices per triangle) instead of every segment's partials
+        const int nt = (int)h.nt;
+        double2* ts = static_cast<double2*>(pl.ws.take(h.ws_tsum, sizeof(double2) * h.nt_pad * pl.ldb));
+        const int64_t total = (int64_t)nt * ea.W;
+        hipLaunchKernelGGL(tri_segsum_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nt, ea.W,
+                           ea.wcs, ea.WCSP, ea.ldb, ea.S, ea.seg_stride, ea.ldp, ea.part, ts);
+        BAEMM_HIP(hipGetLastError());
+        ea.tsum = ts;
+    }
+    bdr_epilogue(h, ea, a.Btype, a.Ctype, s);
 }
 
 void apply_boundary_dev(Handle& h, const ApplyArgs& a, hipStream_t s, char* scratch, size_t scratch_bytes)

@@ -51,7 +51,7 @@ void apply_prec(Handle& h, const double2* in, double2* out, double cg_tol, hipSt
 // re-captured when any of it changes; solves that reload the same operator
 // (every DtN solve at the same wave numbers, for instance) replay it.
 struct GraphKeyHead {
-    const void* ptr[16];
+    const void* ptr[20];
     int64_t val[12];
     double tol, gm_tol;
 };
@@ -63,8 +63,8 @@ std::vector<char> graph_key(Handle& h, int j, double cg_tol, double gm_tol)
     const Workspace& ws = h.sws;
     const void* p[] = {ws.V.ptr, ws.w.ptr, ws.t.ptr, ws.hc.ptr, ws.sc.ptr, ws.red.ptr, ws.cg.ptr, ws.scal.ptr,
                        ws.cgctr.ptr, h.ws_b32.ptr, h.ws_bop.ptr, h.ws_part.ptr, h.ws_scale.ptr, h.ws_misc.ptr,
-                       h.ws_nrm.ptr, ws.gm.ptr};
-    static_assert(sizeof(p) / sizeof(p[0]) <= 16, "key size");
+                       h.ws_nrm.ptr, ws.gm.ptr, h.ws_tsum.ptr};
+    static_assert(sizeof(p) / sizeof(p[0]) <= 20, "key size");
     for (size_t i = 0; i < sizeof(p) / sizeof(p[0]); ++i) k.ptr[i] = p[i];
     int rmin_bits = 0;
     std::memcpy(&rmin_bits, &h.rmin, sizeof(float));

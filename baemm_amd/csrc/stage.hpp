@@ -143,7 +143,16 @@ __device__ __forceinline__ void epi_vertex(const EpiArgs& a, int v, int lane, in
                     t[u] = a.vt_idx[min(e + u, e1 - 1)];
                     pr[u] = pi[u] = 0.0;
                 }
-                for (int s0 = 0; s0 < a.S; s0 += EPI_SEG) {
+                if (a.tsum) {
+                    // the same sums, formed once per triangle beforehand
+#pragma unroll
+                    for (int u = 0; u < EPI_TRI; ++u) {
+                        const double2 ts = a.tsum[(int64_t)t[u] * a.ldb + pc];
+                        pr[u] = ts.x;
+                        pi[u] = ts.y;
+                    }
+                }
+                for (int s0 = 0; s0 < (a.tsum ? 0 : a.S); s0 += EPI_SEG) {
                     float xr[EPI_TRI][EPI_SEG], xi[EPI_TRI][EPI_SEG];
 #pragma unroll
                     for (int u = 0; u < EPI_TRI; ++u) {
