@@ -272,8 +272,10 @@ def run_potential_leg(repeats=3):
     sphere of radius 1.5 (Fibonacci lattice) around the 81,920-triangle
     icosphere, the same 64 vertex-value columns and kappa {8, 16}.  Kernel time
     (HIP events around the pair kernels) and host-path wall time per apply in
-    each precision; the MFMA modes run the wave-specialised boundary kernel with
-    the points as targets."""
+    each precision (host arrays: the wall time includes the PCIe transfers of
+    B, the points and the 65,536 x 64 result); mixed precision runs the
+    automatic MFMA kernel choice (the single-wave kernel for this operator) and
+    FP32 MFMA the wave-specialised kernel, both with the points as targets."""
     from baemm_amd import Helmholtz_HIP
     from baemm_amd.meshes import fibonacci_sphere
 
