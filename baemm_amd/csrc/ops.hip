@@ -520,8 +520,8 @@ static bool mixed_uses_ws(const Handle& h, const BdrPlan& pl)
     return pl.f32 || h.mixed_kernel == 1 || (h.mixed_kernel == 2 && pl.adj);
 }
 
-// Whether the vertex epilogue reads per-triangle segment sums formed
-// beforehand (tri_segsum_kernel): with four or more segments, where re-reading
+// Whether the vertex epilogue reads per-triangle values formed
+// beforehand (tri_epilogue_kernel): with four or more segments, where re-reading
 // every segment's partials for each of a triangle's three vertices costs more
 // than one extra pass.
 static bool use_tsum(const BdrPlan& pl) { return pl.kernel_on && pl.S >= 4; }
@@ -814,22 +814,26 @@ void bdr_setup(Handle& h, BdrPlan& pl)
 }
 
 // Per triangle t and column k of chunk q: the sum over the segments s (in
-// ascending order, FP64) of part_s[t][q][k], real and imaginary parts, at
-// tsum[t * ldb + q * WCSP + k % wcs]: the sums the vertex epilogue forms.
-__global__ void tri_segsum_kernel(int nt, int W, int wcs, int WCSP, int ldb, int S, int64_t seg_stride, int ldp,
-                                  const float* __restrict__ part, double2* __restrict__ tsum)
+// ascending order, FP64) of part_s[t][q][k], real and imaginary parts, turned
+// into the triangle's value (epi_tri_value) at tsum[t * ldb + q * WCSP + k % wcs]:
+// the values the vertex epilogue forms, once per triangle instead of once per
+// incident vertex.
+__global__ __launch_bounds__(256) void tri_epilogue_kernel(EpiArgs a, int nt, double2* __restrict__ tsum)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)nt * W) return;
-    const int t = (int)(idx / W), k = (int)(idx % W);
-    const int q = k / wcs, kk = k % wcs;
-    const float* pp = part + (int64_t)t * ldp + q * 2 * WCSP + kk;
+    if (idx >= (int64_t)nt * a.W) return;
+    const int t = (int)(idx / a.W), k = (int)(idx % a.W);
+    const int q = k / a.wcs, kk = k % a.wcs;
+    const int pc = q * a.WCSP + kk;
+    const float* pp = a.part + (int64_t)t * a.ldp + q * 2 * a.WCSP + kk;
     double pr = 0.0, pi = 0.0;
-    for (int s = 0; s < S; ++s) {
-        pr += (double)pp[(int64_t)s * seg_stride];
-        pi += (double)pp[(int64_t)s * seg_stride + WCSP];
+    for (int s = 0; s < a.S; ++s) {
+        pr += (double)pp[(int64_t)s * a.seg_stride];
+        pi += (double)pp[(int64_t)s * a.seg_stride + a.WCSP];
     }
-    tsum[(int64_t)t * ldb + q * WCSP + kk] = make_double2(pr, pi);
+    double qr, qi;
+    epi_tri_value(a, t, q, pc, pr, pi, qr, qi);
+    tsum[(int64_t)t * a.ldb + pc] = make_double2(qr, qi);
 }
 
 size_t bdr_workspace_bytes(const Handle& h, const BdrPlan& pl, bool need_rmin)
@@ -942,13 +946,12 @@ void bdr_end(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     EpiArgs& ea = pl.ea;
     ea.tsum = nullptr;
     if (use_tsum(pl)) {
-        // the segment sums once per triangle, then the vertex gather reads them
-        // (three vertices per triangle) instead of every segment's partials
+        // each triangle's value once, then the vertex gather reads them (three
+        // vertices per triangle) instead of every segment's partials
         const int nt = (int)h.nt;
         double2* ts = static_cast<double2*>(pl.ws.take(h.ws_tsum, sizeof(double2) * h.nt_pad * pl.ldb));
         const int64_t total = (int64_t)nt * ea.W;
-        hipLaunchKernelGGL(tri_segsum_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nt, ea.W,
-                           ea.wcs, ea.WCSP, ea.ldb, ea.S, ea.seg_stride, ea.ldp, ea.part, ts);
+        hipLaunchKernelGGL(tri_epilogue_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, ea, nt, ts);
         BAEMM_HIP(hipGetLastError());
         ea.tsum = ts;
     }

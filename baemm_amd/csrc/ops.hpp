@@ -22,7 +22,7 @@ struct EpiArgs {
     const float* area;
     const float* sdiag;
     const float* part;
-    const double2* tsum;   // non-null: the segment sums of part per triangle, ldb apart (tri_segsum_kernel)
+    const double2* tsum;   // non-null: the value of each triangle, ldb apart (tri_epilogue_kernel)
     const float2* b;
     const float* unscale;
     const int32_t* M_ptr;

@@ -113,6 +113,35 @@ constexpr int EPI_SEG = 8;
 // right-hand sides leave no lanes idle.
 __host__ __device__ inline int epi_lanes(int W) { return W <= 16 ? 16 : (W <= 32 ? 32 : 64); }
 
+// Value of triangle t in column pc = q * WCSP + kk of the boundary epilogue,
+// from its segment sums (pr, pi): unscaled, multiplied by the chunk's phase,
+// plus the single-layer diagonal term D_t b[t][pc].  The vertex epilogue and
+// tri_epilogue_kernel both form it here, so their values are the same bits.
+__device__ __forceinline__ void epi_tri_value(const EpiArgs& a, int t, int q, int pc, double pr, double pi,
+                                              double& qr, double& qi)
+{
+    const double us = a.unscale ? (double)a.unscale[pc] : 1.0;
+    qr = pr * us;
+    qi = pi * us;
+    {
+        const double rr = a.rotr[q], ri = a.roti[q];
+        const double tr = qr * rr - qi * ri;
+        qi = qr * ri + qi * rr;
+        qr = tr;
+    }
+    if (a.diag_on) {
+        const float2 bv = a.b[(int64_t)t * a.ldb + pc];
+        const double fr = (double)a.c1r[q] * a.sdiag[t] - (double)a.c1i[q] * a.kap[q];
+        const double fi = (double)a.c1i[q] * a.sdiag[t] + (double)a.c1r[q] * a.kap[q];
+        qr += fr * bv.x - fi * bv.y;
+        qi += fr * bv.y + fi * bv.x;
+    }
+}
+
+// Incident triangles per batch of the vertex gather over triangle values
+// (tri_epilogue_kernel): a vertex of a closed triangle mesh has about six.
+constexpr int EPI_TRI_VAL = 8;
+
 // Vertex v of the boundary epilogue, columns lane, lane + lpv, ...:
 // C[v][k] = alpha * ( sum_{t ni v} a_t/3 (sum_s part_s[t][k] * unscale + D_t b[t][k])
 //                     + c0_q (M B)[v][k] ) + beta * C[v][k]
@@ -127,8 +156,31 @@ __device__ __forceinline__ void epi_vertex(const EpiArgs& a, int v, int lane, in
         const int q = k / a.wcs, kk = k % a.wcs;
         const int pc = q * a.WCSP + kk;
         double sr = 0.0, si = 0.0;
-        if (a.kernel_on) {
-            const double us = a.unscale ? (double)a.unscale[pc] : 1.0;
+        if (a.kernel_on && a.tsum) {
+            // the triangles' values, formed once per triangle beforehand; all
+            // loads of a batch are issued before the first is added, and the
+            // sums run in the order of the loop below
+            const int e0 = a.vt_ptr[v], e1 = a.vt_ptr[v + 1];
+            for (int e = e0; e < e1; e += EPI_TRI_VAL) {
+                int t[EPI_TRI_VAL];
+                double2 tv[EPI_TRI_VAL];
+                float ar[EPI_TRI_VAL];
+#pragma unroll
+                for (int u = 0; u < EPI_TRI_VAL; ++u) t[u] = a.vt_idx[min(e + u, e1 - 1)];
+#pragma unroll
+                for (int u = 0; u < EPI_TRI_VAL; ++u) {
+                    tv[u] = a.tsum[(int64_t)t[u] * a.ldb + pc];
+                    ar[u] = a.area[t[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < EPI_TRI_VAL; ++u) {
+                    if (e + u >= e1) break;
+                    const double w = (double)ar[u] / 3.0;
+                    sr += w * tv[u].x;
+                    si += w * tv[u].y;
+                }
+            }
+        } else if (a.kernel_on) {
             const int e0 = a.vt_ptr[v], e1 = a.vt_ptr[v + 1];
             // EPI_TRI triangles and EPI_SEG segments per batch: every load of a
             // batch is issued before the first is added (indices clamped to the
@@ -143,16 +195,7 @@ __device__ __forceinline__ void epi_vertex(const EpiArgs& a, int v, int lane, in
                     t[u] = a.vt_idx[min(e + u, e1 - 1)];
                     pr[u] = pi[u] = 0.0;
                 }
-                if (a.tsum) {
-                    // the same sums, formed once per triangle beforehand
-#pragma unroll
-                    for (int u = 0; u < EPI_TRI; ++u) {
-                        const double2 ts = a.tsum[(int64_t)t[u] * a.ldb + pc];
-                        pr[u] = ts.x;
-                        pi[u] = ts.y;
-                    }
-                }
-                for (int s0 = 0; s0 < (a.tsum ? 0 : a.S); s0 += EPI_SEG) {
+                for (int s0 = 0; s0 < a.S; s0 += EPI_SEG) {
                     float xr[EPI_TRI][EPI_SEG], xi[EPI_TRI][EPI_SEG];
 #pragma unroll
                     for (int u = 0; u < EPI_TRI; ++u) {
@@ -176,20 +219,8 @@ __device__ __forceinline__ void epi_vertex(const EpiArgs& a, int v, int lane, in
 #pragma unroll
                 for (int u = 0; u < EPI_TRI; ++u) {
                     if (e + u >= e1) break;
-                    double qr = pr[u] * us, qi = pi[u] * us;
-                    {
-                        const double rr = a.rotr[q], ri = a.roti[q];
-                        const double tr = qr * rr - qi * ri;
-                        qi = qr * ri + qi * rr;
-                        qr = tr;
-                    }
-                    if (a.diag_on) {
-                        const float2 bv = a.b[(int64_t)t[u] * a.ldb + pc];
-                        const double fr = (double)a.c1r[q] * a.sdiag[t[u]] - (double)a.c1i[q] * a.kap[q];
-                        const double fi = (double)a.c1i[q] * a.sdiag[t[u]] + (double)a.c1r[q] * a.kap[q];
-                        qr += fr * bv.x - fi * bv.y;
-                        qi += fr * bv.y + fi * bv.x;
-                    }
+                    double qr, qi;
+                    epi_tri_value(a, t[u], q, pc, pr[u], pi[u], qr, qi);
                     const double w = (double)a.area[t[u]] / 3.0;
                     sr += w * qr;
                     si += w * qi;
