@@ -827,9 +827,21 @@ __global__ __launch_bounds__(256) void tri_epilogue_kernel(EpiArgs a, int nt, do
     const int pc = q * a.WCSP + kk;
     const float* pp = a.part + (int64_t)t * a.ldp + q * 2 * a.WCSP + kk;
     double pr = 0.0, pi = 0.0;
-    for (int s = 0; s < a.S; ++s) {
-        pr += (double)pp[(int64_t)s * a.seg_stride];
-        pi += (double)pp[(int64_t)s * a.seg_stride + a.WCSP];
+    // EPI_SEG segments per batch, every load issued before the first is added
+    for (int s0 = 0; s0 < a.S; s0 += EPI_SEG) {
+        float xr[EPI_SEG], xi[EPI_SEG];
+#pragma unroll
+        for (int j = 0; j < EPI_SEG; ++j) {
+            const int64_t o = (int64_t)min(s0 + j, a.S - 1) * a.seg_stride;
+            xr[j] = pp[o];
+            xi[j] = pp[o + a.WCSP];
+        }
+#pragma unroll
+        for (int j = 0; j < EPI_SEG; ++j)
+            if (s0 + j < a.S) {
+                pr += (double)xr[j];
+                pi += (double)xi[j];
+            }
     }
     double qr, qi;
     epi_tri_value(a, t, q, pc, pr, pi, qr, qi);
