@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void batch_colmax_kernel(const StageDesc* __re
 
 // Per MFMA instance: the hi/lo B operand (K x nst x NCB x 64 elements) with
 // the column scales taken from the maxima, then the scales and unscales
-// (ldb), as colscale_kernel and split_kernel compute them.
+// (ldb), as split_kernel computes them.
 __global__ void batch_split_kernel(const StageDesc* __restrict__ sd, const int64_t* __restrict__ off, int n)
 {
     const int64_t total = off[n];

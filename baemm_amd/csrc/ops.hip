@@ -66,12 +66,6 @@ __global__ void fill_u32_kernel(unsigned* __restrict__ p, int n, unsigned v)
     if (i < n) p[i] = v;
 }
 
-__global__ void zero_u32_kernel(unsigned* __restrict__ p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
 // Per-column max of |re|, |im| (positive floats order like their bit patterns).
 // Block: 4 row lanes x 64 columns; blocks stride over rows; one atomicMax per
 // column and block.  colmax must be zero on entry.
@@ -97,39 +91,39 @@ __global__ __launch_bounds__(256) void colmax_kernel(const float2* __restrict__ 
     }
 }
 
-// Power-of-two scale per column so that max |s b| lies in [2^13, 2^14);
-// unscale[pc] = 1 / (sA_q * s_pc).
+// The power-of-two A scale sA_q of each chunk q.  The B scale s_pc of column
+// pc is col_scale(colmax[pc]), a power of two that puts max |s b| in
+// [2^13, 2^14); the epilogue multiplies by unscale[pc] = 1 / (sA_q * s_pc).
 struct ChunkScales {
     float sA[64];
 };
 
-__global__ void colscale_kernel(const unsigned* __restrict__ colmax, int ncols, int WCSP, ChunkScales cs,
-                                float* __restrict__ scale, float* __restrict__ unscale)
-{
-    const int pc = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pc >= ncols) return;
-    const float s = col_scale(colmax[pc]);
-    scale[pc] = s;
-    unscale[pc] = 1.0f / (s * cs.sA[pc / WCSP]);
-}
-
-// B-operand of the MFMA kernel (split_elem), with the column scales of colscale_kernel.
+// B-operand of the MFMA kernel (split_elem) with the column scales
+// col_scale(colmax[pc]); the first ncols threads also write
+// unscale[pc] = 1 / (sA_q * s_pc) (ncols < the element count).
 __global__ void split_kernel(const float2* __restrict__ b, int ldb, int nst, int WCSP, int K,
-                             const float* __restrict__ scale, uint4* __restrict__ bop)
+                             const unsigned* __restrict__ colmax, int ncols, ChunkScales cs,
+                             float* __restrict__ unscale, uint4* __restrict__ bop)
 {
     const int NCB = 2 * WCSP / 32;
     const int64_t total = (int64_t)K * nst * NCB * 64;
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    split_elem(b, ldb, nst, WCSP, idx, [&](int pc) { return scale[pc]; }, bop);
+    if (idx < ncols) {
+        const int pc = (int)idx;
+        unscale[pc] = 1.0f / (col_scale(colmax[pc]) * cs.sA[pc / WCSP]);
+    }
+    split_elem(b, ldb, nst, WCSP, idx, [&](int pc) { return col_scale(colmax[pc]); }, bop);
 }
 
-// out[j] = s * nrm[j]: source normals carrying the common scale of the
-// non-adjoint MFMA kernel values.
-__global__ void scale_nrm_kernel(const float4* __restrict__ nrm, int n, float s, float4* __restrict__ out)
+// out[j] = s * nrm[j] for j < n: source normals carrying the common scale of
+// the non-adjoint MFMA kernel values (out NULL: none); zero[j] = 0 for j < nzero.
+__global__ void scale_nrm_kernel(const float4* __restrict__ nrm, int n, float s, float4* __restrict__ out,
+                                 unsigned* __restrict__ zero, int nzero)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+    if (j < nzero) zero[j] = 0u;
+    if (j >= n || !out) return;
     const float4 v = nrm[j];
     out[j] = make_float4(s * v.x, s * v.y, s * v.z, v.w);
 }
@@ -674,6 +668,8 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
     pl.sA.assign(p.K, 1.0f);
     pl.rot.assign(p.K, cfloat(1.0f, 0.0f));
     ChunkScales csc{};
+    float nrm_sig = 0.0f;      // the common scale of the source normals (non-adjoint form)
+    float4* nrm_out = nullptr;  // the scaled normals (non-adjoint form)
     if (!pl.adj) {
         // Every chunk has c2 != 0: the kernel values are taken as A' = s A / c2
         // with one scale s for all chunks, carried by the source normals
@@ -699,10 +695,9 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
             pl.sd.nrm_in = h.d_nrm;
             pl.sd.nrm_out = nrm;
             pl.sd.nrm_s = sig;
-        } else {
-            hipLaunchKernelGGL(scale_nrm_kernel, dim3((unsigned)((h.nt_pad + 255) / 256)), dim3(256), 0, s, h.d_nrm,
-                               (int)h.nt_pad, sig, nrm);
         }
+        nrm_sig = sig;
+        nrm_out = nrm;
         pl.nrm = nrm;
     } else {
         for (int q = 0; q < p.K; ++q) {
@@ -738,12 +733,15 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
         for (int q = 0; q < p.K; ++q) d.sA[q] = csc.sA[q];
         return;
     }
-    hipLaunchKernelGGL(zero_u32_kernel, dim3((ncols + 255) / 256), dim3(256), 0, s, colmax, ncols);
+    // three launches: the scaled source normals (non-adjoint form) with the
+    // column maxima cleared; the column maxima; the split operand with the
+    // column scales and their inverses
+    const int nn = pl.adj ? 0 : (int)h.nt_pad;
+    hipLaunchKernelGGL(scale_nrm_kernel, dim3((unsigned)((std::max(nn, ncols) + 255) / 256)), dim3(256), 0, s,
+                       h.d_nrm, nn, nrm_sig, nrm_out, colmax, ncols);
     hipLaunchKernelGGL(colmax_kernel, dim3(256), dim3(256), 0, s, pl.b, (int)h.nt, pl.ldb, colmax);
-    hipLaunchKernelGGL(colscale_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, colmax, ncols, pl.WCSP, csc, scale,
-                       unscale);
     hipLaunchKernelGGL(split_kernel, dim3((unsigned)((nbop + 255) / 256)), dim3(256), 0, s, pl.b, pl.ldb, pl.nst,
-                       pl.WCSP, (int)p.K, scale, bop);
+                       pl.WCSP, (int)p.K, colmax, ncols, csc, unscale, bop);
     BAEMM_HIP(hipGetLastError());
 }
 
