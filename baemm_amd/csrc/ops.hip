@@ -1,6 +1,7 @@
 // ops.hip -- device-resident operator applies: staging, pair-kernel dispatch,
 // epilogues (AvT, SL diagonal, mass, alpha/beta) and the incident waves.
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -89,6 +90,39 @@ __global__ __launch_bounds__(256) void colmax_kernel(const float2* __restrict__ 
         }
         __syncthreads();
     }
+}
+
+// b = Av B (av_elem) and the maxima of |re|, |im| over each column of b
+// (positive floats order like their bit patterns).  Threads stride over the
+// elements by the grid's thread count, a multiple of ldb, so each thread stays
+// in one column: its running maximum, then per block in LDS (ldb entries of
+// dynamic shared memory), then one atomicMax per column and block.  colmax
+// must be zero on entry.
+// Largest grid of av_gather_max_kernel (more blocks: the separate gather and
+// colmax_kernel).
+constexpr int64_t kGatherMaxBlocks = 512;
+
+template <int BT>
+__global__ __launch_bounds__(256) void av_gather_max_kernel(const int4* __restrict__ tris,
+                                                            const float* __restrict__ area, int nt, int64_t total,
+                                                            const void* __restrict__ Bv, int64_t ldB, int wcs,
+                                                            int WCSP, int ldb, float2* __restrict__ b,
+                                                            unsigned* __restrict__ colmax)
+{
+    extern __shared__ unsigned s_max[];
+    for (int c = threadIdx.x; c < ldb; c += blockDim.x) s_max[c] = 0u;
+    __syncthreads();
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float m = 0.0f;
+    for (int64_t idx = first; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+        const float2 v = av_elem<BT>(tris, area, nt, Bv, ldB, wcs, WCSP, ldb, idx);
+        b[idx] = v;
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    if (m > 0.0f) atomicMax(&s_max[(int)(first % ldb)], __float_as_uint(m));
+    __syncthreads();
+    for (int c = threadIdx.x; c < ldb; c += blockDim.x)
+        if (s_max[c]) atomicMax(&colmax[c], s_max[c]);
 }
 
 // The power-of-two A scale sA_q of each chunk q.  The B scale s_pc of column
@@ -642,11 +676,12 @@ static void launch_av(Handle& h, const void* B, int64_t ldB, int Btype, int WCSP
 // scales pl.sA folded into the coefficients, the source normals pl.nrm the
 // kernel reads, and pl.unscale (per column, null for FP32 MFMA).  rmin: lower
 // bound of the target-centroid distances (mixed precision only).
-static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
+static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s, const void* B, int64_t ldB, int Btype)
 {
     const Params& p = h.prm;
     if (pl.f32) {
         if (pl.defer) throw ArgError{"internal: deferred staging of the FP32 MFMA operand"};
+        launch_av(h, B, ldB, Btype, pl.WCSP, pl.b, pl.ldb, s);
         // FP32 MFMA: no FP16 range to fit, no scales; the non-adjoint form still
         // divides each chunk by c2 (the epilogue multiplies it back)
         pl.sA.assign(p.K, 1.0f);
@@ -733,13 +768,31 @@ static void stage_mfma(Handle& h, BdrPlan& pl, float rmin, hipStream_t s)
         for (int q = 0; q < p.K; ++q) d.sA[q] = csc.sA[q];
         return;
     }
-    // three launches: the scaled source normals (non-adjoint form) with the
-    // column maxima cleared; the column maxima; the split operand with the
-    // column scales and their inverses
+    // the scaled source normals (non-adjoint form) with the column maxima
+    // cleared; the Av gather and the column maxima (one launch for small
+    // meshes, where the launches are the cost; two above, where the blocks'
+    // atomics on the few maxima of the one launch cost more); the split operand
+    // with the column scales and their inverses
     const int nn = pl.adj ? 0 : (int)h.nt_pad;
     hipLaunchKernelGGL(scale_nrm_kernel, dim3((unsigned)((std::max(nn, ncols) + 255) / 256)), dim3(256), 0, s,
                        h.d_nrm, nn, nrm_sig, nrm_out, colmax, ncols);
-    hipLaunchKernelGGL(colmax_kernel, dim3(256), dim3(256), 0, s, pl.b, (int)h.nt, pl.ldb, colmax);
+    const int64_t total = h.nt_pad * (int64_t)pl.ldb;
+    if ((total + 255) / 256 <= kGatherMaxBlocks) {
+        // a multiple of ldb / gcd(ldb, 256) blocks: the grid's thread count is
+        // a multiple of ldb
+        const int64_t unit = pl.ldb / std::gcd(pl.ldb, 256);
+        const dim3 grid((unsigned)(((total + 255) / 256 + unit - 1) / unit * unit));
+        const size_t lds = sizeof(unsigned) * (size_t)pl.ldb;
+        if (Btype == kC128)
+            hipLaunchKernelGGL(av_gather_max_kernel<kC128>, grid, dim3(256), lds, s, h.d_tris, h.d_area, (int)h.nt,
+                               total, B, ldB, (int)p.wcs, pl.WCSP, pl.ldb, pl.b, colmax);
+        else
+            hipLaunchKernelGGL(av_gather_max_kernel<kC64>, grid, dim3(256), lds, s, h.d_tris, h.d_area, (int)h.nt,
+                               total, B, ldB, (int)p.wcs, pl.WCSP, pl.ldb, pl.b, colmax);
+    } else {
+        launch_av(h, B, ldB, Btype, pl.WCSP, pl.b, pl.ldb, s);
+        hipLaunchKernelGGL(colmax_kernel, dim3(256), dim3(256), 0, s, pl.b, (int)h.nt, pl.ldb, colmax);
+    }
     hipLaunchKernelGGL(split_kernel, dim3((unsigned)((nbop + 255) / 256)), dim3(256), 0, s, pl.b, pl.ldb, pl.nst,
                        pl.WCSP, (int)p.K, colmax, ncols, csc, unscale, bop);
     BAEMM_HIP(hipGetLastError());
@@ -891,7 +944,7 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
         d.WCSP = pl.WCSP;
         d.ldb = pl.ldb;
         d.b = pl.b;
-    } else {
+    } else if (!pl.mixed) {
         launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
     }
     ea.b = pl.b;
@@ -899,7 +952,8 @@ void bdr_begin(Handle& h, const ApplyArgs& a, BdrPlan& pl, hipStream_t s)
     ea.ldb = pl.ldb;
     ea.unscale = nullptr;
     if (!pl.mixed) return;
-    stage_mfma(h, pl, pl.f32 ? 0.0f : compute_rmin(h, pl.ws, s), s);
+    // (not deferred: the Av gather too)
+    stage_mfma(h, pl, pl.f32 ? 0.0f : compute_rmin(h, pl.ws, s), s, a.B, a.ldB, a.Btype);
     ea.unscale = pl.unscale;
     for (int q = 0; q < p.K; ++q) {
         ea.rotr[q] = pl.rot[q].real();
@@ -1072,11 +1126,10 @@ static void apply_near_mfma(Handle& h, const ApplyArgs& a, const float4* d_pts, 
     pl.nrm = h.d_nrm;
     pl.nst = (int)(h.nt_pad / 8);
     pl.b = static_cast<float2*>(pl.ws.take(h.ws_b32, sizeof(float2) * h.nt_pad * pl.ldb));
-    launch_av(h, a.B, a.ldB, a.Btype, pl.WCSP, pl.b, pl.ldb, s);
     stage_mfma(h, pl,
                pl.f32 ? 0.0f
                       : min_distance(h, d_pts, P, false, static_cast<unsigned*>(pl.ws.take(h.ws_misc, 256)), s),
-               s);
+               s, a.B, a.ldB, a.Btype);
     const int P_pad = (P + kMixTile - 1) / kMixTile * kMixTile;
     const int nst = pl.nst;
     const int tiles = P_pad / kMixTile;
