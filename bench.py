@@ -18,6 +18,8 @@ outputs resident in HBM.  Reported:
                      FP16 MFMA peak; roofline_valu: kernel-generation FLOP/s vs FP32 peak
   cpu_baseline       FP32 restatement of the reference OpenCL kernel (oracle) on the
                      host cores, bounded sample of target rows, same workload
+  mesh_sizes         (--mesh-sizes 3,4,...) the same operator on icospheres of other
+                     sizes, mixed and FP32 step and kernel times
 """
 import argparse
 import json
@@ -307,6 +309,72 @@ def run_potential_leg(repeats=3):
     return out
 
 
+def run_mesh_size_leg(levels):
+    """The operator of the headline workload (SL+DL, kappa {8, 16} x 32 RHS =
+    64 columns, complex N(0,1) vertex values) on icospheres of the given
+    subdivision levels (3: 1,280 ... 7: 327,680 triangles), device-resident
+    applies in mixed precision (automatic kernel choice) and FP32: wall time per
+    step over enough steps for about 0.2 s, and the pair kernels' HIP-event time
+    per step."""
+    import ctypes
+
+    from baemm_amd import Helmholtz_HIP, _lib
+    from baemm_amd.meshes import icosphere
+
+    L = _lib.load()
+    W, wcs = 64, 32
+    kappa = [8.0, 16.0]
+    coeff = np.array([0, 1, 1, 0, 0.5, -1j * 16.0, 1, 0], np.complex128)
+    one = (ctypes.c_double * 2)(1.0, 0.0)
+    zero = (ctypes.c_double * 2)(0.0, 0.0)
+    out = {}
+    for level in levels:
+        V, T = icosphere(level)
+        nt, nv = T.shape[0], V.shape[0]
+        log(f"mesh-size leg: {nt} triangles")
+        rng = np.random.default_rng(1)
+        B = np.ascontiguousarray(rng.standard_normal((nv, W)) + 1j * rng.standard_normal((nv, W)))
+        nbytes = B.nbytes
+        dB, dC = ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(L.baemm_dev_alloc(ctypes.byref(dB), nbytes))
+        _lib.check(L.baemm_dev_alloc(ctypes.byref(dC), nbytes))
+        _lib.check(L.baemm_dev_upload(dB, B.ctypes.data_as(ctypes.c_void_p), nbytes))
+        H = Helmholtz_HIP(V, T)
+        h = H._h
+        H.LoadBoundaryOperators_PL(kappa, coeff, W, wcs)
+        res = {"triangles": nt}
+        for mode in ("mixed", "fp32"):
+            H.SetPrecision(mode)
+
+            def step():
+                _lib.check(L.baemm_apply_boundary_dev(h, one, dB, W, _lib.C128, zero, dC, W, _lib.C128, None), h)
+
+            step()
+            _lib.check(L.baemm_dev_synchronize())
+            t0 = time.perf_counter()
+            step()
+            _lib.check(L.baemm_dev_synchronize())
+            n = int(min(2000, max(3, 0.2 / max(time.perf_counter() - t0, 1e-6))))
+            t0 = time.perf_counter()
+            for _ in range(n):
+                step()
+            _lib.check(L.baemm_dev_synchronize())
+            wall = (time.perf_counter() - t0) / n
+            H.SetTiming(True)
+            for _ in range(n):
+                step()
+            _lib.check(L.baemm_dev_synchronize())
+            kms, _ = H.GetTiming()
+            H.SetTiming(False)
+            res[mode] = {"steps": n, "ms_per_step": wall * 1e3, "kernel_ms": kms / n,
+                         "pair_rhs_per_s": float(nt) * nt * W / wall}
+        H.close()
+        L.baemm_dev_free(dB)
+        L.baemm_dev_free(dC)
+        out[str(nt)] = res
+    return out
+
+
 def run_batch_leg(repeats=5, seed=3, n_inst=24, levels=(3, 6), gmres=True):
     """Batched uneven instances: n_inst perturbed icospheres (radius 1 + 0.1 sum
     of real spherical harmonics of degree <= 4, seeded N(0,1)/(l+1)^2 weights)
@@ -507,11 +575,20 @@ def main():
     ap.add_argument("--serve-max-level", type=int, default=6,
                     help="largest icosphere level of the serving leg (6: 81920, 7: 327680 triangles)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--mesh-sizes", default="",
+                    help="comma-separated icosphere levels (3: 1,280 ... 7: 327,680 triangles) for an extra leg "
+                         "with the headline operator on each mesh (default: none)")
+    ap.add_argument("--only-mesh-sizes", action="store_true",
+                    help="run only the --mesh-sizes leg and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
               "the apply would shard by target rows (one all-reduce of vertex sums) or by request.", file=sys.stderr)
         sys.exit(2)
+    levels = [int(x) for x in args.mesh_sizes.split(",") if x.strip()]
+    if args.only_mesh_sizes:
+        print(json.dumps({"mesh_sizes": run_mesh_size_leg(levels)}))
+        return
     nt, nv, W, res, rel = run_gpu(args)
     pairs_rhs = float(nt) * nt * W
     mixed = res["mixed"]
@@ -607,6 +684,8 @@ def main():
             "workload": "96 perturbed icospheres of 320 or 1280 triangles, otherwise as above", **small}
     if not args.no_potential:
         line["potential"] = run_potential_leg()
+    if levels:
+        line["mesh_sizes"] = run_mesh_size_leg(levels)
     if not args.no_solve:
         line["solve"] = dict(workload="sound-soft torus 38400 triangles, kappa 10, 32 plane waves, FarField, "
                                       "GMRES(30) 1e-6, 256 far-field directions", **run_solve_leg(graphs=not args.no_graphs))
