@@ -39,7 +39,10 @@ def sphere3():
 @pytest.mark.parametrize(
     "W,wcs,kappa",
     [(16, 16, [4.0]), (8, 4, [4.0, 2.5]), (12, 3, [1.0, 2.0, 3.0, 4.0]), (64, 32, [8.0, 16.0]), (1, 1, [3.0]),
-     (64, 16, [1.0, 2.0, 3.0, 5.0]), (33, 1, [1.0 + 0.25 * k for k in range(33)])],
+     (64, 16, [1.0, 2.0, 3.0, 5.0]), (33, 1, [1.0 + 0.25 * k for k in range(33)]),
+     # staged widths of 48 and 80 columns (three and five chunks of 16), not
+     # powers of two, on a mesh small enough for the one-launch gather
+     (36, 12, [1.5, 3.0, 4.5]), (20, 4, [1.0, 2.0, 3.0, 4.0, 5.0])],
 )
 def test_boundary_apply_matches_oracle(sphere3, mode, W, wcs, kappa):
     m, H = sphere3
